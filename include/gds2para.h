@@ -1,0 +1,247 @@
+/*
+ * gds2para.h -- C ABI of the MI355X (gfx950) simulate-mode hot path.
+ *
+ * Plain pointers, fixed-width integers, no C++ or torch types.  Every entry
+ * point returns a gp_status; nothing calls exit().  A gp_layout owns the
+ * device-resident copies of one mesh, its stiffness matrix S, its V0 basis and
+ * its multigrid hierarchy, so repeated solves on the same layout re-use them.
+ *
+ * Reference interfaces replaced (paths relative to the gds2Para tree):
+ *   gp_assemble_stiffness   int generateStiff(fdtdMesh*)            src/fdtd.hpp:4448, src/generateStiff.cpp:6
+ *                           + mklMatrixMulti_nt / COO2CSR_malloc    src/generateStiff.cpp:871-980
+ *   gp_build_v0             fdtdMesh::merge_v0d1 / generateAd /
+ *                           merge_v0c / generateAc                  src/fdtd.hpp:1823,2778,2866,3481
+ *   gp_solve_fullwave       int reference(fdtdMesh*, int, myint*, myint*, double*)  src/generateStiff.cpp:568
+ *                           void fdtdMesh::reference1(int, int, complex<double>*)   src/fdtd.hpp:3963
+ *   gp_solve_real           int hypreSolve(fdtdMesh*, myint*, myint*, double*, myint,
+ *                                          double*, myint, double*)  src/hypreSolver.h:14
+ *   gp_extract_z            fdtdMesh::Construct_Z_V0_Vh             src/fdtd.hpp:4133
+ *   gp_freq_of              fdtdMesh::freqNo2freq                   src/fdtd.hpp:4319
+ *   gp_edge_map             fdtdMesh::setMapEdge                    src/fdtd.hpp:1508
+ */
+#ifndef GDS2PARA_H
+#define GDS2PARA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* status                                                                    */
+/* ------------------------------------------------------------------------ */
+typedef enum gp_status {
+  GP_OK = 0,
+  GP_ERR_ARG = 1,      /* malformed mesh or argument                         */
+  GP_ERR_HIP = 2,      /* a HIP runtime call failed                          */
+  GP_ERR_NOMEM = 3,    /* device or host allocation failed / over budget     */
+  GP_ERR_NOCONV = 4,   /* solve stopped at max_iters above the tolerance     */
+  GP_ERR_STATE = 5,    /* stage called before the stage it depends on        */
+  GP_ERR_NODEVICE = 6  /* no gfx950 device visible                           */
+} gp_status;
+
+/* Human-readable text of the last error on this thread. */
+const char *gp_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* mesh contract (what meshAndMark / portSet leave in fdtdMesh)              */
+/* ------------------------------------------------------------------------ */
+/*
+ * Numbering follows the reference (src/fdtd.hpp:4107-4130, src/mesh.cpp:395-410):
+ *   Es = Ny(Nx+1) + Nx(Ny+1) surface edges per plane, Ev = (Nx+1)(Ny+1) vertical
+ *   edges per layer, plane iz holds Es surface edges then (iz<Nz) Ev z-edges.
+ *   y-edge (ix,iy):  iz(Es+Ev) + ix*Ny + iy
+ *   x-edge (ix,iy):  iz(Es+Ev) + Ny(Nx+1) + ix(Ny+1) + iy
+ *   z-edge (ix,iy):  iz(Es+Ev) + Es + ix(Ny+1) + iy
+ *   node:            iz*Ev + ix(Ny+1) + iy
+ * with Nx = nx-1 cells etc.  All arrays are host pointers owned by the caller.
+ */
+typedef struct gp_mesh {
+  int64_t nx, ny, nz;             /* node counts per axis (N_cell_* + 1)          */
+  const double *xn, *yn, *zn;     /* node coordinates (m), strictly increasing    */
+  const double *stack_epsn;       /* [nz] relative permittivity by z-plane        */
+  const double *stack_sign;       /* [nz] conductivity (S/m) by z-plane           */
+  const int32_t *mark_edge;       /* [N_edge] conductor id (1-based) or 0         */
+  const int32_t *mark_node;       /* [N_node] conductor id (1-based) or 0         */
+  int32_t num_cdt;                /* number of isolated conductors                */
+  const int32_t *cdt_mark_port;   /* [num_cdt] -1 reference, 0 floating, p+1 port */
+  int64_t num_bd_edges;           /* |lbde u ubde|                                */
+  const int64_t *bd_edges;        /* sorted ascending                             */
+  int64_t num_bd_nodes;           /* |lbdn u ubdn|                                */
+  const int64_t *bd_nodes;        /* sorted ascending                             */
+  int32_t num_ports;
+  const int32_t *port_side_ptr;   /* [num_ports+1] offsets into sides             */
+  const int64_t *side_edge_ptr;   /* [num_sides+1] offsets into side_edges        */
+  const int64_t *side_edges;      /* portEdge lists (original edge ids)           */
+  const double *side_area;        /* [num_sides] portArea (m^2)                   */
+  const int32_t *side_dir;        /* [num_sides] portDirection (+1/-1)            */
+  const int32_t *side_cnd;        /* [num_sides] portCnd (1-based conductor id)   */
+  double freq_unit, freq_start, freq_end;
+  int32_t nfreq, freq_scale;      /* freq_scale 1 = linear, otherwise log         */
+} gp_mesh;
+
+typedef struct gp_mesh_sizes {
+  int64_t n_edge, n_edge_s, n_edge_v, n_node, n_node_s, n_patch;
+  int64_t n_unknowns;             /* N_edge - bden                                */
+  int64_t nnz_s;                  /* nnz of S after PEC removal (closed form)     */
+  int32_t num_sides;
+} gp_mesh_sizes;
+
+/* Closed-form sizes; pure host arithmetic, valid without a GPU. */
+gp_status gp_mesh_sizes_of(const gp_mesh *mesh, gp_mesh_sizes *out);
+
+/* Structural validation of a mesh (bounds, sortedness, boundary sets).  Host only. */
+gp_status gp_mesh_validate(const gp_mesh *mesh);
+
+/* mapEdge[N_edge] (new id or -1) and mapEdgeR[n] exactly as setMapEdge builds them. Host only. */
+gp_status gp_edge_map(const gp_mesh *mesh, int64_t *map_edge, int64_t *map_edge_r);
+
+/* freqNo2freq for the mesh's frequency grid. Host only. */
+double gp_freq_of(const gp_mesh *mesh, int32_t freq_no);
+
+/* ------------------------------------------------------------------------ */
+/* synthetic layouts (stand-in for the GDSII/sim_input front end)            */
+/* ------------------------------------------------------------------------ */
+typedef enum gp_gen_kind {
+  GP_GEN_MICROSTRIP = 0,          /* one strip over a ground plane, one z-port    */
+  GP_GEN_MANHATTAN = 1            /* random Manhattan metal layers with vias      */
+} gp_gen_kind;
+
+typedef struct gp_gen_params {
+  int32_t kind;                   /* gp_gen_kind                                  */
+  int32_t nx_cells, ny_cells, nz_cells;
+  double dx, dy, dz;              /* nominal cell sizes (m)                       */
+  double grade;                   /* 0 = uniform; >0 = sinusoidal spacing ripple  */
+  int32_t n_metal;                /* metal layers (MANHATTAN)                     */
+  double fill;                    /* target area fill per metal layer             */
+  int32_t wmin, wmax;             /* rectangle widths in cells                    */
+  double via_density;             /* fraction of cells holding a via              */
+  double eps_lo, eps_hi;          /* per-layer relative permittivity range        */
+  double sigma;                   /* conductor conductivity (S/m)                 */
+  int32_t n_ports;
+  int32_t nfreq, freq_scale;
+  double f_start, f_end;          /* Hz                                           */
+  uint64_t seed;
+} gp_gen_params;
+
+typedef struct gp_owned_mesh gp_owned_mesh;   /* opaque; owns every array of .mesh */
+
+gp_status gp_generate(const gp_gen_params *params, gp_owned_mesh **out);
+const gp_mesh *gp_owned_mesh_view(const gp_owned_mesh *m);
+void gp_owned_mesh_free(gp_owned_mesh *m);
+
+/* ------------------------------------------------------------------------ */
+/* device runtime                                                            */
+/* ------------------------------------------------------------------------ */
+typedef struct gp_device_info {
+  int32_t device_count;
+  char arch[64];                  /* gcnArchName of device 0                      */
+  int64_t total_mem;              /* bytes                                        */
+  int32_t cu_count;
+} gp_device_info;
+
+gp_status gp_device_query(gp_device_info *out);
+
+/* ------------------------------------------------------------------------ */
+/* layout handle: device-resident mesh, S, V0, hierarchy                     */
+/* ------------------------------------------------------------------------ */
+typedef struct gp_layout gp_layout;
+
+/* Validates and uploads the mesh; builds mapEdge on the device. */
+gp_status gp_layout_create(const gp_mesh *mesh, gp_layout **out);
+void gp_layout_destroy(gp_layout *lay);
+
+/* Device bytes currently held by the layout (matrices, hierarchy, workspaces). */
+int64_t gp_layout_device_bytes(const gp_layout *lay);
+
+/* ---- stiffness (generateStiff + mklMatrixMulti_nt) ---------------------- */
+/* Assembles S on the device: CSR, int64 row pointers, int32 columns in the
+ * renumbered (mapEdge) index space, FP64 values v/MU.  Rows are the kept edges
+ * in ascending order, columns ascending within a row. */
+gp_status gp_assemble_stiffness(gp_layout *lay);
+/* Host copy in CSR form; arrays sized n+1, nnz, nnz. */
+gp_status gp_stiffness_copy_csr(const gp_layout *lay, int64_t *row_ptr, int32_t *col, double *val);
+/* Host copy in the reference's COO layout (SRowId, SColId, Sval). */
+gp_status gp_stiffness_copy_coo(const gp_layout *lay, int64_t *row, int64_t *col, double *val);
+
+/* ---- V0 basis and nodal Laplacians ------------------------------------- */
+typedef struct gp_v0_sizes {
+  int64_t v0d_nnz, v0d_cols;      /* v0d1num, leng_v0d1                           */
+  int64_t ad_nnz, ad_len;         /* kept entries, leng_Ad (incl. trailing zeros) */
+  int64_t v0c_nnz, v0c_cols;      /* v0cnum, leng_v0c                             */
+  int64_t ac_nnz;                 /* leng_Ac                                      */
+} gp_v0_sizes;
+
+gp_status gp_build_v0(gp_layout *lay);
+gp_status gp_v0_sizes_get(const gp_layout *lay, gp_v0_sizes *out);
+/* V0d in the reference's COO entry order: row = original edge id, col, val, aval
+ * (before the sqrt(eps) scaling that paraGenerator applies). */
+gp_status gp_v0d_copy(const gp_layout *lay, int64_t *row, int64_t *col, double *val, double *aval);
+/* Ad COO sorted (row, col), |v| > 1e-8 entries only. */
+gp_status gp_ad_copy(const gp_layout *lay, int64_t *row, int64_t *col, double *val);
+gp_status gp_v0c_copy(const gp_layout *lay, int64_t *row, int64_t *col, double *val, double *aval);
+gp_status gp_ac_copy(const gp_layout *lay, int64_t *row, int64_t *col, double *val);
+
+/* ---- full-wave solve (reference / reference1) -------------------------- */
+typedef enum gp_precision {
+  GP_PREC_FP64 = 0,               /* FP64 preconditioner, single RHS per solve    */
+  GP_PREC_MIXED = 1               /* FP32 hierarchy + V-cycle, FP64 outer FGMRES  */
+} gp_precision;
+
+typedef struct gp_solve_opts {
+  double rtol;                    /* relative residual ||b-Ax||/||b||             */
+  int32_t max_iters;              /* total Krylov iterations per column           */
+  int32_t restart;                /* FGMRES restart length m                      */
+  int32_t precision;              /* gp_precision                                 */
+  int32_t batch;                  /* columns per batch; 0 = autotuned, 1 = single */
+  int32_t verbose;
+} gp_solve_opts;
+
+typedef struct gp_solve_stats {
+  int32_t columns;                /* port x frequency columns solved              */
+  int32_t converged;              /* columns that reached rtol                    */
+  int32_t max_iters_used;
+  int32_t batches;
+  double max_rel_residual;        /* FP64 true residual, max over columns         */
+  double setup_ms, solve_ms;      /* device wall time                             */
+} gp_solve_stats;
+
+/* Fills opts with the defaults (rtol 1e-8, restart 30, MIXED, batch 0). */
+void gp_solve_opts_default(gp_solve_opts *opts);
+
+/* Builds (or re-uses) the multigrid hierarchy for the layout. */
+gp_status gp_prepare_solver(gp_layout *lay, int32_t precision);
+
+/* Solves (S + diag(-w^2 eps + i w sigma)) x = -i w J for every (port, freq)
+ * pair, exactly the system reference() hands to PARDISO.  freqs in Hz.
+ * X (optional): complex interleaved (re,im), column c = f*np + p, n rows each.
+ * Z (optional): nf*np*np complex, index k*np*np + resp + np*src, the layout
+ * Construct_Z_V0_Vh writes into fdtdMesh::x. */
+gp_status gp_solve_fullwave(gp_layout *lay, const double *freqs, int32_t nf,
+                            const int32_t *ports, int32_t np,
+                            const gp_solve_opts *opts,
+                            double *X, double *Z, gp_solve_stats *stats);
+
+/* Construct_Z_V0_Vh for one solution vector x (n complex, renumbered space):
+ * accumulates column src of the np*np block Zk (complex interleaved). Host only. */
+gp_status gp_extract_z(const gp_mesh *mesh, const double *x, int32_t source_port, double *Zk);
+
+/* ---- real nodal solve (hypreSolve) ------------------------------------- */
+/* Solves A x = b for a real sparse A given in the reference's row-sorted COO,
+ * FGMRES + multigrid on the device.  rtol 0 selects the reference's 1e-4. */
+gp_status gp_solve_real(const int64_t *row, const int64_t *col, const double *val, int64_t nnz,
+                        int64_t n, const double *b, double *x, double rtol,
+                        int32_t *iters_out, double *relres_out);
+
+/* ---- measurement hooks -------------------------------------------------- */
+/* Average device time (ms, HIP events on the library stream) of one complex
+ * A(w)*X product with k right-hand sides.  precision selects FP64 CSR or the
+ * FP32 level-0 operator. */
+gp_status gp_bench_spmv(gp_layout *lay, int32_t k, int32_t precision, int32_t iters, double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GDS2PARA_H */

@@ -1,0 +1,316 @@
+"""CPU checks that pin the oracle (oracle/gp_oracle.c) independently of itself.
+
+None of these re-types the oracle's formulas: they test closed forms (counts),
+mathematical identities of the discrete curl-curl operator (curl grad = 0,
+curl curl of polynomial fields, the gradient null space, the dual-volume
+symmetrizer), an independent assembly route (dense Sh @ Se.T), and physical
+properties of direct solves (reciprocity, capacitive scaling).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle
+from gds2para_amd import mesh as M
+
+MU = 4 * np.pi * 1e-7
+
+
+def grid_geometry(mesh):
+    """Edge directions, index triples, midpoints and lengths from the numbering contract."""
+    xn, yn, zn = mesh.coords()
+    Nx, Ny, Nz = len(xn) - 1, len(yn) - 1, len(zn) - 1
+    Es = Ny * (Nx + 1) + Nx * (Ny + 1)
+    Ev = (Nx + 1) * (Ny + 1)
+    El = Es + Ev
+    Ne = Es * (Nz + 1) + Ev * Nz
+    e = np.arange(Ne)
+    iz = e // El
+    r = e % El
+    d = np.where(r >= Es, 2, np.where(r >= Ny * (Nx + 1), 1, 0))
+    ix = np.where(d == 2, (r - Es) // (Ny + 1), np.where(d == 1, (r - Ny * (Nx + 1)) // (Ny + 1), r // Ny))
+    iy = np.where(d == 2, (r - Es) % (Ny + 1), np.where(d == 1, (r - Ny * (Nx + 1)) % (Ny + 1), r % Ny))
+    n1 = iz * Ev + ix * (Ny + 1) + iy
+    n2 = np.where(d == 2, n1 + Ev, np.where(d == 1, n1 + (Ny + 1), n1 + 1))
+    x0, y0, z0 = xn[ix], yn[iy], zn[iz]
+    x1 = np.where(d == 1, xn[np.minimum(ix + 1, Nx)], x0)
+    y1 = np.where(d == 0, yn[np.minimum(iy + 1, Ny)], y0)
+    z1 = np.where(d == 2, zn[np.minimum(iz + 1, Nz)], z0)
+    L = np.abs(x1 - x0) + np.abs(y1 - y0) + np.abs(z1 - z0)
+    return dict(d=d, iz=iz, ix=ix, iy=iy, n1=n1, n2=n2, x0=x0, y0=y0, z0=z0, x1=x1, y1=y1, z1=z1, L=L,
+                Nx=Nx, Ny=Ny, Nz=Nz, Es=Es, Ev=Ev, Ne=Ne, Nn=(Nz + 1) * Ev)
+
+
+def gradient(mesh, geo):
+    """Discrete gradient (edges x nodes): (phi(n2) - phi(n1)) / length."""
+    Ne, Nn = geo["Ne"], geo["Nn"]
+    rows = np.concatenate([np.arange(Ne), np.arange(Ne)])
+    cols = np.concatenate([geo["n1"], geo["n2"]])
+    vals = np.concatenate([-1.0 / geo["L"], 1.0 / geo["L"]])
+    return sp.csr_matrix((vals, (rows, cols)), shape=(Ne, Nn))
+
+
+def kept(mesh):
+    m, r = mesh.edge_map()
+    return r
+
+
+def tiny(cells, grade=0.0, seed=5):
+    """A strip-over-ground mesh of the given cell counts, graded on all three axes."""
+    spec = M.microstrip_spec()
+    spec.cells = cells
+    spec.grade = grade
+    spec.seed = seed
+    return M.Mesh(spec)
+
+
+# ---------------------------------------------------------------- stiffness
+
+def test_nnz_closed_form_microstrip(microstrip):
+    Nx, Ny, Nz = 60, 20, 8
+    Es = Ny * (Nx + 1) + Nx * (Ny + 1)
+    Ev = (Nx + 1) * (Ny + 1)
+    Ne = Es * (Nz + 1) + Ev * Nz
+    Np = Nx * Ny * (Nz + 1) + ((Nx + 1) * Ny + (Ny + 1) * Nx) * Nz
+    Ps, Pv = Nx * Ny, (Nx + 1) * Ny + (Ny + 1) * Nx
+    expect = Ne + 12 * Np - (12 * Ps + 6 * Pv + Es)
+    assert expect == 368488
+    rp, ci, v = oracle.stiffness(microstrip)
+    assert len(v) == expect == microstrip.sizes.nnz_s
+    assert microstrip.n == 30088 and microstrip.n_edge == 32568
+
+
+@pytest.mark.parametrize("cells", [(200, 200, 24), (300, 300, 32), (400, 400, 32)])
+def test_size_formulas_of_large_workloads(cells):
+    # N_edge and nnz(S) of the measured workloads (closed form, no assembly)
+    Nx, Ny, Nz = cells
+    Es = Ny * (Nx + 1) + Nx * (Ny + 1)
+    Ev = (Nx + 1) * (Ny + 1)
+    Ne = Es * (Nz + 1) + Ev * Nz
+    Np = Nx * Ny * (Nz + 1) + ((Nx + 1) * Ny + (Ny + 1) * Nx) * Nz
+    Ps, Pv = Nx * Ny, (Nx + 1) * Ny + (Ny + 1) * Nx
+    nnz = Ne + 12 * Np - (12 * Ps + 6 * Pv + Es)
+    table = {(200, 200, 24): (2979624, 37092024), (300, 300, 32): (8859032, 111505232),
+             (400, 400, 32): (15732032, 198113632)}
+    assert (Ne, nnz) == table[cells]
+
+
+@pytest.mark.parametrize("cells,grade", [((4, 4, 2), 0.0), ((5, 4, 3), 0.3), ((4, 5, 4), 0.2)])
+def test_dense_product_matches(cells, grade):
+    # Independent assembly route: dense Sh @ Se.T from the raw incidences.
+    m = tiny(cells, grade)
+    row, col, se, sh = oracle.se_sh(m)
+    Ne, Nf = m.n_edge, int(col.max()) + 1
+    Se = np.zeros((Ne, Nf))
+    Sh = np.zeros((Ne, Nf))
+    np.add.at(Se, (row, col), se)
+    np.add.at(Sh, (row, col), sh)
+    assert np.count_nonzero(Se, axis=0).tolist() == [4] * Nf
+    full = Sh @ Se.T / MU
+    r = kept(m)
+    dense = full[np.ix_(r, r)]
+    S = oracle.stiffness_csr(m).toarray()
+    assert np.array_equal(dense != 0, S != 0)
+    nz = S != 0
+    ulp = np.spacing(np.abs(S[nz]))
+    assert np.all(np.abs(dense[nz] - S[nz]) <= 2 * ulp)
+
+
+def test_curl_of_gradient_vanishes(small_manhattan):
+    m = small_manhattan
+    geo = grid_geometry(m)
+    G = gradient(m, geo)
+    r = kept(m)
+    S = oracle.stiffness_csr(m)
+    Nns = geo["Ev"]
+    Gk = G[r][:, Nns:]  # potentials vanishing on the PEC plane
+    SG = S @ Gk
+    scale = abs(S).max() * abs(Gk).max()
+    assert abs(SG).max() <= 1e-12 * scale
+
+
+def test_null_space_dimension():
+    m = tiny((4, 4, 3), 0.25)
+    geo = grid_geometry(m)
+    S = oracle.stiffness_csr(m).toarray()
+    r = kept(m)
+    # dual volume of each edge: dual widths across the edge times its length
+    xn, yn, zn = m.coords()
+
+    def dual(c, i):
+        i = np.asarray(i)
+        n = len(c) - 1
+        lo, hi = np.maximum(i - 1, 0), np.minimum(i + 1, n)
+        w = (c[hi] - c[lo]) / 2
+        return np.where(i == 0, c[1] - c[0], np.where(i == n, c[n] - c[n - 1], w))
+
+    d, iz, ix, iy = geo["d"], geo["iz"], geo["ix"], geo["iy"]
+    W = np.where(d == 0, dual(xn, ix) * dual(zn, iz), np.where(d == 1, dual(yn, iy) * dual(zn, iz),
+                 dual(xn, ix) * dual(yn, iy))) * geo["L"]
+    W = W[r]
+    K = (W[:, None] * S)
+    assert np.abs(K - K.T).max() <= 1e-12 * np.abs(K).max()
+    Ks = np.sqrt(W)[:, None] * S / np.sqrt(W)[None, :]
+    ev = np.linalg.eigvalsh((Ks + Ks.T) / 2)
+    tol = 1e-9 * ev.max()
+    assert ev.min() > -tol
+    assert int(np.sum(np.abs(ev) < tol)) == geo["Nn"] - geo["Ev"]
+
+
+def _apply_field(m, geo, fx, fy, fz):
+    """Sample a polynomial field as edge values (tangential component at the edge)."""
+    d = geo["d"]
+    xm, ym, zm = (geo["x0"] + geo["x1"]) / 2, (geo["y0"] + geo["y1"]) / 2, (geo["z0"] + geo["z1"]) / 2
+    return np.where(d == 1, fx(xm, ym, zm), np.where(d == 0, fy(xm, ym, zm), fz(xm, ym, zm)))
+
+
+@pytest.mark.parametrize("which", ["x", "y", "z"])
+def test_curl_curl_of_quadratic_fields(which):
+    # F=(y^2,0,0): curl curl F=(-2,0,0); F=(0,z^2,0): (0,-2,0); F=(0,0,x^2): (0,0,-2).
+    # Exact on graded grids at interior edges, pinning the dual widths.
+    m = tiny((6, 5, 6), 0.35, seed=3)
+    geo = grid_geometry(m)
+    zero = lambda x, y, z: 0 * x
+    f = {"x": (lambda x, y, z: y * y, zero, zero), "y": (zero, lambda x, y, z: z * z, zero),
+         "z": (zero, zero, lambda x, y, z: x * x)}[which]
+    E = _apply_field(m, geo, *f)
+    r = kept(m)
+    S = oracle.stiffness_csr(m)
+    y = S @ E[r]
+    d, iz, ix, iy = geo["d"][r], geo["iz"][r], geo["ix"][r], geo["iy"][r]
+    Nx, Ny, Nz = geo["Nx"], geo["Ny"], geo["Nz"]
+    if which == "x":
+        # plane-1 rows see the PEC plane through their xz faces (E = 0 there)
+        sel = (d == 1) & (iy >= 1) & (iy <= Ny - 1) & (iz >= 2)
+    elif which == "y":
+        sel = (d == 0) & (iz >= 1) & (iz <= Nz - 1)
+    else:
+        sel = (d == 2) & (ix >= 1) & (ix <= Nx - 1) & (iy >= 1) & (iy <= Ny - 1)
+    assert sel.sum() > 10
+    np.testing.assert_allclose(y[sel], -2 / MU, rtol=1e-9)
+    other = ~sel & (d != {"x": 1, "y": 0, "z": 2}[which])
+    interior = (ix >= 1) & (ix <= Nx - 1) & (iy >= 1) & (iy <= Ny - 1) & (iz >= 1) & (iz <= Nz - 1)
+    assert np.abs(y[other & interior]).max() <= 1e-9 * 2 / MU
+
+
+# ---------------------------------------------------------------- V0 basis
+
+def test_v0d_columns_are_gradients(small_manhattan):
+    m = small_manhattan
+    geo = grid_geometry(m)
+    v = oracle.v0d(m)
+    mark_node = m.mark_node()
+    Nn, Nns = geo["Nn"], geo["Ev"]
+    bd = np.zeros(Nn, bool)
+    bd[np.ctypeslib.as_array(m.c.bd_nodes, shape=(m.c.num_bd_nodes,))] = True
+    iz = np.arange(Nn) // Nns
+    elig = (mark_node == 0) & ~(((iz == 0) | (iz == m.c.nz - 1)) & bd)
+    n_diel = int(elig.sum())
+    V = sp.csr_matrix((v["val"], (v["row"], v["col"])), shape=(geo["Ne"], v["cols"]))
+    G = gradient(m, geo)
+    # dielectric columns are -grad of the hat function of each eligible node, in node order
+    nodes = np.nonzero(elig)[0]
+    np.testing.assert_array_equal((-G[:, nodes]).toarray(), V[:, :n_diel].toarray())
+    # conductor columns are gradients too: the curl of every column vanishes
+    row, col, se, sh = oracle.se_sh(m)
+    Se = sp.csr_matrix((se, (row, col)), shape=(geo["Ne"], int(col.max()) + 1))
+    CV = Se.T @ V
+    assert abs(CV).max() <= 1e-12 * abs(Se).max() * abs(V).max()
+    mp = np.ctypeslib.as_array(m.c.cdt_mark_port, shape=(m.c.num_cdt,))
+    assert v["cols"] == n_diel + int((mp != -1).sum())
+
+
+def test_ad_is_the_petrov_galerkin_product(small_manhattan):
+    m = small_manhattan
+    geo = grid_geometry(m)
+    v = oracle.v0d(m)
+    eps0 = 1.0 / (299792458.0 ** 2 * MU)
+    epsn = np.ctypeslib.as_array(m.c.stack_epsn, shape=(m.c.nz,))
+    V = sp.csr_matrix((v["val"], (v["row"], v["col"])), shape=(geo["Ne"], v["cols"]))
+    Va = sp.csr_matrix((v["aval"], (v["row"], v["col"])), shape=(geo["Ne"], v["cols"]))
+    D = sp.diags(epsn[geo["iz"]] * eps0)
+    P = (Va.T @ D @ V).tocoo()
+    Pd = P.toarray()
+    Ad = np.zeros_like(Pd)
+    Ad[v["ad_row"], v["ad_col"]] = v["ad_val"]
+    keep = np.abs(Pd) > 1e-8
+    np.testing.assert_allclose(Ad[keep], Pd[keep], rtol=1e-12, atol=0)
+    assert np.all(Ad[~keep] == 0)
+    # rows of dielectric columns away from boundaries sum to zero (Neumann-type Laplacian)
+    assert v["ad_len"] >= len(v["ad_val"])
+
+
+def test_ac_is_the_conductor_galerkin_product(small_manhattan):
+    m = small_manhattan
+    geo = grid_geometry(m)
+    v = oracle.v0c(m)
+    sign = np.ctypeslib.as_array(m.c.stack_sign, shape=(m.c.nz,))
+    me = m.mark_edge()
+    V = sp.csr_matrix((v["val"], (v["row"], v["col"])), shape=(geo["Ne"], v["cols"]))
+    Va = sp.csr_matrix((v["aval"], (v["row"], v["col"])), shape=(geo["Ne"], v["cols"]))
+    D = sp.diags(np.where(me != 0, sign[geo["iz"]], 0.0))
+    P = (Va.T @ D @ V).toarray()
+    Ac = np.zeros_like(P)
+    Ac[v["ac_row"], v["ac_col"]] = v["ac_val"]
+    np.testing.assert_allclose(Ac, P, rtol=1e-12, atol=1e-12 * np.abs(P).max())
+    # conductor nodes: one column each, minus the last node of every non-reference conductor
+    mn = m.mark_node()
+    mp = np.ctypeslib.as_array(m.c.cdt_mark_port, shape=(m.c.num_cdt,))
+    bd = np.zeros(len(mn), bool)
+    bd[np.ctypeslib.as_array(m.c.bd_nodes, shape=(m.c.num_bd_nodes,))] = True
+    expect = 0
+    for c in range(m.c.num_cdt):
+        nodes = np.nonzero(mn == c + 1)[0]
+        if mp[c] <= -1:
+            expect += int((~bd[nodes]).sum())
+        elif mp[c] == 0:
+            expect += int((~bd[nodes[:-1]]).sum())
+        else:
+            expect += len(nodes) - 1
+    assert v["cols"] == expect
+
+
+# ---------------------------------------------------------------- full-wave oracle
+
+def test_microstrip_is_capacitive(microstrip):
+    m = microstrip
+    f0 = oracle.freq(m, 0)
+    assert f0 == 1e9
+    _, Z = oracle.direct_solve(m, [f0, f0 / 10])
+    z1, z2 = Z[0][0], Z[1][0]
+    w = 2 * np.pi * f0
+    C = -1.0 / (w * z1.imag)
+    # strip 100 x 2 um, 2 um above ground, er 3.9: parallel plate 3.5 fF plus fringing
+    assert 3e-15 < C < 2e-14
+    assert abs(z1.real) < 1e-2 * abs(z1.imag)
+    np.testing.assert_allclose(z2.imag, 10 * z1.imag, rtol=2e-3)
+
+
+def test_reciprocity(small_manhattan):
+    m = small_manhattan
+    fs = [oracle.freq(m, k) for k in range(m.c.nfreq)]
+    np_ = m.num_ports
+    _, Z0 = oracle.direct_solve(m, fs)
+    _, Z = oracle.direct_solve(m, fs, refine=True)
+    # Plain FP64 LU loses ~1e-7..1e-6 at 100 MHz (gradient modes sit ~1e11
+    # below the curl modes); long-double refinement brings the Z matrix back
+    # to the reciprocity the rounded FP64 entries of S allow (~1e-8 here).
+    for Zk, Zk0 in zip(Z, Z0):
+        Zm, Zm0 = Zk.reshape(np_, np_), Zk0.reshape(np_, np_)  # [src, resp]
+        asym = np.abs(Zm - Zm.T).max() / np.abs(Zm).max()
+        asym0 = np.abs(Zm0 - Zm0.T).max() / np.abs(Zm0).max()
+        assert asym < 2e-7
+        assert asym <= asym0 + 1e-12
+        np.testing.assert_allclose(Zk0, Zk, rtol=1e-5)
+
+
+def test_rhs_and_diag_layout(microstrip):
+    m = microstrip
+    f = 1e9
+    b = oracle.rhs(m, f, 0)
+    nz = np.nonzero(b)[0]
+    assert len(nz) == 4  # four z-edges from the ground plane to the strip
+    assert np.all(b[nz].real == 0) and np.allclose(b[nz].imag, -2 * np.pi * f)
+    d = oracle.fullwave_diag(m, f)
+    assert np.all(d.real < 0)
+    assert np.count_nonzero(d.imag) > 0
