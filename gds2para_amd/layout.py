@@ -1,0 +1,129 @@
+"""Device-resident layout: the Python face of gp_layout.
+
+Mirrors the reference's call order on fdtdMesh (src/TestMain.cpp:343-370):
+generateStiff -> V0 basis -> per-port / per-frequency solves -> Z.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _capi as capi
+
+i64p, i32p, f64p = C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_double)
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+class Layout:
+    def __init__(self, mesh):
+        self.lib = capi.load()
+        self.mesh = mesh
+        h = C.c_void_p()
+        capi.check(self.lib.gp_layout_create(mesh.ptr, C.byref(h)), "gp_layout_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gp_layout_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def device_bytes(self):
+        return int(self.lib.gp_layout_device_bytes(self.h))
+
+    # generateStiff --------------------------------------------------------
+    def assemble(self):
+        capi.check(self.lib.gp_assemble_stiffness(self.h), "gp_assemble_stiffness")
+
+    def stiffness(self):
+        """(row_ptr, col, val) of S on the host."""
+        n, nnz = self.mesh.n, int(self.mesh.sizes.nnz_s)
+        rp = np.empty(n + 1, np.int64)
+        ci = np.empty(nnz, np.int32)
+        v = np.empty(nnz)
+        capi.check(self.lib.gp_stiffness_copy_csr(self.h, _p(rp, i64p), _p(ci, i32p), _p(v, f64p)),
+                   "gp_stiffness_copy_csr")
+        return rp, ci, v
+
+    def stiffness_coo(self):
+        nnz = int(self.mesh.sizes.nnz_s)
+        r, c, v = np.empty(nnz, np.int64), np.empty(nnz, np.int64), np.empty(nnz)
+        capi.check(self.lib.gp_stiffness_copy_coo(self.h, _p(r, i64p), _p(c, i64p), _p(v, f64p)),
+                   "gp_stiffness_copy_coo")
+        return r, c, v
+
+    # V0 basis -------------------------------------------------------------
+    def build_v0(self):
+        capi.check(self.lib.gp_build_v0(self.h), "gp_build_v0")
+
+    def v0_sizes(self):
+        s = capi.GpV0Sizes()
+        capi.check(self.lib.gp_v0_sizes_get(self.h, C.byref(s)), "gp_v0_sizes_get")
+        return dict(v0d_nnz=s.v0d_nnz, v0d_cols=s.v0d_cols, ad_nnz=s.ad_nnz, ad_len=s.ad_len,
+                    v0c_nnz=s.v0c_nnz, v0c_cols=s.v0c_cols, ac_nnz=s.ac_nnz)
+
+    def v0d(self):
+        s = self.v0_sizes()
+        n = s["v0d_nnz"]
+        r, c, v, a = np.empty(n, np.int64), np.empty(n, np.int64), np.empty(n), np.empty(n)
+        capi.check(self.lib.gp_v0d_copy(self.h, _p(r, i64p), _p(c, i64p), _p(v, f64p), _p(a, f64p)), "gp_v0d_copy")
+        m = s["ad_nnz"]
+        ar, ac, av = np.empty(m, np.int64), np.empty(m, np.int64), np.empty(m)
+        capi.check(self.lib.gp_ad_copy(self.h, _p(ar, i64p), _p(ac, i64p), _p(av, f64p)), "gp_ad_copy")
+        return dict(row=r, col=c, val=v, aval=a, cols=s["v0d_cols"], ad_row=ar, ad_col=ac, ad_val=av,
+                    ad_len=s["ad_len"])
+
+    def v0c(self):
+        s = self.v0_sizes()
+        n = s["v0c_nnz"]
+        r, c, v, a = np.empty(n, np.int64), np.empty(n, np.int64), np.empty(n), np.empty(n)
+        capi.check(self.lib.gp_v0c_copy(self.h, _p(r, i64p), _p(c, i64p), _p(v, f64p), _p(a, f64p)), "gp_v0c_copy")
+        m = s["ac_nnz"]
+        ar, ac, av = np.empty(m, np.int64), np.empty(m, np.int64), np.empty(m)
+        capi.check(self.lib.gp_ac_copy(self.h, _p(ar, i64p), _p(ac, i64p), _p(av, f64p)), "gp_ac_copy")
+        return dict(row=r, col=c, val=v, aval=a, cols=s["v0c_cols"], ac_row=ar, ac_col=ac, ac_val=av)
+
+    # full-wave solve --------------------------------------------------------
+    def prepare(self, precision=capi.GP_PREC_MIXED):
+        capi.check(self.lib.gp_prepare_solver(self.h, int(precision)), "gp_prepare_solver")
+
+    def solve(self, freqs, ports=None, precision=capi.GP_PREC_MIXED, rtol=1e-8, batch=0, restart=30,
+              max_iters=2000, want_x=False, verbose=0):
+        """Returns (Z[nf, np*np] complex, stats, X or None)."""
+        ports = list(range(self.mesh.num_ports)) if ports is None else list(ports)
+        freqs = np.ascontiguousarray(np.asarray(freqs, np.float64))
+        pr = np.ascontiguousarray(np.asarray(ports, np.int32))
+        nf, npp, npt = len(freqs), len(pr), self.mesh.num_ports
+        o = capi.GpSolveOpts()
+        self.lib.gp_solve_opts_default(C.byref(o))
+        o.rtol, o.precision, o.batch, o.restart, o.max_iters, o.verbose = rtol, precision, batch, restart, max_iters, verbose
+        Z = np.zeros(2 * nf * npt * npt)
+        X = np.zeros(2 * self.mesh.n * nf * npp) if want_x else None
+        st = capi.GpSolveStats()
+        code = self.lib.gp_solve_fullwave(self.h, _p(freqs, f64p), nf, _p(pr, i32p), npp, C.byref(o),
+                                          _p(X, f64p) if want_x else None, _p(Z, f64p), C.byref(st))
+        if code not in (capi.GP_OK, capi.GP_ERR_NOCONV):
+            capi.check(code, "gp_solve_fullwave")
+        Zc = (Z[0::2] + 1j * Z[1::2]).reshape(nf, npt * npt)
+        Xc = None
+        if want_x:
+            Xc = (X[0::2] + 1j * X[1::2]).reshape(nf * npp, self.mesh.n)
+        stats = dict(columns=st.columns, converged=st.converged, max_iters=st.max_iters_used, batches=st.batches,
+                     max_rel_residual=st.max_rel_residual, setup_ms=st.setup_ms, solve_ms=st.solve_ms,
+                     status=code)
+        return Zc, stats, Xc
+
+    def bench_spmv(self, k=1, precision=capi.GP_PREC_FP64, iters=20):
+        ms = C.c_double()
+        capi.check(self.lib.gp_bench_spmv(self.h, int(k), int(precision), int(iters), C.byref(ms)), "gp_bench_spmv")
+        return ms.value
