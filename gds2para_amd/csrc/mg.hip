@@ -1,0 +1,587 @@
+// mg.hip -- multigrid hierarchy setup (GPU Galerkin products) and the
+// split-form hybrid V-cycle, templated on the working precision.
+#include <chrono>
+#include <cmath>
+#include <complex>
+
+#include "mg.hpp"
+
+namespace gp {
+
+namespace {
+
+constexpr double kEps0v = kEps0;
+
+// ---------------------------------------------------------------- setup kernels
+
+// K0 = W * [S, eps_r eps0 on the diagonal, sigma on the diagonal]
+__global__ void k_level0_K(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                           const double *__restrict__ s, const double *__restrict__ W, const double *__restrict__ epsr,
+                           const double *__restrict__ sig, int32_t *__restrict__ oci, double *__restrict__ o0,
+                           double *__restrict__ o1, double *__restrict__ o2) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double w = W[i];
+  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) {
+    const bool d = ci[p] == (int32_t)i;
+    oci[p] = ci[p];
+    o0[p] = w * s[p];
+    o1[p] = d ? w * epsr[i] * kEps0v : 0.;
+    o2[p] = d ? w * sig[i] : 0.;
+  }
+}
+
+// B0 = W * [eps_r eps0, sigma] * G   (row scaling of the gradient)
+__global__ void k_level0_B(int64_t n, const int64_t *__restrict__ rp, const double *__restrict__ g,
+                           const double *__restrict__ W, const double *__restrict__ epsr, const double *__restrict__ sig,
+                           double *__restrict__ o0, double *__restrict__ o1) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = W[i] * epsr[i] * kEps0v, b = W[i] * sig[i];
+  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) { o0[p] = a * g[p]; o1[p] = b * g[p]; }
+}
+
+__global__ void k_diag_l1(int64_t nr, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci, int nv,
+                          const double *__restrict__ v0, const double *__restrict__ v1, const double *__restrict__ v2,
+                          double *__restrict__ d0, double *__restrict__ d1, double *__restrict__ d2,
+                          double *__restrict__ l0, double *__restrict__ l1, double *__restrict__ l2) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nr) return;
+  const double *v[3] = {v0, v1, v2};
+  double d[3] = {0, 0, 0}, l[3] = {0, 0, 0};
+  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) {
+    const bool diag = ci[p] == (int32_t)i;
+    for (int k = 0; k < nv; ++k) {
+      if (diag) d[k] += v[k][p];
+      else l[k] += fabs(v[k][p]);
+    }
+  }
+  double *dd[3] = {d0, d1, d2}, *ll[3] = {l0, l1, l2};
+  for (int k = 0; k < nv; ++k) { dd[k][i] = d[k]; ll[k][i] = l[k]; }
+}
+
+__global__ void k_d2f_arr(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+gp_status smoother_data(Arena &A, const Csr &M, double **d, double **l, hipStream_t s) {
+  for (int k = 0; k < M.nv; ++k) {
+    GP_TRY(A.alloc(&d[k], M.nr));
+    GP_TRY(A.alloc(&l[k], M.nr));
+  }
+  k_diag_l1<<<blocks_for(M.nr), 256, 0, s>>>(M.nr, M.rp, M.ci, M.nv, M.v[0], M.nv > 1 ? M.v[1] : nullptr,
+                                              M.nv > 2 ? M.v[2] : nullptr, d[0], M.nv > 1 ? d[1] : nullptr,
+                                              M.nv > 2 ? d[2] : nullptr, l[0], M.nv > 1 ? l[1] : nullptr,
+                                              M.nv > 2 ? l[2] : nullptr);
+  GP_HIP(hipStreamSynchronize(s));
+  return GP_OK;
+}
+
+gp_status to_f32(Arena &A, const double *src, int64_t n, float **dst, hipStream_t s) {
+  GP_TRY(A.alloc(dst, std::max<int64_t>(n, 1)));
+  if (n) k_d2f_arr<<<blocks_for(n), 256, 0, s>>>(n, src, *dst);
+  return GP_OK;
+}
+
+gp_status dense_host(const Csr &M, int which, int64_t nr, int64_t nc, std::vector<double> *out) {
+  HostCsr h;
+  GP_TRY(csr_host_copy(M, &h, which));
+  out->assign(nr * nc, 0.0);
+  for (int64_t i = 0; i < nr; ++i)
+    for (int64_t p = h.rp[i]; p < h.rp[i + 1]; ++p) (*out)[i * nc + h.ci[p]] += h.v[p];
+  return GP_OK;
+}
+
+}  // namespace
+
+gp_status build_hierarchy(gp_layout *lay, Hierarchy *H, bool want_f32, int max_levels) {
+  auto t0 = std::chrono::steady_clock::now();
+  if (!lay->s_ready) GP_TRY(gp_assemble_stiffness(lay));
+  hipStream_t s = lay->stream;
+  Arena &A = lay->arena;
+  const gp_mesh &m = lay->hm.view;
+  std::vector<LevelGrid> grids(1);
+  level0_grid(m, &grids[0]);
+  LevelGrid &g0 = grids[0];
+  const int64_t n = g0.ne();
+  if (n != lay->n) return fail(GP_ERR_STATE, "level-0 edge count differs from the layout");
+  // level-0 per-edge data
+  std::vector<double> W, epsr(n), sig(n);
+  edge_dual_volume(g0, &W);
+  for (int64_t k = 0; k < n; ++k) {
+    int64_t e = g0.erev[k];
+    int64_t L = e / g0.g.El;
+    epsr[k] = m.stack_epsn[L];
+    sig[k] = m.mark_edge[e] != 0 ? m.stack_sign[L] : 0.;
+  }
+  GP_TRY(A.alloc(&H->W, n)); GP_TRY(A.alloc(&H->epsr, n)); GP_TRY(A.alloc(&H->sig, n));
+  GP_HIP(hipMemcpyAsync(H->W, W.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemcpyAsync(H->epsr, epsr.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemcpyAsync(H->sig, sig.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+  H->L.clear();
+  H->L.emplace_back();
+  {
+    MgLevel &L0 = H->L[0];
+    L0.ne = n;
+    L0.nn = g0.nn();
+    const CsrD &S = lay->S;
+    GP_TRY(csr_alloc(A, n, n, S.nnz, 3, &L0.K));
+    GP_HIP(hipMemcpyAsync(L0.K.rp, S.rp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToDevice, s));
+    k_level0_K<<<blocks_for(n), 256, 0, s>>>(n, S.rp, S.ci, S.v, H->W, H->epsr, H->sig, L0.K.ci, L0.K.v[0], L0.K.v[1],
+                                             L0.K.v[2]);
+    HostCsr hg;
+    build_gradient(g0, &hg);
+    GP_TRY(csr_upload(A, hg, &L0.G, s));
+    GP_TRY(csr_alloc(A, n, L0.nn, L0.G.nnz, 2, &L0.B));
+    GP_HIP(hipMemcpyAsync(L0.B.rp, L0.G.rp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToDevice, s));
+    GP_HIP(hipMemcpyAsync(L0.B.ci, L0.G.ci, sizeof(int32_t) * L0.G.nnz, hipMemcpyDeviceToDevice, s));
+    k_level0_B<<<blocks_for(n), 256, 0, s>>>(n, L0.G.rp, L0.G.v[0], H->W, H->epsr, H->sig, L0.B.v[0], L0.B.v[1]);
+    GP_HIP(hipStreamSynchronize(s));
+    GP_TRY(transpose(A, L0.G, &L0.GT, s));
+    GP_TRY(spgemm(A, L0.GT, L0.B, &L0.N, s));
+  }
+  // coarser levels
+  while ((int)H->L.size() < max_levels && grids.back().ne() > 400) {
+    LevelGrid c;
+    if (!coarsen_grid(grids.back(), &c)) break;
+    if (c.ne() == 0 || c.nn() == 0) break;
+    MgLevel &F = H->L.back();
+    HostCsr hp, hpn, hg;
+    build_edge_prolongation(grids.back(), c, &hp);
+    build_node_prolongation(grids.back(), c, &hpn);
+    build_gradient(c, &hg);
+    GP_TRY(csr_upload(A, hp, &F.P, s));
+    GP_TRY(csr_upload(A, hpn, &F.Pn, s));
+    GP_TRY(transpose(A, F.P, &F.PT, s));
+    GP_TRY(transpose(A, F.Pn, &F.PnT, s));
+    MgLevel Cl;
+    Cl.ne = c.ne();
+    Cl.nn = c.nn();
+    Csr tmpc;
+    GP_TRY(spgemm(A, F.K, F.P, &tmpc, s));
+    GP_TRY(spgemm(A, F.PT, tmpc, &Cl.K, s));
+    csr_release(A, tmpc);
+    GP_TRY(spgemm(A, F.B, F.Pn, &tmpc, s));
+    GP_TRY(spgemm(A, F.PT, tmpc, &Cl.B, s));
+    csr_release(A, tmpc);
+    GP_TRY(spgemm(A, F.N, F.Pn, &tmpc, s));
+    GP_TRY(spgemm(A, F.PnT, tmpc, &Cl.N, s));
+    csr_release(A, tmpc);
+    GP_TRY(csr_upload(A, hg, &Cl.G, s));
+    GP_TRY(transpose(A, Cl.G, &Cl.GT, s));
+    grids.push_back(std::move(c));
+    H->L.push_back(Cl);
+  }
+  // smoother data and FP32 copies
+  for (MgLevel &L : H->L) {
+    GP_TRY(smoother_data(A, L.K, L.kd, L.kl, s));
+    GP_TRY(smoother_data(A, L.N, L.nd, L.nl, s));
+  }
+  if (want_f32) {
+    GP_TRY(to_f32(A, H->W, n, &H->Wf, s));
+    for (MgLevel &L : H->L) {
+      GP_TRY(csr_to_float(A, L.K, &L.Kf, s));
+      GP_TRY(csr_to_float(A, L.B, &L.Bf, s));
+      GP_TRY(csr_to_float(A, L.N, &L.Nf, s));
+      GP_TRY(csr_to_float(A, L.G, &L.Gf, s));
+      GP_TRY(csr_to_float(A, L.GT, &L.GTf, s));
+      if (L.P.nr) {
+        GP_TRY(csr_to_float(A, L.P, &L.Pf, s));
+        GP_TRY(csr_to_float(A, L.PT, &L.PTf, s));
+        GP_TRY(csr_to_float(A, L.Pn, &L.Pnf, s));
+        GP_TRY(csr_to_float(A, L.PnT, &L.PnTf, s));
+      }
+      for (int k = 0; k < 3; ++k) {
+        GP_TRY(to_f32(A, L.kd[k], L.ne, &L.kdf[k], s));
+        GP_TRY(to_f32(A, L.kl[k], L.ne, &L.klf[k], s));
+      }
+      for (int k = 0; k < 2; ++k) {
+        GP_TRY(to_f32(A, L.nd[k], L.nn, &L.ndf[k], s));
+        GP_TRY(to_f32(A, L.nl[k], L.nn, &L.nlf[k], s));
+      }
+    }
+    H->has_f32 = true;
+  }
+  // coarsest level to the host, dense
+  const MgLevel &C = H->L.back();
+  H->neL = C.ne;
+  H->nnL = C.nn;
+  for (int k = 0; k < 3; ++k) GP_TRY(dense_host(C.K, k, C.ne, C.ne, &H->KL[k]));
+  for (int k = 0; k < 2; ++k) GP_TRY(dense_host(C.B, k, C.ne, C.nn, &H->BL[k]));
+  for (int k = 0; k < 2; ++k) GP_TRY(dense_host(C.N, k, C.nn, C.nn, &H->NL[k]));
+  GP_TRY(dense_host(C.GT, 0, C.nn, C.ne, &H->GTL));
+  GP_HIP(hipStreamSynchronize(s));
+  H->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return GP_OK;
+}
+
+void free_hierarchy(gp_layout *lay, Hierarchy *H) {
+  // the layout arena owns every buffer; a rebuilt hierarchy simply adds new ones
+  (void)lay;
+  H->L.clear();
+}
+
+// ---------------------------------------------------------------- dense inverse (host)
+
+namespace {
+
+// Gauss-Jordan inverse with partial pivoting, complex FP64, row-major.
+bool invert(std::vector<std::complex<double>> &a, int64_t n) {
+  std::vector<std::complex<double>> inv(n * n, 0.0);
+  for (int64_t i = 0; i < n; ++i) inv[i * n + i] = 1.0;
+  for (int64_t c = 0; c < n; ++c) {
+    int64_t piv = c;
+    double best = std::abs(a[c * n + c]);
+    for (int64_t r = c + 1; r < n; ++r)
+      if (std::abs(a[r * n + c]) > best) { best = std::abs(a[r * n + c]); piv = r; }
+    if (best == 0.0) return false;
+    if (piv != c)
+      for (int64_t j = 0; j < n; ++j) { std::swap(a[c * n + j], a[piv * n + j]); std::swap(inv[c * n + j], inv[piv * n + j]); }
+    std::complex<double> d = 1.0 / a[c * n + c];
+    for (int64_t j = 0; j < n; ++j) { a[c * n + j] *= d; inv[c * n + j] *= d; }
+    for (int64_t r = 0; r < n; ++r) {
+      if (r == c) continue;
+      std::complex<double> f = a[r * n + c];
+      if (f == 0.0) continue;
+      for (int64_t j = 0; j < n; ++j) { a[r * n + j] -= f * a[c * n + j]; inv[r * n + j] -= f * inv[c * n + j]; }
+    }
+  }
+  a.swap(inv);
+  return true;
+}
+
+}  // namespace
+
+template <class T>
+gp_status vwork_create(gp_layout *lay, const Hierarchy &H, int k, const double *omega_cols, const int32_t *fidx_cols,
+                       const std::vector<double> &omegas, VWork<T> *w) {
+  hipStream_t s = lay->stream;
+  Arena &A = w->arena;
+  w->k = k;
+  w->nf = (int)omegas.size();
+  std::vector<T> c1(k), c2(k);
+  for (int j = 0; j < k; ++j) { c1[j] = (T)(-omega_cols[j] * omega_cols[j]); c2[j] = (T)omega_cols[j]; }
+  GP_TRY(A.alloc(&w->c1, k)); GP_TRY(A.alloc(&w->c2, k)); GP_TRY(A.alloc(&w->fidx, k));
+  GP_HIP(hipMemcpyAsync(w->c1, c1.data(), sizeof(T) * k, hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemcpyAsync(w->c2, c2.data(), sizeof(T) * k, hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemcpyAsync(w->fidx, fidx_cols, sizeof(int32_t) * k, hipMemcpyHostToDevice, s));
+  const size_t nl = H.L.size();
+  w->r.assign(nl, nullptr); w->u.assign(nl, nullptr); w->phi.assign(nl, nullptr); w->res.assign(nl, nullptr);
+  w->tmp.assign(nl, nullptr); w->rn.assign(nl, nullptr); w->dn.assign(nl, nullptr); w->dn2.assign(nl, nullptr);
+  for (size_t l = 0; l < nl; ++l) {
+    const int64_t ne = H.L[l].ne * k, nn = H.L[l].nn * k;
+    GP_TRY(A.alloc(&w->r[l], ne)); GP_TRY(A.alloc(&w->u[l], ne)); GP_TRY(A.alloc(&w->phi[l], nn));
+    GP_TRY(A.alloc(&w->res[l], ne)); GP_TRY(A.alloc(&w->tmp[l], ne)); GP_TRY(A.alloc(&w->rn[l], nn));
+    GP_TRY(A.alloc(&w->dn[l], nn)); GP_TRY(A.alloc(&w->dn2[l], nn));
+  }
+  // coarsest-level inverses per distinct frequency
+  const int64_t ne = H.neL, nn = H.nnL;
+  std::vector<double> kinv(2 * ne * ne * w->nf), ninv(2 * nn * nn * w->nf), bl(2 * ne * nn * w->nf);
+  for (int f = 0; f < w->nf; ++f) {
+    const double om = omegas[f], a1 = -om * om;
+    std::vector<std::complex<double>> a(ne * ne), b(nn * nn);
+    for (int64_t i = 0; i < ne * ne; ++i) a[i] = {H.KL[0][i] + a1 * H.KL[1][i], om * H.KL[2][i]};
+    for (int64_t i = 0; i < nn * nn; ++i) b[i] = {a1 * H.NL[0][i], om * H.NL[1][i]};
+    if (!invert(a, ne)) return fail(GP_ERR_ARG, "singular coarsest edge operator");
+    if (nn && !invert(b, nn)) return fail(GP_ERR_ARG, "singular coarsest nodal operator");
+    for (int64_t i = 0; i < ne * ne; ++i) { kinv[2 * (f * ne * ne + i)] = a[i].real(); kinv[2 * (f * ne * ne + i) + 1] = a[i].imag(); }
+    for (int64_t i = 0; i < nn * nn; ++i) { ninv[2 * (f * nn * nn + i)] = b[i].real(); ninv[2 * (f * nn * nn + i) + 1] = b[i].imag(); }
+    for (int64_t i = 0; i < ne * nn; ++i) { bl[2 * (f * ne * nn + i)] = a1 * H.BL[0][i]; bl[2 * (f * ne * nn + i) + 1] = om * H.BL[1][i]; }
+  }
+  GP_TRY(A.alloc(&w->kinv, 2 * ne * ne * w->nf));
+  GP_TRY(A.alloc(&w->ninv, std::max<int64_t>(2 * nn * nn * w->nf, 1)));
+  GP_TRY(A.alloc(&w->bl, std::max<int64_t>(2 * ne * nn * w->nf, 1)));
+  GP_TRY(A.alloc(&w->gtl, std::max<int64_t>(nn * ne, 1)));
+  GP_HIP(hipMemcpyAsync(w->kinv, kinv.data(), sizeof(double) * kinv.size(), hipMemcpyHostToDevice, s));
+  if (nn) {
+    GP_HIP(hipMemcpyAsync(w->ninv, ninv.data(), sizeof(double) * ninv.size(), hipMemcpyHostToDevice, s));
+    GP_HIP(hipMemcpyAsync(w->bl, bl.data(), sizeof(double) * bl.size(), hipMemcpyHostToDevice, s));
+    GP_HIP(hipMemcpyAsync(w->gtl, H.GTL.data(), sizeof(double) * H.GTL.size(), hipMemcpyHostToDevice, s));
+  }
+  GP_HIP(hipStreamSynchronize(s));
+  return GP_OK;
+}
+
+template <class T>
+void vwork_free(VWork<T> *w) {
+  w->arena.free_all();
+}
+
+// ---------------------------------------------------------------- V-cycle kernels
+
+namespace {
+
+template <class T>
+__device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+template <class T>
+__device__ __forceinline__ C2<T> cdiv(C2<T> a, C2<T> b) {
+  T d = b.x * b.x + b.y * b.y;
+  return {(a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d};
+}
+
+// l1-scaled diagonal: a_ii * (1 + L / |a_ii|), L a bound on the off-diagonal row sum of |a_ij(w)|
+template <class T>
+__device__ __forceinline__ C2<T> l1diag(C2<T> a, T L) {
+  T m = sqrt(a.x * a.x + a.y * a.y);
+  T f = m > (T)0 ? (T)1 + L / m : (T)1;
+  return {a.x * f, a.y * f};
+}
+
+struct EdgeMode { enum { RES = 0, JACOBI = 1, JACOBI0 = 2 }; };
+
+// Edge kernel, one thread per (row, column):
+//   acc = r - K(w) u - B(w) phi
+//   RES:     res = acc
+//   JACOBI:  u_out = u + acc / d_l1
+//   JACOBI0: u_out = r / d_l1           (u = phi = 0)
+template <class T>
+__global__ void k_edge(int64_t nr, int k, MatV<T> K, MatV<T> B, const T *__restrict__ c1v, const T *__restrict__ c2v,
+                       const T *__restrict__ kd0, const T *__restrict__ kd1, const T *__restrict__ kd2,
+                       const T *__restrict__ kl0, const T *__restrict__ kl1, const T *__restrict__ kl2,
+                       const C2<T> *__restrict__ r, const C2<T> *__restrict__ u, const C2<T> *__restrict__ phi,
+                       C2<T> *__restrict__ out, int mode, int use_phi, T damp) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t i = t / k;
+  int j = (int)(t - i * k);
+  if (i >= nr) return;
+  const T c1 = c1v[j], c2 = c2v[j];
+  C2<T> acc = r[i * k + j];
+  if (mode != EdgeMode::JACOBI0) {
+    for (int64_t p = K.rp[i]; p < K.rp[i + 1]; ++p) {
+      C2<T> a = {K.v[0][p] + c1 * K.v[1][p], c2 * K.v[2][p]};
+      C2<T> x = u[(int64_t)K.ci[p] * k + j];
+      acc.x -= a.x * x.x - a.y * x.y;
+      acc.y -= a.x * x.y + a.y * x.x;
+    }
+    if (use_phi) {
+      for (int64_t p = B.rp[i]; p < B.rp[i + 1]; ++p) {
+        C2<T> b = {c1 * B.v[0][p], c2 * B.v[1][p]};
+        C2<T> x = phi[(int64_t)B.ci[p] * k + j];
+        acc.x -= b.x * x.x - b.y * x.y;
+        acc.y -= b.x * x.y + b.y * x.x;
+      }
+    }
+  }
+  if (mode == EdgeMode::RES) { out[i * k + j] = acc; return; }
+  C2<T> d = l1diag<T>({kd0[i] + c1 * kd1[i], c2 * kd2[i]}, kl0[i] + fabs(c1) * kl1[i] + c2 * kl2[i]);
+  C2<T> q = cdiv(acc, d);
+  if (mode == EdgeMode::JACOBI0) { out[i * k + j] = {damp * q.x, damp * q.y}; return; }
+  C2<T> u0 = u[i * k + j];
+  out[i * k + j] = {u0.x + damp * q.x, u0.y + damp * q.y};
+}
+
+// Nodal l1-Jacobi on N(w) = -w^2 N_eps + i w N_sig:
+//   first (d == null): d_out = rn / dn;  else d_out = d + (rn - N d) / dn
+template <class T>
+__global__ void k_node(int64_t nr, int k, MatV<T> N, const T *__restrict__ c1v, const T *__restrict__ c2v,
+                       const T *__restrict__ nd0, const T *__restrict__ nd1, const T *__restrict__ nl0,
+                       const T *__restrict__ nl1, const C2<T> *__restrict__ rn, const C2<T> *__restrict__ d,
+                       C2<T> *__restrict__ dout, T damp) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t i = t / k;
+  int j = (int)(t - i * k);
+  if (i >= nr) return;
+  const T c1 = c1v[j], c2 = c2v[j];
+  C2<T> acc = rn[i * k + j];
+  if (d) {
+    for (int64_t p = N.rp[i]; p < N.rp[i + 1]; ++p) {
+      C2<T> a = {c1 * N.v[0][p], c2 * N.v[1][p]};
+      C2<T> x = d[(int64_t)N.ci[p] * k + j];
+      acc.x -= a.x * x.x - a.y * x.y;
+      acc.y -= a.x * x.y + a.y * x.x;
+    }
+  }
+  C2<T> dg = l1diag<T>({c1 * nd0[i], c2 * nd1[i]}, fabs(c1) * nl0[i] + c2 * nl1[i]);
+  C2<T> q = cdiv(acc, dg);
+  if (!d) { dout[i * k + j] = {damp * q.x, damp * q.y}; return; }
+  C2<T> d0 = d[i * k + j];
+  dout[i * k + j] = {d0.x + damp * q.x, d0.y + damp * q.y};
+}
+
+// y = M x (real matrix, complex multi-column vectors); accumulate adds to y.
+template <class T>
+__global__ void k_spmv_real(int64_t nr, int k, MatV<T> M, const C2<T> *__restrict__ x, C2<T> *__restrict__ y, int accumulate) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t i = t / k;
+  int j = (int)(t - i * k);
+  if (i >= nr) return;
+  C2<T> acc = {0, 0};
+  for (int64_t p = M.rp[i]; p < M.rp[i + 1]; ++p) {
+    const T a = M.v[0][p];
+    C2<T> v = x[(int64_t)M.ci[p] * k + j];
+    acc.x += a * v.x;
+    acc.y += a * v.y;
+  }
+  if (accumulate) { C2<T> o = y[i * k + j]; acc.x += o.x; acc.y += o.y; }
+  y[i * k + j] = acc;
+}
+
+template <class T>
+__global__ void k_add(int64_t n, C2<T> *__restrict__ a, const C2<T> *__restrict__ b) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) { a[i].x += b[i].x; a[i].y += b[i].y; }
+}
+
+// Exact coarsest solve in FP64, one block per column:
+//   phi = N^-1 (G^T r);  u = K^-1 (r - B phi)
+template <class T>
+__global__ void k_coarse(int64_t ne, int64_t nn, int k, const int32_t *__restrict__ fidx,
+                         const double *__restrict__ kinv, const double *__restrict__ ninv, const double *__restrict__ bl,
+                         const double *__restrict__ gtl, const C2<T> *__restrict__ r, C2<T> *__restrict__ u,
+                         C2<T> *__restrict__ phi) {
+  extern __shared__ double sm[];
+  double *rr = sm;               // 2*ne
+  double *rn = rr + 2 * ne;      // 2*nn
+  double *ph = rn + 2 * nn;      // 2*nn
+  double *rs = ph + 2 * nn;      // 2*ne
+  const int j = blockIdx.x;
+  const int f = fidx[j];
+  for (int64_t i = threadIdx.x; i < ne; i += blockDim.x) { rr[2 * i] = (double)r[i * k + j].x; rr[2 * i + 1] = (double)r[i * k + j].y; }
+  __syncthreads();
+  for (int64_t a = threadIdx.x; a < nn; a += blockDim.x) {
+    double sx = 0, sy = 0;
+    for (int64_t i = 0; i < ne; ++i) { double g = gtl[a * ne + i]; sx += g * rr[2 * i]; sy += g * rr[2 * i + 1]; }
+    rn[2 * a] = sx; rn[2 * a + 1] = sy;
+  }
+  __syncthreads();
+  const double *Ni = ninv + 2 * (int64_t)f * nn * nn;
+  for (int64_t a = threadIdx.x; a < nn; a += blockDim.x) {
+    double sx = 0, sy = 0;
+    for (int64_t b = 0; b < nn; ++b) {
+      double mx = Ni[2 * (a * nn + b)], my = Ni[2 * (a * nn + b) + 1];
+      sx += mx * rn[2 * b] - my * rn[2 * b + 1];
+      sy += mx * rn[2 * b + 1] + my * rn[2 * b];
+    }
+    ph[2 * a] = sx; ph[2 * a + 1] = sy;
+  }
+  __syncthreads();
+  const double *Bf = bl + 2 * (int64_t)f * ne * nn;
+  for (int64_t i = threadIdx.x; i < ne; i += blockDim.x) {
+    double sx = rr[2 * i], sy = rr[2 * i + 1];
+    for (int64_t a = 0; a < nn; ++a) {
+      double mx = Bf[2 * (i * nn + a)], my = Bf[2 * (i * nn + a) + 1];
+      sx -= mx * ph[2 * a] - my * ph[2 * a + 1];
+      sy -= mx * ph[2 * a + 1] + my * ph[2 * a];
+    }
+    rs[2 * i] = sx; rs[2 * i + 1] = sy;
+  }
+  __syncthreads();
+  const double *Ki = kinv + 2 * (int64_t)f * ne * ne;
+  for (int64_t i = threadIdx.x; i < ne; i += blockDim.x) {
+    double sx = 0, sy = 0;
+    for (int64_t b = 0; b < ne; ++b) {
+      double mx = Ki[2 * (i * ne + b)], my = Ki[2 * (i * ne + b) + 1];
+      sx += mx * rs[2 * b] - my * rs[2 * b + 1];
+      sy += mx * rs[2 * b + 1] + my * rs[2 * b];
+    }
+    u[i * k + j] = {(T)sx, (T)sy};
+  }
+  for (int64_t a = threadIdx.x; a < nn; a += blockDim.x) phi[a * k + j] = {(T)ph[2 * a], (T)ph[2 * a + 1]};
+}
+
+template <class T>
+MatV<T> view(const Csr &c, const CsrF &f);
+template <>
+MatV<double> view<double>(const Csr &c, const CsrF &) {
+  MatV<double> v; v.nr = c.nr; v.nc = c.nc; v.rp = c.rp; v.ci = c.ci;
+  for (int k = 0; k < c.nv; ++k) v.v[k] = c.v[k];
+  return v;
+}
+template <>
+MatV<float> view<float>(const Csr &, const CsrF &f) {
+  MatV<float> v; v.nr = f.nr; v.nc = f.nc; v.rp = f.rp; v.ci = f.ci;
+  for (int k = 0; k < f.nv; ++k) v.v[k] = f.v[k];
+  return v;
+}
+
+template <class T> const T *arr(double *d, float *f);
+template <> const double *arr<double>(double *d, float *) { return d; }
+template <> const float *arr<float>(double *, float *f) { return f; }
+
+inline unsigned grid_rows(int64_t nr, int k) { return blocks_for(nr * k); }
+
+}  // namespace
+
+template <class T>
+static gp_status vcycle_level(const Hierarchy &H, VWork<T> &w, hipStream_t s, int l, int nu_pre, int nu_post) {
+  const MgLevel &L = H.L[l];
+  const int k = w.k;
+  const int64_t ne = L.ne, nn = L.nn;
+  C2<T> *r = w.r[l], *u = w.u[l], *phi = w.phi[l], *res = w.res[l], *tmp = w.tmp[l];
+  C2<T> *rn = w.rn[l], *d1 = w.dn[l], *d2 = w.dn2[l];
+  if (l + 1 == (int)H.L.size()) {
+    size_t shm = sizeof(double) * (4 * ne + 4 * nn);
+    k_coarse<T><<<k, 256, shm, s>>>(ne, nn, k, w.fidx, w.kinv, w.ninv, w.bl, w.gtl, r, u, phi);
+    return GP_OK;
+  }
+  const MatV<T> K = view<T>(L.K, L.Kf), B = view<T>(L.B, L.Bf), N = view<T>(L.N, L.Nf);
+  const MatV<T> GT = view<T>(L.GT, L.GTf), PT = view<T>(L.PT, L.PTf), P = view<T>(L.P, L.Pf), Pn = view<T>(L.Pn, L.Pnf);
+  const T *kd0 = arr<T>(L.kd[0], L.kdf[0]), *kd1 = arr<T>(L.kd[1], L.kdf[1]), *kd2 = arr<T>(L.kd[2], L.kdf[2]);
+  const T *kl0 = arr<T>(L.kl[0], L.klf[0]), *kl1 = arr<T>(L.kl[1], L.klf[1]), *kl2 = arr<T>(L.kl[2], L.klf[2]);
+  const T *nd0 = arr<T>(L.nd[0], L.ndf[0]), *nd1 = arr<T>(L.nd[1], L.ndf[1]);
+  const T *nl0 = arr<T>(L.nl[0], L.nlf[0]), *nl1 = arr<T>(L.nl[1], L.nlf[1]);
+  const T damp = (T)1.0;
+  const unsigned ge = grid_rows(ne, k), gn = grid_rows(nn, k);
+  // pre-smoothing on the edges (phi = 0)
+  k_edge<T><<<ge, 256, 0, s>>>(ne, k, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, nullptr, nullptr, u,
+                               EdgeMode::JACOBI0, 0, damp);
+  for (int it = 1; it < nu_pre; ++it) {
+    k_edge<T><<<ge, 256, 0, s>>>(ne, k, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, nullptr, tmp,
+                                 EdgeMode::JACOBI, 0, damp);
+    std::swap(u, tmp);
+  }
+  // pre-smoothing on the nodal auxiliary space
+  k_edge<T><<<ge, 256, 0, s>>>(ne, k, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, nullptr, res,
+                               EdgeMode::RES, 0, damp);
+  k_spmv_real<T><<<gn, 256, 0, s>>>(nn, k, GT, res, rn, 0);
+  k_node<T><<<gn, 256, 0, s>>>(nn, k, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, nullptr, phi, damp);
+  for (int it = 1; it < nu_pre; ++it) {
+    k_node<T><<<gn, 256, 0, s>>>(nn, k, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, phi, d2, damp);
+    std::swap(phi, d2);
+  }
+  // coarse-grid correction
+  k_edge<T><<<ge, 256, 0, s>>>(ne, k, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, phi, res,
+                               EdgeMode::RES, 1, damp);
+  const int64_t nec = H.L[l + 1].ne, nnc = H.L[l + 1].nn;
+  k_spmv_real<T><<<grid_rows(nec, k), 256, 0, s>>>(nec, k, PT, res, w.r[l + 1], 0);
+  GP_TRY(vcycle_level<T>(H, w, s, l + 1, nu_pre, nu_post));
+  k_spmv_real<T><<<ge, 256, 0, s>>>(ne, k, P, w.u[l + 1], u, 1);
+  k_spmv_real<T><<<gn, 256, 0, s>>>(nn, k, Pn, w.phi[l + 1], phi, 1);
+  (void)nnc;
+  // post-smoothing: nodal, then edges
+  k_edge<T><<<ge, 256, 0, s>>>(ne, k, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, phi, res,
+                               EdgeMode::RES, 1, damp);
+  k_spmv_real<T><<<gn, 256, 0, s>>>(nn, k, GT, res, rn, 0);
+  k_node<T><<<gn, 256, 0, s>>>(nn, k, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, nullptr, d1, damp);
+  for (int it = 1; it < nu_post; ++it) {
+    k_node<T><<<gn, 256, 0, s>>>(nn, k, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, d1, d2, damp);
+    std::swap(d1, d2);
+  }
+  k_add<T><<<gn, 256, 0, s>>>(nn * k, phi, d1);
+  for (int it = 0; it < nu_post; ++it) {
+    k_edge<T><<<ge, 256, 0, s>>>(ne, k, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, phi, tmp,
+                                 EdgeMode::JACOBI, 1, damp);
+    std::swap(u, tmp);
+  }
+  // results must land in the level's canonical buffers
+  if (u != w.u[l]) GP_HIP(hipMemcpyAsync(w.u[l], u, sizeof(C2<T>) * ne * k, hipMemcpyDeviceToDevice, s));
+  if (phi != w.phi[l]) GP_HIP(hipMemcpyAsync(w.phi[l], phi, sizeof(C2<T>) * nn * k, hipMemcpyDeviceToDevice, s));
+  return GP_OK;
+}
+
+template <class T>
+gp_status vcycle(const Hierarchy &H, VWork<T> &w, hipStream_t s, int nu_pre, int nu_post) {
+  return vcycle_level<T>(H, w, s, 0, nu_pre, nu_post);
+}
+
+template gp_status vwork_create<float>(gp_layout *, const Hierarchy &, int, const double *, const int32_t *,
+                                       const std::vector<double> &, VWork<float> *);
+template gp_status vwork_create<double>(gp_layout *, const Hierarchy &, int, const double *, const int32_t *,
+                                        const std::vector<double> &, VWork<double> *);
+template void vwork_free<float>(VWork<float> *);
+template void vwork_free<double>(VWork<double> *);
+template gp_status vcycle<float>(const Hierarchy &, VWork<float> &, hipStream_t, int, int);
+template gp_status vcycle<double>(const Hierarchy &, VWork<double> &, hipStream_t, int, int);
+
+}  // namespace gp

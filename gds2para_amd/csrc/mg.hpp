@@ -1,0 +1,83 @@
+// mg.hpp -- geometric-algebraic multigrid for A(w) = S + diag(-w^2 eps + i w sigma).
+//
+// The V-cycle works on the symmetrized operator K(w) = W A(w) (W = edge dual
+// volume, diag(W) S symmetric) and carries every iterate in split form
+// x = u + G phi.  Because S G = 0, K G = W D(w) G is assembled directly (the
+// B blocks) and residuals r - K u - B phi never cancel the huge gradient
+// component against the curl-curl part.  Each level holds three real
+// Galerkin blocks sharing one pattern:
+//   K = [K_S, K_eps, K_sig]  -> K(w)  = K_S - w^2 K_eps + i w K_sig
+//   B = [B_eps, B_sig]       -> B(w)  = -w^2 B_eps + i w B_sig      (= K(w) G)
+//   N = [N_eps, N_sig]       -> N(w)  = -w^2 N_eps + i w N_sig      (= G^T K(w) G)
+// built with the sort-based SpGEMM from the structured transfer operators.
+// Smoothing is hybrid: l1-Jacobi on the edges, l1-Jacobi on the nodal
+// auxiliary problem, both with per-column frequencies.  The coarsest level is
+// solved exactly in FP64 (dense inverses of N_L(w) and K_L(w) per frequency).
+#pragma once
+#include <vector>
+
+#include "sparse.hpp"
+#include "transfer.hpp"
+
+namespace gp {
+
+template <class T>
+struct C2 { T x, y; };
+
+template <class T>
+struct MatV {  // read-only view of a CSR in precision T
+  int64_t nr = 0, nc = 0;
+  const int64_t *rp = nullptr;
+  const int32_t *ci = nullptr;
+  const T *v[3] = {nullptr, nullptr, nullptr};
+};
+
+struct MgLevel {
+  int64_t ne = 0, nn = 0;
+  Csr K, B, N, G, GT, P, PT, Pn, PnT;
+  CsrF Kf, Bf, Nf, Gf, GTf, Pf, PTf, Pnf, PnTf;
+  double *kd[3] = {}, *kl[3] = {}, *nd[2] = {}, *nl[2] = {};
+  float *kdf[3] = {}, *klf[3] = {}, *ndf[2] = {}, *nlf[2] = {};
+};
+
+struct Hierarchy {
+  std::vector<MgLevel> L;
+  bool has_f32 = false;
+  // level-0 data for the FP64 outer operator and the V-cycle input scaling
+  double *W = nullptr;        // edge dual volume
+  float *Wf = nullptr;
+  double *epsr = nullptr;     // stack_epsn of each kept edge's layer
+  double *sig = nullptr;      // stack_sign if the edge is a conductor edge, else 0
+  // coarsest level, dense on the host (row-major), per block
+  int64_t neL = 0, nnL = 0;
+  std::vector<double> KL[3], BL[2], NL[2], GTL;
+  double setup_ms = 0;
+};
+
+gp_status build_hierarchy(gp_layout *lay, Hierarchy *H, bool want_f32, int max_levels);
+void free_hierarchy(gp_layout *lay, Hierarchy *H);
+
+// Per-batch data: column frequencies and the coarsest-level inverses.
+template <class T>
+struct VWork {
+  int k = 0;
+  T *c1 = nullptr, *c2 = nullptr;     // per column: -w^2 and w
+  int32_t *fidx = nullptr;            // per column: index into the batch's frequency list
+  int nf = 0;
+  double *kinv = nullptr, *ninv = nullptr;  // [nf][neL*neL] / [nf][nnL*nnL] complex (re,im)
+  double *gtl = nullptr, *bl = nullptr;     // coarsest GT (nnL x neL) real, B per freq [nf][neL*nnL] complex
+  std::vector<C2<T> *> r, u, phi, res, tmp, rn, dn, dn2;
+  Arena arena;
+};
+
+template <class T>
+gp_status vwork_create(gp_layout *lay, const Hierarchy &H, int k, const double *omega_cols, const int32_t *fidx_cols,
+                       const std::vector<double> &omegas, VWork<T> *w);
+template <class T>
+void vwork_free(VWork<T> *w);
+
+// One V-cycle: input w->r[0] (K-form residual, n x k), output w->u[0], w->phi[0].
+template <class T>
+gp_status vcycle(const Hierarchy &H, VWork<T> &w, hipStream_t s, int nu_pre, int nu_post);
+
+}  // namespace gp

@@ -1,16 +1,540 @@
-// solver.hip -- full-wave solve orchestration.
-#include "gp_dev.hpp"
+// solver.hip -- full-wave solve (the system reference() hands to PARDISO,
+// src/generateStiff.cpp:568-699) as batched flexible GMRES in FP64 with a
+// multigrid preconditioner, and Z extraction (Construct_Z_V0_Vh,
+// src/fdtd.hpp:4133-4170).
+//
+// Columns are (port, frequency) pairs.  A batch of k columns shares every
+// sparse matrix read (SpMM); each column keeps its own Arnoldi recurrence,
+// Hessenberg matrix and Givens rotations.  Iterates are kept in split form
+// x = u + G phi so that A x = S u + D(w) (u + G phi) is evaluated without
+// cancellation (S G = 0), which is what lets a 1e-8 relative residual be met
+// at low frequency where the gradient part of x dominates by ~1e10.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <map>
+
+#include "mg.hpp"
 
 namespace gp {
-struct Solver {};
+
+struct Solver {
+  Hierarchy H;
+  bool f32 = false;
+};
+
 void destroy_solver(Solver *s) { delete s; }
+
+namespace {
+
+using cd = std::complex<double>;
+typedef C2<double> Z2;
+
+// ---------------------------------------------------------------- FP64 outer kernels
+
+// out = S u + D(w)(u + G phi)   (mode 0)   or   out = b - (...)   (mode 1)
+// D(w) = w * complex(-w * eps_r * EPSILON0, sigma), the reference's diagonal
+// expression (src/generateStiff.cpp:620-626).  use_phi = 0 drops the G phi term.
+__global__ void k_outer(int64_t n, int k, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                        const double *__restrict__ sv, const int64_t *__restrict__ grp, const int32_t *__restrict__ gci,
+                        const double *__restrict__ gv, const double *__restrict__ epsr, const double *__restrict__ sig,
+                        const double *__restrict__ omega, const Z2 *__restrict__ u, const Z2 *__restrict__ phi,
+                        const Z2 *__restrict__ b, Z2 *__restrict__ out, int mode, int use_phi) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t i = t / k;
+  int j = (int)(t - i * k);
+  if (i >= n) return;
+  Z2 acc = {0., 0.};
+  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) {
+    const double a = sv[p];
+    const Z2 x = u[(int64_t)ci[p] * k + j];
+    acc.x += a * x.x;
+    acc.y += a * x.y;
+  }
+  Z2 xi = u[i * k + j];
+  if (use_phi) {
+    for (int64_t p = grp[i]; p < grp[i + 1]; ++p) {
+      const double g = gv[p];
+      const Z2 x = phi[(int64_t)gci[p] * k + j];
+      xi.x += g * x.x;
+      xi.y += g * x.y;
+    }
+  }
+  const double w = omega[j];
+  const double dr = w * (-(w)*epsr[i] * kEps0), di = w * sig[i];
+  acc.x += dr * xi.x - di * xi.y;
+  acc.y += dr * xi.y + di * xi.x;
+  if (mode == 1) { const Z2 bb = b[i * k + j]; acc.x = bb.x - acc.x; acc.y = bb.y - acc.y; }
+  out[i * k + j] = acc;
+}
+
+// partial[t][blk][j] = sum over this block's rows of conj(V_t) * w   (k a power of two <= 256)
+__global__ void k_dots(int64_t n, int k, const Z2 *__restrict__ V, int64_t vstride, const Z2 *__restrict__ w,
+                       Z2 *__restrict__ partial) {
+  __shared__ Z2 sh[256];
+  const int t = blockIdx.y;
+  const Z2 *Vt = V + (int64_t)t * vstride;
+  const int rows_per = blockDim.x / k;
+  const int j = threadIdx.x % k;
+  int64_t i = (int64_t)blockIdx.x * rows_per + threadIdx.x / k;
+  const int64_t step = (int64_t)gridDim.x * rows_per;
+  Z2 acc = {0., 0.};
+  for (; i < n; i += step) {
+    Z2 a = Vt[i * k + j], b = w[i * k + j];
+    acc.x += a.x * b.x + a.y * b.y;
+    acc.y += a.x * b.y - a.y * b.x;
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s >= k; s >>= 1) {
+    if ((int)threadIdx.x < s) { sh[threadIdx.x].x += sh[threadIdx.x + s].x; sh[threadIdx.x].y += sh[threadIdx.x + s].y; }
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < k) partial[((int64_t)t * gridDim.x + blockIdx.x) * k + threadIdx.x] = sh[threadIdx.x];
+}
+
+__global__ void k_dots_final(int nvec, int nb, int k, const Z2 *__restrict__ partial, Z2 *__restrict__ h) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nvec * k) return;
+  int t = q / k, j = q % k;
+  Z2 s = {0., 0.};
+  for (int b = 0; b < nb; ++b) { Z2 v = partial[((int64_t)t * nb + b) * k + j]; s.x += v.x; s.y += v.y; }
+  h[q] = s;
+}
+
+// w -= sum_t V_t h_t (per column)
+__global__ void k_gs_update(int64_t n, int k, int nvec, const Z2 *__restrict__ V, int64_t vstride,
+                            const Z2 *__restrict__ h, Z2 *__restrict__ w) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n * k) return;
+  int j = (int)(q % k);
+  Z2 acc = w[q];
+  for (int t = 0; t < nvec; ++t) {
+    Z2 a = V[(int64_t)t * vstride + q], c = h[t * k + j];
+    acc.x -= a.x * c.x - a.y * c.y;
+    acc.y -= a.x * c.y + a.y * c.x;
+  }
+  w[q] = acc;
+}
+
+// x += sum_t Z_t y_t (per column)
+__global__ void k_xupdate(int64_t n, int k, int nvec, const Z2 *__restrict__ Zb, int64_t zstride,
+                          const Z2 *__restrict__ y, Z2 *__restrict__ x) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n * k) return;
+  int j = (int)(q % k);
+  Z2 acc = x[q];
+  for (int t = 0; t < nvec; ++t) {
+    Z2 a = Zb[(int64_t)t * zstride + q], c = y[t * k + j];
+    acc.x += a.x * c.x - a.y * c.y;
+    acc.y += a.x * c.y + a.y * c.x;
+  }
+  x[q] = acc;
+}
+
+__global__ void k_scale_cols(int64_t n, int k, const Z2 *__restrict__ w, const double *__restrict__ inv, Z2 *__restrict__ out) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n * k) return;
+  const double s = inv[q % k];
+  out[q] = {w[q].x * s, w[q].y * s};
+}
+
+template <class T>
+__global__ void k_weight_in(int64_t n, int k, const double *__restrict__ W, const Z2 *__restrict__ v, C2<T> *__restrict__ out) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n * k) return;
+  const double w = W[q / k];
+  out[q] = {(T)(w * v[q].x), (T)(w * v[q].y)};
+}
+
+template <class T>
+__global__ void k_widen(int64_t n, const C2<T> *__restrict__ in, Z2 *__restrict__ out) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) out[q] = {(double)in[q].x, (double)in[q].y};
+}
+
+// b[row*k + j] = (0, -((dir * f) * 2) * pi) at the port's edges (src/generateStiff.cpp:583-593)
+__global__ void k_rhs(int64_t nent, int k, const int64_t *__restrict__ row, const int32_t *__restrict__ col,
+                      const double *__restrict__ val, Z2 *__restrict__ b) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nent) return;
+  b[row[q] * k + col[q]] = {0., val[q]};
+}
+
+// x = u + G phi at selected rows (port edges) or at every row (idx == null)
+__global__ void k_assemble_x(int64_t nidx, const int64_t *__restrict__ idx, int k, const int64_t *__restrict__ grp,
+                             const int32_t *__restrict__ gci, const double *__restrict__ gv, const Z2 *__restrict__ u,
+                             const Z2 *__restrict__ phi, Z2 *__restrict__ out) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t a = q / k;
+  int j = (int)(q - a * k);
+  if (a >= nidx) return;
+  int64_t i = idx ? idx[a] : a;
+  Z2 x = u[i * k + j];
+  for (int64_t p = grp[i]; p < grp[i + 1]; ++p) {
+    const Z2 f = phi[(int64_t)gci[p] * k + j];
+    x.x += gv[p] * f.x;
+    x.y += gv[p] * f.y;
+  }
+  out[a * k + j] = x;
+}
+
+struct DevBuf {
+  Arena A;
+  ~DevBuf() { A.free_all(); }
+};
+
+// Column-batched dot products V_t^H w for t < nvec; result h[t*k + j] on the host.
+gp_status dots(hipStream_t s, int64_t n, int k, const Z2 *V, int64_t vstride, int nvec, const Z2 *w, Z2 *partial,
+               Z2 *hd, std::vector<cd> &h, int nb) {
+  dim3 grid(nb, nvec);
+  k_dots<<<grid, 256, 0, s>>>(n, k, V, vstride, w, partial);
+  k_dots_final<<<(nvec * k + 255) / 256, 256, 0, s>>>(nvec, nb, k, partial, hd);
+  h.resize((size_t)nvec * k);
+  GP_HIP(hipMemcpyAsync(h.data(), hd, sizeof(Z2) * nvec * k, hipMemcpyDeviceToHost, s));
+  GP_HIP(hipStreamSynchronize(s));
+  return GP_OK;
+}
+
+int pow2_at_least(int x) { int p = 1; while (p < x) p <<= 1; return p; }
+
+struct Column { int32_t port; int32_t fslot; };
+
+// One batch: k columns (k a power of two; padded columns repeat the last real column).
+template <class T>
+gp_status solve_batch(gp_layout *lay, Solver *sv, const std::vector<Column> &cols, int kreal,
+                      const std::vector<double> &freqs, const gp_solve_opts &o, double *X, int64_t xcol0,
+                      std::vector<std::vector<cd>> &zcols, gp_solve_stats &st) {
+  const Hierarchy &H = sv->H;
+  hipStream_t s = lay->stream;
+  const int k = (int)cols.size();
+  const int64_t n = lay->n, nn = H.L[0].nn;
+  const int m = std::max(2, o.restart);
+  const gp_mesh &mesh = lay->hm.view;
+  // per-column frequencies and the batch's distinct frequency list
+  std::vector<double> omega_col(k), omegas;
+  std::vector<int32_t> fidx(k);
+  std::map<int32_t, int> slot;
+  for (int j = 0; j < k; ++j) {
+    int32_t fs = cols[j].fslot;
+    if (!slot.count(fs)) { slot[fs] = (int)omegas.size(); omegas.push_back(2. * kPi * freqs[fs]); }
+    fidx[j] = slot[fs];
+    omega_col[j] = 2. * kPi * freqs[fs];
+  }
+  VWork<T> vw;
+  gp_status stt = vwork_create<T>(lay, H, k, omega_col.data(), fidx.data(), omegas, &vw);
+  if (stt != GP_OK) { vwork_free(&vw); return stt; }
+  DevBuf D;
+  Arena &A = D.A;
+  Z2 *b, *xu, *xphi, *V, *Zu, *Zp, *w, *partial, *hd;
+  double *om_d, *inv_d;
+  const int nb = 512;
+  GP_TRY(A.alloc(&b, n * k)); GP_TRY(A.alloc(&xu, n * k)); GP_TRY(A.alloc(&xphi, nn * k));
+  GP_TRY(A.alloc(&V, (int64_t)(m + 1) * n * k));
+  GP_TRY(A.alloc(&Zu, (int64_t)m * n * k)); GP_TRY(A.alloc(&Zp, (int64_t)m * nn * k));
+  GP_TRY(A.alloc(&w, n * k)); GP_TRY(A.alloc(&partial, (int64_t)nb * (m + 2) * k));
+  GP_TRY(A.alloc(&hd, (int64_t)(m + 2) * k));
+  GP_TRY(A.alloc(&om_d, k)); GP_TRY(A.alloc(&inv_d, k));
+  GP_HIP(hipMemcpyAsync(om_d, omega_col.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemsetAsync(b, 0, sizeof(Z2) * n * k, s));
+  GP_HIP(hipMemsetAsync(xu, 0, sizeof(Z2) * n * k, s));
+  GP_HIP(hipMemsetAsync(xphi, 0, sizeof(Z2) * nn * k, s));
+  {  // right-hand sides
+    std::vector<int64_t> row;
+    std::vector<int32_t> col;
+    std::vector<double> val;
+    std::vector<int64_t> map(lay->g.Ne);
+    gp_edge_map(&mesh, map.data(), nullptr);
+    for (int j = 0; j < k; ++j) {
+      const int32_t p = cols[j].port;
+      const double f = freqs[cols[j].fslot];
+      for (int32_t sd = mesh.port_side_ptr[p]; sd < mesh.port_side_ptr[p + 1]; ++sd)
+        for (int64_t q = mesh.side_edge_ptr[sd]; q < mesh.side_edge_ptr[sd + 1]; ++q) {
+          row.push_back(map[mesh.side_edges[q]]);
+          col.push_back(j);
+          val.push_back(0. - (double)mesh.side_dir[sd] * f * 2. * kPi);
+        }
+    }
+    int64_t *drow; int32_t *dcol; double *dval;
+    GP_TRY(A.alloc(&drow, (int64_t)row.size())); GP_TRY(A.alloc(&dcol, (int64_t)row.size())); GP_TRY(A.alloc(&dval, (int64_t)row.size()));
+    GP_HIP(hipMemcpyAsync(drow, row.data(), sizeof(int64_t) * row.size(), hipMemcpyHostToDevice, s));
+    GP_HIP(hipMemcpyAsync(dcol, col.data(), sizeof(int32_t) * row.size(), hipMemcpyHostToDevice, s));
+    GP_HIP(hipMemcpyAsync(dval, val.data(), sizeof(double) * row.size(), hipMemcpyHostToDevice, s));
+    k_rhs<<<blocks_for((int64_t)row.size()), 256, 0, s>>>((int64_t)row.size(), k, drow, dcol, dval, b);
+  }
+  const MgLevel &L0 = H.L[0];
+  const CsrD &S = lay->S;
+  const unsigned gnk = blocks_for(n * k), gnnk = blocks_for(nn * k);
+  std::vector<cd> h, bnorm2;
+  GP_TRY(dots(s, n, k, b, 0, 1, b, partial, hd, bnorm2, nb));
+  std::vector<double> bnorm(k), beta(k);
+  for (int j = 0; j < k; ++j) bnorm[j] = std::sqrt(bnorm2[j].real());
+  // Arnoldi state per column
+  std::vector<cd> Hm((size_t)(m + 1) * m * k), cs((size_t)m * k), sn((size_t)m * k), g((size_t)(m + 1) * k), y((size_t)m * k);
+  std::vector<double> invh(k);
+  int iters = 0;
+  bool done = false;
+  double relres_max = 0;
+  while (!done) {
+    // explicit FP64 residual of the split iterate
+    k_outer<<<gnk, 256, 0, s>>>(n, k, S.rp, S.ci, S.v, L0.G.rp, L0.G.ci, L0.G.v[0], H.epsr, H.sig, om_d, xu, xphi, b, V, 1, 1);
+    GP_TRY(dots(s, n, k, V, 0, 1, V, partial, hd, h, nb));
+    bool all = true;
+    relres_max = 0;
+    for (int j = 0; j < k; ++j) {
+      beta[j] = std::sqrt(h[j].real());
+      double rr = bnorm[j] > 0 ? beta[j] / bnorm[j] : 0.;
+      if (j < kreal) relres_max = std::max(relres_max, rr);
+      if (rr > o.rtol) all = false;
+    }
+    if (all || iters >= o.max_iters) break;
+    for (int j = 0; j < k; ++j) invh[j] = beta[j] > 0 ? 1.0 / beta[j] : 0.0;
+    GP_HIP(hipMemcpyAsync(inv_d, invh.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+    k_scale_cols<<<gnk, 256, 0, s>>>(n, k, V, inv_d, V);
+    std::fill(g.begin(), g.end(), cd(0));
+    for (int j = 0; j < k; ++j) g[j] = beta[j];
+    int used = 0;
+    for (int i = 0; i < m && iters < o.max_iters; ++i) {
+      Z2 *Vi = V + (int64_t)i * n * k;
+      // preconditioner: (u, phi) = Vcycle(W v)
+      k_weight_in<T><<<gnk, 256, 0, s>>>(n, k, H.W, Vi, vw.r[0]);
+      GP_TRY(vcycle<T>(H, vw, s, 1, 1));
+      Z2 *zu = Zu + (int64_t)i * n * k, *zp = Zp + (int64_t)i * nn * k;
+      k_widen<T><<<gnk, 256, 0, s>>>(n * k, vw.u[0], zu);
+      k_widen<T><<<gnnk, 256, 0, s>>>(nn * k, vw.phi[0], zp);
+      // w = A z (split form, FP64)
+      k_outer<<<gnk, 256, 0, s>>>(n, k, S.rp, S.ci, S.v, L0.G.rp, L0.G.ci, L0.G.v[0], H.epsr, H.sig, om_d, zu, zp, nullptr, w, 0, 1);
+      // classical Gram-Schmidt with one re-orthogonalization pass
+      GP_TRY(dots(s, n, k, V, n * k, i + 1, w, partial, hd, h, nb));
+      k_gs_update<<<gnk, 256, 0, s>>>(n, k, i + 1, V, n * k, hd, w);
+      std::vector<cd> h2;
+      GP_TRY(dots(s, n, k, V, n * k, i + 1, w, partial, hd, h2, nb));
+      k_gs_update<<<gnk, 256, 0, s>>>(n, k, i + 1, V, n * k, hd, w);
+      std::vector<cd> nrm;
+      GP_TRY(dots(s, n, k, w, 0, 1, w, partial, hd, nrm, nb));
+      bool all_in = true;
+      for (int j = 0; j < k; ++j) {
+        cd *Hc = &Hm[(size_t)j * (m + 1) * m];
+        for (int t = 0; t <= i; ++t) Hc[t * m + i] = h[t * k + j] + h2[t * k + j];
+        double hn = std::sqrt(std::max(nrm[j].real(), 0.0));
+        Hc[(i + 1) * m + i] = hn;
+        invh[j] = hn > 0 ? 1.0 / hn : 0.0;
+        // apply previous rotations, then form the new one
+        for (int t = 0; t < i; ++t) {
+          cd c = cs[t * k + j], sn_ = sn[t * k + j];
+          cd a = Hc[t * m + i], bq = Hc[(t + 1) * m + i];
+          Hc[t * m + i] = std::conj(c) * a + std::conj(sn_) * bq;
+          Hc[(t + 1) * m + i] = -sn_ * a + c * bq;
+        }
+        cd a = Hc[i * m + i], bq = Hc[(i + 1) * m + i];
+        double den = std::sqrt(std::norm(a) + std::norm(bq));
+        cd c = den > 0 ? a / den : cd(1), sn_ = den > 0 ? bq / den : cd(0);
+        cs[i * k + j] = c;
+        sn[i * k + j] = sn_;
+        Hc[i * m + i] = std::conj(c) * a + std::conj(sn_) * bq;
+        Hc[(i + 1) * m + i] = 0;
+        cd gi = g[i * k + j];
+        g[i * k + j] = std::conj(c) * gi;
+        g[(i + 1) * k + j] = -sn_ * gi;
+        if (bnorm[j] > 0 && std::abs(g[(i + 1) * k + j]) > o.rtol * bnorm[j]) all_in = false;
+      }
+      GP_HIP(hipMemcpyAsync(inv_d, invh.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+      k_scale_cols<<<gnk, 256, 0, s>>>(n, k, w, inv_d, V + (int64_t)(i + 1) * n * k);
+      ++iters;
+      used = i + 1;
+      if (all_in) break;
+    }
+    // y = R^-1 g per column, then x += Z y
+    for (int j = 0; j < k; ++j) {
+      cd *Hc = &Hm[(size_t)j * (m + 1) * m];
+      for (int t = used - 1; t >= 0; --t) {
+        cd acc = g[t * k + j];
+        for (int q = t + 1; q < used; ++q) acc -= Hc[t * m + q] * y[q * k + j];
+        y[t * k + j] = Hc[t * m + t] != cd(0) ? acc / Hc[t * m + t] : cd(0);
+      }
+    }
+    std::vector<Z2> yh((size_t)used * k);
+    for (int t = 0; t < used; ++t)
+      for (int j = 0; j < k; ++j) yh[t * k + j] = {y[t * k + j].real(), y[t * k + j].imag()};
+    GP_HIP(hipMemcpyAsync(hd, yh.data(), sizeof(Z2) * used * k, hipMemcpyHostToDevice, s));
+    k_xupdate<<<gnk, 256, 0, s>>>(n, k, used, Zu, n * k, hd, xu);
+    k_xupdate<<<gnnk, 256, 0, s>>>(nn, k, used, Zp, nn * k, hd, xphi);
+    if (o.verbose) std::fprintf(stderr, "[gp] batch k=%d restart after %d its, est %.3e\n", k, iters, relres_max);
+  }
+  // outputs: Z from the port edges; optional full solution vectors
+  {
+    std::vector<int64_t> idx;
+    std::vector<int64_t> map(lay->g.Ne);
+    gp_edge_map(&mesh, map.data(), nullptr);
+    for (int32_t p = 0; p < mesh.num_ports; ++p) {
+      int32_t s0 = mesh.port_side_ptr[p];
+      for (int64_t q = mesh.side_edge_ptr[s0]; q < mesh.side_edge_ptr[s0 + 1]; ++q) idx.push_back(map[mesh.side_edges[q]]);
+    }
+    int64_t *didx; Z2 *xo;
+    GP_TRY(A.alloc(&didx, (int64_t)idx.size())); GP_TRY(A.alloc(&xo, (int64_t)idx.size() * k));
+    GP_HIP(hipMemcpyAsync(didx, idx.data(), sizeof(int64_t) * idx.size(), hipMemcpyHostToDevice, s));
+    k_assemble_x<<<blocks_for((int64_t)idx.size() * k), 256, 0, s>>>((int64_t)idx.size(), didx, k, L0.G.rp, L0.G.ci,
+                                                                      L0.G.v[0], xu, xphi, xo);
+    std::vector<Z2> xh(idx.size() * k);
+    GP_HIP(hipMemcpyAsync(xh.data(), xo, sizeof(Z2) * xh.size(), hipMemcpyDeviceToHost, s));
+    GP_HIP(hipStreamSynchronize(s));
+    zcols.assign(kreal, {});
+    for (int j = 0; j < kreal; ++j) {
+      zcols[j].resize(idx.size());
+      for (size_t a = 0; a < idx.size(); ++a) zcols[j][a] = cd(xh[a * k + j].x, xh[a * k + j].y);
+    }
+    if (X) {
+      Z2 *xfull;
+      GP_TRY(A.alloc(&xfull, n * k));
+      k_assemble_x<<<gnk, 256, 0, s>>>(n, nullptr, k, L0.G.rp, L0.G.ci, L0.G.v[0], xu, xphi, xfull);
+      std::vector<Z2> xf(n * k);
+      GP_HIP(hipMemcpyAsync(xf.data(), xfull, sizeof(Z2) * n * k, hipMemcpyDeviceToHost, s));
+      GP_HIP(hipStreamSynchronize(s));
+      for (int j = 0; j < kreal; ++j)
+        for (int64_t i = 0; i < n; ++i) {
+          X[2 * ((xcol0 + j) * n + i)] = xf[i * k + j].x;
+          X[2 * ((xcol0 + j) * n + i) + 1] = xf[i * k + j].y;
+        }
+    }
+  }
+  GP_HIP(hipGetLastError());
+  vwork_free(&vw);
+  st.max_iters_used = std::max(st.max_iters_used, iters);
+  st.max_rel_residual = std::max(st.max_rel_residual, relres_max);
+  st.batches += 1;
+  st.columns += kreal;
+  if (relres_max <= o.rtol) st.converged += kreal;
+  return GP_OK;
+}
+
+}  // namespace
+
 }  // namespace gp
+
 using namespace gp;
 
 extern "C" void gp_solve_opts_default(gp_solve_opts *o) {
-  o->rtol = 1e-8; o->max_iters = 2000; o->restart = 30; o->precision = GP_PREC_MIXED; o->batch = 0; o->verbose = 0;
+  o->rtol = 1e-8;
+  o->max_iters = 500;
+  o->restart = 30;
+  o->precision = GP_PREC_MIXED;
+  o->batch = 0;
+  o->verbose = 0;
 }
-extern "C" gp_status gp_prepare_solver(gp_layout *, int32_t) { return fail(GP_ERR_STATE, "solver not built yet"); }
-extern "C" gp_status gp_solve_fullwave(gp_layout *, const double *, int32_t, const int32_t *, int32_t, const gp_solve_opts *, double *, double *, gp_solve_stats *) { return fail(GP_ERR_STATE, "solver not built yet"); }
-extern "C" gp_status gp_solve_real(const int64_t *, const int64_t *, const double *, int64_t, int64_t, const double *, double *, double, int32_t *, double *) { return fail(GP_ERR_STATE, "solver not built yet"); }
-extern "C" gp_status gp_bench_spmv(gp_layout *, int32_t, int32_t, int32_t, double *) { return fail(GP_ERR_STATE, "solver not built yet"); }
+
+extern "C" gp_status gp_prepare_solver(gp_layout *lay, int32_t precision) {
+  if (!lay) return fail(GP_ERR_ARG, "null layout");
+  const bool want32 = precision == GP_PREC_MIXED;
+  if (lay->solver && (!want32 || lay->solver->f32)) return GP_OK;
+  if (lay->solver) { destroy_solver(lay->solver); lay->solver = nullptr; }
+  auto *sv = new Solver();
+  gp_status st = build_hierarchy(lay, &sv->H, want32, 12);
+  if (st != GP_OK) { delete sv; return st; }
+  sv->f32 = want32;
+  lay->solver = sv;
+  return GP_OK;
+}
+
+extern "C" gp_status gp_solve_fullwave(gp_layout *lay, const double *freqs, int32_t nf, const int32_t *ports, int32_t np,
+                                       const gp_solve_opts *opts, double *X, double *Z, gp_solve_stats *stats) {
+  if (!lay || !freqs || nf < 1 || !ports || np < 1) return fail(GP_ERR_ARG, "bad solve arguments");
+  gp_solve_opts o;
+  if (opts) o = *opts; else gp_solve_opts_default(&o);
+  const gp_mesh &mesh = lay->hm.view;
+  for (int32_t a = 0; a < np; ++a)
+    if (ports[a] < 0 || ports[a] >= mesh.num_ports) return fail(GP_ERR_ARG, "port index out of range");
+  for (int32_t f = 0; f < nf; ++f)
+    if (!(freqs[f] > 0)) return fail(GP_ERR_ARG, "frequencies must be positive");
+  auto t0 = std::chrono::steady_clock::now();
+  GP_TRY(gp_prepare_solver(lay, o.precision));
+  auto t1 = std::chrono::steady_clock::now();
+  Solver *sv = lay->solver;
+  gp_solve_stats st{};
+  // columns ordered frequency-major: c = f*np + p
+  std::vector<Column> all;
+  for (int32_t f = 0; f < nf; ++f)
+    for (int32_t a = 0; a < np; ++a) all.push_back({ports[a], f});
+  int kb = o.batch > 0 ? o.batch : (o.precision == GP_PREC_MIXED ? 16 : 1);
+  kb = std::min(kb, 64);
+  const int32_t npt = mesh.num_ports;
+  std::vector<double> zbuf(Z ? 0 : 0);
+  if (Z) std::fill(Z, Z + 2 * (size_t)nf * npt * npt, 0.0);
+  std::vector<double> fr(freqs, freqs + nf);
+  for (size_t c0 = 0; c0 < all.size(); c0 += kb) {
+    int kreal = (int)std::min<size_t>(kb, all.size() - c0);
+    int k = pow2_at_least(kreal);
+    std::vector<Column> cols(all.begin() + c0, all.begin() + c0 + kreal);
+    while ((int)cols.size() < k) cols.push_back(cols.back());
+    std::vector<std::vector<cd>> xport;
+    gp_status s2 = sv->f32 && o.precision == GP_PREC_MIXED
+                       ? solve_batch<float>(lay, sv, cols, kreal, fr, o, X, (int64_t)c0, xport, st)
+                       : solve_batch<double>(lay, sv, cols, kreal, fr, o, X, (int64_t)c0, xport, st);
+    if (s2 != GP_OK) return s2;
+    if (Z) {
+      // Construct_Z_V0_Vh on the first side of every port
+      for (int j = 0; j < kreal; ++j) {
+        const int32_t src = cols[j].port, f = cols[j].fslot;
+        double cur = 0.;
+        for (int32_t sd = mesh.port_side_ptr[src]; sd < mesh.port_side_ptr[src + 1]; ++sd) cur += mesh.side_area[sd];
+        size_t a = 0;
+        for (int32_t p = 0; p < npt; ++p) {
+          const int32_t s0 = mesh.port_side_ptr[p];
+          cd z(0., 0.);
+          for (int64_t q = mesh.side_edge_ptr[s0]; q < mesh.side_edge_ptr[s0 + 1]; ++q, ++a) {
+            int dir; int64_t iz, ix, iy;
+            lay->g.decode(mesh.side_edges[q], dir, iz, ix, iy);
+            double leng = dir == 2 ? mesh.zn[iz + 1] - mesh.zn[iz] : dir == 1 ? mesh.xn[ix + 1] - mesh.xn[ix] : mesh.yn[iy + 1] - mesh.yn[iy];
+            z -= xport[j][a] * leng * (mesh.side_dir[s0] * 1.0);
+          }
+          z /= cur;
+          size_t zi = (size_t)f * npt * npt + p + (size_t)npt * src;
+          Z[2 * zi] = z.real();
+          Z[2 * zi + 1] = z.imag();
+        }
+      }
+    }
+  }
+  auto t2 = std::chrono::steady_clock::now();
+  st.setup_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  st.solve_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+  if (stats) *stats = st;
+  return st.converged == st.columns ? GP_OK : fail(GP_ERR_NOCONV, "some columns stopped above the tolerance");
+}
+
+extern "C" gp_status gp_solve_real(const int64_t *, const int64_t *, const double *, int64_t, int64_t, const double *,
+                                   double *, double, int32_t *, double *) {
+  return fail(GP_ERR_STATE, "real nodal solve not built yet");
+}
+
+extern "C" gp_status gp_bench_spmv(gp_layout *lay, int32_t k, int32_t precision, int32_t iters, double *avg_ms) {
+  if (!lay || k < 1 || iters < 1 || !avg_ms) return fail(GP_ERR_ARG, "bad arguments");
+  GP_TRY(gp_prepare_solver(lay, GP_PREC_FP64));
+  Solver *sv = lay->solver;
+  const int64_t n = lay->n;
+  hipStream_t s = lay->stream;
+  DevBuf D;
+  Z2 *x, *y;
+  double *om;
+  GP_TRY(D.A.alloc(&x, n * k)); GP_TRY(D.A.alloc(&y, n * k)); GP_TRY(D.A.alloc(&om, k));
+  std::vector<double> omh(k, 2 * kPi * 1e9);
+  GP_HIP(hipMemcpyAsync(om, omh.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemsetAsync(x, 0, sizeof(Z2) * n * k, s));
+  const CsrD &S = lay->S;
+  const MgLevel &L0 = sv->H.L[0];
+  (void)precision;
+  hipEvent_t e0, e1;
+  GP_HIP(hipEventCreate(&e0)); GP_HIP(hipEventCreate(&e1));
+  k_outer<<<blocks_for(n * k), 256, 0, s>>>(n, k, S.rp, S.ci, S.v, L0.G.rp, L0.G.ci, L0.G.v[0], sv->H.epsr, sv->H.sig, om, x, nullptr, nullptr, y, 0, 0);
+  GP_HIP(hipEventRecord(e0, s));
+  for (int it = 0; it < iters; ++it)
+    k_outer<<<blocks_for(n * k), 256, 0, s>>>(n, k, S.rp, S.ci, S.v, L0.G.rp, L0.G.ci, L0.G.v[0], sv->H.epsr, sv->H.sig, om, x, nullptr, nullptr, y, 0, 0);
+  GP_HIP(hipEventRecord(e1, s));
+  GP_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  GP_HIP(hipEventElapsedTime(&ms, e0, e1));
+  *avg_ms = ms / iters;
+  (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+  return GP_OK;
+}

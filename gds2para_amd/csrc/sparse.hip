@@ -1,0 +1,283 @@
+// sparse.hip -- sort-based SpGEMM and transpose for the Galerkin hierarchy.
+//
+// C = A*B is formed by expansion: every product a_ik * b_kj becomes a
+// (key = i*nc + j, index) pair, the pairs are radix-sorted by key (stable, so
+// equal keys keep expansion order and the sums are deterministic), runs of
+// equal keys are summed and the row pointers come from a histogram + scan.
+// All arithmetic in FP64.
+#include <hipcub/hipcub.hpp>
+
+#include "sparse.hpp"
+
+namespace gp {
+
+namespace {
+
+__global__ void k_expand_count(int64_t nr, const int64_t *__restrict__ arp, const int32_t *__restrict__ aci,
+                               const int64_t *__restrict__ brp, int64_t *__restrict__ cnt) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  (void)nr;
+  // one thread per nonzero of A: number of products it spawns
+  if (p >= arp[nr]) return;
+  int32_t k = aci[p];
+  cnt[p] = brp[k + 1] - brp[k];
+}
+
+// row index of every nonzero of a CSR (binary search would also do; a scatter is simpler)
+__global__ void k_row_of(int64_t nr, const int64_t *__restrict__ rp, int32_t *__restrict__ row) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nr) return;
+  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) row[p] = (int32_t)i;
+}
+
+__global__ void k_expand(int64_t annz, const int32_t *__restrict__ arow, const int32_t *__restrict__ aci,
+                         const int64_t *__restrict__ brp, const int32_t *__restrict__ bci,
+                         const int64_t *__restrict__ off, int64_t nc, uint64_t *__restrict__ key,
+                         uint32_t *__restrict__ src_a, uint32_t *__restrict__ src_b) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= annz) return;
+  int64_t o = off[p];
+  const int64_t i = arow[p];
+  const int32_t k = aci[p];
+  for (int64_t q = brp[k]; q < brp[k + 1]; ++q, ++o) {
+    key[o] = (uint64_t)i * (uint64_t)nc + (uint64_t)bci[q];
+    src_a[o] = (uint32_t)p;
+    src_b[o] = (uint32_t)q;
+  }
+}
+
+__global__ void k_iota(int64_t n, uint32_t *__restrict__ a) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = (uint32_t)i;
+}
+
+__global__ void k_heads(int64_t n, const uint64_t *__restrict__ key, int64_t *__restrict__ head) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  head[i] = (i == 0 || key[i] != key[i - 1]) ? 1 : 0;
+}
+
+// seg_start[s] = first sorted position of segment s
+__global__ void k_seg_start(int64_t n, const int64_t *__restrict__ head, const int64_t *__restrict__ hscan,
+                            int64_t *__restrict__ start) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (head[i]) start[hscan[i]] = i;
+}
+
+struct ValSrc {
+  const double *a[3];
+  const double *b[3];
+  int na, nb, nout;
+};
+
+__global__ void k_reduce(int64_t nseg, int64_t n, const int64_t *__restrict__ start, const uint64_t *__restrict__ key,
+                         const uint32_t *__restrict__ perm, const uint32_t *__restrict__ src_a,
+                         const uint32_t *__restrict__ src_b, ValSrc vs, int64_t nc, int32_t *__restrict__ ci,
+                         int64_t *__restrict__ rowcnt, double *__restrict__ o0, double *__restrict__ o1,
+                         double *__restrict__ o2) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  int64_t b = start[s], e = (s + 1 < nseg) ? start[s + 1] : n;
+  double acc[3] = {0., 0., 0.};
+  for (int64_t q = b; q < e; ++q) {
+    uint32_t t = perm[q];
+    uint32_t pa = src_a ? src_a[t] : t, pb = src_b ? src_b[t] : 0;
+    for (int c = 0; c < vs.nout; ++c) {
+      double va = vs.a[vs.na > 1 ? c : 0][pa];
+      double vb = vs.nb > 0 ? vs.b[vs.nb > 1 ? c : 0][pb] : 1.0;
+      acc[c] += va * vb;
+    }
+  }
+  uint64_t k = key[b];
+  int64_t row = (int64_t)(k / (uint64_t)nc);
+  ci[s] = (int32_t)(k % (uint64_t)nc);
+  atomicAdd((unsigned long long *)&rowcnt[row], 1ULL);
+  o0[s] = acc[0];
+  if (o1) o1[s] = acc[1];
+  if (o2) o2[s] = acc[2];
+}
+
+int bits_for(uint64_t maxkey) {
+  int b = 1;
+  while (b < 64 && (maxkey >> b) != 0) ++b;
+  return b;
+}
+
+// Sort (key, src) pairs and reduce equal keys into a CSR with nout value arrays.
+gp_status sort_reduce(Arena &A, int64_t nr, int64_t nc, int64_t nprod, uint64_t *key, const uint32_t *src_a,
+                      const uint32_t *src_b, ValSrc vs, Csr *out, hipStream_t s) {
+  Arena tmp;
+  gp_status st = GP_OK;
+  uint64_t *key2 = nullptr;
+  uint32_t *perm = nullptr, *perm2 = nullptr;
+  int64_t *head = nullptr, *hscan = nullptr, *start = nullptr, *rowcnt = nullptr;
+  if (nprod > 2147483647LL) return fail(GP_ERR_NOMEM, "Galerkin expansion exceeds 2^31 products");
+  do {
+    if ((st = tmp.alloc(&key2, nprod)) != GP_OK) break;
+    if ((st = tmp.alloc(&perm, nprod)) != GP_OK) break;
+    if ((st = tmp.alloc(&perm2, nprod)) != GP_OK) break;
+    k_iota<<<blocks_for(nprod), 256, 0, s>>>(nprod, perm);
+    int endbit = bits_for((uint64_t)nr * (uint64_t)nc);
+    size_t bytes = 0;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key, key2, perm, perm2, (int)nprod, 0, endbit, s) != hipSuccess) { st = fail(GP_ERR_HIP, "radix sort size"); break; }
+    uint8_t *buf = nullptr;
+    if ((st = tmp.alloc(&buf, (int64_t)bytes + 16)) != GP_OK) break;
+    if (hipcub::DeviceRadixSort::SortPairs(buf, bytes, key, key2, perm, perm2, (int)nprod, 0, endbit, s) != hipSuccess) { st = fail(GP_ERR_HIP, "radix sort"); break; }
+    tmp.release(buf, (int64_t)bytes + 16);
+    tmp.release(perm, nprod);
+    if ((st = tmp.alloc(&head, nprod)) != GP_OK) break;
+    if ((st = tmp.alloc(&hscan, nprod + 1)) != GP_OK) break;
+    k_heads<<<blocks_for(nprod), 256, 0, s>>>(nprod, key2, head);
+    if ((st = exclusive_scan_i64(head, hscan, nprod, s, tmp)) != GP_OK) break;
+    int64_t nseg = 0;
+    if (hipMemcpyAsync(&nseg, hscan + nprod, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) { st = fail(GP_ERR_HIP, "segment count"); break; }
+    if ((st = tmp.alloc(&start, nseg)) != GP_OK) break;
+    k_seg_start<<<blocks_for(nprod), 256, 0, s>>>(nprod, head, hscan, start);
+    if ((st = csr_alloc(A, nr, nc, nseg, vs.nout, out)) != GP_OK) break;
+    if ((st = tmp.alloc(&rowcnt, nr)) != GP_OK) break;
+    if (hipMemsetAsync(rowcnt, 0, sizeof(int64_t) * nr, s) != hipSuccess) { st = fail(GP_ERR_HIP, "memset"); break; }
+    k_reduce<<<blocks_for(nseg), 256, 0, s>>>(nseg, nprod, start, key2, perm2, src_a, src_b, vs, nc, out->ci, rowcnt,
+                                               out->v[0], vs.nout > 1 ? out->v[1] : nullptr,
+                                               vs.nout > 2 ? out->v[2] : nullptr);
+    if ((st = exclusive_scan_i64(rowcnt, out->rp, nr, s, tmp)) != GP_OK) break;
+    if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess) { st = fail(GP_ERR_HIP, "sort_reduce kernels"); break; }
+  } while (0);
+  (void)hipStreamSynchronize(s);
+  tmp.free_all();
+  return st;
+}
+
+}  // namespace
+
+gp_status csr_alloc(Arena &A, int64_t nr, int64_t nc, int64_t nnz, int nv, Csr *out) {
+  *out = Csr();
+  out->nr = nr; out->nc = nc; out->nnz = nnz; out->nv = nv;
+  GP_TRY(A.alloc(&out->rp, nr + 1));
+  GP_TRY(A.alloc(&out->ci, std::max<int64_t>(nnz, 1)));
+  for (int k = 0; k < nv; ++k) GP_TRY(A.alloc(&out->v[k], std::max<int64_t>(nnz, 1)));
+  return GP_OK;
+}
+
+void csr_release(Arena &A, Csr &c) {
+  A.release(c.rp, c.nr + 1);
+  A.release(c.ci, std::max<int64_t>(c.nnz, 1));
+  for (int k = 0; k < c.nv; ++k) A.release(c.v[k], std::max<int64_t>(c.nnz, 1));
+  c = Csr();
+}
+
+gp_status csr_upload(Arena &A, const HostCsr &h, Csr *out, hipStream_t s) {
+  int64_t nnz = (int64_t)h.ci.size();
+  GP_TRY(csr_alloc(A, h.nr, h.nc, nnz, 1, out));
+  GP_HIP(hipMemcpyAsync(out->rp, h.rp.data(), sizeof(int64_t) * (h.nr + 1), hipMemcpyHostToDevice, s));
+  if (nnz) {
+    GP_HIP(hipMemcpyAsync(out->ci, h.ci.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s));
+    GP_HIP(hipMemcpyAsync(out->v[0], h.v.data(), sizeof(double) * nnz, hipMemcpyHostToDevice, s));
+  }
+  GP_HIP(hipStreamSynchronize(s));
+  return GP_OK;
+}
+
+gp_status spgemm(Arena &A, const Csr &a, const Csr &b, Csr *c, hipStream_t s) {
+  if (a.nc != b.nr) return fail(GP_ERR_ARG, "spgemm shape mismatch");
+  if (a.nv > 1 && b.nv > 1 && a.nv != b.nv) return fail(GP_ERR_ARG, "spgemm value arrays mismatch");
+  Arena tmp;
+  gp_status st = GP_OK;
+  int64_t *cnt = nullptr, *off = nullptr;
+  int32_t *arow = nullptr;
+  uint64_t *key = nullptr;
+  uint32_t *sa = nullptr, *sb = nullptr;
+  do {
+    if (a.nnz == 0) { st = csr_alloc(A, a.nr, b.nc, 0, std::max(a.nv, b.nv), c);
+      if (st == GP_OK && hipMemsetAsync(c->rp, 0, sizeof(int64_t) * (a.nr + 1), s) != hipSuccess) st = fail(GP_ERR_HIP, "memset");
+      break; }
+    if ((st = tmp.alloc(&cnt, a.nnz)) != GP_OK) break;
+    if ((st = tmp.alloc(&off, a.nnz + 1)) != GP_OK) break;
+    if ((st = tmp.alloc(&arow, a.nnz)) != GP_OK) break;
+    k_row_of<<<blocks_for(a.nr), 256, 0, s>>>(a.nr, a.rp, arow);
+    k_expand_count<<<blocks_for(a.nnz), 256, 0, s>>>(a.nr, a.rp, a.ci, b.rp, cnt);
+    if ((st = exclusive_scan_i64(cnt, off, a.nnz, s, tmp)) != GP_OK) break;
+    int64_t nprod = 0;
+    if (hipMemcpyAsync(&nprod, off + a.nnz, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) { st = fail(GP_ERR_HIP, "product count"); break; }
+    tmp.release(cnt, a.nnz);
+    if ((st = tmp.alloc(&key, std::max<int64_t>(nprod, 1))) != GP_OK) break;
+    if ((st = tmp.alloc(&sa, std::max<int64_t>(nprod, 1))) != GP_OK) break;
+    if ((st = tmp.alloc(&sb, std::max<int64_t>(nprod, 1))) != GP_OK) break;
+    k_expand<<<blocks_for(a.nnz), 256, 0, s>>>(a.nnz, arow, a.ci, b.rp, b.ci, off, b.nc, key, sa, sb);
+    ValSrc vs{};
+    vs.na = a.nv; vs.nb = b.nv; vs.nout = std::max(a.nv, b.nv);
+    for (int k = 0; k < 3; ++k) { vs.a[k] = a.v[k]; vs.b[k] = b.v[k]; }
+    if (nprod == 0) { st = csr_alloc(A, a.nr, b.nc, 0, vs.nout, c);
+      if (st == GP_OK && hipMemsetAsync(c->rp, 0, sizeof(int64_t) * (a.nr + 1), s) != hipSuccess) st = fail(GP_ERR_HIP, "memset");
+      break; }
+    st = sort_reduce(A, a.nr, b.nc, nprod, key, sa, sb, vs, c, s);
+  } while (0);
+  (void)hipStreamSynchronize(s);
+  tmp.free_all();
+  return st;
+}
+
+namespace {
+__global__ void k_transpose_keys(int64_t nr, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                 int64_t nrow_t, uint64_t *__restrict__ key) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nr) return;
+  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) key[p] = (uint64_t)ci[p] * (uint64_t)nr + (uint64_t)i;
+  (void)nrow_t;
+}
+}  // namespace
+
+gp_status transpose(Arena &A, const Csr &a, Csr *t, hipStream_t s) {
+  Arena tmp;
+  uint64_t *key = nullptr;
+  gp_status st = GP_OK;
+  do {
+    if (a.nnz == 0) { st = csr_alloc(A, a.nc, a.nr, 0, a.nv, t);
+      if (st == GP_OK && hipMemsetAsync(t->rp, 0, sizeof(int64_t) * (a.nc + 1), s) != hipSuccess) st = fail(GP_ERR_HIP, "memset");
+      break; }
+    if ((st = tmp.alloc(&key, a.nnz)) != GP_OK) break;
+    k_transpose_keys<<<blocks_for(a.nr), 256, 0, s>>>(a.nr, a.rp, a.ci, a.nc, key);
+    ValSrc vs{};
+    vs.na = a.nv; vs.nb = 0; vs.nout = a.nv;
+    for (int k = 0; k < 3; ++k) vs.a[k] = a.v[k];
+    st = sort_reduce(A, a.nc, a.nr, a.nnz, key, nullptr, nullptr, vs, t, s);
+  } while (0);
+  (void)hipStreamSynchronize(s);
+  tmp.free_all();
+  return st;
+}
+
+namespace {
+__global__ void k_d2f(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+}  // namespace
+
+gp_status csr_to_float(Arena &A, const Csr &a, CsrF *f, hipStream_t s) {
+  *f = CsrF();
+  f->nr = a.nr; f->nc = a.nc; f->nnz = a.nnz; f->rp = a.rp; f->ci = a.ci; f->nv = a.nv;
+  for (int k = 0; k < a.nv; ++k) {
+    GP_TRY(A.alloc(&f->v[k], std::max<int64_t>(a.nnz, 1)));
+    if (a.nnz) k_d2f<<<blocks_for(a.nnz), 256, 0, s>>>(a.nnz, a.v[k], f->v[k]);
+  }
+  GP_HIP(hipStreamSynchronize(s));
+  return GP_OK;
+}
+
+gp_status csr_host_copy(const Csr &a, HostCsr *h, int which) {
+  h->nr = a.nr; h->nc = a.nc;
+  h->rp.resize(a.nr + 1);
+  h->ci.resize(a.nnz);
+  h->v.resize(a.nnz);
+  GP_HIP(hipMemcpy(h->rp.data(), a.rp, sizeof(int64_t) * (a.nr + 1), hipMemcpyDeviceToHost));
+  if (a.nnz) {
+    GP_HIP(hipMemcpy(h->ci.data(), a.ci, sizeof(int32_t) * a.nnz, hipMemcpyDeviceToHost));
+    GP_HIP(hipMemcpy(h->v.data(), a.v[which], sizeof(double) * a.nnz, hipMemcpyDeviceToHost));
+  }
+  return GP_OK;
+}
+
+}  // namespace gp

@@ -1,0 +1,42 @@
+// sparse.hpp -- device sparse matrices used by the multigrid hierarchy:
+// CSR with int64 row pointers, int32 columns and up to three FP64 value arrays
+// sharing one pattern, plus the sort-based (expand / radix-sort / reduce)
+// SpGEMM and transpose that build the Galerkin coarse operators on the GPU.
+#pragma once
+#include <vector>
+
+#include "gp_dev.hpp"
+#include "host_csr.hpp"
+
+namespace gp {
+
+struct Csr {
+  int64_t nr = 0, nc = 0, nnz = 0;
+  int64_t *rp = nullptr;
+  int32_t *ci = nullptr;
+  int nv = 0;
+  double *v[3] = {nullptr, nullptr, nullptr};
+};
+
+// FP32 copy of a Csr's values; row pointers and columns are shared.
+struct CsrF {
+  int64_t nr = 0, nc = 0, nnz = 0;
+  const int64_t *rp = nullptr;
+  const int32_t *ci = nullptr;
+  int nv = 0;
+  float *v[3] = {nullptr, nullptr, nullptr};
+};
+
+gp_status csr_upload(Arena &A, const HostCsr &h, Csr *out, hipStream_t s);
+gp_status csr_alloc(Arena &A, int64_t nr, int64_t nc, int64_t nnz, int nv, Csr *out);
+void csr_release(Arena &A, Csr &c);
+// C = A * B; the value arrays multiply pairwise when both have nv > 1
+// (nv must match), otherwise the single array scales every array of the other.
+gp_status spgemm(Arena &A, const Csr &a, const Csr &b, Csr *c, hipStream_t s);
+gp_status transpose(Arena &A, const Csr &a, Csr *t, hipStream_t s);
+// FP32 copy of every value array
+gp_status csr_to_float(Arena &A, const Csr &a, CsrF *f, hipStream_t s);
+// host copy of value array `which` of a
+gp_status csr_host_copy(const Csr &a, HostCsr *h, int which);
+
+}  // namespace gp

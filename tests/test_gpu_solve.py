@@ -1,0 +1,64 @@
+"""GPU parity of the full-wave solve (reference()/reference1() PARDISO path)
+against the refined FP64 direct solve of the oracle: Z within 1e-6 relative,
+solution vectors within 1e-6, true residual <= 1e-8, for the plain FP64
+single-column path and the mixed-precision batched path."""
+import numpy as np
+import pytest
+
+import oracle
+from gds2para_amd import _capi as capi
+from gds2para_amd import mesh as M
+from gds2para_amd.layout import Layout
+
+pytestmark = pytest.mark.gpu
+
+ZTOL = 1e-6
+
+
+def _relerr(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.fixture(scope="module")
+def strip():
+    m = M.Mesh(M.microstrip_spec())
+    _, Zr = oracle.direct_solve(m, [1e9], refine=True)
+    return m, Zr
+
+
+@pytest.fixture(scope="module")
+def manh():
+    m = M.Mesh(M.manhattan_spec((24, 20, 10), n_ports=3, nfreq=3, f_range=(1e8, 1e10), seed=11))
+    fs = m.freqs()
+    X, Zr = oracle.direct_solve(m, fs, refine=True)
+    return m, fs, X, Zr
+
+
+@pytest.mark.parametrize("prec", [capi.GP_PREC_FP64, capi.GP_PREC_MIXED])
+def test_microstrip_z(strip, prec):
+    m, Zr = strip
+    print(f"[solve] microstrip prec={prec}", flush=True)
+    lay = Layout(m)
+    Z, st, _ = lay.solve([1e9], precision=prec, rtol=1e-8)
+    print(st, Z, Zr, flush=True)
+    assert st["status"] == capi.GP_OK and st["max_rel_residual"] <= 1e-8
+    assert _relerr(Z, Zr) < ZTOL
+    lay.close()
+
+
+@pytest.mark.parametrize("prec,batch", [(capi.GP_PREC_FP64, 1), (capi.GP_PREC_MIXED, 8), (capi.GP_PREC_MIXED, 4)])
+def test_manhattan_z_and_x(manh, prec, batch):
+    m, fs, Xr, Zr = manh
+    print(f"[solve] manhattan prec={prec} batch={batch}", flush=True)
+    lay = Layout(m)
+    Z, st, X = lay.solve(fs, precision=prec, rtol=1e-8, batch=batch, want_x=True)
+    print(st, flush=True)
+    assert st["status"] == capi.GP_OK and st["max_rel_residual"] <= 1e-8
+    for k in range(len(fs)):
+        assert _relerr(Z[k], Zr[k]) < ZTOL, (k, _relerr(Z[k], Zr[k]))
+    npt = m.num_ports
+    for k in range(len(fs)):
+        for p in range(npt):
+            x, xr = X[k * npt + p], Xr[k][p]
+            assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-6
+    lay.close()
