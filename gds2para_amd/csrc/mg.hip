@@ -31,6 +31,17 @@ __global__ void k_level0_K(int64_t n, const int64_t *__restrict__ rp, const int3
   }
 }
 
+// d[i] = M_ii (0 when row i stores no diagonal entry)
+__global__ void k_diag(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                       const double *__restrict__ v, double *__restrict__ d) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double x = 0.;
+  for (int64_t p = rp[i]; p < rp[i + 1]; ++p)
+    if (ci[p] == i) x = v[p];
+  d[i] = x;
+}
+
 // B0 = W * [eps_r eps0, sigma] * G   (row scaling of the gradient)
 __global__ void k_level0_B(int64_t n, const int64_t *__restrict__ rp, const double *__restrict__ g,
                            const double *__restrict__ W, const double *__restrict__ epsr, const double *__restrict__ sig,
@@ -84,6 +95,20 @@ gp_status to_f32(Arena &A, const double *src, int64_t n, float **dst, hipStream_
   return GP_OK;
 }
 
+}  // namespace
+
+gp_status aux_smoother_data(Arena &A, const Csr &M, double **d, double **l, hipStream_t s) {
+  return smoother_data(A, M, d, l, s);
+}
+
+gp_status d2f_copy(const double *src, float *dst, int64_t n, hipStream_t s) {
+  if (n) k_d2f_arr<<<blocks_for(n), 256, 0, s>>>(n, src, dst);
+  GP_HIP(hipGetLastError());
+  return GP_OK;
+}
+
+namespace {
+
 gp_status dense_host(const Csr &M, int which, int64_t nr, int64_t nc, std::vector<double> *out) {
   HostCsr h;
   GP_TRY(csr_host_copy(M, &h, which));
@@ -130,6 +155,8 @@ gp_status build_hierarchy(gp_layout *lay, Hierarchy *H, bool want_f32, int max_l
     GP_HIP(hipMemcpyAsync(L0.K.rp, S.rp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToDevice, s));
     k_level0_K<<<blocks_for(n), 256, 0, s>>>(n, S.rp, S.ci, S.v, H->W, H->epsr, H->sig, L0.K.ci, L0.K.v[0], L0.K.v[1],
                                              L0.K.v[2]);
+    GP_TRY(A.alloc(&H->sdiag, n));
+    k_diag<<<blocks_for(n), 256, 0, s>>>(n, S.rp, S.ci, S.v, H->sdiag);
     HostCsr hg;
     build_gradient(g0, &hg);
     GP_TRY(csr_upload(A, hg, &L0.G, s));
@@ -212,6 +239,13 @@ gp_status build_hierarchy(gp_layout *lay, Hierarchy *H, bool want_f32, int max_l
   for (int k = 0; k < 2; ++k) GP_TRY(dense_host(C.N, k, C.nn, C.nn, &H->NL[k]));
   GP_TRY(dense_host(C.GT, 0, C.nn, C.ne, &H->GTL));
   GP_HIP(hipStreamSynchronize(s));
+  GP_TRY(build_aux(lay, H, want_f32));
+  {
+    Csr Sv;
+    Sv.nr = lay->S.n; Sv.nc = lay->S.n; Sv.nnz = lay->S.nnz; Sv.rp = lay->S.rp; Sv.ci = lay->S.ci; Sv.nv = 1; Sv.v[0] = lay->S.v;
+    GP_TRY(spgemm_ex(A, Sv, H->L[0].G, &H->SG, s, 1));
+    GP_TRY(spgemm_ex(A, Sv, H->aux.Gx, &H->SGx, s, 1));
+  }
   H->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return GP_OK;
 }
@@ -288,6 +322,27 @@ gp_status vwork_create(gp_layout *lay, const Hierarchy &H, int k, const double *
     for (int64_t i = 0; i < ne * ne; ++i) { kinv[2 * (f * ne * ne + i)] = a[i].real(); kinv[2 * (f * ne * ne + i) + 1] = a[i].imag(); }
     for (int64_t i = 0; i < nn * nn; ++i) { ninv[2 * (f * nn * nn + i)] = b[i].real(); ninv[2 * (f * nn * nn + i) + 1] = b[i].imag(); }
     for (int64_t i = 0; i < ne * nn; ++i) { bl[2 * (f * ne * nn + i)] = a1 * H.BL[0][i]; bl[2 * (f * ne * nn + i) + 1] = om * H.BL[1][i]; }
+  }
+  {  // auxiliary AMG buffers and coarsest inverses
+    const AuxAMG &X = H.aux;
+    w->ar.assign(X.L.size(), nullptr); w->ax.assign(X.L.size(), nullptr); w->at.assign(X.L.size(), nullptr);
+    for (size_t l = 0; l < X.L.size(); ++l) {
+      GP_TRY(A.alloc(&w->ar[l], X.L[l].n * k)); GP_TRY(A.alloc(&w->ax[l], X.L[l].n * k)); GP_TRY(A.alloc(&w->at[l], X.L[l].n * k));
+    }
+    GP_TRY(A.alloc(&w->a, X.naux * k));
+    GP_TRY(A.alloc(&w->r0, H.L[0].ne * k));
+    const int64_t na = X.nL;
+    std::vector<double> ainv(2 * na * na * w->nf);
+    for (int f = 0; f < w->nf; ++f) {
+      const double om = omegas[f], a1 = -om * om;
+      std::vector<std::complex<double>> a(na * na);
+      for (int64_t i = 0; i < na * na; ++i) a[i] = {a1 * X.NL[0][i], om * X.NL[1][i]};
+      if (!invert(a, na)) return fail(GP_ERR_ARG, "singular coarsest auxiliary operator");
+      for (int64_t i = 0; i < na * na; ++i) { ainv[2 * (f * na * na + i)] = a[i].real(); ainv[2 * (f * na * na + i) + 1] = a[i].imag(); }
+    }
+    GP_TRY(A.alloc(&w->ainv, std::max<int64_t>(2 * na * na * w->nf, 1)));
+    GP_HIP(hipMemcpyAsync(w->ainv, ainv.data(), sizeof(double) * ainv.size(), hipMemcpyHostToDevice, s));
+    GP_HIP(hipStreamSynchronize(s));
   }
   GP_TRY(A.alloc(&w->kinv, 2 * ne * ne * w->nf));
   GP_TRY(A.alloc(&w->ninv, std::max<int64_t>(2 * nn * nn * w->nf, 1)));
@@ -480,6 +535,53 @@ __global__ void k_coarse(int64_t ne, int64_t nn, int k, const int32_t *__restric
   for (int64_t a = threadIdx.x; a < nn; a += blockDim.x) phi[a * k + j] = {(T)ph[2 * a], (T)ph[2 * a + 1]};
 }
 
+// y = r - M2(w) x  (mode 0),  y -= M2(w) x  (mode 1),  y = M2(w) x (mode 2); M2 = c1 v0 + i c2 v1
+template <class T>
+__global__ void k_spmv2(int64_t nr, int k, MatV<T> M, const T *__restrict__ c1v, const T *__restrict__ c2v,
+                        const C2<T> *__restrict__ r, const C2<T> *__restrict__ x, C2<T> *__restrict__ y, int mode) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t i = t / k;
+  int j = (int)(t - i * k);
+  if (i >= nr) return;
+  const T c1 = c1v[j], c2 = c2v[j];
+  C2<T> acc = {0, 0};
+  for (int64_t p = M.rp[i]; p < M.rp[i + 1]; ++p) {
+    C2<T> a = {c1 * M.v[0][p], c2 * M.v[1][p]};
+    C2<T> v = x[(int64_t)M.ci[p] * k + j];
+    acc.x += a.x * v.x - a.y * v.y;
+    acc.y += a.x * v.y + a.y * v.x;
+  }
+  if (mode == 0) { C2<T> b = r[i * k + j]; y[i * k + j] = {b.x - acc.x, b.y - acc.y}; }
+  else if (mode == 1) { C2<T> b = y[i * k + j]; y[i * k + j] = {b.x - acc.x, b.y - acc.y}; }
+  else y[i * k + j] = acc;
+}
+
+// x = A_f^-1 r for each column (dense complex FP64 inverse per frequency), one block per column
+template <class T>
+__global__ void k_dense_apply(int64_t n, int k, const int32_t *__restrict__ fidx, const double *__restrict__ inv,
+                              const C2<T> *__restrict__ r, C2<T> *__restrict__ x) {
+  extern __shared__ double sr[];
+  const int j = blockIdx.x;
+  const double *M = inv + 2 * (int64_t)fidx[j] * n * n;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) { sr[2 * i] = (double)r[i * k + j].x; sr[2 * i + 1] = (double)r[i * k + j].y; }
+  __syncthreads();
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    double ax = 0, ay = 0;
+    for (int64_t b = 0; b < n; ++b) {
+      const double mx = M[2 * (i * n + b)], my = M[2 * (i * n + b) + 1];
+      ax += mx * sr[2 * b] - my * sr[2 * b + 1];
+      ay += mx * sr[2 * b + 1] + my * sr[2 * b];
+    }
+    x[i * k + j] = {(T)ax, (T)ay};
+  }
+}
+
+template <class T>
+__global__ void k_copy(int64_t n, const C2<T> *__restrict__ a, C2<T> *__restrict__ b) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
 template <class T>
 MatV<T> view(const Csr &c, const CsrF &f);
 template <>
@@ -574,6 +676,73 @@ template <class T>
 gp_status vcycle(const Hierarchy &H, VWork<T> &w, hipStream_t s, int nu_pre, int nu_post) {
   return vcycle_level<T>(H, w, s, 0, nu_pre, nu_post);
 }
+
+// Auxiliary AMG V-cycle on level l: input w.ar[l], output w.ax[l].
+template <class T>
+static gp_status aux_cycle(const Hierarchy &H, VWork<T> &w, hipStream_t s, int l) {
+  const AuxAMG &X = H.aux;
+  const AuxLevel &L = X.L[l];
+  const int k = w.k;
+  const int64_t n = L.n;
+  C2<T> *r = w.ar[l], *x = w.ax[l], *t = w.at[l];
+  if (l + 1 == (int)X.L.size()) {
+    k_dense_apply<T><<<k, 256, sizeof(double) * 2 * n, s>>>(n, k, w.fidx, w.ainv, r, x);
+    return GP_OK;
+  }
+  const MatV<T> N = view<T>(L.N, L.Nf), P = view<T>(L.P, L.Pf), PT = view<T>(L.PT, L.PTf);
+  const T *d0 = arr<T>(L.nd[0], L.ndf[0]), *d1 = arr<T>(L.nd[1], L.ndf[1]);
+  const T *l0 = arr<T>(L.nl[0], L.nlf[0]), *l1 = arr<T>(L.nl[1], L.nlf[1]);
+  const unsigned gr = grid_rows(n, k);
+  const int nu = w.nu_aux;
+  k_node<T><<<gr, 256, 0, s>>>(n, k, N, w.c1, w.c2, d0, d1, l0, l1, r, nullptr, x, (T)1);
+  for (int it = 1; it < nu; ++it) {
+    k_node<T><<<gr, 256, 0, s>>>(n, k, N, w.c1, w.c2, d0, d1, l0, l1, r, x, t, (T)1);
+    std::swap(x, t);
+  }
+  k_spmv2<T><<<gr, 256, 0, s>>>(n, k, N, w.c1, w.c2, r, x, t, 0);
+  const int64_t nc = X.L[l + 1].n;
+  k_spmv_real<T><<<grid_rows(nc, k), 256, 0, s>>>(nc, k, PT, t, w.ar[l + 1], 0);
+  GP_TRY(aux_cycle<T>(H, w, s, l + 1));
+  k_spmv_real<T><<<gr, 256, 0, s>>>(n, k, P, w.ax[l + 1], x, 1);
+  for (int it = 0; it < nu; ++it) {
+    k_node<T><<<gr, 256, 0, s>>>(n, k, N, w.c1, w.c2, d0, d1, l0, l1, r, x, t, (T)1);
+    std::swap(x, t);
+  }
+  if (x != w.ax[l]) GP_HIP(hipMemcpyAsync(w.ax[l], x, sizeof(C2<T>) * n * k, hipMemcpyDeviceToDevice, s));
+  return GP_OK;
+}
+
+template <class T>
+gp_status precond(const Hierarchy &H, VWork<T> &w, hipStream_t s, int nu_pre, int nu_post) {
+  const AuxAMG &X = H.aux;
+  const MgLevel &L0 = H.L[0];
+  const int k = w.k;
+  const int64_t n = L0.ne, na = X.naux;
+  const MatV<T> GxT = view<T>(X.GxT, X.GxTf), Bx = view<T>(X.Bx, X.Bxf);
+  const MatV<T> K = view<T>(L0.K, L0.Kf), B = view<T>(L0.B, L0.Bf);
+  const unsigned gn = grid_rows(n, k), ga = grid_rows(na, k);
+  // auxiliary correction: a = AMG(Gx^T r), r <- r - Bx a
+  k_copy<T><<<blocks_for(n * k), 256, 0, s>>>(n * k, w.r[0], w.r0);
+  k_spmv_real<T><<<ga, 256, 0, s>>>(na, k, GxT, w.r[0], w.ar[0], 0);
+  GP_TRY(aux_cycle<T>(H, w, s, 0));
+  k_copy<T><<<blocks_for(na * k), 256, 0, s>>>(na * k, w.ax[0], w.a);
+  k_spmv2<T><<<gn, 256, 0, s>>>(n, k, Bx, w.c1, w.c2, nullptr, w.a, w.r[0], 1);
+  // edge V-cycle on the remaining residual
+  GP_TRY(vcycle<T>(H, w, s, nu_pre, nu_post));
+  // auxiliary correction on r0 - K u - B phi - Bx a
+  const T *kd0 = arr<T>(L0.kd[0], L0.kdf[0]);
+  (void)kd0;
+  k_edge<T><<<gn, 256, 0, s>>>(n, k, K, B, w.c1, w.c2, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, w.r0,
+                               w.u[0], w.phi[0], w.res[0], EdgeMode::RES, 1, (T)1);
+  k_spmv2<T><<<gn, 256, 0, s>>>(n, k, Bx, w.c1, w.c2, nullptr, w.a, w.res[0], 1);
+  k_spmv_real<T><<<ga, 256, 0, s>>>(na, k, GxT, w.res[0], w.ar[0], 0);
+  GP_TRY(aux_cycle<T>(H, w, s, 0));
+  k_add<T><<<blocks_for(na * k), 256, 0, s>>>(na * k, w.a, w.ax[0]);
+  return GP_OK;
+}
+
+template gp_status precond<float>(const Hierarchy &, VWork<float> &, hipStream_t, int, int);
+template gp_status precond<double>(const Hierarchy &, VWork<double> &, hipStream_t, int, int);
 
 template gp_status vwork_create<float>(gp_layout *, const Hierarchy &, int, const double *, const int32_t *,
                                        const std::vector<double> &, VWork<float> *);

@@ -40,14 +40,36 @@ struct MgLevel {
   float *kdf[3] = {}, *klf[3] = {}, *ndf[2] = {}, *nlf[2] = {};
 };
 
+struct AuxLevel {
+  int64_t n = 0;
+  Csr N, P, PT;                  // N = [N_eps, N_sig]; P to the next coarser level
+  CsrF Nf, Pf, PTf;
+  double *nd[2] = {}, *nl[2] = {};
+  float *ndf[2] = {}, *nlf[2] = {};
+};
+
+// Auxiliary nodal space [V0d | V0c] and its class-aware smoothed-aggregation hierarchy.
+struct AuxAMG {
+  int64_t naux = 0, ndiel = 0, nL = 0;
+  Csr Gx, GxT, Bx;               // Gx: kept edges x naux; Bx = W [eps_r eps0, sigma] Gx
+  CsrF Gxf, GxTf, Bxf;
+  std::vector<AuxLevel> L;
+  std::vector<double> NL[2];     // coarsest level, dense (host)
+};
+
 struct Hierarchy {
   std::vector<MgLevel> L;
+  AuxAMG aux;
+  // S G and S Gx of the rounded FP64 stiffness, formed exactly (double-double):
+  // A (u + G phi + Gx a) = S u + SG phi + SGx a + D (u + G phi + Gx a)
+  Csr SG, SGx;
   bool has_f32 = false;
   // level-0 data for the FP64 outer operator and the V-cycle input scaling
   double *W = nullptr;        // edge dual volume
   float *Wf = nullptr;
   double *epsr = nullptr;     // stack_epsn of each kept edge's layer
   double *sig = nullptr;      // stack_sign if the edge is a conductor edge, else 0
+  double *sdiag = nullptr;    // S_ii: the system diagonal is the FP64 sum S_ii + D_ii(w)
   // coarsest level, dense on the host (row-major), per block
   int64_t neL = 0, nnL = 0;
   std::vector<double> KL[3], BL[2], NL[2], GTL;
@@ -55,6 +77,9 @@ struct Hierarchy {
 };
 
 gp_status build_hierarchy(gp_layout *lay, Hierarchy *H, bool want_f32, int max_levels);
+gp_status build_aux(gp_layout *lay, Hierarchy *H, bool want_f32);
+gp_status aux_smoother_data(Arena &A, const Csr &M, double **d, double **l, hipStream_t s);
+gp_status d2f_copy(const double *src, float *dst, int64_t n, hipStream_t s);
 void free_hierarchy(gp_layout *lay, Hierarchy *H);
 
 // Per-batch data: column frequencies and the coarsest-level inverses.
@@ -67,6 +92,11 @@ struct VWork {
   double *kinv = nullptr, *ninv = nullptr;  // [nf][neL*neL] / [nf][nnL*nnL] complex (re,im)
   double *gtl = nullptr, *bl = nullptr;     // coarsest GT (nnL x neL) real, B per freq [nf][neL*nnL] complex
   std::vector<C2<T> *> r, u, phi, res, tmp, rn, dn, dn2;
+  // auxiliary AMG: per-level rhs / iterate / scratch, level-0 output, saved input
+  std::vector<C2<T> *> ar, ax, at;
+  C2<T> *a = nullptr, *r0 = nullptr;
+  double *ainv = nullptr;                   // [nf][nL*nL] complex, coarsest aux level
+  int nu_aux = 2;
   Arena arena;
 };
 
@@ -79,5 +109,10 @@ void vwork_free(VWork<T> *w);
 // One V-cycle: input w->r[0] (K-form residual, n x k), output w->u[0], w->phi[0].
 template <class T>
 gp_status vcycle(const Hierarchy &H, VWork<T> &w, hipStream_t s, int nu_pre, int nu_post);
+// Full preconditioner: auxiliary-space AMG correction, edge V-cycle, auxiliary
+// correction again.  Input w->r[0]; output (w->u[0], w->phi[0], w->a) with
+// x = u + G phi + Gx a.  w->r[0] is left modified.
+template <class T>
+gp_status precond(const Hierarchy &H, VWork<T> &w, hipStream_t s, int nu_pre, int nu_post);
 
 }  // namespace gp

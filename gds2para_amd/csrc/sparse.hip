@@ -69,7 +69,16 @@ struct ValSrc {
   const double *a[3];
   const double *b[3];
   int na, nb, nout;
+  int exact;  // 1: products and sums carried in double-double, rounded once at the end
 };
+
+// error-free transformations (Dekker/Knuth): a*b = p + e exactly, a+b = s + t exactly
+__device__ __forceinline__ void two_prod(double a, double b, double &p, double &e) { p = a * b; e = fma(a, b, -p); }
+__device__ __forceinline__ void two_sum(double a, double b, double &s, double &t) {
+  s = a + b;
+  double bb = s - a;
+  t = (a - (s - bb)) + (b - bb);
+}
 
 __global__ void k_reduce(int64_t nseg, int64_t n, const int64_t *__restrict__ start, const uint64_t *__restrict__ key,
                          const uint32_t *__restrict__ perm, const uint32_t *__restrict__ src_a,
@@ -79,16 +88,26 @@ __global__ void k_reduce(int64_t nseg, int64_t n, const int64_t *__restrict__ st
   int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nseg) return;
   int64_t b = start[s], e = (s + 1 < nseg) ? start[s + 1] : n;
-  double acc[3] = {0., 0., 0.};
+  double acc[3] = {0., 0., 0.}, lo[3] = {0., 0., 0.};
   for (int64_t q = b; q < e; ++q) {
     uint32_t t = perm[q];
     uint32_t pa = src_a ? src_a[t] : t, pb = src_b ? src_b[t] : 0;
     for (int c = 0; c < vs.nout; ++c) {
       double va = vs.a[vs.na > 1 ? c : 0][pa];
       double vb = vs.nb > 0 ? vs.b[vs.nb > 1 ? c : 0][pb] : 1.0;
-      acc[c] += va * vb;
+      if (vs.exact) {
+        double p, pe, s2, t2;
+        two_prod(va, vb, p, pe);
+        two_sum(acc[c], p, s2, t2);
+        acc[c] = s2;
+        lo[c] += t2 + pe;
+      } else {
+        acc[c] += va * vb;
+      }
     }
   }
+  if (vs.exact)
+    for (int c = 0; c < vs.nout; ++c) acc[c] += lo[c];
   uint64_t k = key[b];
   int64_t row = (int64_t)(k / (uint64_t)nc);
   ci[s] = (int32_t)(k % (uint64_t)nc);
@@ -151,6 +170,102 @@ gp_status sort_reduce(Arena &A, int64_t nr, int64_t nc, int64_t nprod, uint64_t 
 
 }  // namespace
 
+namespace {
+__global__ void k_coo_keys(int64_t n, const int64_t *__restrict__ r, const int64_t *__restrict__ c, int64_t nc,
+                           uint64_t *__restrict__ key) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) key[i] = (uint64_t)r[i] * (uint64_t)nc + (uint64_t)c[i];
+}
+__global__ void k_csr_triplets(int64_t nr, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                               const double *__restrict__ v, double alpha, int64_t base, int64_t *__restrict__ r,
+                               int64_t *__restrict__ c, double *__restrict__ ov) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nr) return;
+  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) { r[base + p] = i; c[base + p] = ci[p]; ov[base + p] = alpha * v[p]; }
+}
+}  // namespace
+
+gp_status csr_from_coo(Arena &A, int64_t nr, int64_t nc, int64_t n, const int64_t *rows, const int64_t *cols,
+                       const double *const *vals, int nv, Csr *out, hipStream_t s) {
+  Arena tmp;
+  uint64_t *key = nullptr;
+  gp_status st = GP_OK;
+  do {
+    if (n == 0) { st = csr_alloc(A, nr, nc, 0, nv, out);
+      if (st == GP_OK && hipMemsetAsync(out->rp, 0, sizeof(int64_t) * (nr + 1), s) != hipSuccess) st = fail(GP_ERR_HIP, "memset");
+      break; }
+    if ((st = tmp.alloc(&key, n)) != GP_OK) break;
+    k_coo_keys<<<blocks_for(n), 256, 0, s>>>(n, rows, cols, nc, key);
+    ValSrc vs{};
+    vs.na = nv; vs.nb = 0; vs.nout = nv;
+    for (int k = 0; k < nv; ++k) vs.a[k] = vals[k];
+    st = sort_reduce(A, nr, nc, n, key, nullptr, nullptr, vs, out, s);
+  } while (0);
+  (void)hipStreamSynchronize(s);
+  tmp.free_all();
+  return st;
+}
+
+gp_status csr_add(Arena &A, const Csr &a, double alpha, const Csr &b, Csr *c, hipStream_t s) {
+  if (a.nr != b.nr || a.nc != b.nc) return fail(GP_ERR_ARG, "csr_add shape mismatch");
+  Arena tmp;
+  int64_t *r = nullptr, *cc = nullptr;
+  double *v = nullptr;
+  const int64_t n = a.nnz + b.nnz;
+  gp_status st = GP_OK;
+  do {
+    if ((st = tmp.alloc(&r, std::max<int64_t>(n, 1))) != GP_OK) break;
+    if ((st = tmp.alloc(&cc, std::max<int64_t>(n, 1))) != GP_OK) break;
+    if ((st = tmp.alloc(&v, std::max<int64_t>(n, 1))) != GP_OK) break;
+    k_csr_triplets<<<blocks_for(a.nr), 256, 0, s>>>(a.nr, a.rp, a.ci, a.v[0], 1.0, 0, r, cc, v);
+    k_csr_triplets<<<blocks_for(b.nr), 256, 0, s>>>(b.nr, b.rp, b.ci, b.v[0], alpha, a.nnz, r, cc, v);
+    const double *vals[1] = {v};
+    st = csr_from_coo(A, a.nr, a.nc, n, r, cc, vals, 1, c, s);
+  } while (0);
+  (void)hipStreamSynchronize(s);
+  tmp.free_all();
+  return st;
+}
+
+namespace {
+__global__ void k_nz_flags(int64_t nnz, const double *__restrict__ v, int64_t *__restrict__ f) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nnz) f[i] = v[i] != 0.0;
+}
+__global__ void k_nz_compact(int64_t nr, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                             const double *__restrict__ v, const int64_t *__restrict__ pos, int64_t *__restrict__ r,
+                             int64_t *__restrict__ c, double *__restrict__ ov) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nr) return;
+  for (int64_t p = rp[i]; p < rp[i + 1]; ++p)
+    if (v[p] != 0.0) { r[pos[p]] = i; c[pos[p]] = ci[p]; ov[pos[p]] = v[p]; }
+}
+}  // namespace
+
+gp_status csr_drop_zeros(Arena &A, const Csr &a, Csr *out, hipStream_t s) {
+  Arena tmp;
+  gp_status st = GP_OK;
+  int64_t *f = nullptr, *pos = nullptr, *r = nullptr, *c = nullptr;
+  double *v = nullptr;
+  do {
+    if ((st = tmp.alloc(&f, std::max<int64_t>(a.nnz, 1))) != GP_OK) break;
+    if ((st = tmp.alloc(&pos, a.nnz + 1)) != GP_OK) break;
+    if (a.nnz) k_nz_flags<<<blocks_for(a.nnz), 256, 0, s>>>(a.nnz, a.v[0], f);
+    if ((st = exclusive_scan_i64(f, pos, a.nnz, s, tmp)) != GP_OK) break;
+    int64_t nk = 0;
+    if (hipMemcpyAsync(&nk, pos + a.nnz, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) { st = fail(GP_ERR_HIP, "count"); break; }
+    if ((st = tmp.alloc(&r, std::max<int64_t>(nk, 1))) != GP_OK) break;
+    if ((st = tmp.alloc(&c, std::max<int64_t>(nk, 1))) != GP_OK) break;
+    if ((st = tmp.alloc(&v, std::max<int64_t>(nk, 1))) != GP_OK) break;
+    k_nz_compact<<<blocks_for(a.nr), 256, 0, s>>>(a.nr, a.rp, a.ci, a.v[0], pos, r, c, v);
+    const double *vals[1] = {v};
+    st = csr_from_coo(A, a.nr, a.nc, nk, r, c, vals, 1, out, s);
+  } while (0);
+  (void)hipStreamSynchronize(s);
+  tmp.free_all();
+  return st;
+}
+
 gp_status csr_alloc(Arena &A, int64_t nr, int64_t nc, int64_t nnz, int nv, Csr *out) {
   *out = Csr();
   out->nr = nr; out->nc = nc; out->nnz = nnz; out->nv = nv;
@@ -179,7 +294,9 @@ gp_status csr_upload(Arena &A, const HostCsr &h, Csr *out, hipStream_t s) {
   return GP_OK;
 }
 
-gp_status spgemm(Arena &A, const Csr &a, const Csr &b, Csr *c, hipStream_t s) {
+gp_status spgemm(Arena &A, const Csr &a, const Csr &b, Csr *c, hipStream_t s) { return spgemm_ex(A, a, b, c, s, 0); }
+
+gp_status spgemm_ex(Arena &A, const Csr &a, const Csr &b, Csr *c, hipStream_t s, int exact) {
   if (a.nc != b.nr) return fail(GP_ERR_ARG, "spgemm shape mismatch");
   if (a.nv > 1 && b.nv > 1 && a.nv != b.nv) return fail(GP_ERR_ARG, "spgemm value arrays mismatch");
   Arena tmp;
@@ -208,6 +325,7 @@ gp_status spgemm(Arena &A, const Csr &a, const Csr &b, Csr *c, hipStream_t s) {
     k_expand<<<blocks_for(a.nnz), 256, 0, s>>>(a.nnz, arow, a.ci, b.rp, b.ci, off, b.nc, key, sa, sb);
     ValSrc vs{};
     vs.na = a.nv; vs.nb = b.nv; vs.nout = std::max(a.nv, b.nv);
+    vs.exact = exact;
     for (int k = 0; k < 3; ++k) { vs.a[k] = a.v[k]; vs.b[k] = b.v[k]; }
     if (nprod == 0) { st = csr_alloc(A, a.nr, b.nc, 0, vs.nout, c);
       if (st == GP_OK && hipMemsetAsync(c->rp, 0, sizeof(int64_t) * (a.nr + 1), s) != hipSuccess) st = fail(GP_ERR_HIP, "memset");

@@ -33,9 +33,19 @@ void csr_release(Arena &A, Csr &c);
 // C = A * B; the value arrays multiply pairwise when both have nv > 1
 // (nv must match), otherwise the single array scales every array of the other.
 gp_status spgemm(Arena &A, const Csr &a, const Csr &b, Csr *c, hipStream_t s);
+// exact = 1: each entry's products and sums are carried in double-double and
+// rounded once (the exact product of the FP64 inputs, to 1 ulp)
+gp_status spgemm_ex(Arena &A, const Csr &a, const Csr &b, Csr *c, hipStream_t s, int exact);
 gp_status transpose(Arena &A, const Csr &a, Csr *t, hipStream_t s);
+// copy of a single-valued CSR without its explicit zeros
+gp_status csr_drop_zeros(Arena &A, const Csr &a, Csr *out, hipStream_t s);
 // FP32 copy of every value array
 gp_status csr_to_float(Arena &A, const Csr &a, CsrF *f, hipStream_t s);
+// CSR from COO triplets (duplicates summed), nv value arrays
+gp_status csr_from_coo(Arena &A, int64_t nr, int64_t nc, int64_t n, const int64_t *rows, const int64_t *cols,
+                       const double *const *vals, int nv, Csr *out, hipStream_t s);
+// C = a + alpha * b (same shape, single value array each)
+gp_status csr_add(Arena &A, const Csr &a, double alpha, const Csr &b, Csr *c, hipStream_t s);
 // host copy of value array `which` of a
 gp_status csr_host_copy(const Csr &a, HostCsr *h, int which);
 
