@@ -70,8 +70,20 @@ class GpV0Sizes(C.Structure):
 class GpSolveOpts(C.Structure):
     _fields_ = [
         ("rtol", f64), ("max_iters", i32), ("restart", i32), ("precision", i32),
-        ("batch", i32), ("verbose", i32),
+        ("batch", i32), ("verbose", i32), ("smooth_edge", i32), ("smooth_aux", i32),
     ]
+
+
+class GpHierarchyInfo(C.Structure):
+    _fields_ = [
+        ("edge_levels", i32), ("ne", i64 * 16), ("nn", i64 * 16), ("nnz_k", i64 * 16),
+        ("aux_levels", i32), ("na", i64 * 16), ("nnz_n", i64 * 16), ("nlong", i64 * 16),
+        ("naux", i64), ("setup_ms", f64),
+    ]
+
+
+class GpColumnReport(C.Structure):
+    _fields_ = [("iters", i32), ("converged", i32), ("rel_residual", f64), ("start_ms", f64), ("done_ms", f64)]
 
 
 class GpSolveStats(C.Structure):
@@ -106,10 +118,16 @@ SIGNATURES = [
     ("gp_ac_copy", i32, [C.c_void_p, P(i64), P(i64), P(f64)]),
     ("gp_solve_opts_default", None, [P(GpSolveOpts)]),
     ("gp_prepare_solver", i32, [C.c_void_p, i32]),
+    ("gp_hierarchy_info_get", i32, [C.c_void_p, P(GpHierarchyInfo)]),
     ("gp_solve_fullwave", i32, [C.c_void_p, P(f64), i32, P(i32), i32, P(GpSolveOpts), P(f64), P(f64), P(GpSolveStats)]),
+    ("gp_solve_fullwave_ex", i32, [C.c_void_p, P(f64), i32, P(i32), i32, P(GpSolveOpts), P(f64), P(f64),
+                                   P(GpSolveStats), P(GpColumnReport)]),
     ("gp_extract_z", i32, [P(GpMesh), P(f64), i32, P(f64)]),
     ("gp_solve_real", i32, [P(i64), P(i64), P(f64), i64, i64, P(f64), P(f64), f64, P(i32), P(f64)]),
     ("gp_bench_spmv", i32, [C.c_void_p, i32, i32, i32, P(f64)]),
+    ("gp_layout_sync", i32, [C.c_void_p]),
+    ("gp_kernel_timer", i32, [C.c_void_p, i32]),
+    ("gp_kernel_timer_read", i32, [C.c_void_p, P(i64), P(f64), P(f64)]),
 ]
 
 _lib = None

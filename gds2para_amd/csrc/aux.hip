@@ -123,6 +123,62 @@ __global__ void k_agg_assign(int64_t n, const int64_t *__restrict__ sidx, const 
   }
 }
 
+// Connected components inside each box: label propagation over the couplings
+// of the class block (N_eps between dielectric nodes, N_sig within a
+// conductor) restricted to pairs in the same box; super nodes stay alone.
+// A box may hold disconnected pieces of a conductor (parallel wires one cell
+// apart), which one constant would tie together and so spoil the coarse space.
+__global__ void k_cc_init(int64_t n, int64_t *__restrict__ lab) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) lab[i] = i;
+}
+
+__global__ void k_cc_prop(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                          const double *__restrict__ ne, const double *__restrict__ ns, const int32_t *__restrict__ cls,
+                          const int64_t *__restrict__ box, const int64_t *__restrict__ lin, int64_t *__restrict__ lout,
+                          int *__restrict__ changed) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t c = cls[i];
+  int64_t best = lin[i];
+  if (c != kSuperClass)
+    for (int64_t p = rp[i]; p < rp[i + 1]; ++p) {
+      const int32_t j = ci[p];
+      if (j == (int32_t)i || box[j] != box[i] || cls[j] != c) continue;
+      const double a = c == 0 ? ne[p] : ns[p];
+      if (a != 0. && lin[j] < best) best = lin[j];
+    }
+  lout[i] = best;
+  if (best != lin[i]) atomicAdd(changed, 1);
+}
+
+__global__ void k_cc_keys(int64_t n, const int64_t *__restrict__ lab, uint64_t *__restrict__ key, int64_t *__restrict__ idx) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  key[i] = (uint64_t)lab[i];
+  idx[i] = i;
+}
+
+// aggregate of each node from the sorted component labels; the coarse node
+// takes the class of its component and the box coordinates (super nodes keep
+// their own coordinates)
+__global__ void k_cc_assign(int64_t n, const int64_t *__restrict__ sidx, const int64_t *__restrict__ hscan,
+                            const int64_t *__restrict__ head, const uint64_t *__restrict__ skey,
+                            const int32_t *__restrict__ cls, const int32_t *__restrict__ crd,
+                            int64_t *__restrict__ agg, int32_t *__restrict__ ccls, int32_t *__restrict__ ccrd) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int64_t g = head[p] ? hscan[p] : hscan[p] - 1;
+  agg[sidx[p]] = g;
+  if (head[p]) {
+    const int64_t lab = (int64_t)skey[p];
+    const int32_t c = cls[lab];
+    ccls[g] = c;
+    const int sh = c == kSuperClass ? 0 : 1;
+    for (int d = 0; d < 3; ++d) ccrd[3 * g + d] = crd[3 * lab + d] >> sh;
+  }
+}
+
 __global__ void k_ptent(int64_t n, const int64_t *__restrict__ agg, int64_t *__restrict__ rp, int32_t *__restrict__ ci,
                         double *__restrict__ v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -271,13 +327,39 @@ gp_status build_aux(gp_layout *lay, Hierarchy *H, bool want_f32) {
       if (hipcub::DeviceRadixSort::SortPairs(buf, bytes, key, key2, idx, idx2, (int)nf, 0, 64, s) != hipSuccess) { st = fail(GP_ERR_HIP, "sort"); break; }
       k_heads64<<<blocks_for(nf), 256, 0, s>>>(nf, key2, head);
       if ((st = exclusive_scan_i64(head, hscan, nf, s, tmp)) != GP_OK) break;
+      int64_t nbox = 0;
+      if (hipMemcpyAsync(&nbox, hscan + nf, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) { st = fail(GP_ERR_HIP, "count"); break; }
+      int32_t *bcls = nullptr, *bcrd = nullptr;
+      if ((st = tmp.alloc(&bcls, nbox)) != GP_OK) break;
+      if ((st = tmp.alloc(&bcrd, 3 * nbox)) != GP_OK) break;
+      k_agg_assign<<<blocks_for(nf), 256, 0, s>>>(nf, idx2, hscan, head, key2, agg, bcls, bcrd);
+      // split every box into its connected components
+      int64_t *la = nullptr, *lb = nullptr;
+      int *chg = nullptr;
+      if ((st = tmp.alloc(&la, nf)) != GP_OK) break;
+      if ((st = tmp.alloc(&lb, nf)) != GP_OK) break;
+      if ((st = tmp.alloc(&chg, 1)) != GP_OK) break;
+      k_cc_init<<<blocks_for(nf), 256, 0, s>>>(nf, la);
+      for (int it = 0; it < 64; ++it) {
+        int hc = 0;
+        if (hipMemsetAsync(chg, 0, sizeof(int), s) != hipSuccess) { st = fail(GP_ERR_HIP, "memset"); break; }
+        k_cc_prop<<<blocks_for(nf), 256, 0, s>>>(nf, F.N.rp, F.N.ci, F.N.v[0], F.N.v[1], cls, agg, la, lb, chg);
+        std::swap(la, lb);
+        if (hipMemcpyAsync(&hc, chg, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) { st = fail(GP_ERR_HIP, "labels"); break; }
+        if (hc == 0) break;
+      }
+      if (st != GP_OK) break;
+      k_cc_keys<<<blocks_for(nf), 256, 0, s>>>(nf, la, key, idx);
+      if (hipcub::DeviceRadixSort::SortPairs(buf, bytes, key, key2, idx, idx2, (int)nf, 0, 64, s) != hipSuccess) { st = fail(GP_ERR_HIP, "sort"); break; }
+      k_heads64<<<blocks_for(nf), 256, 0, s>>>(nf, key2, head);
+      if ((st = exclusive_scan_i64(head, hscan, nf, s, tmp)) != GP_OK) break;
       int64_t ncoarse = 0;
       if (hipMemcpyAsync(&ncoarse, hscan + nf, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) { st = fail(GP_ERR_HIP, "count"); break; }
       if (ncoarse > (int64_t)(0.85 * (double)nf)) break;
       int32_t *ccls = nullptr, *ccrd = nullptr;
       if ((st = tmp.alloc(&ccls, ncoarse)) != GP_OK) break;
       if ((st = tmp.alloc(&ccrd, 3 * ncoarse)) != GP_OK) break;
-      k_agg_assign<<<blocks_for(nf), 256, 0, s>>>(nf, idx2, hscan, head, key2, agg, ccls, ccrd);
+      k_cc_assign<<<blocks_for(nf), 256, 0, s>>>(nf, idx2, hscan, head, key2, cls, crd, agg, ccls, ccrd);
       Csr Pt;
       if ((st = csr_alloc(tmp, nf, ncoarse, nf, 1, &Pt)) != GP_OK) break;
       k_ptent<<<blocks_for(nf + 1), 256, 0, s>>>(nf, agg, Pt.rp, Pt.ci, Pt.v[0]);
@@ -318,6 +400,11 @@ gp_status build_aux(gp_layout *lay, Hierarchy *H, bool want_f32) {
     }
     if (st != GP_OK) break;
     for (AuxLevel &L : X.L) GP_TRY(aux_smoother_data(A, L.N, L.nd, L.nl, s));
+    GP_TRY(csr_mark_long(A, X.GxT, s));
+    for (AuxLevel &L : X.L) {
+      GP_TRY(csr_mark_long(A, L.N, s));
+      if (L.P.nr) { GP_TRY(csr_mark_long(A, L.P, s)); GP_TRY(csr_mark_long(A, L.PT, s)); }
+    }
     if (want_f32) {
       GP_TRY(csr_to_float(A, X.Gx, &X.Gxf, s));
       GP_TRY(csr_to_float(A, X.GxT, &X.GxTf, s));

@@ -93,7 +93,40 @@ void destroy_solver(Solver *);
 
 }  // namespace gp
 
+namespace gp {
+// Device time of one kernel class, measured with HIP events recorded on the
+// launching stream around each launch, together with the algorithmic bytes of
+// each launch (for the achieved-bandwidth figure).
+struct KTimer {
+  bool on = false;
+  std::vector<hipEvent_t> ev;     // begin/end pairs
+  std::vector<double> bytes;      // per pair
+  size_t used = 0;
+  void begin(hipStream_t s) {
+    if (!on) return;
+    if (2 * used + 2 > ev.size()) {
+      for (int q = 0; q < 256; ++q) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) { on = false; return; }
+        ev.push_back(e);
+      }
+    }
+    (void)hipEventRecord(ev[2 * used], s);
+  }
+  void end(hipStream_t s, double b) {
+    if (!on) return;
+    (void)hipEventRecord(ev[2 * used + 1], s);
+    if (bytes.size() <= used) bytes.resize(used + 1);
+    bytes[used] = b;
+    ++used;
+  }
+  void reset() { used = 0; }
+  ~KTimer() { for (hipEvent_t e : ev) (void)hipEventDestroy(e); }
+};
+}  // namespace gp
+
 struct gp_layout {
+  gp::KTimer ktimer;                    // level-0 edge smoother launches
   gp::HostMesh hm;
   gp::Grid g{};
   int64_t n = 0;                        // unknowns after PEC removal

@@ -240,6 +240,21 @@ gp_status build_hierarchy(gp_layout *lay, Hierarchy *H, bool want_f32, int max_l
   GP_TRY(dense_host(C.GT, 0, C.nn, C.ne, &H->GTL));
   GP_HIP(hipStreamSynchronize(s));
   GP_TRY(build_aux(lay, H, want_f32));
+  {  // dense coarsest data on the device, for the per-frequency inverses
+    auto up = [&](const std::vector<double> &h, double **d) -> gp_status {
+      GP_TRY(A.alloc(d, std::max<int64_t>((int64_t)h.size(), 1)));
+      if (!h.empty()) GP_HIP(hipMemcpyAsync(*d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, s));
+      return GP_OK;
+    };
+    for (int k = 0; k < 3; ++k) GP_TRY(up(H->KL[k], &H->KLd[k]));
+    for (int k = 0; k < 2; ++k) {
+      GP_TRY(up(H->BL[k], &H->BLd[k]));
+      GP_TRY(up(H->NL[k], &H->NLd[k]));
+      GP_TRY(up(H->aux.NL[k], &H->aux.NLd[k]));
+    }
+    GP_TRY(up(H->GTL, &H->GTLd));
+    GP_HIP(hipStreamSynchronize(s));
+  }
   {
     Csr Sv;
     Sv.nr = lay->S.n; Sv.nc = lay->S.n; Sv.nnz = lay->S.nnz; Sv.rp = lay->S.rp; Sv.ci = lay->S.ci; Sv.nv = 1; Sv.v[0] = lay->S.v;
@@ -256,50 +271,99 @@ void free_hierarchy(gp_layout *lay, Hierarchy *H) {
   H->L.clear();
 }
 
-// ---------------------------------------------------------------- dense inverse (host)
+// ---------------------------------------------------------------- coarsest levels per frequency
 
 namespace {
 
-// Gauss-Jordan inverse with partial pivoting, complex FP64, row-major.
-bool invert(std::vector<std::complex<double>> &a, int64_t n) {
-  std::vector<std::complex<double>> inv(n * n, 0.0);
-  for (int64_t i = 0; i < n; ++i) inv[i * n + i] = 1.0;
-  for (int64_t c = 0; c < n; ++c) {
-    int64_t piv = c;
-    double best = std::abs(a[c * n + c]);
-    for (int64_t r = c + 1; r < n; ++r)
-      if (std::abs(a[r * n + c]) > best) { best = std::abs(a[r * n + c]); piv = r; }
-    if (best == 0.0) return false;
-    if (piv != c)
-      for (int64_t j = 0; j < n; ++j) { std::swap(a[c * n + j], a[piv * n + j]); std::swap(inv[c * n + j], inv[piv * n + j]); }
-    std::complex<double> d = 1.0 / a[c * n + c];
-    for (int64_t j = 0; j < n; ++j) { a[c * n + j] *= d; inv[c * n + j] *= d; }
-    for (int64_t r = 0; r < n; ++r) {
-      if (r == c) continue;
-      std::complex<double> f = a[r * n + c];
-      if (f == 0.0) continue;
-      for (int64_t j = 0; j < n; ++j) { a[r * n + j] -= f * a[c * n + j]; inv[r * n + j] -= f * inv[c * n + j]; }
-    }
-  }
-  a.swap(inv);
-  return true;
+// out[f][i] = c0 * A0[i] + a1_f * A1[i] + i w_f * A2[i]; A0 may be null (zero), n entries per matrix
+__global__ void k_form_freq(int64_t n, int nf, const double *__restrict__ om, const double *__restrict__ A0,
+                            const double *__restrict__ A1, const double *__restrict__ A2, C2<double> *__restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * nf) return;
+  const int f = (int)(t / n);
+  const int64_t i = t - (int64_t)f * n;
+  const double w = om[f], a1 = -w * w;
+  out[t] = {(A0 ? A0[i] : 0.0) + a1 * A1[i], w * A2[i]};
 }
 
 }  // namespace
 
+gp_status coarse_cache_lookup(const Hierarchy &H, CoarseCache &C, const std::vector<double> &omegas,
+                              std::vector<int32_t> *slots, hipStream_t s) {
+  slots->assign(omegas.size(), -1);
+  std::vector<double> fresh;
+  for (size_t i = 0; i < omegas.size(); ++i) {
+    for (size_t q = 0; q < C.omegas.size(); ++q)
+      if (C.omegas[q] == omegas[i]) { (*slots)[i] = (int32_t)q; break; }
+    if ((*slots)[i] < 0) {
+      bool dup = false;
+      for (double f : fresh) dup = dup || f == omegas[i];
+      if (!dup) fresh.push_back(omegas[i]);
+    }
+  }
+  if (!fresh.empty()) {
+    const int nb = (int)fresh.size();
+    const int64_t ne = H.neL, nn = H.nnL, na = H.aux.nL;
+    Arena &A = C.arena;
+    double *om;
+    C2<double> *kb, *nbuf, *bb, *ab;
+    GP_TRY(A.alloc(&om, nb));
+    GP_TRY(A.alloc(&kb, std::max<int64_t>(ne * ne * nb, 1)));
+    GP_TRY(A.alloc(&nbuf, std::max<int64_t>(nn * nn * nb, 1)));
+    GP_TRY(A.alloc(&bb, std::max<int64_t>(ne * nn * nb, 1)));
+    GP_TRY(A.alloc(&ab, std::max<int64_t>(na * na * nb, 1)));
+    GP_HIP(hipMemcpyAsync(om, fresh.data(), sizeof(double) * nb, hipMemcpyHostToDevice, s));
+    if (ne) k_form_freq<<<blocks_for(ne * ne * nb), 256, 0, s>>>(ne * ne, nb, om, H.KLd[0], H.KLd[1], H.KLd[2], kb);
+    if (nn) {
+      k_form_freq<<<blocks_for(nn * nn * nb), 256, 0, s>>>(nn * nn, nb, om, nullptr, H.NLd[0], H.NLd[1], nbuf);
+      k_form_freq<<<blocks_for(ne * nn * nb), 256, 0, s>>>(ne * nn, nb, om, nullptr, H.BLd[0], H.BLd[1], bb);
+    }
+    if (na) k_form_freq<<<blocks_for(na * na * nb), 256, 0, s>>>(na * na, nb, om, nullptr, H.aux.NLd[0], H.aux.NLd[1], ab);
+    GP_TRY(dense_inverse_batched(kb, (int)ne, nb, s));
+    GP_TRY(dense_inverse_batched(nbuf, (int)nn, nb, s));
+    GP_TRY(dense_inverse_batched(ab, (int)na, nb, s));
+    for (int f = 0; f < nb; ++f) {
+      C.omegas.push_back(fresh[f]);
+      C.kinv.push_back((double *)(kb + (int64_t)f * ne * ne));
+      C.ninv.push_back((double *)(nbuf + (int64_t)f * nn * nn));
+      C.bl.push_back((double *)(bb + (int64_t)f * ne * nn));
+      C.ainv.push_back((double *)(ab + (int64_t)f * na * na));
+    }
+    const int need = (int)C.omegas.size();
+    if (need > C.cap) {
+      int cap = std::max(16, C.cap);
+      while (cap < need) cap *= 2;
+      const double **tab;
+      GP_TRY(A.alloc(&tab, 4 * (int64_t)cap));
+      C.tab = tab;
+      C.cap = cap;
+    }
+    std::vector<const double *> h(4 * (size_t)C.cap, nullptr);
+    for (int q = 0; q < need; ++q) {
+      h[q] = C.kinv[q]; h[C.cap + q] = C.ninv[q]; h[2 * C.cap + q] = C.bl[q]; h[3 * C.cap + q] = C.ainv[q];
+    }
+    GP_HIP(hipMemcpyAsync((void *)C.tab, h.data(), sizeof(double *) * h.size(), hipMemcpyHostToDevice, s));
+    GP_HIP(hipStreamSynchronize(s));
+    for (size_t i = 0; i < omegas.size(); ++i)
+      if ((*slots)[i] < 0)
+        for (size_t q = 0; q < C.omegas.size(); ++q)
+          if (C.omegas[q] == omegas[i]) { (*slots)[i] = (int32_t)q; break; }
+  }
+  return GP_OK;
+}
+
+void coarse_cache_free(CoarseCache &C) {
+  C.arena.free_all();
+  C.omegas.clear(); C.kinv.clear(); C.ninv.clear(); C.bl.clear(); C.ainv.clear();
+  C.tab = nullptr; C.cap = 0;
+}
+
 template <class T>
-gp_status vwork_create(gp_layout *lay, const Hierarchy &H, int k, const double *omega_cols, const int32_t *fidx_cols,
-                       const std::vector<double> &omegas, VWork<T> *w) {
-  hipStream_t s = lay->stream;
+gp_status vwork_create(gp_layout *lay, const Hierarchy &H, int k, VWork<T> *w) {
+  (void)lay;
   Arena &A = w->arena;
   w->k = k;
-  w->nf = (int)omegas.size();
-  std::vector<T> c1(k), c2(k);
-  for (int j = 0; j < k; ++j) { c1[j] = (T)(-omega_cols[j] * omega_cols[j]); c2[j] = (T)omega_cols[j]; }
   GP_TRY(A.alloc(&w->c1, k)); GP_TRY(A.alloc(&w->c2, k)); GP_TRY(A.alloc(&w->fidx, k));
-  GP_HIP(hipMemcpyAsync(w->c1, c1.data(), sizeof(T) * k, hipMemcpyHostToDevice, s));
-  GP_HIP(hipMemcpyAsync(w->c2, c2.data(), sizeof(T) * k, hipMemcpyHostToDevice, s));
-  GP_HIP(hipMemcpyAsync(w->fidx, fidx_cols, sizeof(int32_t) * k, hipMemcpyHostToDevice, s));
   const size_t nl = H.L.size();
   w->r.assign(nl, nullptr); w->u.assign(nl, nullptr); w->phi.assign(nl, nullptr); w->res.assign(nl, nullptr);
   w->tmp.assign(nl, nullptr); w->rn.assign(nl, nullptr); w->dn.assign(nl, nullptr); w->dn2.assign(nl, nullptr);
@@ -309,52 +373,38 @@ gp_status vwork_create(gp_layout *lay, const Hierarchy &H, int k, const double *
     GP_TRY(A.alloc(&w->res[l], ne)); GP_TRY(A.alloc(&w->tmp[l], ne)); GP_TRY(A.alloc(&w->rn[l], nn));
     GP_TRY(A.alloc(&w->dn[l], nn)); GP_TRY(A.alloc(&w->dn2[l], nn));
   }
-  // coarsest-level inverses per distinct frequency
-  const int64_t ne = H.neL, nn = H.nnL;
-  std::vector<double> kinv(2 * ne * ne * w->nf), ninv(2 * nn * nn * w->nf), bl(2 * ne * nn * w->nf);
-  for (int f = 0; f < w->nf; ++f) {
-    const double om = omegas[f], a1 = -om * om;
-    std::vector<std::complex<double>> a(ne * ne), b(nn * nn);
-    for (int64_t i = 0; i < ne * ne; ++i) a[i] = {H.KL[0][i] + a1 * H.KL[1][i], om * H.KL[2][i]};
-    for (int64_t i = 0; i < nn * nn; ++i) b[i] = {a1 * H.NL[0][i], om * H.NL[1][i]};
-    if (!invert(a, ne)) return fail(GP_ERR_ARG, "singular coarsest edge operator");
-    if (nn && !invert(b, nn)) return fail(GP_ERR_ARG, "singular coarsest nodal operator");
-    for (int64_t i = 0; i < ne * ne; ++i) { kinv[2 * (f * ne * ne + i)] = a[i].real(); kinv[2 * (f * ne * ne + i) + 1] = a[i].imag(); }
-    for (int64_t i = 0; i < nn * nn; ++i) { ninv[2 * (f * nn * nn + i)] = b[i].real(); ninv[2 * (f * nn * nn + i) + 1] = b[i].imag(); }
-    for (int64_t i = 0; i < ne * nn; ++i) { bl[2 * (f * ne * nn + i)] = a1 * H.BL[0][i]; bl[2 * (f * ne * nn + i) + 1] = om * H.BL[1][i]; }
+  w->gtl = H.GTLd;
+  const AuxAMG &X = H.aux;
+  w->ar.assign(X.L.size(), nullptr); w->ax.assign(X.L.size(), nullptr); w->at.assign(X.L.size(), nullptr);
+  for (size_t l = 0; l < X.L.size(); ++l) {
+    GP_TRY(A.alloc(&w->ar[l], X.L[l].n * k)); GP_TRY(A.alloc(&w->ax[l], X.L[l].n * k)); GP_TRY(A.alloc(&w->at[l], X.L[l].n * k));
   }
-  {  // auxiliary AMG buffers and coarsest inverses
-    const AuxAMG &X = H.aux;
-    w->ar.assign(X.L.size(), nullptr); w->ax.assign(X.L.size(), nullptr); w->at.assign(X.L.size(), nullptr);
-    for (size_t l = 0; l < X.L.size(); ++l) {
-      GP_TRY(A.alloc(&w->ar[l], X.L[l].n * k)); GP_TRY(A.alloc(&w->ax[l], X.L[l].n * k)); GP_TRY(A.alloc(&w->at[l], X.L[l].n * k));
-    }
-    GP_TRY(A.alloc(&w->a, X.naux * k));
-    GP_TRY(A.alloc(&w->r0, H.L[0].ne * k));
-    const int64_t na = X.nL;
-    std::vector<double> ainv(2 * na * na * w->nf);
-    for (int f = 0; f < w->nf; ++f) {
-      const double om = omegas[f], a1 = -om * om;
-      std::vector<std::complex<double>> a(na * na);
-      for (int64_t i = 0; i < na * na; ++i) a[i] = {a1 * X.NL[0][i], om * X.NL[1][i]};
-      if (!invert(a, na)) return fail(GP_ERR_ARG, "singular coarsest auxiliary operator");
-      for (int64_t i = 0; i < na * na; ++i) { ainv[2 * (f * na * na + i)] = a[i].real(); ainv[2 * (f * na * na + i) + 1] = a[i].imag(); }
-    }
-    GP_TRY(A.alloc(&w->ainv, std::max<int64_t>(2 * na * na * w->nf, 1)));
-    GP_HIP(hipMemcpyAsync(w->ainv, ainv.data(), sizeof(double) * ainv.size(), hipMemcpyHostToDevice, s));
-    GP_HIP(hipStreamSynchronize(s));
+  GP_TRY(A.alloc(&w->a, X.naux * k));
+  GP_TRY(A.alloc(&w->r0, H.L[0].ne * k));
+  int64_t nlp = (int64_t)X.GxT.nlong * k, ncp = (int64_t)X.GxT.nchunk * k;
+  for (const AuxLevel &L : X.L) {
+    nlp = std::max(nlp, (int64_t)L.N.nlong * 2 * k);
+    nlp = std::max(nlp, (int64_t)std::max(L.P.nlong, L.PT.nlong) * k);
+    ncp = std::max(ncp, (int64_t)L.N.nchunk * 2 * k);
+    ncp = std::max(ncp, (int64_t)std::max(L.P.nchunk, L.PT.nchunk) * k);
   }
-  GP_TRY(A.alloc(&w->kinv, 2 * ne * ne * w->nf));
-  GP_TRY(A.alloc(&w->ninv, std::max<int64_t>(2 * nn * nn * w->nf, 1)));
-  GP_TRY(A.alloc(&w->bl, std::max<int64_t>(2 * ne * nn * w->nf, 1)));
-  GP_TRY(A.alloc(&w->gtl, std::max<int64_t>(nn * ne, 1)));
-  GP_HIP(hipMemcpyAsync(w->kinv, kinv.data(), sizeof(double) * kinv.size(), hipMemcpyHostToDevice, s));
-  if (nn) {
-    GP_HIP(hipMemcpyAsync(w->ninv, ninv.data(), sizeof(double) * ninv.size(), hipMemcpyHostToDevice, s));
-    GP_HIP(hipMemcpyAsync(w->bl, bl.data(), sizeof(double) * bl.size(), hipMemcpyHostToDevice, s));
-    GP_HIP(hipMemcpyAsync(w->gtl, H.GTL.data(), sizeof(double) * H.GTL.size(), hipMemcpyHostToDevice, s));
-  }
-  GP_HIP(hipStreamSynchronize(s));
+  GP_TRY(A.alloc(&w->lp, std::max<int64_t>(nlp, 1)));
+  GP_TRY(A.alloc(&w->lpart, std::max<int64_t>(ncp, 1)));
+  return GP_OK;
+}
+
+template <class T>
+gp_status vwork_set_columns(VWork<T> &w, const double *omega_cols, const int32_t *fslot_cols, const CoarseCache &C,
+                            hipStream_t s) {
+  const int k = w.k;
+  std::vector<T> c1(k), c2(k);
+  for (int j = 0; j < k; ++j) { c1[j] = (T)(-omega_cols[j] * omega_cols[j]); c2[j] = (T)omega_cols[j]; }
+  GP_HIP(hipMemcpyAsync(w.c1, c1.data(), sizeof(T) * k, hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemcpyAsync(w.c2, c2.data(), sizeof(T) * k, hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemcpyAsync(w.fidx, fslot_cols, sizeof(int32_t) * k, hipMemcpyHostToDevice, s));
+  GP_HIP(hipStreamSynchronize(s));  // the host arrays may go out of scope
+  w.tab = C.tab;
+  w.cap = C.cap;
   return GP_OK;
 }
 
@@ -381,6 +431,55 @@ __device__ __forceinline__ C2<T> l1diag(C2<T> a, T L) {
   T m = sqrt(a.x * a.x + a.y * a.y);
   T f = m > (T)0 ? (T)1 + L / m : (T)1;
   return {a.x * f, a.y * f};
+}
+
+// Products of the long rows (kLongRow), in two passes.  k_long_part: one block
+// per chunk of kLongChunk entries, threads = (column j, entry group g); the
+// groups' sums are combined in LDS in a fixed order into
+// part[(c*nv + b)*k + j] = sum_{p in chunk c} v_b[p] x[ci[p]][j].
+// k_long_fin: lp[(q*nv + b)*k + j] = sum of row q's chunk partials, in order.
+template <class T>
+__global__ void __launch_bounds__(256) k_long_part(MatV<T> M, int k, const C2<T> *__restrict__ x, C2<T> *__restrict__ part) {
+  __shared__ C2<T> red[3][256];
+  const int c = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int j = tid % k, g = tid / k, G = blockDim.x / k;
+  C2<T> s[3] = {{0, 0}, {0, 0}, {0, 0}};
+  const int64_t pe = M.lce[c];
+#pragma unroll 4
+  for (int64_t p = M.lcb[c] + g; p < pe; p += G) {
+    const C2<T> xv = x[(int64_t)M.ci[p] * k + j];
+    for (int b = 0; b < M.nv; ++b) {
+      const T a = M.v[b][p];
+      s[b].x += a * xv.x;
+      s[b].y += a * xv.y;
+    }
+  }
+  for (int b = 0; b < M.nv; ++b) red[b][tid] = s[b];
+  __syncthreads();
+  if (tid < k) {
+    for (int b = 0; b < M.nv; ++b) {
+      C2<T> t = {0, 0};
+      for (int gg = 0; gg < G; ++gg) { t.x += red[b][gg * k + j].x; t.y += red[b][gg * k + j].y; }
+      part[((int64_t)c * M.nv + b) * k + j] = t;
+    }
+  }
+}
+
+template <class T>
+__global__ void k_long_fin(MatV<T> M, int k, const C2<T> *__restrict__ part, C2<T> *__restrict__ lp) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)M.nv * k;
+  if (t >= (int64_t)M.nlong * per) return;
+  const int q = (int)(t / per);
+  const int64_t bj = t - (int64_t)q * per;  // b*k + j
+  C2<T> acc = {0, 0};
+  for (int c = M.lqp[q]; c < M.lqp[q + 1]; ++c) {
+    const C2<T> v = part[(int64_t)c * per + bj];
+    acc.x += v.x;
+    acc.y += v.y;
+  }
+  lp[t] = acc;
 }
 
 struct EdgeMode { enum { RES = 0, JACOBI = 1, JACOBI0 = 2 }; };
@@ -432,14 +531,19 @@ template <class T>
 __global__ void k_node(int64_t nr, int k, MatV<T> N, const T *__restrict__ c1v, const T *__restrict__ c2v,
                        const T *__restrict__ nd0, const T *__restrict__ nd1, const T *__restrict__ nl0,
                        const T *__restrict__ nl1, const C2<T> *__restrict__ rn, const C2<T> *__restrict__ d,
-                       C2<T> *__restrict__ dout, T damp) {
+                       C2<T> *__restrict__ dout, T damp, const C2<T> *__restrict__ lp) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t i = t / k;
   int j = (int)(t - i * k);
   if (i >= nr) return;
   const T c1 = c1v[j], c2 = c2v[j];
   C2<T> acc = rn[i * k + j];
-  if (d) {
+  const int32_t q = (d && N.lidx) ? N.lidx[i] : -1;
+  if (q >= 0) {
+    const C2<T> s0 = lp[((int64_t)q * 2) * k + j], s1 = lp[((int64_t)q * 2 + 1) * k + j];
+    acc.x -= c1 * s0.x - c2 * s1.y;
+    acc.y -= c1 * s0.y + c2 * s1.x;
+  } else if (d) {
     for (int64_t p = N.rp[i]; p < N.rp[i + 1]; ++p) {
       C2<T> a = {c1 * N.v[0][p], c2 * N.v[1][p]};
       C2<T> x = d[(int64_t)N.ci[p] * k + j];
@@ -448,21 +552,24 @@ __global__ void k_node(int64_t nr, int k, MatV<T> N, const T *__restrict__ c1v, 
     }
   }
   C2<T> dg = l1diag<T>({c1 * nd0[i], c2 * nd1[i]}, fabs(c1) * nl0[i] + c2 * nl1[i]);
-  C2<T> q = cdiv(acc, dg);
-  if (!d) { dout[i * k + j] = {damp * q.x, damp * q.y}; return; }
+  C2<T> dq = cdiv(acc, dg);
+  if (!d) { dout[i * k + j] = {damp * dq.x, damp * dq.y}; return; }
   C2<T> d0 = d[i * k + j];
-  dout[i * k + j] = {d0.x + damp * q.x, d0.y + damp * q.y};
+  dout[i * k + j] = {d0.x + damp * dq.x, d0.y + damp * dq.y};
 }
 
 // y = M x (real matrix, complex multi-column vectors); accumulate adds to y.
 template <class T>
-__global__ void k_spmv_real(int64_t nr, int k, MatV<T> M, const C2<T> *__restrict__ x, C2<T> *__restrict__ y, int accumulate) {
+__global__ void k_spmv_real(int64_t nr, int k, MatV<T> M, const C2<T> *__restrict__ x, C2<T> *__restrict__ y, int accumulate,
+                            const C2<T> *__restrict__ lp) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t i = t / k;
   int j = (int)(t - i * k);
   if (i >= nr) return;
   C2<T> acc = {0, 0};
-  for (int64_t p = M.rp[i]; p < M.rp[i + 1]; ++p) {
+  const int32_t q = M.lidx ? M.lidx[i] : -1;
+  if (q >= 0) acc = lp[(int64_t)q * k + j];
+  else for (int64_t p = M.rp[i]; p < M.rp[i + 1]; ++p) {
     const T a = M.v[0][p];
     C2<T> v = x[(int64_t)M.ci[p] * k + j];
     acc.x += a * v.x;
@@ -482,7 +589,7 @@ __global__ void k_add(int64_t n, C2<T> *__restrict__ a, const C2<T> *__restrict_
 //   phi = N^-1 (G^T r);  u = K^-1 (r - B phi)
 template <class T>
 __global__ void k_coarse(int64_t ne, int64_t nn, int k, const int32_t *__restrict__ fidx,
-                         const double *__restrict__ kinv, const double *__restrict__ ninv, const double *__restrict__ bl,
+                         const double *const *__restrict__ tab, int cap,
                          const double *__restrict__ gtl, const C2<T> *__restrict__ r, C2<T> *__restrict__ u,
                          C2<T> *__restrict__ phi) {
   extern __shared__ double sm[];
@@ -500,7 +607,7 @@ __global__ void k_coarse(int64_t ne, int64_t nn, int k, const int32_t *__restric
     rn[2 * a] = sx; rn[2 * a + 1] = sy;
   }
   __syncthreads();
-  const double *Ni = ninv + 2 * (int64_t)f * nn * nn;
+  const double *Ni = tab[cap + f];
   for (int64_t a = threadIdx.x; a < nn; a += blockDim.x) {
     double sx = 0, sy = 0;
     for (int64_t b = 0; b < nn; ++b) {
@@ -511,7 +618,7 @@ __global__ void k_coarse(int64_t ne, int64_t nn, int k, const int32_t *__restric
     ph[2 * a] = sx; ph[2 * a + 1] = sy;
   }
   __syncthreads();
-  const double *Bf = bl + 2 * (int64_t)f * ne * nn;
+  const double *Bf = tab[2 * cap + f];
   for (int64_t i = threadIdx.x; i < ne; i += blockDim.x) {
     double sx = rr[2 * i], sy = rr[2 * i + 1];
     for (int64_t a = 0; a < nn; ++a) {
@@ -522,7 +629,7 @@ __global__ void k_coarse(int64_t ne, int64_t nn, int k, const int32_t *__restric
     rs[2 * i] = sx; rs[2 * i + 1] = sy;
   }
   __syncthreads();
-  const double *Ki = kinv + 2 * (int64_t)f * ne * ne;
+  const double *Ki = tab[f];
   for (int64_t i = threadIdx.x; i < ne; i += blockDim.x) {
     double sx = 0, sy = 0;
     for (int64_t b = 0; b < ne; ++b) {
@@ -538,14 +645,19 @@ __global__ void k_coarse(int64_t ne, int64_t nn, int k, const int32_t *__restric
 // y = r - M2(w) x  (mode 0),  y -= M2(w) x  (mode 1),  y = M2(w) x (mode 2); M2 = c1 v0 + i c2 v1
 template <class T>
 __global__ void k_spmv2(int64_t nr, int k, MatV<T> M, const T *__restrict__ c1v, const T *__restrict__ c2v,
-                        const C2<T> *__restrict__ r, const C2<T> *__restrict__ x, C2<T> *__restrict__ y, int mode) {
+                        const C2<T> *__restrict__ r, const C2<T> *__restrict__ x, C2<T> *__restrict__ y, int mode,
+                        const C2<T> *__restrict__ lp) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t i = t / k;
   int j = (int)(t - i * k);
   if (i >= nr) return;
   const T c1 = c1v[j], c2 = c2v[j];
   C2<T> acc = {0, 0};
-  for (int64_t p = M.rp[i]; p < M.rp[i + 1]; ++p) {
+  const int32_t q = M.lidx ? M.lidx[i] : -1;
+  if (q >= 0) {
+    const C2<T> s0 = lp[((int64_t)q * 2) * k + j], s1 = lp[((int64_t)q * 2 + 1) * k + j];
+    acc = {c1 * s0.x - c2 * s1.y, c1 * s0.y + c2 * s1.x};
+  } else for (int64_t p = M.rp[i]; p < M.rp[i + 1]; ++p) {
     C2<T> a = {c1 * M.v[0][p], c2 * M.v[1][p]};
     C2<T> v = x[(int64_t)M.ci[p] * k + j];
     acc.x += a.x * v.x - a.y * v.y;
@@ -558,11 +670,11 @@ __global__ void k_spmv2(int64_t nr, int k, MatV<T> M, const T *__restrict__ c1v,
 
 // x = A_f^-1 r for each column (dense complex FP64 inverse per frequency), one block per column
 template <class T>
-__global__ void k_dense_apply(int64_t n, int k, const int32_t *__restrict__ fidx, const double *__restrict__ inv,
+__global__ void k_dense_apply(int64_t n, int k, const int32_t *__restrict__ fidx, const double *const *__restrict__ tab,
                               const C2<T> *__restrict__ r, C2<T> *__restrict__ x) {
   extern __shared__ double sr[];
   const int j = blockIdx.x;
-  const double *M = inv + 2 * (int64_t)fidx[j] * n * n;
+  const double *M = tab[fidx[j]];
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) { sr[2 * i] = (double)r[i * k + j].x; sr[2 * i + 1] = (double)r[i * k + j].y; }
   __syncthreads();
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
@@ -586,14 +698,18 @@ template <class T>
 MatV<T> view(const Csr &c, const CsrF &f);
 template <>
 MatV<double> view<double>(const Csr &c, const CsrF &) {
-  MatV<double> v; v.nr = c.nr; v.nc = c.nc; v.rp = c.rp; v.ci = c.ci;
+  MatV<double> v; v.nr = c.nr; v.nc = c.nc; v.rp = c.rp; v.ci = c.ci; v.nv = c.nv;
   for (int k = 0; k < c.nv; ++k) v.v[k] = c.v[k];
+  v.nlong = c.nlong; v.lrows = c.lrows; v.lidx = c.lidx;
+  v.nchunk = c.nchunk; v.lcb = c.lcb; v.lce = c.lce; v.lqp = c.lqp;
   return v;
 }
 template <>
 MatV<float> view<float>(const Csr &, const CsrF &f) {
-  MatV<float> v; v.nr = f.nr; v.nc = f.nc; v.rp = f.rp; v.ci = f.ci;
+  MatV<float> v; v.nr = f.nr; v.nc = f.nc; v.rp = f.rp; v.ci = f.ci; v.nv = f.nv;
   for (int k = 0; k < f.nv; ++k) v.v[k] = f.v[k];
+  v.nlong = f.nlong; v.lrows = f.lrows; v.lidx = f.lidx;
+  v.nchunk = f.nchunk; v.lcb = f.lcb; v.lce = f.lce; v.lqp = f.lqp;
   return v;
 }
 
@@ -602,6 +718,13 @@ template <> const double *arr<double>(double *d, float *) { return d; }
 template <> const float *arr<float>(double *, float *f) { return f; }
 
 inline unsigned grid_rows(int64_t nr, int k) { return blocks_for(nr * k); }
+
+template <class T>
+void long_pass(const MatV<T> &M, int k, const C2<T> *x, C2<T> *part, C2<T> *lp, hipStream_t s) {
+  if (M.nlong <= 0) return;
+  k_long_part<T><<<M.nchunk, 256, 0, s>>>(M, k, x, part);
+  k_long_fin<T><<<blocks_for((int64_t)M.nlong * M.nv * k), 256, 0, s>>>(M, k, part, lp);
+}
 
 }  // namespace
 
@@ -614,7 +737,7 @@ static gp_status vcycle_level(const Hierarchy &H, VWork<T> &w, hipStream_t s, in
   C2<T> *rn = w.rn[l], *d1 = w.dn[l], *d2 = w.dn2[l];
   if (l + 1 == (int)H.L.size()) {
     size_t shm = sizeof(double) * (4 * ne + 4 * nn);
-    k_coarse<T><<<k, 256, shm, s>>>(ne, nn, k, w.fidx, w.kinv, w.ninv, w.bl, w.gtl, r, u, phi);
+    k_coarse<T><<<k, 256, shm, s>>>(ne, nn, k, w.fidx, w.tab, w.cap, w.gtl, r, u, phi);
     return GP_OK;
   }
   const MatV<T> K = view<T>(L.K, L.Kf), B = view<T>(L.B, L.Bf), N = view<T>(L.N, L.Nf);
@@ -625,45 +748,70 @@ static gp_status vcycle_level(const Hierarchy &H, VWork<T> &w, hipStream_t s, in
   const T *nl0 = arr<T>(L.nl[0], L.nlf[0]), *nl1 = arr<T>(L.nl[1], L.nlf[1]);
   const T damp = (T)1.0;
   const unsigned ge = grid_rows(ne, k), gn = grid_rows(nn, k);
+  // algorithmic bytes of one edge-kernel launch: K (columns + three values) and
+  // its row pointers, B when phi is used, and each vector once
+  KTimer *kt = (l == 0) ? w.kt : nullptr;
+  auto edge_bytes = [&](int mode, int use_phi) {
+    const double vb = (double)sizeof(C2<T>) * k, tb = (double)sizeof(T);
+    double b = (double)L.K.nnz * (4 + 3 * tb) + 8.0 * (ne + 1) + vb * ne * 2;  // K, r in, out
+    if (mode != EdgeMode::JACOBI0) b += vb * ne;                                // u
+    if (use_phi) b += (double)L.B.nnz * (4 + 2 * tb) + 8.0 * (ne + 1) + vb * nn;
+    if (mode != EdgeMode::RES) b += 6 * tb * ne;                                // smoother diagonals
+    return b;
+  };
+  auto tb0 = [&]() { if (kt) kt->begin(s); };
+  auto te0 = [&](int mode, int use_phi) { if (kt) kt->end(s, edge_bytes(mode, use_phi)); };
   // pre-smoothing on the edges (phi = 0)
+  tb0();
   k_edge<T><<<ge, 256, 0, s>>>(ne, k, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, nullptr, nullptr, u,
                                EdgeMode::JACOBI0, 0, damp);
+  te0(EdgeMode::JACOBI0, 0);
   for (int it = 1; it < nu_pre; ++it) {
+    tb0();
     k_edge<T><<<ge, 256, 0, s>>>(ne, k, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, nullptr, tmp,
-                                 EdgeMode::JACOBI, 0, damp);
+                                   EdgeMode::JACOBI, 0, damp);
+    te0(EdgeMode::JACOBI, 0);
     std::swap(u, tmp);
   }
   // pre-smoothing on the nodal auxiliary space
+  tb0();
   k_edge<T><<<ge, 256, 0, s>>>(ne, k, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, nullptr, res,
                                EdgeMode::RES, 0, damp);
-  k_spmv_real<T><<<gn, 256, 0, s>>>(nn, k, GT, res, rn, 0);
-  k_node<T><<<gn, 256, 0, s>>>(nn, k, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, nullptr, phi, damp);
+  te0(EdgeMode::RES, 0);
+  k_spmv_real<T><<<gn, 256, 0, s>>>(nn, k, GT, res, rn, 0, w.lp);
+  k_node<T><<<gn, 256, 0, s>>>(nn, k, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, nullptr, phi, damp, w.lp);
   for (int it = 1; it < nu_pre; ++it) {
-    k_node<T><<<gn, 256, 0, s>>>(nn, k, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, phi, d2, damp);
+    k_node<T><<<gn, 256, 0, s>>>(nn, k, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, phi, d2, damp, w.lp);
     std::swap(phi, d2);
   }
   // coarse-grid correction
+  tb0();
   k_edge<T><<<ge, 256, 0, s>>>(ne, k, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, phi, res,
                                EdgeMode::RES, 1, damp);
+  te0(EdgeMode::RES, 1);
   const int64_t nec = H.L[l + 1].ne, nnc = H.L[l + 1].nn;
-  k_spmv_real<T><<<grid_rows(nec, k), 256, 0, s>>>(nec, k, PT, res, w.r[l + 1], 0);
+  k_spmv_real<T><<<grid_rows(nec, k), 256, 0, s>>>(nec, k, PT, res, w.r[l + 1], 0, w.lp);
   GP_TRY(vcycle_level<T>(H, w, s, l + 1, nu_pre, nu_post));
-  k_spmv_real<T><<<ge, 256, 0, s>>>(ne, k, P, w.u[l + 1], u, 1);
-  k_spmv_real<T><<<gn, 256, 0, s>>>(nn, k, Pn, w.phi[l + 1], phi, 1);
+  k_spmv_real<T><<<ge, 256, 0, s>>>(ne, k, P, w.u[l + 1], u, 1, w.lp);
+  k_spmv_real<T><<<gn, 256, 0, s>>>(nn, k, Pn, w.phi[l + 1], phi, 1, w.lp);
   (void)nnc;
   // post-smoothing: nodal, then edges
+  tb0();
   k_edge<T><<<ge, 256, 0, s>>>(ne, k, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, phi, res,
                                EdgeMode::RES, 1, damp);
-  k_spmv_real<T><<<gn, 256, 0, s>>>(nn, k, GT, res, rn, 0);
-  k_node<T><<<gn, 256, 0, s>>>(nn, k, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, nullptr, d1, damp);
+  te0(EdgeMode::RES, 1);
+  k_spmv_real<T><<<gn, 256, 0, s>>>(nn, k, GT, res, rn, 0, w.lp);
+  k_node<T><<<gn, 256, 0, s>>>(nn, k, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, nullptr, d1, damp, w.lp);
   for (int it = 1; it < nu_post; ++it) {
-    k_node<T><<<gn, 256, 0, s>>>(nn, k, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, d1, d2, damp);
+    k_node<T><<<gn, 256, 0, s>>>(nn, k, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, d1, d2, damp, w.lp);
     std::swap(d1, d2);
   }
   k_add<T><<<gn, 256, 0, s>>>(nn * k, phi, d1);
   for (int it = 0; it < nu_post; ++it) {
+    tb0();
     k_edge<T><<<ge, 256, 0, s>>>(ne, k, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, phi, tmp,
-                                 EdgeMode::JACOBI, 1, damp);
+                                   EdgeMode::JACOBI, 1, damp);
+    te0(EdgeMode::JACOBI, 1);
     std::swap(u, tmp);
   }
   // results must land in the level's canonical buffers
@@ -686,7 +834,7 @@ static gp_status aux_cycle(const Hierarchy &H, VWork<T> &w, hipStream_t s, int l
   const int64_t n = L.n;
   C2<T> *r = w.ar[l], *x = w.ax[l], *t = w.at[l];
   if (l + 1 == (int)X.L.size()) {
-    k_dense_apply<T><<<k, 256, sizeof(double) * 2 * n, s>>>(n, k, w.fidx, w.ainv, r, x);
+    k_dense_apply<T><<<k, 256, sizeof(double) * 2 * n, s>>>(n, k, w.fidx, w.tab + 3 * w.cap, r, x);
     return GP_OK;
   }
   const MatV<T> N = view<T>(L.N, L.Nf), P = view<T>(L.P, L.Pf), PT = view<T>(L.PT, L.PTf);
@@ -694,18 +842,23 @@ static gp_status aux_cycle(const Hierarchy &H, VWork<T> &w, hipStream_t s, int l
   const T *l0 = arr<T>(L.nl[0], L.nlf[0]), *l1 = arr<T>(L.nl[1], L.nlf[1]);
   const unsigned gr = grid_rows(n, k);
   const int nu = w.nu_aux;
-  k_node<T><<<gr, 256, 0, s>>>(n, k, N, w.c1, w.c2, d0, d1, l0, l1, r, nullptr, x, (T)1);
+  k_node<T><<<gr, 256, 0, s>>>(n, k, N, w.c1, w.c2, d0, d1, l0, l1, r, nullptr, x, (T)1, w.lp);
   for (int it = 1; it < nu; ++it) {
-    k_node<T><<<gr, 256, 0, s>>>(n, k, N, w.c1, w.c2, d0, d1, l0, l1, r, x, t, (T)1);
+    long_pass<T>(N, k, x, w.lpart, w.lp, s);
+    k_node<T><<<gr, 256, 0, s>>>(n, k, N, w.c1, w.c2, d0, d1, l0, l1, r, x, t, (T)1, w.lp);
     std::swap(x, t);
   }
-  k_spmv2<T><<<gr, 256, 0, s>>>(n, k, N, w.c1, w.c2, r, x, t, 0);
+  long_pass<T>(N, k, x, w.lpart, w.lp, s);
+  k_spmv2<T><<<gr, 256, 0, s>>>(n, k, N, w.c1, w.c2, r, x, t, 0, w.lp);
   const int64_t nc = X.L[l + 1].n;
-  k_spmv_real<T><<<grid_rows(nc, k), 256, 0, s>>>(nc, k, PT, t, w.ar[l + 1], 0);
+  long_pass<T>(PT, k, t, w.lpart, w.lp, s);
+  k_spmv_real<T><<<grid_rows(nc, k), 256, 0, s>>>(nc, k, PT, t, w.ar[l + 1], 0, w.lp);
   GP_TRY(aux_cycle<T>(H, w, s, l + 1));
-  k_spmv_real<T><<<gr, 256, 0, s>>>(n, k, P, w.ax[l + 1], x, 1);
+  long_pass<T>(P, k, w.ax[l + 1], w.lpart, w.lp, s);
+  k_spmv_real<T><<<gr, 256, 0, s>>>(n, k, P, w.ax[l + 1], x, 1, w.lp);
   for (int it = 0; it < nu; ++it) {
-    k_node<T><<<gr, 256, 0, s>>>(n, k, N, w.c1, w.c2, d0, d1, l0, l1, r, x, t, (T)1);
+    long_pass<T>(N, k, x, w.lpart, w.lp, s);
+    k_node<T><<<gr, 256, 0, s>>>(n, k, N, w.c1, w.c2, d0, d1, l0, l1, r, x, t, (T)1, w.lp);
     std::swap(x, t);
   }
   if (x != w.ax[l]) GP_HIP(hipMemcpyAsync(w.ax[l], x, sizeof(C2<T>) * n * k, hipMemcpyDeviceToDevice, s));
@@ -723,10 +876,11 @@ gp_status precond(const Hierarchy &H, VWork<T> &w, hipStream_t s, int nu_pre, in
   const unsigned gn = grid_rows(n, k), ga = grid_rows(na, k);
   // auxiliary correction: a = AMG(Gx^T r), r <- r - Bx a
   k_copy<T><<<blocks_for(n * k), 256, 0, s>>>(n * k, w.r[0], w.r0);
-  k_spmv_real<T><<<ga, 256, 0, s>>>(na, k, GxT, w.r[0], w.ar[0], 0);
+  long_pass<T>(GxT, k, w.r[0], w.lpart, w.lp, s);
+  k_spmv_real<T><<<ga, 256, 0, s>>>(na, k, GxT, w.r[0], w.ar[0], 0, w.lp);
   GP_TRY(aux_cycle<T>(H, w, s, 0));
   k_copy<T><<<blocks_for(na * k), 256, 0, s>>>(na * k, w.ax[0], w.a);
-  k_spmv2<T><<<gn, 256, 0, s>>>(n, k, Bx, w.c1, w.c2, nullptr, w.a, w.r[0], 1);
+  k_spmv2<T><<<gn, 256, 0, s>>>(n, k, Bx, w.c1, w.c2, nullptr, w.a, w.r[0], 1, w.lp);
   // edge V-cycle on the remaining residual
   GP_TRY(vcycle<T>(H, w, s, nu_pre, nu_post));
   // auxiliary correction on r0 - K u - B phi - Bx a
@@ -734,8 +888,9 @@ gp_status precond(const Hierarchy &H, VWork<T> &w, hipStream_t s, int nu_pre, in
   (void)kd0;
   k_edge<T><<<gn, 256, 0, s>>>(n, k, K, B, w.c1, w.c2, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, w.r0,
                                w.u[0], w.phi[0], w.res[0], EdgeMode::RES, 1, (T)1);
-  k_spmv2<T><<<gn, 256, 0, s>>>(n, k, Bx, w.c1, w.c2, nullptr, w.a, w.res[0], 1);
-  k_spmv_real<T><<<ga, 256, 0, s>>>(na, k, GxT, w.res[0], w.ar[0], 0);
+  k_spmv2<T><<<gn, 256, 0, s>>>(n, k, Bx, w.c1, w.c2, nullptr, w.a, w.res[0], 1, w.lp);
+  long_pass<T>(GxT, k, w.res[0], w.lpart, w.lp, s);
+  k_spmv_real<T><<<ga, 256, 0, s>>>(na, k, GxT, w.res[0], w.ar[0], 0, w.lp);
   GP_TRY(aux_cycle<T>(H, w, s, 0));
   k_add<T><<<blocks_for(na * k), 256, 0, s>>>(na * k, w.a, w.ax[0]);
   return GP_OK;
@@ -744,10 +899,10 @@ gp_status precond(const Hierarchy &H, VWork<T> &w, hipStream_t s, int nu_pre, in
 template gp_status precond<float>(const Hierarchy &, VWork<float> &, hipStream_t, int, int);
 template gp_status precond<double>(const Hierarchy &, VWork<double> &, hipStream_t, int, int);
 
-template gp_status vwork_create<float>(gp_layout *, const Hierarchy &, int, const double *, const int32_t *,
-                                       const std::vector<double> &, VWork<float> *);
-template gp_status vwork_create<double>(gp_layout *, const Hierarchy &, int, const double *, const int32_t *,
-                                        const std::vector<double> &, VWork<double> *);
+template gp_status vwork_create<float>(gp_layout *, const Hierarchy &, int, VWork<float> *);
+template gp_status vwork_create<double>(gp_layout *, const Hierarchy &, int, VWork<double> *);
+template gp_status vwork_set_columns<float>(VWork<float> &, const double *, const int32_t *, const CoarseCache &, hipStream_t);
+template gp_status vwork_set_columns<double>(VWork<double> &, const double *, const int32_t *, const CoarseCache &, hipStream_t);
 template void vwork_free<float>(VWork<float> *);
 template void vwork_free<double>(VWork<double> *);
 template gp_status vcycle<float>(const Hierarchy &, VWork<float> &, hipStream_t, int, int);

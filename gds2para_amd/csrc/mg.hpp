@@ -16,13 +16,11 @@
 #pragma once
 #include <vector>
 
+#include "dense.hpp"
 #include "sparse.hpp"
 #include "transfer.hpp"
 
 namespace gp {
-
-template <class T>
-struct C2 { T x, y; };
 
 template <class T>
 struct MatV {  // read-only view of a CSR in precision T
@@ -30,6 +28,12 @@ struct MatV {  // read-only view of a CSR in precision T
   const int64_t *rp = nullptr;
   const int32_t *ci = nullptr;
   const T *v[3] = {nullptr, nullptr, nullptr};
+  int nv = 0;
+  int32_t nlong = 0;                 // rows reduced block-wise beforehand (kLongRow)
+  const int32_t *lrows = nullptr, *lidx = nullptr;
+  int32_t nchunk = 0;
+  const int64_t *lcb = nullptr, *lce = nullptr;
+  const int32_t *lqp = nullptr;
 };
 
 struct MgLevel {
@@ -55,6 +59,7 @@ struct AuxAMG {
   CsrF Gxf, GxTf, Bxf;
   std::vector<AuxLevel> L;
   std::vector<double> NL[2];     // coarsest level, dense (host)
+  double *NLd[2] = {};           // same, on the device
 };
 
 struct Hierarchy {
@@ -73,8 +78,25 @@ struct Hierarchy {
   // coarsest level, dense on the host (row-major), per block
   int64_t neL = 0, nnL = 0;
   std::vector<double> KL[3], BL[2], NL[2], GTL;
+  double *KLd[3] = {}, *BLd[2] = {}, *NLd[2] = {}, *GTLd = nullptr;  // device copies
   double setup_ms = 0;
 };
+
+// Coarsest-level operators per frequency, formed and inverted on the device
+// once per distinct frequency and kept for the layout's lifetime.  tab holds
+// device pointers [kinv | ninv | bl | ainv] x cap, indexed by frequency slot.
+struct CoarseCache {
+  std::vector<double> omegas;
+  std::vector<double *> kinv, ninv, bl, ainv;
+  const double **tab = nullptr;
+  int cap = 0;
+  Arena arena;
+};
+
+// Slots of the given angular frequencies, adding (and inverting) new ones.
+gp_status coarse_cache_lookup(const Hierarchy &H, CoarseCache &C, const std::vector<double> &omegas,
+                              std::vector<int32_t> *slots, hipStream_t s);
+void coarse_cache_free(CoarseCache &C);
 
 gp_status build_hierarchy(gp_layout *lay, Hierarchy *H, bool want_f32, int max_levels);
 gp_status build_aux(gp_layout *lay, Hierarchy *H, bool want_f32);
@@ -82,27 +104,32 @@ gp_status aux_smoother_data(Arena &A, const Csr &M, double **d, double **l, hipS
 gp_status d2f_copy(const double *src, float *dst, int64_t n, hipStream_t s);
 void free_hierarchy(gp_layout *lay, Hierarchy *H);
 
-// Per-batch data: column frequencies and the coarsest-level inverses.
+// Per-batch work space: per-slot frequencies and every level's vectors.
 template <class T>
 struct VWork {
   int k = 0;
-  T *c1 = nullptr, *c2 = nullptr;     // per column: -w^2 and w
-  int32_t *fidx = nullptr;            // per column: index into the batch's frequency list
-  int nf = 0;
-  double *kinv = nullptr, *ninv = nullptr;  // [nf][neL*neL] / [nf][nnL*nnL] complex (re,im)
-  double *gtl = nullptr, *bl = nullptr;     // coarsest GT (nnL x neL) real, B per freq [nf][neL*nnL] complex
+  T *c1 = nullptr, *c2 = nullptr;     // per slot: -w^2 and w
+  int32_t *fidx = nullptr;            // per slot: frequency slot in the CoarseCache
+  const double *const *tab = nullptr; // CoarseCache::tab
+  int cap = 0;
+  const double *gtl = nullptr;        // coarsest G^T (nnL x neL), real
   std::vector<C2<T> *> r, u, phi, res, tmp, rn, dn, dn2;
   // auxiliary AMG: per-level rhs / iterate / scratch, level-0 output, saved input
   std::vector<C2<T> *> ar, ax, at;
   C2<T> *a = nullptr, *r0 = nullptr;
-  double *ainv = nullptr;                   // [nf][nL*nL] complex, coarsest aux level
+  C2<T> *lp = nullptr;                      // long-row products [nlong][nv][k]
+  C2<T> *lpart = nullptr;                   // per-chunk partial products [nchunk][nv][k]
   int nu_aux = 2;
+  KTimer *kt = nullptr;               // times the level-0 edge smoother launches
   Arena arena;
 };
 
 template <class T>
-gp_status vwork_create(gp_layout *lay, const Hierarchy &H, int k, const double *omega_cols, const int32_t *fidx_cols,
-                       const std::vector<double> &omegas, VWork<T> *w);
+gp_status vwork_create(gp_layout *lay, const Hierarchy &H, int k, VWork<T> *w);
+// per-slot angular frequencies and CoarseCache slots (host arrays of length k)
+template <class T>
+gp_status vwork_set_columns(VWork<T> &w, const double *omega_cols, const int32_t *fslot_cols, const CoarseCache &C,
+                            hipStream_t s);
 template <class T>
 void vwork_free(VWork<T> *w);
 

@@ -3,17 +3,24 @@
 // multigrid preconditioner, and Z extraction (Construct_Z_V0_Vh,
 // src/fdtd.hpp:4133-4170).
 //
-// Columns are (port, frequency) pairs.  A batch of k columns shares every
-// sparse matrix read (SpMM); each column keeps its own Arnoldi recurrence,
-// Hessenberg matrix and Givens rotations.  Iterates are kept in split form
-// x = u + G phi so that A x = S u + D(w) (u + G phi) is evaluated without
-// cancellation (S G = 0), which is what lets a 1e-8 relative residual be met
-// at low frequency where the gradient part of x dominates by ~1e10.
+// Columns are (port, frequency) pairs scheduled onto k device slots.  The k
+// slots share every sparse matrix read (SpMM); each keeps its own Arnoldi
+// recurrence, Hessenberg matrix and Givens rotations.  At every restart the
+// FP64 residual of each slot is formed exactly; a slot whose column has
+// converged hands its solution to Z extraction and takes the next pending
+// column (continuous batching), so slots never wait for the slowest column.
+//
+// Iterates are kept in split form x = u + G phi + Gx a (G the nodal gradient,
+// Gx the [V0d | V0c] basis) so that A x = S u + (S G) phi + (S Gx) a +
+// D(w) x is evaluated without cancellation: S G and S Gx vanish up to the
+// rounding of S and are formed exactly, which is what lets a 1e-8 relative
+// residual be met at low frequency where the gradient part of x dominates.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <deque>
 #include <map>
 
 #include "mg.hpp"
@@ -23,9 +30,14 @@ namespace gp {
 struct Solver {
   Hierarchy H;
   bool f32 = false;
+  CoarseCache cc;
 };
 
-void destroy_solver(Solver *s) { delete s; }
+void destroy_solver(Solver *s) {
+  if (!s) return;
+  coarse_cache_free(s->cc);
+  delete s;
+}
 
 namespace {
 
@@ -48,101 +60,123 @@ __device__ __forceinline__ double reference_diag_re(double sii, double w, double
   return t - sii;
 }
 
-// out = S u + D(w)(u + G phi + Gx a)   (mode 0)   or   out = b - (...)   (mode 1)
+struct OuterOp {
+  int64_t n;
+  const int64_t *rp; const int32_t *ci; const double *sv;       // S
+  const int64_t *grp; const int32_t *gci; const double *gv;     // G
+  const int64_t *xrp; const int32_t *xci; const double *xv;     // Gx
+  const int64_t *sgrp; const int32_t *sgci; const double *sgv;  // S G (exact)
+  const int64_t *sxrp; const int32_t *sxci; const double *sxv;  // S Gx (exact)
+  const double *epsr, *sig, *sdiag;
+};
+
+// out = S u [+ SG phi + SGx a] + D(w)(u + G phi + Gx a)   (mode 0)   or b - (...)   (mode 1)
 // D(w) = w * complex(-w * eps_r * EPSILON0, sigma) with the real part rounded as
-// above (src/generateStiff.cpp:620-626).  use_phi = 0 drops the G phi term.
-__global__ void k_outer(int64_t n, int k, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                        const double *__restrict__ sv, const int64_t *__restrict__ grp, const int32_t *__restrict__ gci,
-                        const double *__restrict__ gv, const int64_t *__restrict__ xrp, const int32_t *__restrict__ xci,
-                        const double *__restrict__ xv, const double *__restrict__ epsr, const double *__restrict__ sig,
-                        const double *__restrict__ sdiag, const double *__restrict__ omega, const Z2 *__restrict__ u, const Z2 *__restrict__ phi,
-                        const Z2 *__restrict__ a, const Z2 *__restrict__ b, Z2 *__restrict__ out, int mode, int use_phi,
-                        const int64_t *__restrict__ sgrp = nullptr, const int32_t *__restrict__ sgci = nullptr,
-                        const double *__restrict__ sgv = nullptr, const int64_t *__restrict__ sxrp = nullptr,
-                        const int32_t *__restrict__ sxci = nullptr, const double *__restrict__ sxv = nullptr) {
+// above (src/generateStiff.cpp:620-626).  exact = 1 adds the S G / S Gx terms.
+template <class TZ>
+__global__ void k_outer(OuterOp op, int k, const double *__restrict__ omega, const C2<TZ> *__restrict__ u,
+                        const C2<TZ> *__restrict__ phi, const C2<TZ> *__restrict__ a, const Z2 *__restrict__ b,
+                        Z2 *__restrict__ out, int mode, int exact) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t i = t / k;
   int j = (int)(t - i * k);
-  if (i >= n) return;
+  if (i >= op.n) return;
   Z2 acc = {0., 0.};
-  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) {
-    const double a = sv[p];
-    const Z2 x = u[(int64_t)ci[p] * k + j];
-    acc.x += a * x.x;
-    acc.y += a * x.y;
+  for (int64_t p = op.rp[i]; p < op.rp[i + 1]; ++p) {
+    const double m = op.sv[p];
+    const C2<TZ> x = u[(int64_t)op.ci[p] * k + j];
+    acc.x += m * (double)x.x;
+    acc.y += m * (double)x.y;
   }
-  Z2 xi = u[i * k + j];
-  if (use_phi) {
-    for (int64_t p = grp[i]; p < grp[i + 1]; ++p) {
-      const double g = gv[p];
-      const Z2 x = phi[(int64_t)gci[p] * k + j];
-      xi.x += g * x.x;
-      xi.y += g * x.y;
-    }
-    for (int64_t p = xrp[i]; p < xrp[i + 1]; ++p) {
-      const double g = xv[p];
-      const Z2 x = a[(int64_t)xci[p] * k + j];
-      xi.x += g * x.x;
-      xi.y += g * x.y;
-    }
+  Z2 xi = {(double)u[i * k + j].x, (double)u[i * k + j].y};
+  for (int64_t p = op.grp[i]; p < op.grp[i + 1]; ++p) {
+    const double g = op.gv[p];
+    const C2<TZ> x = phi[(int64_t)op.gci[p] * k + j];
+    xi.x += g * (double)x.x;
+    xi.y += g * (double)x.y;
   }
-  if (sgrp) {  // rounding residue of S on the gradient parts (exact products, tiny)
-    for (int64_t p = sgrp[i]; p < sgrp[i + 1]; ++p) {
-      const Z2 x = phi[(int64_t)sgci[p] * k + j];
-      acc.x += sgv[p] * x.x;
-      acc.y += sgv[p] * x.y;
+  for (int64_t p = op.xrp[i]; p < op.xrp[i + 1]; ++p) {
+    const double g = op.xv[p];
+    const C2<TZ> x = a[(int64_t)op.xci[p] * k + j];
+    xi.x += g * (double)x.x;
+    xi.y += g * (double)x.y;
+  }
+  if (exact) {  // rounding residue of S on the gradient parts (exact products, tiny)
+    for (int64_t p = op.sgrp[i]; p < op.sgrp[i + 1]; ++p) {
+      const C2<TZ> x = phi[(int64_t)op.sgci[p] * k + j];
+      acc.x += op.sgv[p] * (double)x.x;
+      acc.y += op.sgv[p] * (double)x.y;
     }
-    for (int64_t p = sxrp[i]; p < sxrp[i + 1]; ++p) {
-      const Z2 x = a[(int64_t)sxci[p] * k + j];
-      acc.x += sxv[p] * x.x;
-      acc.y += sxv[p] * x.y;
+    for (int64_t p = op.sxrp[i]; p < op.sxrp[i + 1]; ++p) {
+      const C2<TZ> x = a[(int64_t)op.sxci[p] * k + j];
+      acc.x += op.sxv[p] * (double)x.x;
+      acc.y += op.sxv[p] * (double)x.y;
     }
   }
   const double w = omega[j];
-  const double dr = reference_diag_re(sdiag[i], w, epsr[i]), di = w * sig[i];
+  const double dr = reference_diag_re(op.sdiag[i], w, op.epsr[i]), di = w * op.sig[i];
   acc.x += dr * xi.x - di * xi.y;
   acc.y += dr * xi.y + di * xi.x;
   if (mode == 1) { const Z2 bb = b[i * k + j]; acc.x = bb.x - acc.x; acc.y = bb.y - acc.y; }
   out[i * k + j] = acc;
 }
 
-// partial[t][blk][j] = sum over this block's rows of conj(V_t) * w   (k a power of two <= 256)
-__global__ void k_dots(int64_t n, int k, const Z2 *__restrict__ V, int64_t vstride, const Z2 *__restrict__ w,
-                       Z2 *__restrict__ partial) {
+// Multi-vector dots.  Block (blk, c) handles vectors t in [8c, 8c+8) of V
+// (stride vstride) against w over a grid-stride range of rows and writes
+// partial[(blk * nout + t) * k + j] = sum conj(V_t) w; with self, chunk 0 also
+// writes t = nvec: sum |w|^2.  nout = nvec + self.
+constexpr int kDotChunk = 8;
+__global__ void __launch_bounds__(256) k_mdots(int64_t n, int k, const Z2 *__restrict__ V, int64_t vstride, int nvec,
+                                               const Z2 *__restrict__ w, int self, Z2 *__restrict__ partial) {
   __shared__ Z2 sh[256];
-  const int t = blockIdx.y;
-  const Z2 *Vt = V + (int64_t)t * vstride;
+  const int c = blockIdx.y;
+  const int t0 = c * kDotChunk;
+  const int nt = min(kDotChunk, nvec - t0);
+  const int do_self = self && c == 0;
   const int rows_per = blockDim.x / k;
   const int j = threadIdx.x % k;
-  int64_t i = (int64_t)blockIdx.x * rows_per + threadIdx.x / k;
+  Z2 acc[kDotChunk + 1];
+#pragma unroll
+  for (int q = 0; q <= kDotChunk; ++q) acc[q] = {0., 0.};
   const int64_t step = (int64_t)gridDim.x * rows_per;
-  Z2 acc = {0., 0.};
-  for (; i < n; i += step) {
-    Z2 a = Vt[i * k + j], b = w[i * k + j];
-    acc.x += a.x * b.x + a.y * b.y;
-    acc.y += a.x * b.y - a.y * b.x;
+  for (int64_t i = (int64_t)blockIdx.x * rows_per + threadIdx.x / k; i < n; i += step) {
+    const Z2 b = w[i * k + j];
+#pragma unroll
+    for (int q = 0; q < kDotChunk; ++q)
+      if (q < nt) {
+        const Z2 a = V[(int64_t)(t0 + q) * vstride + i * k + j];
+        acc[q].x += a.x * b.x + a.y * b.y;
+        acc[q].y += a.x * b.y - a.y * b.x;
+      }
+    if (do_self) acc[kDotChunk].x += b.x * b.x + b.y * b.y;
   }
-  sh[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s >= k; s >>= 1) {
-    if ((int)threadIdx.x < s) { sh[threadIdx.x].x += sh[threadIdx.x + s].x; sh[threadIdx.x].y += sh[threadIdx.x + s].y; }
+  const int nout = nvec + self;
+  for (int q = 0; q <= kDotChunk; ++q) {
+    const bool live = q < nt || (q == kDotChunk && do_self);
+    if (!live) continue;  // uniform across the block
+    sh[threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int s = blockDim.x / 2; s >= k; s >>= 1) {
+      if ((int)threadIdx.x < s) { sh[threadIdx.x].x += sh[threadIdx.x + s].x; sh[threadIdx.x].y += sh[threadIdx.x + s].y; }
+      __syncthreads();
+    }
+    const int t = q < kDotChunk ? t0 + q : nvec;
+    if ((int)threadIdx.x < k) partial[((int64_t)blockIdx.x * nout + t) * k + threadIdx.x] = sh[threadIdx.x];
     __syncthreads();
   }
-  if ((int)threadIdx.x < k) partial[((int64_t)t * gridDim.x + blockIdx.x) * k + threadIdx.x] = sh[threadIdx.x];
 }
 
-__global__ void k_dots_final(int nvec, int nb, int k, const Z2 *__restrict__ partial, Z2 *__restrict__ h) {
+__global__ void k_mdots_final(int nout, int nb, int k, const Z2 *__restrict__ partial, Z2 *__restrict__ h) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nvec * k) return;
-  int t = q / k, j = q % k;
+  if (q >= nout * k) return;
   Z2 s = {0., 0.};
-  for (int b = 0; b < nb; ++b) { Z2 v = partial[((int64_t)t * nb + b) * k + j]; s.x += v.x; s.y += v.y; }
+  for (int b = 0; b < nb; ++b) { Z2 v = partial[(int64_t)b * nout * k + q]; s.x += v.x; s.y += v.y; }
   h[q] = s;
 }
 
-// w -= sum_t V_t h_t (per column)
+// w = (w - sum_{t<nvec} V_t h_t) * scale_j   (per column)
 __global__ void k_gs_update(int64_t n, int k, int nvec, const Z2 *__restrict__ V, int64_t vstride,
-                            const Z2 *__restrict__ h, Z2 *__restrict__ w) {
+                            const Z2 *__restrict__ h, const double *__restrict__ scale, Z2 *__restrict__ w) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n * k) return;
   int j = (int)(q % k);
@@ -152,20 +186,23 @@ __global__ void k_gs_update(int64_t n, int k, int nvec, const Z2 *__restrict__ V
     acc.x -= a.x * c.x - a.y * c.y;
     acc.y -= a.x * c.y + a.y * c.x;
   }
-  w[q] = acc;
+  const double sc = scale[j];
+  w[q] = {acc.x * sc, acc.y * sc};
 }
 
 // x += sum_t Z_t y_t (per column)
-__global__ void k_xupdate(int64_t n, int k, int nvec, const Z2 *__restrict__ Zb, int64_t zstride,
+template <class TZ>
+__global__ void k_xupdate(int64_t n, int k, int nvec, const C2<TZ> *__restrict__ Zb, int64_t zstride,
                           const Z2 *__restrict__ y, Z2 *__restrict__ x) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n * k) return;
   int j = (int)(q % k);
   Z2 acc = x[q];
   for (int t = 0; t < nvec; ++t) {
-    Z2 a = Zb[(int64_t)t * zstride + q], c = y[t * k + j];
-    acc.x += a.x * c.x - a.y * c.y;
-    acc.y += a.x * c.y + a.y * c.x;
+    const C2<TZ> a = Zb[(int64_t)t * zstride + q];
+    const Z2 c = y[t * k + j];
+    acc.x += (double)a.x * c.x - (double)a.y * c.y;
+    acc.y += (double)a.x * c.y + (double)a.y * c.x;
   }
   x[q] = acc;
 }
@@ -186,9 +223,16 @@ __global__ void k_weight_in(int64_t n, int k, const double *__restrict__ W, cons
 }
 
 template <class T>
-__global__ void k_widen(int64_t n, const C2<T> *__restrict__ in, Z2 *__restrict__ out) {
+__global__ void k_copy_t(int64_t n, const C2<T> *__restrict__ in, C2<T> *__restrict__ out) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < n) out[q] = {(double)in[q].x, (double)in[q].y};
+  if (q < n) out[q] = in[q];
+}
+
+// v[i*k + j] = 0 for every column j with mask[j]
+__global__ void k_zero_cols(int64_t n, int k, const int32_t *__restrict__ mask, Z2 *__restrict__ v) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n * k) return;
+  if (mask[q % k]) v[q] = {0., 0.};
 }
 
 // b[row*k + j] = (0, -((dir * f) * 2) * pi) at the port's edges (src/generateStiff.cpp:583-593)
@@ -199,7 +243,7 @@ __global__ void k_rhs(int64_t nent, int k, const int64_t *__restrict__ row, cons
   b[row[q] * k + col[q]] = {0., val[q]};
 }
 
-// x = u + G phi at selected rows (port edges) or at every row (idx == null)
+// x = u + G phi + Gx a at selected rows (idx) or at every row (idx == null)
 __global__ void k_assemble_x(int64_t nidx, const int64_t *__restrict__ idx, int k, const int64_t *__restrict__ grp,
                              const int32_t *__restrict__ gci, const double *__restrict__ gv,
                              const int64_t *__restrict__ xrp, const int32_t *__restrict__ xci,
@@ -221,7 +265,7 @@ __global__ void k_assemble_x(int64_t nidx, const int64_t *__restrict__ idx, int 
     x.x += xv[p] * f.x;
     x.y += xv[p] * f.y;
   }
-  out[q / k * k + j] = x;
+  out[r * k + j] = x;
 }
 
 struct DevBuf {
@@ -229,151 +273,297 @@ struct DevBuf {
   ~DevBuf() { A.free_all(); }
 };
 
-// Column-batched dot products V_t^H w for t < nvec; result h[t*k + j] on the host.
-gp_status dots(hipStream_t s, int64_t n, int k, const Z2 *V, int64_t vstride, int nvec, const Z2 *w, Z2 *partial,
-               Z2 *hd, std::vector<cd> &h, int nb) {
-  dim3 grid(nb, nvec);
-  k_dots<<<grid, 256, 0, s>>>(n, k, V, vstride, w, partial);
-  k_dots_final<<<(nvec * k + 255) / 256, 256, 0, s>>>(nvec, nb, k, partial, hd);
-  h.resize((size_t)nvec * k);
-  GP_HIP(hipMemcpyAsync(h.data(), hd, sizeof(Z2) * nvec * k, hipMemcpyDeviceToHost, s));
-  GP_HIP(hipStreamSynchronize(s));
-  return GP_OK;
-}
-
 int pow2_at_least(int x) { int p = 1; while (p < x) p <<= 1; return p; }
 
-struct Column { int32_t port; int32_t fslot; };
+OuterOp outer_op(gp_layout *lay, const Hierarchy &H) {
+  OuterOp op;
+  const CsrD &S = lay->S;
+  const Csr &G = H.L[0].G, &Gx = H.aux.Gx;
+  op.n = lay->n;
+  op.rp = S.rp; op.ci = S.ci; op.sv = S.v;
+  op.grp = G.rp; op.gci = G.ci; op.gv = G.v[0];
+  op.xrp = Gx.rp; op.xci = Gx.ci; op.xv = Gx.v[0];
+  op.sgrp = H.SG.rp; op.sgci = H.SG.ci; op.sgv = H.SG.v[0];
+  op.sxrp = H.SGx.rp; op.sxci = H.SGx.ci; op.sxv = H.SGx.v[0];
+  op.epsr = H.epsr; op.sig = H.sig; op.sdiag = H.sdiag;
+  return op;
+}
 
-// One batch: k columns (k a power of two; padded columns repeat the last real column).
-template <class T>
-gp_status solve_batch(gp_layout *lay, Solver *sv, const std::vector<Column> &cols, int kreal,
-                      const std::vector<double> &freqs, const gp_solve_opts &o, double *X, int64_t xcol0,
-                      std::vector<std::vector<cd>> &zcols, gp_solve_stats &st) {
-  const Hierarchy &H = sv->H;
-  hipStream_t s = lay->stream;
-  const int k = (int)cols.size();
-  const int64_t n = lay->n, nn = H.L[0].nn, na = H.aux.naux;
-  const int m = std::max(2, o.restart);
-  const int nu_edge = 2;
-  const gp_mesh &mesh = lay->hm.view;
-  // per-column frequencies and the batch's distinct frequency list
-  std::vector<double> omega_col(k), omegas;
-  std::vector<int32_t> fidx(k);
-  std::map<int32_t, int> slot;
-  for (int j = 0; j < k; ++j) {
-    int32_t fs = cols[j].fslot;
-    if (!slot.count(fs)) { slot[fs] = (int)omegas.size(); omegas.push_back(2. * kPi * freqs[fs]); }
-    fidx[j] = slot[fs];
-    omega_col[j] = 2. * kPi * freqs[fs];
+// Column-batched dot products on the device, copied to the host:
+// h[t*k + j] = V_t^H w for t < nvec, and h[nvec*k + j] = |w_j|^2 if self.
+struct Dots {
+  int nb = 512;
+  Z2 *partial = nullptr, *hd = nullptr;
+  gp_status init(Arena &A, int k, int maxv) {
+    GP_TRY(A.alloc(&partial, (int64_t)nb * (maxv + 1) * k));
+    GP_TRY(A.alloc(&hd, (int64_t)(maxv + 1) * k));
+    return GP_OK;
   }
+  gp_status run(hipStream_t s, int64_t n, int k, const Z2 *V, int64_t vstride, int nvec, const Z2 *w, int self,
+                std::vector<cd> &h) {
+    const int nout = nvec + self;
+    const int nch = std::max(1, (nvec + kDotChunk - 1) / kDotChunk);
+    k_mdots<<<dim3(nb, nch), 256, 0, s>>>(n, k, V, vstride, nvec, w, self, partial);
+    k_mdots_final<<<(nout * k + 255) / 256, 256, 0, s>>>(nout, nb, k, partial, hd);
+    std::vector<Z2> tmp((size_t)nout * k);
+    GP_HIP(hipMemcpyAsync(tmp.data(), hd, sizeof(Z2) * nout * k, hipMemcpyDeviceToHost, s));
+    GP_HIP(hipStreamSynchronize(s));
+    h.resize(tmp.size());
+    for (size_t q = 0; q < tmp.size(); ++q) h[q] = cd(tmp[q].x, tmp[q].y);
+    return GP_OK;
+  }
+};
+
+struct Column { int32_t port; int32_t fslot; int32_t id; };
+
+struct SolveCtx {
+  gp_layout *lay;
+  Solver *sv;
+  const gp_solve_opts *o;
+  const std::vector<double> *freqs;        // Hz, per frequency of the request
+  const std::vector<int32_t> *cslot;       // CoarseCache slot per request frequency
+  double *X;                               // optional full solutions, column id order
+  std::vector<std::vector<cd>> *xport;     // port-edge values per column id
+  gp_column_report *rep;                   // per column id
+  gp_solve_stats *st;
+  std::chrono::steady_clock::time_point t0;
+};
+
+// Port-edge rows of every port's first side, in Construct_Z_V0_Vh order.
+std::vector<int64_t> port_rows(gp_layout *lay) {
+  const gp_mesh &mesh = lay->hm.view;
+  std::vector<int64_t> map(lay->g.Ne), idx;
+  gp_edge_map(&mesh, map.data(), nullptr);
+  for (int32_t p = 0; p < mesh.num_ports; ++p) {
+    int32_t s0 = mesh.port_side_ptr[p];
+    for (int64_t q = mesh.side_edge_ptr[s0]; q < mesh.side_edge_ptr[s0 + 1]; ++q) idx.push_back(map[mesh.side_edges[q]]);
+  }
+  return idx;
+}
+
+// Solve the pending columns on k slots with preconditioner precision T.
+template <class T>
+gp_status solve_slots(SolveCtx &cx, std::deque<Column> &pending, int k) {
+  gp_layout *lay = cx.lay;
+  const Hierarchy &H = cx.sv->H;
+  const gp_solve_opts &o = *cx.o;
+  hipStream_t s = lay->stream;
+  const int64_t n = lay->n, nn = H.L[0].nn, na = H.aux.naux;
+  const int m = std::min(std::max(2, o.restart), 64);
+  const int nu_edge = o.smooth_edge > 0 ? std::min(o.smooth_edge, 8) : 2;
+  const gp_mesh &mesh = lay->hm.view;
+  const OuterOp op = outer_op(lay, H);
+  const Csr &G0 = H.L[0].G, &Gx = H.aux.Gx;
+
   VWork<T> vw;
-  gp_status stt = vwork_create<T>(lay, H, k, omega_col.data(), fidx.data(), omegas, &vw);
-  if (stt != GP_OK) { vwork_free(&vw); return stt; }
+  struct VwGuard { VWork<T> *w; ~VwGuard() { vwork_free(w); } } vg{&vw};
+  GP_TRY(vwork_create<T>(lay, H, k, &vw));
+  vw.nu_aux = o.smooth_aux > 0 ? std::min(o.smooth_aux, 8) : 2;
+  vw.kt = &lay->ktimer;
   DevBuf D;
   Arena &A = D.A;
-  Z2 *b, *xu, *xphi, *xa, *V, *Zu, *Zp, *Za, *w, *partial, *hd;
-  double *om_d, *inv_d;
-  const int nb = 512;
-  GP_TRY(A.alloc(&b, n * k)); GP_TRY(A.alloc(&xu, n * k)); GP_TRY(A.alloc(&xphi, nn * k));
+  Z2 *b, *xu, *xphi, *xa, *V, *yd;
+  C2<T> *Zu, *Zp, *Za;
+  double *om_d, *sc_d;
+  int32_t *mask_d;
+  GP_TRY(A.alloc(&b, n * k)); GP_TRY(A.alloc(&xu, n * k)); GP_TRY(A.alloc(&xphi, nn * k)); GP_TRY(A.alloc(&xa, na * k));
   GP_TRY(A.alloc(&V, (int64_t)(m + 1) * n * k));
-  GP_TRY(A.alloc(&Zu, (int64_t)m * n * k)); GP_TRY(A.alloc(&Zp, (int64_t)m * nn * k));
-  GP_TRY(A.alloc(&Za, (int64_t)m * na * k)); GP_TRY(A.alloc(&xa, na * k));
-  GP_TRY(A.alloc(&w, n * k)); GP_TRY(A.alloc(&partial, (int64_t)nb * (m + 2) * k));
-  GP_TRY(A.alloc(&hd, (int64_t)(m + 2) * k));
-  GP_TRY(A.alloc(&om_d, k)); GP_TRY(A.alloc(&inv_d, k));
-  GP_HIP(hipMemcpyAsync(om_d, omega_col.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+  GP_TRY(A.alloc(&Zu, (int64_t)m * n * k)); GP_TRY(A.alloc(&Zp, (int64_t)m * nn * k)); GP_TRY(A.alloc(&Za, (int64_t)m * na * k));
+  GP_TRY(A.alloc(&om_d, k)); GP_TRY(A.alloc(&sc_d, k)); GP_TRY(A.alloc(&mask_d, k)); GP_TRY(A.alloc(&yd, (int64_t)m * k));
+  Dots dots;
+  GP_TRY(dots.init(A, k, m + 1));
   GP_HIP(hipMemsetAsync(b, 0, sizeof(Z2) * n * k, s));
   GP_HIP(hipMemsetAsync(xu, 0, sizeof(Z2) * n * k, s));
   GP_HIP(hipMemsetAsync(xphi, 0, sizeof(Z2) * nn * k, s));
   GP_HIP(hipMemsetAsync(xa, 0, sizeof(Z2) * na * k, s));
-  {  // right-hand sides
+  std::vector<int64_t> prow = port_rows(lay), map(lay->g.Ne);
+  gp_edge_map(&mesh, map.data(), nullptr);
+  int64_t *prow_d;
+  Z2 *xo;
+  GP_TRY(A.alloc(&prow_d, (int64_t)prow.size())); GP_TRY(A.alloc(&xo, (int64_t)prow.size() * k));
+  GP_HIP(hipMemcpyAsync(prow_d, prow.data(), sizeof(int64_t) * prow.size(), hipMemcpyHostToDevice, s));
+  Z2 *xfull = nullptr;
+  if (cx.X) GP_TRY(A.alloc(&xfull, n * k));
+
+  const unsigned gnk = blocks_for(n * k), gnnk = blocks_for(nn * k), gnak = blocks_for(na * k);
+  // slot state
+  std::vector<int32_t> cid(k, -1), its(k, 0), fsl(k, 0);
+  std::vector<double> t_in(k, 0.);   // when each slot's column entered (ms since the call)
+  auto now_ms = [&]() { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - cx.t0).count(); };
+  std::vector<double> bnorm(k, 0.), omega(k, 2. * kPi * (*cx.freqs)[0]), beta(k, 0.);
+  std::vector<cd> Hm((size_t)(m + 1) * m * k), cs((size_t)m * k), sn((size_t)m * k), g((size_t)(m + 1) * k), y((size_t)m * k);
+  std::vector<double> sch(k);
+  std::vector<cd> h, h2;
+  std::vector<Z2> yh((size_t)m * k);
+  int cycles = 0;
+
+  // place pending columns into the slots listed in mask (idle when none is left)
+  auto refill = [&](const std::vector<int32_t> &mask) -> gp_status {
     std::vector<int64_t> row;
     std::vector<int32_t> col;
     std::vector<double> val;
-    std::vector<int64_t> map(lay->g.Ne);
-    gp_edge_map(&mesh, map.data(), nullptr);
     for (int j = 0; j < k; ++j) {
-      const int32_t p = cols[j].port;
-      const double f = freqs[cols[j].fslot];
-      for (int32_t sd = mesh.port_side_ptr[p]; sd < mesh.port_side_ptr[p + 1]; ++sd)
+      if (!mask[j]) continue;
+      if (pending.empty()) { cid[j] = -1; bnorm[j] = 0.; continue; }
+      Column c = pending.front();
+      pending.pop_front();
+      cid[j] = c.id; its[j] = 0; t_in[j] = now_ms();
+      const double f = (*cx.freqs)[c.fslot];
+      omega[j] = 2. * kPi * f;
+      fsl[j] = (*cx.cslot)[c.fslot];
+      double b2 = 0.;
+      for (int32_t sd = mesh.port_side_ptr[c.port]; sd < mesh.port_side_ptr[c.port + 1]; ++sd)
         for (int64_t q = mesh.side_edge_ptr[sd]; q < mesh.side_edge_ptr[sd + 1]; ++q) {
+          const double v = 0. - (double)mesh.side_dir[sd] * f * 2. * kPi;
           row.push_back(map[mesh.side_edges[q]]);
           col.push_back(j);
-          val.push_back(0. - (double)mesh.side_dir[sd] * f * 2. * kPi);
+          val.push_back(v);
+          b2 += v * v;
         }
+      bnorm[j] = std::sqrt(b2);
     }
-    int64_t *drow; int32_t *dcol; double *dval;
-    GP_TRY(A.alloc(&drow, (int64_t)row.size())); GP_TRY(A.alloc(&dcol, (int64_t)row.size())); GP_TRY(A.alloc(&dval, (int64_t)row.size()));
-    GP_HIP(hipMemcpyAsync(drow, row.data(), sizeof(int64_t) * row.size(), hipMemcpyHostToDevice, s));
-    GP_HIP(hipMemcpyAsync(dcol, col.data(), sizeof(int32_t) * row.size(), hipMemcpyHostToDevice, s));
-    GP_HIP(hipMemcpyAsync(dval, val.data(), sizeof(double) * row.size(), hipMemcpyHostToDevice, s));
-    k_rhs<<<blocks_for((int64_t)row.size()), 256, 0, s>>>((int64_t)row.size(), k, drow, dcol, dval, b);
-  }
-  const MgLevel &L0 = H.L[0];
-  const Csr &Gx = H.aux.Gx;
-  const CsrD &S = lay->S;
-  const unsigned gnk = blocks_for(n * k), gnnk = blocks_for(nn * k);
-  std::vector<cd> h, bnorm2;
-  GP_TRY(dots(s, n, k, b, 0, 1, b, partial, hd, bnorm2, nb));
-  std::vector<double> bnorm(k), beta(k);
-  for (int j = 0; j < k; ++j) bnorm[j] = std::sqrt(bnorm2[j].real());
-  // Arnoldi state per column
-  std::vector<cd> Hm((size_t)(m + 1) * m * k), cs((size_t)m * k), sn((size_t)m * k), g((size_t)(m + 1) * k), y((size_t)m * k);
-  std::vector<double> invh(k);
-  int iters = 0;
-  bool done = false;
-  double relres_max = 0;
-  while (!done) {
-    // explicit FP64 residual of the split iterate
-    // the restart residual uses the rounded S exactly (S G, S Gx included), so the
-    // refinement converges to the solution of the reference's FP64 system
-    k_outer<<<gnk, 256, 0, s>>>(n, k, S.rp, S.ci, S.v, L0.G.rp, L0.G.ci, L0.G.v[0], Gx.rp, Gx.ci, Gx.v[0], H.epsr, H.sig, H.sdiag,
-                                om_d, xu, xphi, xa, b, V, 1, 1, H.SG.rp, H.SG.ci, H.SG.v[0], H.SGx.rp, H.SGx.ci,
-                                H.SGx.v[0]);
-    GP_TRY(dots(s, n, k, V, 0, 1, V, partial, hd, h, nb));
-    bool all = true;
-    relres_max = 0;
+    GP_HIP(hipMemcpyAsync(mask_d, mask.data(), sizeof(int32_t) * k, hipMemcpyHostToDevice, s));
+    k_zero_cols<<<gnk, 256, 0, s>>>(n, k, mask_d, b);
+    k_zero_cols<<<gnk, 256, 0, s>>>(n, k, mask_d, xu);
+    k_zero_cols<<<gnnk, 256, 0, s>>>(nn, k, mask_d, xphi);
+    k_zero_cols<<<gnak, 256, 0, s>>>(na, k, mask_d, xa);
+    if (!row.empty()) {
+      DevBuf R;
+      int64_t *drow; int32_t *dcol; double *dval;
+      GP_TRY(R.A.alloc(&drow, (int64_t)row.size())); GP_TRY(R.A.alloc(&dcol, (int64_t)row.size()));
+      GP_TRY(R.A.alloc(&dval, (int64_t)row.size()));
+      GP_HIP(hipMemcpyAsync(drow, row.data(), sizeof(int64_t) * row.size(), hipMemcpyHostToDevice, s));
+      GP_HIP(hipMemcpyAsync(dcol, col.data(), sizeof(int32_t) * row.size(), hipMemcpyHostToDevice, s));
+      GP_HIP(hipMemcpyAsync(dval, val.data(), sizeof(double) * row.size(), hipMemcpyHostToDevice, s));
+      k_rhs<<<blocks_for((int64_t)row.size()), 256, 0, s>>>((int64_t)row.size(), k, drow, dcol, dval, b);
+      GP_HIP(hipStreamSynchronize(s));
+    }
+    GP_HIP(hipMemcpyAsync(om_d, omega.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+    GP_TRY(vwork_set_columns<T>(vw, omega.data(), fsl.data(), cx.sv->cc, s));
+    return GP_OK;
+  };
+
+  // hand finished slots (mask) to the caller: port-edge values, optional full x, reports
+  auto finish = [&](const std::vector<int32_t> &mask, const std::vector<double> &relres) -> gp_status {
+    k_assemble_x<<<blocks_for((int64_t)prow.size() * k), 256, 0, s>>>((int64_t)prow.size(), prow_d, k, G0.rp, G0.ci,
+                                                                      G0.v[0], Gx.rp, Gx.ci, Gx.v[0], xu, xphi, xa, xo);
+    std::vector<Z2> xh(prow.size() * k);
+    GP_HIP(hipMemcpyAsync(xh.data(), xo, sizeof(Z2) * xh.size(), hipMemcpyDeviceToHost, s));
+    std::vector<Z2> xf;
+    if (cx.X) {
+      k_assemble_x<<<gnk, 256, 0, s>>>(n, nullptr, k, G0.rp, G0.ci, G0.v[0], Gx.rp, Gx.ci, Gx.v[0], xu, xphi, xa, xfull);
+      xf.resize(n * k);
+      GP_HIP(hipMemcpyAsync(xf.data(), xfull, sizeof(Z2) * n * k, hipMemcpyDeviceToHost, s));
+    }
+    GP_HIP(hipStreamSynchronize(s));
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - cx.t0).count();
     for (int j = 0; j < k; ++j) {
-      beta[j] = std::sqrt(h[j].real());
-      double rr = bnorm[j] > 0 ? beta[j] / bnorm[j] : 0.;
-      if (j < kreal) relres_max = std::max(relres_max, rr);
-      if (rr > o.rtol) all = false;
+      if (!mask[j] || cid[j] < 0) continue;
+      const int id = cid[j];
+      auto &xp = (*cx.xport)[id];
+      xp.resize(prow.size());
+      for (size_t a = 0; a < prow.size(); ++a) xp[a] = cd(xh[a * k + j].x, xh[a * k + j].y);
+      if (cx.X)
+        for (int64_t i = 0; i < n; ++i) {
+          cx.X[2 * ((int64_t)id * n + i)] = xf[i * k + j].x;
+          cx.X[2 * ((int64_t)id * n + i) + 1] = xf[i * k + j].y;
+        }
+      const bool ok = relres[j] <= o.rtol;
+      cx.rep[id] = gp_column_report{its[j], ok ? 1 : 0, relres[j], t_in[j], ms};
+      cx.st->columns += 1;
+      cx.st->converged += ok ? 1 : 0;
+      cx.st->max_iters_used = std::max(cx.st->max_iters_used, its[j]);
+      cx.st->max_rel_residual = std::max(cx.st->max_rel_residual, relres[j]);
     }
-    if (all || iters >= o.max_iters) break;
-    for (int j = 0; j < k; ++j) invh[j] = beta[j] > 0 ? 1.0 / beta[j] : 0.0;
-    GP_HIP(hipMemcpyAsync(inv_d, invh.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-    k_scale_cols<<<gnk, 256, 0, s>>>(n, k, V, inv_d, V);
+    return GP_OK;
+  };
+
+  {  // initial fill
+    std::vector<int32_t> all(k, 1);
+    GP_TRY(refill(all));
+  }
+  for (;;) {
+    // explicit FP64 residual of the split iterate (S G, S Gx included exactly)
+    k_outer<double><<<gnk, 256, 0, s>>>(op, k, om_d, xu, xphi, xa, b, V, 1, 1);
+    GP_TRY(dots.run(s, n, k, V, 0, 0, V, 1, h));
+    std::vector<double> relres(k, 0.);
+    std::vector<int32_t> done(k, 0);
+    int ndone = 0, nactive = 0;
+    for (int j = 0; j < k; ++j) {
+      beta[j] = std::sqrt(std::max(h[j].real(), 0.0));
+      if (cid[j] < 0) continue;
+      relres[j] = bnorm[j] > 0 ? beta[j] / bnorm[j] : 0.;
+      if (relres[j] <= o.rtol || its[j] >= o.max_iters) { done[j] = 1; ++ndone; }
+      else ++nactive;
+    }
+    if (ndone) {
+      GP_TRY(finish(done, relres));
+      // refill the finished slots (slots left without a column carry zero vectors)
+      GP_TRY(refill(done));
+      k_zero_cols<<<gnk, 256, 0, s>>>(n, k, mask_d, V);
+      k_outer<double><<<gnk, 256, 0, s>>>(op, k, om_d, xu, xphi, xa, b, V, 1, 1);
+      GP_TRY(dots.run(s, n, k, V, 0, 0, V, 1, h));
+      nactive = 0;
+      for (int j = 0; j < k; ++j) {
+        beta[j] = cid[j] >= 0 ? std::sqrt(std::max(h[j].real(), 0.0)) : 0.;
+        if (cid[j] >= 0) ++nactive;
+      }
+    }
+    if (nactive == 0) break;
+    ++cycles;
+    for (int j = 0; j < k; ++j) sch[j] = (cid[j] >= 0 && beta[j] > 0) ? 1.0 / beta[j] : 0.0;
+    GP_HIP(hipMemcpyAsync(sc_d, sch.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+    k_scale_cols<<<gnk, 256, 0, s>>>(n, k, V, sc_d, V);
     std::fill(g.begin(), g.end(), cd(0));
-    for (int j = 0; j < k; ++j) g[j] = beta[j];
+    for (int j = 0; j < k; ++j) g[j] = (cid[j] >= 0) ? beta[j] : 0.;
     int used = 0;
-    for (int i = 0; i < m && iters < o.max_iters; ++i) {
-      Z2 *Vi = V + (int64_t)i * n * k;
-      // preconditioner: (u, phi) = Vcycle(W v)
+    for (int i = 0; i < m; ++i) {
+      Z2 *Vi = V + (int64_t)i * n * k, *Vn = V + (int64_t)(i + 1) * n * k;
+      // preconditioner: (u, phi, a) = M(W v)
       k_weight_in<T><<<gnk, 256, 0, s>>>(n, k, H.W, Vi, vw.r[0]);
       GP_TRY(precond<T>(H, vw, s, nu_edge, nu_edge));
-      Z2 *zu = Zu + (int64_t)i * n * k, *zp = Zp + (int64_t)i * nn * k, *za = Za + (int64_t)i * na * k;
-      k_widen<T><<<gnk, 256, 0, s>>>(n * k, vw.u[0], zu);
-      k_widen<T><<<gnnk, 256, 0, s>>>(nn * k, vw.phi[0], zp);
-      k_widen<T><<<blocks_for(na * k), 256, 0, s>>>(na * k, vw.a, za);
-      // w = A z (split form, FP64)
-      k_outer<<<gnk, 256, 0, s>>>(n, k, S.rp, S.ci, S.v, L0.G.rp, L0.G.ci, L0.G.v[0], Gx.rp, Gx.ci, Gx.v[0], H.epsr, H.sig, H.sdiag,
-                                  om_d, zu, zp, za, nullptr, w, 0, 1);
-      // classical Gram-Schmidt with one re-orthogonalization pass
-      GP_TRY(dots(s, n, k, V, n * k, i + 1, w, partial, hd, h, nb));
-      k_gs_update<<<gnk, 256, 0, s>>>(n, k, i + 1, V, n * k, hd, w);
-      std::vector<cd> h2;
-      GP_TRY(dots(s, n, k, V, n * k, i + 1, w, partial, hd, h2, nb));
-      k_gs_update<<<gnk, 256, 0, s>>>(n, k, i + 1, V, n * k, hd, w);
-      std::vector<cd> nrm;
-      GP_TRY(dots(s, n, k, w, 0, 1, w, partial, hd, nrm, nb));
+      C2<T> *zu = Zu + (int64_t)i * n * k, *zp = Zp + (int64_t)i * nn * k, *za = Za + (int64_t)i * na * k;
+      k_copy_t<T><<<gnk, 256, 0, s>>>(n * k, vw.u[0], zu);
+      k_copy_t<T><<<gnnk, 256, 0, s>>>(nn * k, vw.phi[0], zp);
+      k_copy_t<T><<<gnak, 256, 0, s>>>(na * k, vw.a, za);
+      // w = A z (split form, FP64) written into the next basis slot
+      k_outer<T><<<gnk, 256, 0, s>>>(op, k, om_d, zu, zp, za, nullptr, Vn, 0, 0);
+      // classical Gram-Schmidt with the norm from the same pass; a second pass
+      // only when the first one cancelled more than half of |w|^2
+      GP_TRY(dots.run(s, n, k, V, n * k, i + 1, Vn, 1, h));
+      bool reorth = false;
+      std::vector<double> rn2(k);
+      for (int j = 0; j < k; ++j) {
+        double hh = 0.;
+        for (int t = 0; t <= i; ++t) hh += std::norm(h[t * k + j]);
+        const double wn2 = h[(i + 1) * k + j].real();
+        rn2[j] = wn2 - hh;
+        if (cid[j] >= 0 && wn2 > 0 && rn2[j] < 0.5 * wn2) reorth = true;
+      }
+      h2.assign((size_t)(i + 1) * k, cd(0));
+      if (reorth) {
+        std::fill(sch.begin(), sch.end(), 1.0);
+        GP_HIP(hipMemcpyAsync(sc_d, sch.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+        k_gs_update<<<gnk, 256, 0, s>>>(n, k, i + 1, V, n * k, dots.hd, sc_d, Vn);
+        GP_TRY(dots.run(s, n, k, V, n * k, i + 1, Vn, 1, h2));
+        for (int j = 0; j < k; ++j) {
+          double hh = 0.;
+          for (int t = 0; t <= i; ++t) hh += std::norm(h2[t * k + j]);
+          rn2[j] = h2[(i + 1) * k + j].real() - hh;
+        }
+      }
       bool all_in = true;
       for (int j = 0; j < k; ++j) {
         cd *Hc = &Hm[(size_t)j * (m + 1) * m];
-        for (int t = 0; t <= i; ++t) Hc[t * m + i] = h[t * k + j] + h2[t * k + j];
-        double hn = std::sqrt(std::max(nrm[j].real(), 0.0));
+        const double hn = std::sqrt(std::max(rn2[j], 0.0));
+        sch[j] = (cid[j] >= 0 && hn > 0) ? 1.0 / hn : 0.0;
+        if (cid[j] < 0) {
+          for (int t = 0; t <= i + 1; ++t) Hc[t * m + i] = 0.;
+          cs[i * k + j] = 1.; sn[i * k + j] = 0.;
+          continue;
+        }
+        for (int t = 0; t <= i; ++t) Hc[t * m + i] = h[t * k + j] + (reorth ? h2[t * k + j] : cd(0));
         Hc[(i + 1) * m + i] = hn;
-        invh[j] = hn > 0 ? 1.0 / hn : 0.0;
         // apply previous rotations, then form the new one
         for (int t = 0; t < i; ++t) {
           cd c = cs[t * k + j], sn_ = sn[t * k + j];
@@ -391,13 +581,22 @@ gp_status solve_batch(gp_layout *lay, Solver *sv, const std::vector<Column> &col
         cd gi = g[i * k + j];
         g[i * k + j] = std::conj(c) * gi;
         g[(i + 1) * k + j] = -sn_ * gi;
-        if (bnorm[j] > 0 && std::abs(g[(i + 1) * k + j]) > o.rtol * bnorm[j]) all_in = false;
+        ++its[j];
+        if (bnorm[j] > 0 && std::abs(g[(i + 1) * k + j]) > o.rtol * bnorm[j] && its[j] < o.max_iters) all_in = false;
       }
-      GP_HIP(hipMemcpyAsync(inv_d, invh.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-      k_scale_cols<<<gnk, 256, 0, s>>>(n, k, w, inv_d, V + (int64_t)(i + 1) * n * k);
-      ++iters;
+      // V_{i+1} = (w - V h) / |w - V h|, with the coefficients of the last pass
+      // (dots.hd holds them: the second pass overwrote the first when it ran)
+      GP_HIP(hipMemcpyAsync(sc_d, sch.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+      k_gs_update<<<gnk, 256, 0, s>>>(n, k, i + 1, V, n * k, dots.hd, sc_d, Vn);
       used = i + 1;
       if (all_in) break;
+      // with columns waiting, restart once half of the slots are done
+      if (!pending.empty()) {
+        int conv = 0;
+        for (int j = 0; j < k; ++j)
+          if (cid[j] < 0 || std::abs(g[(i + 1) * k + j]) <= o.rtol * bnorm[j] || its[j] >= o.max_iters) ++conv;
+        if (2 * conv >= k) break;
+      }
     }
     // y = R^-1 g per column, then x += Z y
     for (int j = 0; j < k; ++j) {
@@ -405,62 +604,61 @@ gp_status solve_batch(gp_layout *lay, Solver *sv, const std::vector<Column> &col
       for (int t = used - 1; t >= 0; --t) {
         cd acc = g[t * k + j];
         for (int q = t + 1; q < used; ++q) acc -= Hc[t * m + q] * y[q * k + j];
-        y[t * k + j] = Hc[t * m + t] != cd(0) ? acc / Hc[t * m + t] : cd(0);
+        y[t * k + j] = (cid[j] >= 0 && Hc[t * m + t] != cd(0)) ? acc / Hc[t * m + t] : cd(0);
       }
     }
-    std::vector<Z2> yh((size_t)used * k);
     for (int t = 0; t < used; ++t)
       for (int j = 0; j < k; ++j) yh[t * k + j] = {y[t * k + j].real(), y[t * k + j].imag()};
-    GP_HIP(hipMemcpyAsync(hd, yh.data(), sizeof(Z2) * used * k, hipMemcpyHostToDevice, s));
-    k_xupdate<<<gnk, 256, 0, s>>>(n, k, used, Zu, n * k, hd, xu);
-    k_xupdate<<<gnnk, 256, 0, s>>>(nn, k, used, Zp, nn * k, hd, xphi);
-    k_xupdate<<<blocks_for(na * k), 256, 0, s>>>(na, k, used, Za, na * k, hd, xa);
-    if (o.verbose) std::fprintf(stderr, "[gp] batch k=%d restart after %d its, est %.3e\n", k, iters, relres_max);
-  }
-  // outputs: Z from the port edges; optional full solution vectors
-  {
-    std::vector<int64_t> idx;
-    std::vector<int64_t> map(lay->g.Ne);
-    gp_edge_map(&mesh, map.data(), nullptr);
-    for (int32_t p = 0; p < mesh.num_ports; ++p) {
-      int32_t s0 = mesh.port_side_ptr[p];
-      for (int64_t q = mesh.side_edge_ptr[s0]; q < mesh.side_edge_ptr[s0 + 1]; ++q) idx.push_back(map[mesh.side_edges[q]]);
-    }
-    int64_t *didx; Z2 *xo;
-    GP_TRY(A.alloc(&didx, (int64_t)idx.size())); GP_TRY(A.alloc(&xo, (int64_t)idx.size() * k));
-    GP_HIP(hipMemcpyAsync(didx, idx.data(), sizeof(int64_t) * idx.size(), hipMemcpyHostToDevice, s));
-    k_assemble_x<<<blocks_for((int64_t)idx.size() * k), 256, 0, s>>>((int64_t)idx.size(), didx, k, L0.G.rp, L0.G.ci,
-                                                                      L0.G.v[0], Gx.rp, Gx.ci, Gx.v[0], xu, xphi, xa, xo);
-    std::vector<Z2> xh(idx.size() * k);
-    GP_HIP(hipMemcpyAsync(xh.data(), xo, sizeof(Z2) * xh.size(), hipMemcpyDeviceToHost, s));
-    GP_HIP(hipStreamSynchronize(s));
-    zcols.assign(kreal, {});
-    for (int j = 0; j < kreal; ++j) {
-      zcols[j].resize(idx.size());
-      for (size_t a = 0; a < idx.size(); ++a) zcols[j][a] = cd(xh[a * k + j].x, xh[a * k + j].y);
-    }
-    if (X) {
-      Z2 *xfull;
-      GP_TRY(A.alloc(&xfull, n * k));
-      k_assemble_x<<<gnk, 256, 0, s>>>(n, nullptr, k, L0.G.rp, L0.G.ci, L0.G.v[0], Gx.rp, Gx.ci, Gx.v[0], xu, xphi, xa, xfull);
-      std::vector<Z2> xf(n * k);
-      GP_HIP(hipMemcpyAsync(xf.data(), xfull, sizeof(Z2) * n * k, hipMemcpyDeviceToHost, s));
-      GP_HIP(hipStreamSynchronize(s));
-      for (int j = 0; j < kreal; ++j)
-        for (int64_t i = 0; i < n; ++i) {
-          X[2 * ((xcol0 + j) * n + i)] = xf[i * k + j].x;
-          X[2 * ((xcol0 + j) * n + i) + 1] = xf[i * k + j].y;
-        }
+    GP_HIP(hipMemcpyAsync(yd, yh.data(), sizeof(Z2) * used * k, hipMemcpyHostToDevice, s));
+    k_xupdate<T><<<gnk, 256, 0, s>>>(n, k, used, Zu, n * k, yd, xu);
+    k_xupdate<T><<<gnnk, 256, 0, s>>>(nn, k, used, Zp, nn * k, yd, xphi);
+    k_xupdate<T><<<gnak, 256, 0, s>>>(na, k, used, Za, na * k, yd, xa);
+    if (o.verbose) {
+      double est = 0.;
+      for (int j = 0; j < k; ++j)
+        if (cid[j] >= 0 && bnorm[j] > 0) est = std::max(est, std::abs(g[used * k + j]) / bnorm[j]);
+      std::fprintf(stderr, "[gp] k=%d cycle %d: %d steps, active %d, pending %zu, est %.3e\n", k, cycles, used,
+                   nactive, pending.size(), est);
     }
   }
   GP_HIP(hipGetLastError());
-  vwork_free(&vw);
-  st.max_iters_used = std::max(st.max_iters_used, iters);
-  st.max_rel_residual = std::max(st.max_rel_residual, relres_max);
-  st.batches += 1;
-  st.columns += kreal;
-  if (relres_max <= o.rtol) st.converged += kreal;
+  cx.st->batches += cycles;
   return GP_OK;
+}
+
+// Z for one finished column (Construct_Z_V0_Vh on the first side of every port,
+// src/fdtd.hpp:4133-4170): Z[f][resp + np*src] = -sum x*leng*dir / area(src)
+void column_z(gp_layout *lay, const Column &c, const std::vector<cd> &xp, double *Z) {
+  const gp_mesh &mesh = lay->hm.view;
+  const int32_t npt = mesh.num_ports, src = c.port, f = c.fslot;
+  double cur = 0.;
+  for (int32_t sd = mesh.port_side_ptr[src]; sd < mesh.port_side_ptr[src + 1]; ++sd) cur += mesh.side_area[sd];
+  size_t a = 0;
+  for (int32_t p = 0; p < npt; ++p) {
+    const int32_t s0 = mesh.port_side_ptr[p];
+    cd z(0., 0.);
+    for (int64_t q = mesh.side_edge_ptr[s0]; q < mesh.side_edge_ptr[s0 + 1]; ++q, ++a) {
+      int dir; int64_t iz, ix, iy;
+      lay->g.decode(mesh.side_edges[q], dir, iz, ix, iy);
+      double leng = dir == 2 ? mesh.zn[iz + 1] - mesh.zn[iz] : dir == 1 ? mesh.xn[ix + 1] - mesh.xn[ix] : mesh.yn[iy + 1] - mesh.yn[iy];
+      z -= xp[a] * leng * (mesh.side_dir[s0] * 1.0);
+    }
+    z /= cur;
+    size_t zi = (size_t)f * npt * npt + p + (size_t)npt * src;
+    Z[2 * zi] = z.real();
+    Z[2 * zi + 1] = z.imag();
+  }
+}
+
+// Device bytes one slot of the batched solve needs (basis, preconditioned
+// basis, iterate, V-cycle vectors), for the choice of the slot count.
+int64_t bytes_per_slot(const Hierarchy &H, int m, bool f32) {
+  const int64_t n = H.L[0].ne, nn = H.L[0].nn, na = H.aux.naux;
+  const int64_t tz = f32 ? 8 : 16;
+  int64_t b = (int64_t)(m + 1) * n * 16 + (int64_t)m * (n + nn + na) * tz + (n * 2 + nn + na) * 16;
+  for (const MgLevel &L : H.L) b += (5 * L.ne + 3 * L.nn) * tz;
+  for (const AuxLevel &L : H.aux.L) b += 3 * L.n * tz;
+  return b + (n + na) * tz;
 }
 
 }  // namespace
@@ -476,6 +674,8 @@ extern "C" void gp_solve_opts_default(gp_solve_opts *o) {
   o->precision = GP_PREC_MIXED;
   o->batch = 0;
   o->verbose = 0;
+  o->smooth_edge = 0;
+  o->smooth_aux = 0;
 }
 
 extern "C" gp_status gp_prepare_solver(gp_layout *lay, int32_t precision) {
@@ -491,11 +691,32 @@ extern "C" gp_status gp_prepare_solver(gp_layout *lay, int32_t precision) {
   return GP_OK;
 }
 
-extern "C" gp_status gp_solve_fullwave(gp_layout *lay, const double *freqs, int32_t nf, const int32_t *ports, int32_t np,
-                                       const gp_solve_opts *opts, double *X, double *Z, gp_solve_stats *stats) {
+extern "C" gp_status gp_hierarchy_info_get(gp_layout *lay, gp_hierarchy_info *out) {
+  if (!lay || !out) return fail(GP_ERR_ARG, "null argument");
+  if (!lay->solver) return fail(GP_ERR_STATE, "solver not prepared");
+  const Hierarchy &H = lay->solver->H;
+  *out = gp_hierarchy_info{};
+  out->edge_levels = (int32_t)std::min<size_t>(H.L.size(), GP_MAX_LEVELS);
+  for (int l = 0; l < out->edge_levels; ++l) {
+    out->ne[l] = H.L[l].ne; out->nn[l] = H.L[l].nn; out->nnz_k[l] = H.L[l].K.nnz;
+  }
+  if (!H.L.empty() && H.L.size() <= GP_MAX_LEVELS) out->nnz_k[H.L.size() - 1] = H.neL * H.neL;
+  out->aux_levels = (int32_t)std::min<size_t>(H.aux.L.size(), GP_MAX_LEVELS);
+  for (int l = 0; l < out->aux_levels; ++l) {
+    out->na[l] = H.aux.L[l].n; out->nnz_n[l] = H.aux.L[l].N.nnz; out->nlong[l] = H.aux.L[l].N.nlong;
+  }
+  out->naux = H.aux.naux;
+  out->setup_ms = H.setup_ms;
+  return GP_OK;
+}
+
+extern "C" gp_status gp_solve_fullwave_ex(gp_layout *lay, const double *freqs, int32_t nf, const int32_t *ports,
+                                          int32_t np, const gp_solve_opts *opts, double *X, double *Z,
+                                          gp_solve_stats *stats, gp_column_report *reports) {
   if (!lay || !freqs || nf < 1 || !ports || np < 1) return fail(GP_ERR_ARG, "bad solve arguments");
   gp_solve_opts o;
   if (opts) o = *opts; else gp_solve_opts_default(&o);
+  if (!(o.rtol > 0) || o.max_iters < 1) return fail(GP_ERR_ARG, "rtol and max_iters must be positive");
   const gp_mesh &mesh = lay->hm.view;
   for (int32_t a = 0; a < np; ++a)
     if (ports[a] < 0 || ports[a] >= mesh.num_ports) return fail(GP_ERR_ARG, "port index out of range");
@@ -503,53 +724,50 @@ extern "C" gp_status gp_solve_fullwave(gp_layout *lay, const double *freqs, int3
     if (!(freqs[f] > 0)) return fail(GP_ERR_ARG, "frequencies must be positive");
   auto t0 = std::chrono::steady_clock::now();
   GP_TRY(gp_prepare_solver(lay, o.precision));
-  auto t1 = std::chrono::steady_clock::now();
   Solver *sv = lay->solver;
+  const bool f32 = sv->f32 && o.precision == GP_PREC_MIXED;
+  // coarsest-level inverses for every requested frequency
+  std::vector<double> fr(freqs, freqs + nf), om(nf);
+  for (int32_t f = 0; f < nf; ++f) om[f] = 2. * kPi * fr[f];
+  std::vector<int32_t> cslot;
+  GP_TRY(coarse_cache_lookup(sv->H, sv->cc, om, &cslot, lay->stream));
+  auto t1 = std::chrono::steady_clock::now();
   gp_solve_stats st{};
-  // columns ordered frequency-major: c = f*np + p
-  std::vector<Column> all;
+  // columns in request order, frequency-major: id = f*np + p
+  const int ncol = nf * np;
+  std::deque<Column> pending;
   for (int32_t f = 0; f < nf; ++f)
-    for (int32_t a = 0; a < np; ++a) all.push_back({ports[a], f});
-  int kb = o.batch > 0 ? o.batch : (o.precision == GP_PREC_MIXED ? 16 : 1);
+    for (int32_t a = 0; a < np; ++a) pending.push_back({ports[a], f, f * np + a});
+  // slot count: requested, else 16 for the mixed path and 1 for plain FP64,
+  // bounded by the columns and by half of the free device memory
+  const int m = std::min(std::max(2, o.restart), 64);
+  int kb = o.batch > 0 ? o.batch : (f32 ? 16 : 1);
   kb = std::min(kb, 64);
-  const int32_t npt = mesh.num_ports;
-  std::vector<double> zbuf(Z ? 0 : 0);
-  if (Z) std::fill(Z, Z + 2 * (size_t)nf * npt * npt, 0.0);
-  std::vector<double> fr(freqs, freqs + nf);
-  for (size_t c0 = 0; c0 < all.size(); c0 += kb) {
-    int kreal = (int)std::min<size_t>(kb, all.size() - c0);
-    int k = pow2_at_least(kreal);
-    std::vector<Column> cols(all.begin() + c0, all.begin() + c0 + kreal);
-    while ((int)cols.size() < k) cols.push_back(cols.back());
-    std::vector<std::vector<cd>> xport;
-    gp_status s2 = sv->f32 && o.precision == GP_PREC_MIXED
-                       ? solve_batch<float>(lay, sv, cols, kreal, fr, o, X, (int64_t)c0, xport, st)
-                       : solve_batch<double>(lay, sv, cols, kreal, fr, o, X, (int64_t)c0, xport, st);
-    if (s2 != GP_OK) return s2;
-    if (Z) {
-      // Construct_Z_V0_Vh on the first side of every port
-      for (int j = 0; j < kreal; ++j) {
-        const int32_t src = cols[j].port, f = cols[j].fslot;
-        double cur = 0.;
-        for (int32_t sd = mesh.port_side_ptr[src]; sd < mesh.port_side_ptr[src + 1]; ++sd) cur += mesh.side_area[sd];
-        size_t a = 0;
-        for (int32_t p = 0; p < npt; ++p) {
-          const int32_t s0 = mesh.port_side_ptr[p];
-          cd z(0., 0.);
-          for (int64_t q = mesh.side_edge_ptr[s0]; q < mesh.side_edge_ptr[s0 + 1]; ++q, ++a) {
-            int dir; int64_t iz, ix, iy;
-            lay->g.decode(mesh.side_edges[q], dir, iz, ix, iy);
-            double leng = dir == 2 ? mesh.zn[iz + 1] - mesh.zn[iz] : dir == 1 ? mesh.xn[ix + 1] - mesh.xn[ix] : mesh.yn[iy + 1] - mesh.yn[iy];
-            z -= xport[j][a] * leng * (mesh.side_dir[s0] * 1.0);
-          }
-          z /= cur;
-          size_t zi = (size_t)f * npt * npt + p + (size_t)npt * src;
-          Z[2 * zi] = z.real();
-          Z[2 * zi + 1] = z.imag();
-        }
-      }
+  kb = std::min(kb, pow2_at_least(ncol));
+  {
+    size_t fre = 0, tot = 0;
+    if (hipMemGetInfo(&fre, &tot) == hipSuccess) {
+      const int64_t per = bytes_per_slot(sv->H, m, f32);
+      while (kb > 1 && (int64_t)kb * per > (int64_t)(fre / 2)) kb /= 2;
     }
   }
+  const int k = pow2_at_least(kb);
+  std::vector<std::vector<cd>> xport(ncol);
+  std::vector<gp_column_report> rep(ncol, gp_column_report{0, 0, 0., 0., 0.});
+  SolveCtx cx{lay, sv, &o, &fr, &cslot, X, &xport, rep.data(), &st, t0};
+  gp_status s2 = f32 ? solve_slots<float>(cx, pending, k) : solve_slots<double>(cx, pending, k);
+  if (s2 != GP_OK) return s2;
+  const int32_t npt = mesh.num_ports;
+  if (Z) {
+    std::fill(Z, Z + 2 * (size_t)nf * npt * npt, 0.0);
+    for (int32_t f = 0; f < nf; ++f)
+      for (int32_t a = 0; a < np; ++a) {
+        const int id = f * np + a;
+        if (!xport[id].empty()) column_z(lay, Column{ports[a], f, id}, xport[id], Z);
+      }
+  }
+  if (reports)
+    for (int id = 0; id < ncol; ++id) reports[id] = rep[id];
   auto t2 = std::chrono::steady_clock::now();
   st.setup_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   st.solve_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
@@ -557,36 +775,70 @@ extern "C" gp_status gp_solve_fullwave(gp_layout *lay, const double *freqs, int3
   return st.converged == st.columns ? GP_OK : fail(GP_ERR_NOCONV, "some columns stopped above the tolerance");
 }
 
+extern "C" gp_status gp_solve_fullwave(gp_layout *lay, const double *freqs, int32_t nf, const int32_t *ports, int32_t np,
+                                       const gp_solve_opts *opts, double *X, double *Z, gp_solve_stats *stats) {
+  return gp_solve_fullwave_ex(lay, freqs, nf, ports, np, opts, X, Z, stats, nullptr);
+}
+
 extern "C" gp_status gp_solve_real(const int64_t *, const int64_t *, const double *, int64_t, int64_t, const double *,
                                    double *, double, int32_t *, double *) {
   return fail(GP_ERR_STATE, "real nodal solve not built yet");
+}
+
+extern "C" gp_status gp_layout_sync(gp_layout *lay) {
+  if (!lay) return fail(GP_ERR_ARG, "null layout");
+  GP_HIP(hipStreamSynchronize(lay->stream));
+  return GP_OK;
+}
+
+extern "C" gp_status gp_kernel_timer(gp_layout *lay, int32_t enable) {
+  if (!lay) return fail(GP_ERR_ARG, "null layout");
+  GP_HIP(hipStreamSynchronize(lay->stream));
+  lay->ktimer.on = enable != 0;
+  if (enable) lay->ktimer.reset();
+  return GP_OK;
+}
+
+extern "C" gp_status gp_kernel_timer_read(gp_layout *lay, int64_t *launches, double *total_ms, double *total_bytes) {
+  if (!lay || !launches || !total_ms || !total_bytes) return fail(GP_ERR_ARG, "null argument");
+  GP_HIP(hipStreamSynchronize(lay->stream));
+  const KTimer &t = lay->ktimer;
+  double ms = 0, b = 0;
+  for (size_t q = 0; q < t.used; ++q) {
+    float e = 0;
+    GP_HIP(hipEventElapsedTime(&e, t.ev[2 * q], t.ev[2 * q + 1]));
+    ms += e;
+    b += t.bytes[q];
+  }
+  *launches = (int64_t)t.used;
+  *total_ms = ms;
+  *total_bytes = b;
+  return GP_OK;
 }
 
 extern "C" gp_status gp_bench_spmv(gp_layout *lay, int32_t k, int32_t precision, int32_t iters, double *avg_ms) {
   if (!lay || k < 1 || iters < 1 || !avg_ms) return fail(GP_ERR_ARG, "bad arguments");
   GP_TRY(gp_prepare_solver(lay, GP_PREC_FP64));
   Solver *sv = lay->solver;
-  const int64_t n = lay->n;
+  const int64_t n = lay->n, nn = sv->H.L[0].nn, na = sv->H.aux.naux;
   hipStream_t s = lay->stream;
   DevBuf D;
-  Z2 *x, *y;
+  Z2 *x, *p, *a, *y;
   double *om;
   GP_TRY(D.A.alloc(&x, n * k)); GP_TRY(D.A.alloc(&y, n * k)); GP_TRY(D.A.alloc(&om, k));
+  GP_TRY(D.A.alloc(&p, nn * k)); GP_TRY(D.A.alloc(&a, na * k));
   std::vector<double> omh(k, 2 * kPi * 1e9);
   GP_HIP(hipMemcpyAsync(om, omh.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
   GP_HIP(hipMemsetAsync(x, 0, sizeof(Z2) * n * k, s));
-  const CsrD &S = lay->S;
-  const MgLevel &L0 = sv->H.L[0];
+  GP_HIP(hipMemsetAsync(p, 0, sizeof(Z2) * nn * k, s));
+  GP_HIP(hipMemsetAsync(a, 0, sizeof(Z2) * na * k, s));
   (void)precision;
+  const OuterOp op = outer_op(lay, sv->H);
   hipEvent_t e0, e1;
   GP_HIP(hipEventCreate(&e0)); GP_HIP(hipEventCreate(&e1));
-  const Csr &Gx = sv->H.aux.Gx;
-  k_outer<<<blocks_for(n * k), 256, 0, s>>>(n, k, S.rp, S.ci, S.v, L0.G.rp, L0.G.ci, L0.G.v[0], Gx.rp, Gx.ci, Gx.v[0],
-                                            sv->H.epsr, sv->H.sig, sv->H.sdiag, om, x, nullptr, nullptr, nullptr, y, 0, 0);
+  k_outer<double><<<blocks_for(n * k), 256, 0, s>>>(op, k, om, x, p, a, nullptr, y, 0, 0);
   GP_HIP(hipEventRecord(e0, s));
-  for (int it = 0; it < iters; ++it)
-    k_outer<<<blocks_for(n * k), 256, 0, s>>>(n, k, S.rp, S.ci, S.v, L0.G.rp, L0.G.ci, L0.G.v[0], Gx.rp, Gx.ci, Gx.v[0],
-                                              sv->H.epsr, sv->H.sig, sv->H.sdiag, om, x, nullptr, nullptr, nullptr, y, 0, 0);
+  for (int it = 0; it < iters; ++it) k_outer<double><<<blocks_for(n * k), 256, 0, s>>>(op, k, om, x, p, a, nullptr, y, 0, 0);
   GP_HIP(hipEventRecord(e1, s));
   GP_HIP(hipEventSynchronize(e1));
   float ms = 0;

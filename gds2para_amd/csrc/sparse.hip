@@ -377,11 +377,49 @@ __global__ void k_d2f(int64_t n, const double *__restrict__ a, float *__restrict
 gp_status csr_to_float(Arena &A, const Csr &a, CsrF *f, hipStream_t s) {
   *f = CsrF();
   f->nr = a.nr; f->nc = a.nc; f->nnz = a.nnz; f->rp = a.rp; f->ci = a.ci; f->nv = a.nv;
+  f->nlong = a.nlong; f->lrows = a.lrows; f->lidx = a.lidx;
+  f->nchunk = a.nchunk; f->lcb = a.lcb; f->lce = a.lce; f->lqp = a.lqp;
   for (int k = 0; k < a.nv; ++k) {
     GP_TRY(A.alloc(&f->v[k], std::max<int64_t>(a.nnz, 1)));
     if (a.nnz) k_d2f<<<blocks_for(a.nnz), 256, 0, s>>>(a.nnz, a.v[k], f->v[k]);
   }
   GP_HIP(hipStreamSynchronize(s));
+  return GP_OK;
+}
+
+gp_status csr_mark_long(Arena &A, Csr &a, hipStream_t s) {
+  a.nlong = 0; a.lrows = nullptr; a.lidx = nullptr;
+  a.nchunk = 0; a.lcb = a.lce = nullptr; a.lqp = nullptr;
+  if (a.nr == 0) return GP_OK;
+  std::vector<int64_t> rp(a.nr + 1);
+  GP_HIP(hipMemcpyAsync(rp.data(), a.rp, sizeof(int64_t) * (a.nr + 1), hipMemcpyDeviceToHost, s));
+  GP_HIP(hipStreamSynchronize(s));
+  std::vector<int32_t> rows, idx(a.nr, -1), qp(1, 0);
+  std::vector<int64_t> cb, ce;
+  for (int64_t i = 0; i < a.nr; ++i)
+    if (rp[i + 1] - rp[i] > kLongRow) {
+      idx[i] = (int32_t)rows.size();
+      rows.push_back((int32_t)i);
+      for (int64_t p = rp[i]; p < rp[i + 1]; p += kLongChunk) {
+        cb.push_back(p);
+        ce.push_back(std::min(p + kLongChunk, rp[i + 1]));
+      }
+      qp.push_back((int32_t)cb.size());
+    }
+  if (rows.empty()) return GP_OK;
+  GP_TRY(A.alloc(&a.lrows, (int64_t)rows.size()));
+  GP_TRY(A.alloc(&a.lidx, a.nr));
+  GP_TRY(A.alloc(&a.lcb, (int64_t)cb.size()));
+  GP_TRY(A.alloc(&a.lce, (int64_t)ce.size()));
+  GP_TRY(A.alloc(&a.lqp, (int64_t)qp.size()));
+  GP_HIP(hipMemcpyAsync(a.lrows, rows.data(), sizeof(int32_t) * rows.size(), hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemcpyAsync(a.lidx, idx.data(), sizeof(int32_t) * a.nr, hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemcpyAsync(a.lcb, cb.data(), sizeof(int64_t) * cb.size(), hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemcpyAsync(a.lce, ce.data(), sizeof(int64_t) * ce.size(), hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemcpyAsync(a.lqp, qp.data(), sizeof(int32_t) * qp.size(), hipMemcpyHostToDevice, s));
+  GP_HIP(hipStreamSynchronize(s));
+  a.nlong = (int32_t)rows.size();
+  a.nchunk = (int32_t)cb.size();
   return GP_OK;
 }
 

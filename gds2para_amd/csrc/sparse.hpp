@@ -10,12 +10,26 @@
 
 namespace gp {
 
+// Rows with more than kLongRow entries (the per-conductor columns of the V0
+// basis couple to a whole conductor surface) are listed in lrows; lidx[i] is
+// the row's position in that list or -1.  Row-parallel kernels take those rows'
+// products from a block-per-row reduction instead of walking them serially.
+constexpr int64_t kLongRow = 256;
+constexpr int64_t kLongChunk = 2048;  // entries per block of the long-row reduction
+
 struct Csr {
   int64_t nr = 0, nc = 0, nnz = 0;
   int64_t *rp = nullptr;
   int32_t *ci = nullptr;
   int nv = 0;
   double *v[3] = {nullptr, nullptr, nullptr};
+  int32_t nlong = 0;
+  int32_t *lrows = nullptr, *lidx = nullptr;
+  // the long rows cut into chunks of kLongChunk entries: chunk c spans
+  // [lcb[c], lce[c]); long row q owns chunks lqp[q] .. lqp[q+1]-1
+  int32_t nchunk = 0;
+  int64_t *lcb = nullptr, *lce = nullptr;
+  int32_t *lqp = nullptr;
 };
 
 // FP32 copy of a Csr's values; row pointers and columns are shared.
@@ -25,6 +39,11 @@ struct CsrF {
   const int32_t *ci = nullptr;
   int nv = 0;
   float *v[3] = {nullptr, nullptr, nullptr};
+  int32_t nlong = 0;
+  const int32_t *lrows = nullptr, *lidx = nullptr;
+  int32_t nchunk = 0;
+  const int64_t *lcb = nullptr, *lce = nullptr;
+  const int32_t *lqp = nullptr;
 };
 
 gp_status csr_upload(Arena &A, const HostCsr &h, Csr *out, hipStream_t s);
@@ -39,7 +58,9 @@ gp_status spgemm_ex(Arena &A, const Csr &a, const Csr &b, Csr *c, hipStream_t s,
 gp_status transpose(Arena &A, const Csr &a, Csr *t, hipStream_t s);
 // copy of a single-valued CSR without its explicit zeros
 gp_status csr_drop_zeros(Arena &A, const Csr &a, Csr *out, hipStream_t s);
-// FP32 copy of every value array
+// fill nlong / lrows / lidx (see kLongRow)
+gp_status csr_mark_long(Arena &A, Csr &a, hipStream_t s);
+// FP32 copy of every value array (long-row lists shared)
 gp_status csr_to_float(Arena &A, const Csr &a, CsrF *f, hipStream_t s);
 // CSR from COO triplets (duplicates summed), nv value arrays
 gp_status csr_from_coo(Arena &A, int64_t nr, int64_t nc, int64_t n, const int64_t *rows, const int64_t *cols,

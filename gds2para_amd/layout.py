@@ -97,8 +97,15 @@ class Layout:
     def prepare(self, precision=capi.GP_PREC_MIXED):
         capi.check(self.lib.gp_prepare_solver(self.h, int(precision)), "gp_prepare_solver")
 
+    def hierarchy_info(self):
+        h = capi.GpHierarchyInfo()
+        capi.check(self.lib.gp_hierarchy_info_get(self.h, C.byref(h)), "gp_hierarchy_info_get")
+        el, al = h.edge_levels, h.aux_levels
+        return dict(edge=[(h.ne[i], h.nn[i], h.nnz_k[i]) for i in range(el)],
+                    aux=[(h.na[i], h.nnz_n[i], h.nlong[i]) for i in range(al)], naux=h.naux, setup_ms=h.setup_ms)
+
     def solve(self, freqs, ports=None, precision=capi.GP_PREC_MIXED, rtol=1e-8, batch=0, restart=30,
-              max_iters=2000, want_x=False, verbose=0):
+              max_iters=2000, want_x=False, verbose=0, smooth_edge=0, smooth_aux=0):
         """Returns (Z[nf, np*np] complex, stats, X or None)."""
         ports = list(range(self.mesh.num_ports)) if ports is None else list(ports)
         freqs = np.ascontiguousarray(np.asarray(freqs, np.float64))
@@ -107,11 +114,13 @@ class Layout:
         o = capi.GpSolveOpts()
         self.lib.gp_solve_opts_default(C.byref(o))
         o.rtol, o.precision, o.batch, o.restart, o.max_iters, o.verbose = rtol, precision, batch, restart, max_iters, verbose
+        o.smooth_edge, o.smooth_aux = smooth_edge, smooth_aux
         Z = np.zeros(2 * nf * npt * npt)
         X = np.zeros(2 * self.mesh.n * nf * npp) if want_x else None
         st = capi.GpSolveStats()
-        code = self.lib.gp_solve_fullwave(self.h, _p(freqs, f64p), nf, _p(pr, i32p), npp, C.byref(o),
-                                          _p(X, f64p) if want_x else None, _p(Z, f64p), C.byref(st))
+        reps = (capi.GpColumnReport * (nf * npp))()
+        code = self.lib.gp_solve_fullwave_ex(self.h, _p(freqs, f64p), nf, _p(pr, i32p), npp, C.byref(o),
+                                             _p(X, f64p) if want_x else None, _p(Z, f64p), C.byref(st), reps)
         if code not in (capi.GP_OK, capi.GP_ERR_NOCONV):
             capi.check(code, "gp_solve_fullwave")
         Zc = (Z[0::2] + 1j * Z[1::2]).reshape(nf, npt * npt)
@@ -120,8 +129,21 @@ class Layout:
             Xc = (X[0::2] + 1j * X[1::2]).reshape(nf * npp, self.mesh.n)
         stats = dict(columns=st.columns, converged=st.converged, max_iters=st.max_iters_used, batches=st.batches,
                      max_rel_residual=st.max_rel_residual, setup_ms=st.setup_ms, solve_ms=st.solve_ms,
-                     status=code)
+                     status=code,
+                     iters=[r.iters for r in reps], rel_residual=[r.rel_residual for r in reps],
+                     start_ms=[r.start_ms for r in reps], done_ms=[r.done_ms for r in reps])
         return Zc, stats, Xc
+
+    def sync(self):
+        capi.check(self.lib.gp_layout_sync(self.h), "gp_layout_sync")
+
+    def kernel_timer(self, enable=True):
+        capi.check(self.lib.gp_kernel_timer(self.h, 1 if enable else 0), "gp_kernel_timer")
+
+    def kernel_timer_read(self):
+        n, ms, b = C.c_int64(), C.c_double(), C.c_double()
+        capi.check(self.lib.gp_kernel_timer_read(self.h, C.byref(n), C.byref(ms), C.byref(b)), "gp_kernel_timer_read")
+        return n.value, ms.value, b.value
 
     def bench_spmv(self, k=1, precision=capi.GP_PREC_FP64, iters=20):
         ms = C.c_double()

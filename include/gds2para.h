@@ -196,6 +196,8 @@ typedef struct gp_solve_opts {
   int32_t precision;              /* gp_precision                                 */
   int32_t batch;                  /* columns per batch; 0 = autotuned, 1 = single */
   int32_t verbose;
+  int32_t smooth_edge;            /* edge/nodal smoothing sweeps per level (0: 2) */
+  int32_t smooth_aux;             /* auxiliary AMG sweeps per level (0: 2)       */
 } gp_solve_opts;
 
 typedef struct gp_solve_stats {
@@ -213,6 +215,19 @@ void gp_solve_opts_default(gp_solve_opts *opts);
 /* Builds (or re-uses) the multigrid hierarchy for the layout. */
 gp_status gp_prepare_solver(gp_layout *lay, int32_t precision);
 
+/* Shape of the prepared hierarchy (no counterpart in the reference, whose
+ * BoomerAMG setup is opaque; reported for diagnostics and benchmarks). */
+#define GP_MAX_LEVELS 16
+typedef struct {
+  int32_t edge_levels;            /* geometric edge levels incl. the coarsest     */
+  int64_t ne[GP_MAX_LEVELS], nn[GP_MAX_LEVELS], nnz_k[GP_MAX_LEVELS];
+  int32_t aux_levels;             /* auxiliary [V0d|V0c] AMG levels               */
+  int64_t na[GP_MAX_LEVELS], nnz_n[GP_MAX_LEVELS], nlong[GP_MAX_LEVELS];
+  int64_t naux;                   /* columns of [V0d|V0c]                          */
+  double setup_ms;
+} gp_hierarchy_info;
+gp_status gp_hierarchy_info_get(gp_layout *lay, gp_hierarchy_info *out);
+
 /* Solves (S + diag(-w^2 eps + i w sigma)) x = -i w J for every (port, freq)
  * pair, exactly the system reference() hands to PARDISO.  freqs in Hz.
  * X (optional): complex interleaved (re,im), column c = f*np + p, n rows each.
@@ -222,6 +237,26 @@ gp_status gp_solve_fullwave(gp_layout *lay, const double *freqs, int32_t nf,
                             const int32_t *ports, int32_t np,
                             const gp_solve_opts *opts,
                             double *X, double *Z, gp_solve_stats *stats);
+
+/* Per-column outcome of a solve (no counterpart in the reference, which solves
+ * every column with the same direct factorization). */
+typedef struct gp_column_report {
+  int32_t iters;                  /* Krylov iterations spent on the column       */
+  int32_t converged;              /* 1 if the true residual reached rtol         */
+  double rel_residual;            /* final FP64 ||b - A x|| / ||b||               */
+  double start_ms;                /* when the column took a device slot          */
+  double done_ms;                 /* completion time; both since the call start  */
+} gp_column_report;
+
+/* gp_solve_fullwave plus per-column reports (nf*np entries, column f*np + p,
+ * may be NULL).  Columns are scheduled onto opts->batch device slots; a slot
+ * whose column converges is refilled with the next pending column at the next
+ * FGMRES restart (continuous batching), so every slot stays busy. */
+gp_status gp_solve_fullwave_ex(gp_layout *lay, const double *freqs, int32_t nf,
+                               const int32_t *ports, int32_t np,
+                               const gp_solve_opts *opts,
+                               double *X, double *Z, gp_solve_stats *stats,
+                               gp_column_report *reports);
 
 /* Construct_Z_V0_Vh for one solution vector x (n complex, renumbered space):
  * accumulates column src of the np*np block Zk (complex interleaved). Host only. */
@@ -235,6 +270,17 @@ gp_status gp_solve_real(const int64_t *row, const int64_t *col, const double *va
                         int32_t *iters_out, double *relres_out);
 
 /* ---- measurement hooks -------------------------------------------------- */
+/* Waits for all work the layout queued on its stream. */
+gp_status gp_layout_sync(gp_layout *lay);
+
+/* Times every launch of the level-0 edge smoother (the solve's dominant
+ * kernel) with HIP events on the layout stream while enabled; enable = 1
+ * also clears the record.  read returns the launch count, the summed device
+ * time and the summed algorithmic bytes (each matrix and vector byte the
+ * launch must move, counted once). */
+gp_status gp_kernel_timer(gp_layout *lay, int32_t enable);
+gp_status gp_kernel_timer_read(gp_layout *lay, int64_t *launches, double *total_ms, double *total_bytes);
+
 /* Average device time (ms, HIP events on the library stream) of one complex
  * A(w)*X product with k right-hand sides.  precision selects FP64 CSR or the
  * FP32 level-0 operator. */
