@@ -93,6 +93,12 @@ def main():
     def step():
         return lay.solve(freqs, **kw)
 
+    # plain-copy reference point (the kernel the HBM counters are scaled by)
+    import ctypes as C
+    cms = C.c_double()
+    capi.check(lay.lib.gp_bench_copy(1 << 30, 5, C.byref(cms)), "gp_bench_copy")
+    copy_gbs = 2.0 * (1 << 30) / (cms.value * 1e-3) / 1e9
+
     for w in range(a.warmup):
         print(f"[bench] warmup step {w + 1}/{a.warmup}", file=sys.stderr, flush=True)
         step()
@@ -116,19 +122,31 @@ def main():
     value = a.steps * ncol / elapsed
 
     # plain FP64 single-RHS path on a sample of the columns (port 0, spread frequencies)
-    fidx = np.linspace(0, len(freqs) - 1, max(1, a.fp64_sample)).round().astype(int)
-    fs = freqs[fidx]
-    lay.prepare(capi.GP_PREC_FP64)
-    t = time.perf_counter()
-    Zd, std, _ = lay.solve(fs, ports=[0], precision=capi.GP_PREC_FP64, rtol=1e-8, batch=1, restart=a.restart,
-                           max_iters=a.max_iters)
-    lay.sync()
-    t64 = time.perf_counter() - t
-    fp64_rate = len(fs) / t64
-    npt = m.num_ports
-    zm = Zm[fidx].reshape(len(fs), npt, npt)[:, 0, :]   # column of source port 0: index resp + np*0
-    zd = Zd.reshape(len(fs), npt, npt)[:, 0, :]
-    zdiff = float(np.abs(zm - zd).max() / np.abs(zd).max())
+    fp64 = None
+    zdiff = None
+    if a.fp64_sample > 0:
+        fidx = np.linspace(0, len(freqs) - 1, a.fp64_sample).round().astype(int)
+        fs = freqs[fidx]
+        lay.prepare(capi.GP_PREC_FP64)
+        t = time.perf_counter()
+        Zd, std, _ = lay.solve(fs, ports=[0], precision=capi.GP_PREC_FP64, rtol=1e-8, batch=1, restart=a.restart,
+                               max_iters=a.max_iters)
+        lay.sync()
+        t64 = time.perf_counter() - t
+        npt = m.num_ports
+        zm = Zm[fidx].reshape(len(fs), npt, npt)[:, 0, :]   # column of source port 0: index resp + np*0
+        zd = Zd.reshape(len(fs), npt, npt)[:, 0, :]
+        zdiff = float(np.abs(zm - zd).max() / np.abs(zd).max())
+        fp64 = {"value": len(fs) / t64, "unit": "solves/s", "columns": len(fs),
+                "freqs_hz": [float(f) for f in fs], "port": 0, "iters": std["iters"],
+                "converged": f"{std['converged']}/{len(fs)}"}
+    # HBM traffic of the dominant kernel, from the committed counter profile of this command
+    traffic, traffic_src = None, None
+    tp = os.path.join(ROOT, "profiles", "edge0_traffic.json")
+    if os.path.exists(tp):
+        with open(tp) as f:
+            tj = json.load(f)
+        traffic, traffic_src = tj.get("bytes_per_launch"), "profiles/edge0_traffic.json"
 
     line = {
         "metric": METRIC,
@@ -151,16 +169,15 @@ def main():
         "iters": {"p50": float(np.percentile(iters, 50)), "max": int(max(iters))},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "setup_s": setup_s,
-        "fp64_single": {"value": fp64_rate, "unit": "solves/s", "columns": len(fs),
-                        "freqs_hz": [float(f) for f in fs], "port": 0, "iters": std["iters"],
-                        "converged": f"{std['converged']}/{len(fs)}"},
-        "speedup_vs_fp64_single": value / fp64_rate,
+        "fp64_single": fp64,
+        "speedup_vs_fp64_single": value / fp64["value"] if fp64 else None,
         "z_rel_diff_mixed_vs_fp64": zdiff,
+        "copy_probe_GBs": copy_gbs,
         "roofline": {"bound": "hbm", "kernel": "level-0 edge smoother (k_edge<float>)",
                      "achieved": kbytes / (kms * 1e-3) / 1e9 if kms > 0 else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s",
                      "frac": (kbytes / (kms * 1e-3) / 1e9) / HBM_PEAK_GBS if kms > 0 else None,
-                     "traffic": None, "launches": launches,
+                     "traffic": traffic, "traffic_source": traffic_src, "launches": launches,
                      "avg_launch_us": kms / launches * 1e3 if launches else None,
                      "bytes_per_launch": kbytes / launches if launches else None},
     }

@@ -126,6 +126,7 @@ SIGNATURES = [
     ("gp_solve_real", i32, [P(i64), P(i64), P(f64), i64, i64, P(f64), P(f64), f64, P(i32), P(f64)]),
     ("gp_bench_spmv", i32, [C.c_void_p, i32, i32, i32, P(f64)]),
     ("gp_layout_sync", i32, [C.c_void_p]),
+    ("gp_bench_copy", i32, [i64, i32, P(f64)]),
     ("gp_kernel_timer", i32, [C.c_void_p, i32]),
     ("gp_kernel_timer_read", i32, [C.c_void_p, P(i64), P(f64), P(f64)]),
 ]

@@ -847,3 +847,33 @@ extern "C" gp_status gp_bench_spmv(gp_layout *lay, int32_t k, int32_t precision,
   (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
   return GP_OK;
 }
+
+namespace gp {
+// plain vector copy, 16 bytes per thread
+__global__ void gp_copy_probe(int64_t n, const double2 *__restrict__ a, double2 *__restrict__ b) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+}  // namespace gp
+
+extern "C" gp_status gp_bench_copy(int64_t bytes, int32_t iters, double *avg_ms) {
+  if (bytes < 16 || iters < 1 || !avg_ms) return fail(GP_ERR_ARG, "bad arguments");
+  GP_TRY(ensure_device());
+  const int64_t n = bytes / 16;
+  DevBuf D;
+  double2 *a, *b;
+  GP_TRY(D.A.alloc(&a, n)); GP_TRY(D.A.alloc(&b, n));
+  GP_HIP(hipMemset(a, 0, sizeof(double2) * n));
+  hipEvent_t e0, e1;
+  GP_HIP(hipEventCreate(&e0)); GP_HIP(hipEventCreate(&e1));
+  gp_copy_probe<<<blocks_for(n), 256>>>(n, a, b);
+  GP_HIP(hipEventRecord(e0, nullptr));
+  for (int it = 0; it < iters; ++it) gp_copy_probe<<<blocks_for(n), 256>>>(n, a, b);
+  GP_HIP(hipEventRecord(e1, nullptr));
+  GP_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  GP_HIP(hipEventElapsedTime(&ms, e0, e1));
+  *avg_ms = ms / iters;
+  (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+  return GP_OK;
+}

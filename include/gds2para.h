@@ -286,6 +286,11 @@ gp_status gp_kernel_timer_read(gp_layout *lay, int64_t *launches, double *total_
  * FP32 level-0 operator. */
 gp_status gp_bench_spmv(gp_layout *lay, int32_t k, int32_t precision, int32_t iters, double *avg_ms);
 
+/* Average device time of a plain copy of `bytes` bytes (one read and one write
+ * of each byte) by the kernel gp_copy_probe: the reference point that scales
+ * the FETCH_SIZE / WRITE_SIZE counters into HBM bytes. */
+gp_status gp_bench_copy(int64_t bytes, int32_t iters, double *avg_ms);
+
 #ifdef __cplusplus
 }
 #endif
