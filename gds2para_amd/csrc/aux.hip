@@ -405,10 +405,27 @@ gp_status build_aux(gp_layout *lay, Hierarchy *H, bool want_f32) {
       GP_TRY(csr_mark_long(A, L.N, s));
       if (L.P.nr) { GP_TRY(csr_mark_long(A, L.P, s)); GP_TRY(csr_mark_long(A, L.PT, s)); }
     }
+    // SELL copies for the cycle kernels (long rows stay with the long-row pass)
+    GP_TRY(sell_from_csr(A, X.GxT, &X.GxTs, s, "X.GxT"));
+    GP_TRY(sell_from_csr(A, X.Bx, &X.Bxs, s, "X.Bx"));
+    for (size_t l = 0; l + 1 < X.L.size(); ++l) {
+      AuxLevel &L = X.L[l];
+      GP_TRY(sell_from_csr(A, L.N, &L.Ns, s, "L.N"));
+      GP_TRY(sell_from_csr(A, L.P, &L.Ps, s, "L.P"));
+      GP_TRY(sell_from_csr(A, L.PT, &L.PTs, s, "L.PT"));
+    }
     if (want_f32) {
       GP_TRY(csr_to_float(A, X.Gx, &X.Gxf, s));
       GP_TRY(csr_to_float(A, X.GxT, &X.GxTf, s));
       GP_TRY(csr_to_float(A, X.Bx, &X.Bxf, s));
+      GP_TRY(sell_to_float(A, X.GxTs, &X.GxTsf, s));
+      GP_TRY(sell_to_float(A, X.Bxs, &X.Bxsf, s));
+      for (size_t l = 0; l + 1 < X.L.size(); ++l) {
+        AuxLevel &L = X.L[l];
+        GP_TRY(sell_to_float(A, L.Ns, &L.Nsf, s));
+        GP_TRY(sell_to_float(A, L.Ps, &L.Psf, s));
+        GP_TRY(sell_to_float(A, L.PTs, &L.PTsf, s));
+      }
       for (AuxLevel &L : X.L) {
         GP_TRY(csr_to_float(A, L.N, &L.Nf, s));
         if (L.P.nr) { GP_TRY(csr_to_float(A, L.P, &L.Pf, s)); GP_TRY(csr_to_float(A, L.PT, &L.PTf, s)); }

@@ -205,10 +205,40 @@ gp_status build_hierarchy(gp_layout *lay, Hierarchy *H, bool want_f32, int max_l
     GP_TRY(smoother_data(A, L.K, L.kd, L.kl, s));
     GP_TRY(smoother_data(A, L.N, L.nd, L.nl, s));
   }
+  if (H->L.size() > 1) {
+    // level 0: K_eps and K_sig are diagonal; keep them as per-row arrays so the
+    // smoother streams one value per entry (the coarse levels are built already)
+    MgLevel &L0 = H->L[0];
+    GP_TRY(A.alloc(&L0.K.dg[0], n)); GP_TRY(A.alloc(&L0.K.dg[1], n));
+    k_diag<<<blocks_for(n), 256, 0, s>>>(n, L0.K.rp, L0.K.ci, L0.K.v[1], L0.K.dg[0]);
+    k_diag<<<blocks_for(n), 256, 0, s>>>(n, L0.K.rp, L0.K.ci, L0.K.v[2], L0.K.dg[1]);
+    GP_HIP(hipStreamSynchronize(s));
+    A.release(L0.K.v[1], L0.K.nnz);
+    A.release(L0.K.v[2], L0.K.nnz);
+    L0.K.nv = 1;
+  }
+  for (size_t l = 0; l + 1 < H->L.size(); ++l) {  // the coarsest level is solved densely
+    MgLevel &L = H->L[l];
+    GP_TRY(sell_from_csr(A, L.K, &L.Ks, s, "L.K"));
+    GP_TRY(sell_from_csr(A, L.B, &L.Bs, s, "L.B"));
+    GP_TRY(sell_from_csr(A, L.N, &L.Ns, s, "L.N"));
+    GP_TRY(sell_from_csr(A, L.GT, &L.GTs, s, "L.GT"));
+    GP_TRY(sell_from_csr(A, L.P, &L.Ps, s, "L.P"));
+    GP_TRY(sell_from_csr(A, L.PT, &L.PTs, s, "L.PT"));
+    GP_TRY(sell_from_csr(A, L.Pn, &L.Pns, s, "L.Pn"));
+  }
   if (want_f32) {
     GP_TRY(to_f32(A, H->W, n, &H->Wf, s));
     for (MgLevel &L : H->L) {
       GP_TRY(csr_to_float(A, L.K, &L.Kf, s));
+      for (int q = 0; q < 2; ++q)
+        if (L.K.dg[q]) GP_TRY(to_f32(A, L.K.dg[q], L.ne, &L.Kf.dg[q], s));
+      if (L.Ks.nr) {
+        GP_TRY(sell_to_float(A, L.Ks, &L.Ksf, s)); GP_TRY(sell_to_float(A, L.Bs, &L.Bsf, s));
+        GP_TRY(sell_to_float(A, L.Ns, &L.Nsf, s)); GP_TRY(sell_to_float(A, L.GTs, &L.GTsf, s));
+        GP_TRY(sell_to_float(A, L.Ps, &L.Psf, s)); GP_TRY(sell_to_float(A, L.PTs, &L.PTsf, s));
+        GP_TRY(sell_to_float(A, L.Pns, &L.Pnsf, s));
+      }
       GP_TRY(csr_to_float(A, L.B, &L.Bf, s));
       GP_TRY(csr_to_float(A, L.N, &L.Nf, s));
       GP_TRY(csr_to_float(A, L.G, &L.Gf, s));
@@ -260,6 +290,7 @@ gp_status build_hierarchy(gp_layout *lay, Hierarchy *H, bool want_f32, int max_l
     Sv.nr = lay->S.n; Sv.nc = lay->S.n; Sv.nnz = lay->S.nnz; Sv.rp = lay->S.rp; Sv.ci = lay->S.ci; Sv.nv = 1; Sv.v[0] = lay->S.v;
     GP_TRY(spgemm_ex(A, Sv, H->L[0].G, &H->SG, s, 1));
     GP_TRY(spgemm_ex(A, Sv, H->aux.Gx, &H->SGx, s, 1));
+    GP_TRY(sell_from_csr(A, Sv, &H->Ss, s, "S"));
   }
   H->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return GP_OK;
@@ -466,20 +497,29 @@ __global__ void __launch_bounds__(256) k_long_part(MatV<T> M, int k, const C2<T>
   }
 }
 
+// one block per long row: thread (bj, g) sums chunks g, g + G, ... of row q
+// for output bj = b*k + j; the G partial sums are combined in LDS in order
 template <class T>
-__global__ void k_long_fin(MatV<T> M, int k, const C2<T> *__restrict__ part, C2<T> *__restrict__ lp) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t per = (int64_t)M.nv * k;
-  if (t >= (int64_t)M.nlong * per) return;
-  const int q = (int)(t / per);
-  const int64_t bj = t - (int64_t)q * per;  // b*k + j
+__global__ void __launch_bounds__(256) k_long_fin(MatV<T> M, int k, const C2<T> *__restrict__ part, C2<T> *__restrict__ lp) {
+  __shared__ C2<T> red[256];
+  const int q = blockIdx.x;
+  const int per = M.nv * k;  // <= 128 (nv <= 2, k <= 64)
+  const int tid = threadIdx.x;
+  const int bj = tid % per, g = tid / per, G = blockDim.x / per;
   C2<T> acc = {0, 0};
-  for (int c = M.lqp[q]; c < M.lqp[q + 1]; ++c) {
-    const C2<T> v = part[(int64_t)c * per + bj];
-    acc.x += v.x;
-    acc.y += v.y;
+  if (g < G)
+    for (int c = M.lqp[q] + g; c < M.lqp[q + 1]; c += G) {
+      const C2<T> v = part[(int64_t)c * per + bj];
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+  red[tid] = acc;
+  __syncthreads();
+  if (tid < per) {
+    C2<T> t = {0, 0};
+    for (int gg = 0; gg < G; ++gg) { t.x += red[gg * per + tid].x; t.y += red[gg * per + tid].y; }
+    lp[(int64_t)q * per + tid] = t;
   }
-  lp[t] = acc;
 }
 
 struct EdgeMode { enum { RES = 0, JACOBI = 1, JACOBI0 = 2 }; };
@@ -495,13 +535,25 @@ __global__ void k_edge(int64_t nr, int k, MatV<T> K, MatV<T> B, const T *__restr
                        const T *__restrict__ kl0, const T *__restrict__ kl1, const T *__restrict__ kl2,
                        const C2<T> *__restrict__ r, const C2<T> *__restrict__ u, const C2<T> *__restrict__ phi,
                        C2<T> *__restrict__ out, int mode, int use_phi, T damp) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t t = xcd_block_id() * blockDim.x + threadIdx.x;
   int64_t i = t / k;
   int j = (int)(t - i * k);
   if (i >= nr) return;
   const T c1 = c1v[j], c2 = c2v[j];
   C2<T> acc = r[i * k + j];
   if (mode != EdgeMode::JACOBI0) {
+    if (K.dg[0]) {  // K_eps and K_sig diagonal (level 0): one value array in the loop
+      for (int64_t p = K.rp[i]; p < K.rp[i + 1]; ++p) {
+        const T a = K.v[0][p];
+        const C2<T> x = u[(int64_t)K.ci[p] * k + j];
+        acc.x -= a * x.x;
+        acc.y -= a * x.y;
+      }
+      const C2<T> a = {c1 * K.dg[0][i], c2 * K.dg[1][i]};
+      const C2<T> x = u[i * k + j];
+      acc.x -= a.x * x.x - a.y * x.y;
+      acc.y -= a.x * x.y + a.y * x.x;
+    } else
     for (int64_t p = K.rp[i]; p < K.rp[i + 1]; ++p) {
       C2<T> a = {K.v[0][p] + c1 * K.v[1][p], c2 * K.v[2][p]};
       C2<T> x = u[(int64_t)K.ci[p] * k + j];
@@ -525,6 +577,249 @@ __global__ void k_edge(int64_t nr, int k, MatV<T> K, MatV<T> B, const T *__restr
   out[i * k + j] = {u0.x + damp * q.x, u0.y + damp * q.y};
 }
 
+// Edge kernel on SELL matrices, one thread per (row, group of KS slots); the
+// same contract as k_edge.  Block = 256 rows (4 slices) of one slot group
+// (blockIdx.y); blocks are numbered per XCD (xcd_block_id).
+template <class T, int KS>
+__global__ void __launch_bounds__(256) k_edge_sell(int64_t nr, int k, SellV<T> K, SellV<T> B, const T *__restrict__ c1v,
+                                                   const T *__restrict__ c2v, const T *__restrict__ kd0,
+                                                   const T *__restrict__ kd1, const T *__restrict__ kd2,
+                                                   const T *__restrict__ kl0, const T *__restrict__ kl1,
+                                                   const T *__restrict__ kl2, const C2<T> *__restrict__ r,
+                                                   const C2<T> *__restrict__ u, const C2<T> *__restrict__ phi,
+                                                   C2<T> *__restrict__ out, int mode, int use_phi, T damp) {
+  const int64_t i = xcd_block_id() * blockDim.x + threadIdx.x;
+  if (i >= nr) return;
+  const int j0 = blockIdx.y * KS;
+  const int64_t sl = i >> 6;
+  const int lane = (int)(i & 63);
+  T c1[KS], c2[KS];
+#pragma unroll
+  for (int q = 0; q < KS; ++q) { c1[q] = c1v[j0 + q]; c2[q] = c2v[j0 + q]; }
+  C2<T> acc[KS];
+  load_slots<T, KS>(r + i * k + j0, acc);
+  if (mode != EdgeMode::JACOBI0) {
+    const int64_t b0 = K.base[sl];
+    const int wd = K.w[sl];
+    if (K.nv == 1) {
+      for (int t = 0; t < wd; ++t) {
+        const int64_t e = b0 + (int64_t)t * 64 + lane;
+        const int32_t c = K.ci[e];
+        const T a = K.v[0][e];
+        C2<T> x[KS];
+        load_slots<T, KS>(u + (int64_t)c * k + j0, x);
+#pragma unroll
+        for (int q = 0; q < KS; ++q) { acc[q].x -= a * x[q].x; acc[q].y -= a * x[q].y; }
+      }
+    } else {
+      for (int t = 0; t < wd; ++t) {
+        const int64_t e = b0 + (int64_t)t * 64 + lane;
+        const int32_t c = K.ci[e];
+        const T a0 = K.v[0][e], a1 = K.v[1][e], a2 = K.v[2][e];
+        C2<T> x[KS];
+        load_slots<T, KS>(u + (int64_t)c * k + j0, x);
+#pragma unroll
+        for (int q = 0; q < KS; ++q) {
+          const T ar = a0 + c1[q] * a1, ai = c2[q] * a2;
+          acc[q].x -= ar * x[q].x - ai * x[q].y;
+          acc[q].y -= ar * x[q].y + ai * x[q].x;
+        }
+      }
+    }
+    if (K.dg[0]) {  // diagonal K_eps, K_sig
+      const T de = K.dg[0][i], ds = K.dg[1][i];
+      C2<T> x[KS];
+      load_slots<T, KS>(u + i * k + j0, x);
+#pragma unroll
+      for (int q = 0; q < KS; ++q) {
+        const T ar = c1[q] * de, ai = c2[q] * ds;
+        acc[q].x -= ar * x[q].x - ai * x[q].y;
+        acc[q].y -= ar * x[q].y + ai * x[q].x;
+      }
+    }
+    if (use_phi) {
+      const int64_t bb = B.base[sl];
+      const int wb = B.w[sl];
+      for (int t = 0; t < wb; ++t) {
+        const int64_t e = bb + (int64_t)t * 64 + lane;
+        const int32_t c = B.ci[e];
+        const T a0 = B.v[0][e], a1 = B.v[1][e];
+        C2<T> x[KS];
+        load_slots<T, KS>(phi + (int64_t)c * k + j0, x);
+#pragma unroll
+        for (int q = 0; q < KS; ++q) {
+          const T br = c1[q] * a0, bi = c2[q] * a1;
+          acc[q].x -= br * x[q].x - bi * x[q].y;
+          acc[q].y -= br * x[q].y + bi * x[q].x;
+        }
+      }
+    }
+  }
+  if (mode == EdgeMode::RES) { store_slots<T, KS>(out + i * k + j0, acc); return; }
+  const T d0 = kd0[i], d1 = kd1[i], d2 = kd2[i], l0 = kl0[i], l1 = kl1[i], l2 = kl2[i];
+  C2<T> u0[KS];
+  if (mode == EdgeMode::JACOBI) load_slots<T, KS>(u + i * k + j0, u0);
+#pragma unroll
+  for (int q = 0; q < KS; ++q) {
+    const C2<T> dq = l1diag<T>({d0 + c1[q] * d1, c2[q] * d2}, l0 + fabs(c1[q]) * l1 + c2[q] * l2);
+    const C2<T> z = cdiv(acc[q], dq);
+    if (mode == EdgeMode::JACOBI0) acc[q] = {damp * z.x, damp * z.y};
+    else acc[q] = {u0[q].x + damp * z.x, u0[q].y + damp * z.y};
+  }
+  store_slots<T, KS>(out + i * k + j0, acc);
+}
+
+// SELL nodal l1-Jacobi (contract of k_node); long rows (lidx >= 0) take their
+// product N_eps d, N_sig d from lp = [nlong][2][k]
+template <class T, int KS>
+__global__ void __launch_bounds__(256) k_node_sell(int64_t nr, int k, SellV<T> N, const T *__restrict__ c1v,
+                                                   const T *__restrict__ c2v, const T *__restrict__ nd0,
+                                                   const T *__restrict__ nd1, const T *__restrict__ nl0,
+                                                   const T *__restrict__ nl1, const C2<T> *__restrict__ rn,
+                                                   const C2<T> *__restrict__ d, C2<T> *__restrict__ dout, T damp,
+                                                   const int32_t *__restrict__ lidx, const C2<T> *__restrict__ lp) {
+  const int64_t i = xcd_block_id() * blockDim.x + threadIdx.x;
+  if (i >= nr) return;
+  const int j0 = blockIdx.y * KS;
+  T c1[KS], c2[KS];
+#pragma unroll
+  for (int q = 0; q < KS; ++q) { c1[q] = c1v[j0 + q]; c2[q] = c2v[j0 + q]; }
+  C2<T> acc[KS];
+  load_slots<T, KS>(rn + i * k + j0, acc);
+  if (d) {
+    const int32_t lq = lidx ? lidx[i] : -1;
+    if (lq >= 0) {
+      C2<T> s0[KS], s1[KS];
+      load_slots<T, KS>(lp + ((int64_t)lq * 2) * k + j0, s0);
+      load_slots<T, KS>(lp + ((int64_t)lq * 2 + 1) * k + j0, s1);
+#pragma unroll
+      for (int q = 0; q < KS; ++q) {
+        acc[q].x -= c1[q] * s0[q].x - c2[q] * s1[q].y;
+        acc[q].y -= c1[q] * s0[q].y + c2[q] * s1[q].x;
+      }
+    } else {
+      const int64_t sl = i >> 6;
+      const int lane = (int)(i & 63);
+      const int64_t b0 = N.base[sl];
+      const int wd = N.w[sl];
+      for (int t = 0; t < wd; ++t) {
+        const int64_t e = b0 + (int64_t)t * 64 + lane;
+        const int32_t c = N.ci[e];
+        const T a0 = N.v[0][e], a1 = N.v[1][e];
+        C2<T> x[KS];
+        load_slots<T, KS>(d + (int64_t)c * k + j0, x);
+#pragma unroll
+        for (int q = 0; q < KS; ++q) {
+          const T ar = c1[q] * a0, ai = c2[q] * a1;
+          acc[q].x -= ar * x[q].x - ai * x[q].y;
+          acc[q].y -= ar * x[q].y + ai * x[q].x;
+        }
+      }
+    }
+  }
+  const T e0 = nd0[i], e1 = nd1[i], m0 = nl0[i], m1 = nl1[i];
+  C2<T> o[KS];
+  if (d) load_slots<T, KS>(d + i * k + j0, o);
+#pragma unroll
+  for (int q = 0; q < KS; ++q) {
+    const C2<T> dg = l1diag<T>({c1[q] * e0, c2[q] * e1}, fabs(c1[q]) * m0 + c2[q] * m1);
+    const C2<T> z = cdiv(acc[q], dg);
+    if (d) o[q] = {o[q].x + damp * z.x, o[q].y + damp * z.y};
+    else o[q] = {damp * z.x, damp * z.y};
+  }
+  store_slots<T, KS>(dout + i * k + j0, o);
+}
+
+// SELL y = M x (real M) or y += M x; long rows from lp = [nlong][1][k]
+template <class T, int KS>
+__global__ void __launch_bounds__(256) k_spmv_real_sell(int64_t nr, int k, SellV<T> M, const C2<T> *__restrict__ x,
+                                                        C2<T> *__restrict__ y, int accumulate,
+                                                        const int32_t *__restrict__ lidx, const C2<T> *__restrict__ lp) {
+  const int64_t i = xcd_block_id() * blockDim.x + threadIdx.x;
+  if (i >= nr) return;
+  const int j0 = blockIdx.y * KS;
+  C2<T> acc[KS];
+  const int32_t lq = lidx ? lidx[i] : -1;
+  if (lq >= 0) {
+    load_slots<T, KS>(lp + (int64_t)lq * k + j0, acc);
+  } else {
+#pragma unroll
+    for (int q = 0; q < KS; ++q) acc[q] = {0, 0};
+    const int64_t sl = i >> 6;
+    const int lane = (int)(i & 63);
+    const int64_t b0 = M.base[sl];
+    const int wd = M.w[sl];
+    for (int t = 0; t < wd; ++t) {
+      const int64_t e = b0 + (int64_t)t * 64 + lane;
+      const int32_t c = M.ci[e];
+      const T a = M.v[0][e];
+      C2<T> xv[KS];
+      load_slots<T, KS>(x + (int64_t)c * k + j0, xv);
+#pragma unroll
+      for (int q = 0; q < KS; ++q) { acc[q].x += a * xv[q].x; acc[q].y += a * xv[q].y; }
+    }
+  }
+  if (accumulate) {
+    C2<T> o[KS];
+    load_slots<T, KS>(y + i * k + j0, o);
+#pragma unroll
+    for (int q = 0; q < KS; ++q) { acc[q].x += o[q].x; acc[q].y += o[q].y; }
+  }
+  store_slots<T, KS>(y + i * k + j0, acc);
+}
+
+// SELL M2(w) = c1 v0 + i c2 v1: y = r - M2 x (mode 0), y -= M2 x (1), y = M2 x (2)
+template <class T, int KS>
+__global__ void __launch_bounds__(256) k_spmv2_sell(int64_t nr, int k, SellV<T> M, const T *__restrict__ c1v,
+                                                    const T *__restrict__ c2v, const C2<T> *__restrict__ r,
+                                                    const C2<T> *__restrict__ x, C2<T> *__restrict__ y, int mode,
+                                                    const int32_t *__restrict__ lidx, const C2<T> *__restrict__ lp) {
+  const int64_t i = xcd_block_id() * blockDim.x + threadIdx.x;
+  if (i >= nr) return;
+  const int j0 = blockIdx.y * KS;
+  T c1[KS], c2[KS];
+#pragma unroll
+  for (int q = 0; q < KS; ++q) { c1[q] = c1v[j0 + q]; c2[q] = c2v[j0 + q]; }
+  C2<T> acc[KS];
+#pragma unroll
+  for (int q = 0; q < KS; ++q) acc[q] = {0, 0};
+  const int32_t lq = lidx ? lidx[i] : -1;
+  if (lq >= 0) {
+    C2<T> s0[KS], s1[KS];
+    load_slots<T, KS>(lp + ((int64_t)lq * 2) * k + j0, s0);
+    load_slots<T, KS>(lp + ((int64_t)lq * 2 + 1) * k + j0, s1);
+#pragma unroll
+    for (int q = 0; q < KS; ++q) acc[q] = {c1[q] * s0[q].x - c2[q] * s1[q].y, c1[q] * s0[q].y + c2[q] * s1[q].x};
+  } else {
+    const int64_t sl = i >> 6;
+    const int lane = (int)(i & 63);
+    const int64_t b0 = M.base[sl];
+    const int wd = M.w[sl];
+    for (int t = 0; t < wd; ++t) {
+      const int64_t e = b0 + (int64_t)t * 64 + lane;
+      const int32_t c = M.ci[e];
+      const T a0 = M.v[0][e], a1 = M.v[1][e];
+      C2<T> xv[KS];
+      load_slots<T, KS>(x + (int64_t)c * k + j0, xv);
+#pragma unroll
+      for (int q = 0; q < KS; ++q) {
+        const T ar = c1[q] * a0, ai = c2[q] * a1;
+        acc[q].x += ar * xv[q].x - ai * xv[q].y;
+        acc[q].y += ar * xv[q].y + ai * xv[q].x;
+      }
+    }
+  }
+  C2<T> o[KS];
+  if (mode == 0) load_slots<T, KS>(r + i * k + j0, o);
+  else if (mode == 1) load_slots<T, KS>(y + i * k + j0, o);
+#pragma unroll
+  for (int q = 0; q < KS; ++q) {
+    if (mode == 2) o[q] = acc[q];
+    else o[q] = {o[q].x - acc[q].x, o[q].y - acc[q].y};
+  }
+  store_slots<T, KS>(y + i * k + j0, o);
+}
+
 // Nodal l1-Jacobi on N(w) = -w^2 N_eps + i w N_sig:
 //   first (d == null): d_out = rn / dn;  else d_out = d + (rn - N d) / dn
 template <class T>
@@ -532,7 +827,7 @@ __global__ void k_node(int64_t nr, int k, MatV<T> N, const T *__restrict__ c1v, 
                        const T *__restrict__ nd0, const T *__restrict__ nd1, const T *__restrict__ nl0,
                        const T *__restrict__ nl1, const C2<T> *__restrict__ rn, const C2<T> *__restrict__ d,
                        C2<T> *__restrict__ dout, T damp, const C2<T> *__restrict__ lp) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t t = xcd_block_id() * blockDim.x + threadIdx.x;
   int64_t i = t / k;
   int j = (int)(t - i * k);
   if (i >= nr) return;
@@ -562,7 +857,7 @@ __global__ void k_node(int64_t nr, int k, MatV<T> N, const T *__restrict__ c1v, 
 template <class T>
 __global__ void k_spmv_real(int64_t nr, int k, MatV<T> M, const C2<T> *__restrict__ x, C2<T> *__restrict__ y, int accumulate,
                             const C2<T> *__restrict__ lp) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t t = xcd_block_id() * blockDim.x + threadIdx.x;
   int64_t i = t / k;
   int j = (int)(t - i * k);
   if (i >= nr) return;
@@ -647,7 +942,7 @@ template <class T>
 __global__ void k_spmv2(int64_t nr, int k, MatV<T> M, const T *__restrict__ c1v, const T *__restrict__ c2v,
                         const C2<T> *__restrict__ r, const C2<T> *__restrict__ x, C2<T> *__restrict__ y, int mode,
                         const C2<T> *__restrict__ lp) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t t = xcd_block_id() * blockDim.x + threadIdx.x;
   int64_t i = t / k;
   int j = (int)(t - i * k);
   if (i >= nr) return;
@@ -668,24 +963,31 @@ __global__ void k_spmv2(int64_t nr, int k, MatV<T> M, const T *__restrict__ c1v,
   else y[i * k + j] = acc;
 }
 
-// x = A_f^-1 r for each column (dense complex FP64 inverse per frequency), one block per column
+// x = A_f^-1 r for each column (dense complex FP64 inverse per frequency):
+// grid (rows/4, columns), one wavefront per matrix row, the column staged in LDS
 template <class T>
-__global__ void k_dense_apply(int64_t n, int k, const int32_t *__restrict__ fidx, const double *const *__restrict__ tab,
-                              const C2<T> *__restrict__ r, C2<T> *__restrict__ x) {
+__global__ void __launch_bounds__(256) k_dense_apply(int64_t n, int k, const int32_t *__restrict__ fidx,
+                                                     const double *const *__restrict__ tab,
+                                                     const C2<T> *__restrict__ r, C2<T> *__restrict__ x) {
   extern __shared__ double sr[];
-  const int j = blockIdx.x;
+  const int j = blockIdx.y;
   const double *M = tab[fidx[j]];
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) { sr[2 * i] = (double)r[i * k + j].x; sr[2 * i + 1] = (double)r[i * k + j].y; }
   __syncthreads();
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-    double ax = 0, ay = 0;
-    for (int64_t b = 0; b < n; ++b) {
-      const double mx = M[2 * (i * n + b)], my = M[2 * (i * n + b) + 1];
-      ax += mx * sr[2 * b] - my * sr[2 * b + 1];
-      ay += mx * sr[2 * b + 1] + my * sr[2 * b];
-    }
-    x[i * k + j] = {(T)ax, (T)ay};
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (i >= n) return;
+  double ax = 0, ay = 0;
+  for (int64_t b = lane; b < n; b += 64) {
+    const double mx = M[2 * (i * n + b)], my = M[2 * (i * n + b) + 1];
+    ax += mx * sr[2 * b] - my * sr[2 * b + 1];
+    ay += mx * sr[2 * b + 1] + my * sr[2 * b];
   }
+  for (int off = 32; off > 0; off >>= 1) {
+    ax += __shfl_xor(ax, off);
+    ay += __shfl_xor(ay, off);
+  }
+  if (lane == 0) x[i * k + j] = {(T)ax, (T)ay};
 }
 
 template <class T>
@@ -702,6 +1004,7 @@ MatV<double> view<double>(const Csr &c, const CsrF &) {
   for (int k = 0; k < c.nv; ++k) v.v[k] = c.v[k];
   v.nlong = c.nlong; v.lrows = c.lrows; v.lidx = c.lidx;
   v.nchunk = c.nchunk; v.lcb = c.lcb; v.lce = c.lce; v.lqp = c.lqp;
+  v.dg[0] = c.dg[0]; v.dg[1] = c.dg[1];
   return v;
 }
 template <>
@@ -710,7 +1013,120 @@ MatV<float> view<float>(const Csr &, const CsrF &f) {
   for (int k = 0; k < f.nv; ++k) v.v[k] = f.v[k];
   v.nlong = f.nlong; v.lrows = f.lrows; v.lidx = f.lidx;
   v.nchunk = f.nchunk; v.lcb = f.lcb; v.lce = f.lce; v.lqp = f.lqp;
+  v.dg[0] = f.dg[0]; v.dg[1] = f.dg[1];
   return v;
+}
+
+template <class T>
+SellV<T> sview(const Sell &c, const SellF &f, const Csr &kc, const CsrF &kf);
+template <>
+SellV<double> sview<double>(const Sell &c, const SellF &, const Csr &kc, const CsrF &) {
+  SellV<double> v; v.nr = c.nr; v.nslice = c.nslice; v.base = c.base; v.w = c.w; v.ci = c.ci; v.nv = c.nv;
+  for (int q = 0; q < c.nv; ++q) v.v[q] = c.v[q];
+  v.dg[0] = kc.dg[0]; v.dg[1] = kc.dg[1];
+  return v;
+}
+template <>
+SellV<float> sview<float>(const Sell &, const SellF &f, const Csr &, const CsrF &kf) {
+  SellV<float> v; v.nr = f.nr; v.nslice = f.nslice; v.base = f.base; v.w = f.w; v.ci = f.ci; v.nv = f.nv;
+  for (int q = 0; q < f.nv; ++q) v.v[q] = f.v[q];
+  v.dg[0] = kf.dg[0]; v.dg[1] = kf.dg[1];
+  return v;
+}
+
+// Kernel choice per operator: SELL with KS slots per thread for the operators
+// with at least sell_min_rows rows (the level-0 edge stencil, ~13 entries per
+// row, where one coalesced matrix read serves all slots of a row), the
+// (row, slot) CSR kernels elsewhere: on the wider Galerkin rows of coarser
+// levels their 16x larger thread count hides the gather latency better
+// (measured at 200x200x24: level-1 edges 340 us CSR vs 609 us SELL).
+inline int pick_ks(int64_t nr, int k) {
+  int ks = k < 16 ? k : 16;
+  while (ks > 1 && nr * (k / ks) < (int64_t(1) << 20)) ks >>= 1;
+  return ks;
+}
+
+template <class T>
+void edge_sell(int64_t nr, int64_t smin, int k, const SellV<T> &K, const SellV<T> &B, const MatV<T> &Kc, const MatV<T> &Bc,
+               const T *c1, const T *c2, const T *kd0, const T *kd1, const T *kd2, const T *kl0, const T *kl1,
+               const T *kl2, const C2<T> *r, const C2<T> *u, const C2<T> *phi, C2<T> *out, int mode, int use_phi,
+               T damp, hipStream_t s) {
+  if (nr < smin) {
+    k_edge<T><<<blocks_for(nr * k), 256, 0, s>>>(nr, k, Kc, Bc, c1, c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, phi, out,
+                                                 mode, use_phi, damp);
+    return;
+  }
+  const unsigned gx = (unsigned)((nr + 255) / 256);
+  const int ks = pick_ks(nr, k);
+#define GP_EDGE_SELL(KS)                                                                                       \
+  k_edge_sell<T, KS><<<dim3(gx, (unsigned)(k / KS)), 256, 0, s>>>(nr, k, K, B, c1, c2, kd0, kd1, kd2, kl0, kl1, kl2, \
+                                                                   r, u, phi, out, mode, use_phi, damp)
+  if (ks == 16) GP_EDGE_SELL(16);
+  else if (ks == 8) GP_EDGE_SELL(8);
+  else if (ks == 4) GP_EDGE_SELL(4);
+  else if (ks == 2) GP_EDGE_SELL(2);
+  else GP_EDGE_SELL(1);
+#undef GP_EDGE_SELL
+}
+
+template <class T>
+SellV<T> sv(const Sell &c, const SellF &f);
+template <>
+SellV<double> sv<double>(const Sell &c, const SellF &) {
+  SellV<double> v; v.nr = c.nr; v.nslice = c.nslice; v.base = c.base; v.w = c.w; v.ci = c.ci; v.nv = c.nv;
+  for (int q = 0; q < c.nv; ++q) v.v[q] = c.v[q];
+  return v;
+}
+template <>
+SellV<float> sv<float>(const Sell &, const SellF &f) {
+  SellV<float> v; v.nr = f.nr; v.nslice = f.nslice; v.base = f.base; v.w = f.w; v.ci = f.ci; v.nv = f.nv;
+  for (int q = 0; q < f.nv; ++q) v.v[q] = f.v[q];
+  return v;
+}
+
+#define GP_KS_DISPATCH(KERNEL, ...)                                                          \
+  do {                                                                                       \
+    const unsigned gx_ = (unsigned)((nr + 255) / 256);                                       \
+    const int ks_ = pick_ks(nr, k);                                                          \
+    if (ks_ == 16) KERNEL<T, 16><<<dim3(gx_, (unsigned)(k / 16)), 256, 0, s>>>(__VA_ARGS__); \
+    else if (ks_ == 8) KERNEL<T, 8><<<dim3(gx_, (unsigned)(k / 8)), 256, 0, s>>>(__VA_ARGS__); \
+    else if (ks_ == 4) KERNEL<T, 4><<<dim3(gx_, (unsigned)(k / 4)), 256, 0, s>>>(__VA_ARGS__); \
+    else if (ks_ == 2) KERNEL<T, 2><<<dim3(gx_, (unsigned)(k / 2)), 256, 0, s>>>(__VA_ARGS__); \
+    else KERNEL<T, 1><<<dim3(gx_, (unsigned)k), 256, 0, s>>>(__VA_ARGS__);                  \
+  } while (0)
+
+template <class T>
+void node_sell(int64_t nr, int64_t smin, int k, const SellV<T> &N, const MatV<T> &Nc, const T *c1, const T *c2, const T *nd0,
+               const T *nd1, const T *nl0, const T *nl1, const C2<T> *rn, const C2<T> *d, C2<T> *dout, T damp,
+               const int32_t *lidx, const C2<T> *lp, hipStream_t s) {
+  if (nr == 0) return;
+  if (nr < smin) {
+    k_node<T><<<blocks_for(nr * k), 256, 0, s>>>(nr, k, Nc, c1, c2, nd0, nd1, nl0, nl1, rn, d, dout, damp, lp);
+    return;
+  }
+  GP_KS_DISPATCH(k_node_sell, nr, k, N, c1, c2, nd0, nd1, nl0, nl1, rn, d, dout, damp, lidx, lp);
+}
+
+template <class T>
+void spmv_real_sell(int64_t nr, int64_t smin, int k, const SellV<T> &M, const MatV<T> &Mc, const C2<T> *x, C2<T> *y, int accumulate,
+                    const int32_t *lidx, const C2<T> *lp, hipStream_t s) {
+  if (nr == 0) return;
+  if (nr < smin) {
+    k_spmv_real<T><<<blocks_for(nr * k), 256, 0, s>>>(nr, k, Mc, x, y, accumulate, lp);
+    return;
+  }
+  GP_KS_DISPATCH(k_spmv_real_sell, nr, k, M, x, y, accumulate, lidx, lp);
+}
+
+template <class T>
+void spmv2_sell(int64_t nr, int64_t smin, int k, const SellV<T> &M, const MatV<T> &Mc, const T *c1, const T *c2, const C2<T> *r,
+                const C2<T> *x, C2<T> *y, int mode, const int32_t *lidx, const C2<T> *lp, hipStream_t s) {
+  if (nr == 0) return;
+  if (nr < smin) {
+    k_spmv2<T><<<blocks_for(nr * k), 256, 0, s>>>(nr, k, Mc, c1, c2, r, x, y, mode, lp);
+    return;
+  }
+  GP_KS_DISPATCH(k_spmv2_sell, nr, k, M, c1, c2, r, x, y, mode, lidx, lp);
 }
 
 template <class T> const T *arr(double *d, float *f);
@@ -723,7 +1139,7 @@ template <class T>
 void long_pass(const MatV<T> &M, int k, const C2<T> *x, C2<T> *part, C2<T> *lp, hipStream_t s) {
   if (M.nlong <= 0) return;
   k_long_part<T><<<M.nchunk, 256, 0, s>>>(M, k, x, part);
-  k_long_fin<T><<<blocks_for((int64_t)M.nlong * M.nv * k), 256, 0, s>>>(M, k, part, lp);
+  k_long_fin<T><<<M.nlong, 256, 0, s>>>(M, k, part, lp);
 }
 
 }  // namespace
@@ -741,6 +1157,9 @@ static gp_status vcycle_level(const Hierarchy &H, VWork<T> &w, hipStream_t s, in
     return GP_OK;
   }
   const MatV<T> K = view<T>(L.K, L.Kf), B = view<T>(L.B, L.Bf), N = view<T>(L.N, L.Nf);
+  const SellV<T> KS = sview<T>(L.Ks, L.Ksf, L.K, L.Kf), BS = sview<T>(L.Bs, L.Bsf, L.B, L.Bf);
+  const SellV<T> NS = sv<T>(L.Ns, L.Nsf), GTS = sv<T>(L.GTs, L.GTsf), PS = sv<T>(L.Ps, L.Psf), PTS = sv<T>(L.PTs, L.PTsf),
+                 PnS = sv<T>(L.Pns, L.Pnsf);
   const MatV<T> GT = view<T>(L.GT, L.GTf), PT = view<T>(L.PT, L.PTf), P = view<T>(L.P, L.Pf), Pn = view<T>(L.Pn, L.Pnf);
   const T *kd0 = arr<T>(L.kd[0], L.kdf[0]), *kd1 = arr<T>(L.kd[1], L.kdf[1]), *kd2 = arr<T>(L.kd[2], L.kdf[2]);
   const T *kl0 = arr<T>(L.kl[0], L.klf[0]), *kl1 = arr<T>(L.kl[1], L.klf[1]), *kl2 = arr<T>(L.kl[2], L.klf[2]);
@@ -753,9 +1172,12 @@ static gp_status vcycle_level(const Hierarchy &H, VWork<T> &w, hipStream_t s, in
   KTimer *kt = (l == 0) ? w.kt : nullptr;
   auto edge_bytes = [&](int mode, int use_phi) {
     const double vb = (double)sizeof(C2<T>) * k, tb = (double)sizeof(T);
-    double b = (double)L.K.nnz * (4 + 3 * tb) + 8.0 * (ne + 1) + vb * ne * 2;  // K, r in, out
-    if (mode != EdgeMode::JACOBI0) b += vb * ne;                                // u
-    if (use_phi) b += (double)L.B.nnz * (4 + 2 * tb) + 8.0 * (ne + 1) + vb * nn;
+    double b = vb * ne * 2;                                                     // r in, out
+    if (mode != EdgeMode::JACOBI0) {
+      b += (double)L.K.nnz * (4 + L.K.nv * tb) + 8.0 * (ne + 1) + vb * ne;       // K, u
+      if (L.K.dg[0]) b += 2 * tb * ne;                                          // diagonal K_eps, K_sig
+      if (use_phi) b += (double)L.B.nnz * (4 + 2 * tb) + 8.0 * (ne + 1) + vb * nn;
+    }
     if (mode != EdgeMode::RES) b += 6 * tb * ne;                                // smoother diagonals
     return b;
   };
@@ -763,54 +1185,54 @@ static gp_status vcycle_level(const Hierarchy &H, VWork<T> &w, hipStream_t s, in
   auto te0 = [&](int mode, int use_phi) { if (kt) kt->end(s, edge_bytes(mode, use_phi)); };
   // pre-smoothing on the edges (phi = 0)
   tb0();
-  k_edge<T><<<ge, 256, 0, s>>>(ne, k, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, nullptr, nullptr, u,
-                               EdgeMode::JACOBI0, 0, damp);
+  edge_sell<T>(ne, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, nullptr, nullptr, u,
+                               EdgeMode::JACOBI0, 0, damp, s);
   te0(EdgeMode::JACOBI0, 0);
   for (int it = 1; it < nu_pre; ++it) {
     tb0();
-    k_edge<T><<<ge, 256, 0, s>>>(ne, k, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, nullptr, tmp,
-                                   EdgeMode::JACOBI, 0, damp);
+    edge_sell<T>(ne, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, nullptr, tmp,
+                                   EdgeMode::JACOBI, 0, damp, s);
     te0(EdgeMode::JACOBI, 0);
     std::swap(u, tmp);
   }
   // pre-smoothing on the nodal auxiliary space
   tb0();
-  k_edge<T><<<ge, 256, 0, s>>>(ne, k, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, nullptr, res,
-                               EdgeMode::RES, 0, damp);
+  edge_sell<T>(ne, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, nullptr, res,
+                               EdgeMode::RES, 0, damp, s);
   te0(EdgeMode::RES, 0);
-  k_spmv_real<T><<<gn, 256, 0, s>>>(nn, k, GT, res, rn, 0, w.lp);
-  k_node<T><<<gn, 256, 0, s>>>(nn, k, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, nullptr, phi, damp, w.lp);
+  spmv_real_sell<T>(nn, w.sell_min_rows, k, GTS, GT, res, rn, 0, nullptr, w.lp, s);
+  node_sell<T>(nn, w.sell_min_rows, k, NS, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, nullptr, phi, damp, nullptr, w.lp, s);
   for (int it = 1; it < nu_pre; ++it) {
-    k_node<T><<<gn, 256, 0, s>>>(nn, k, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, phi, d2, damp, w.lp);
+    node_sell<T>(nn, w.sell_min_rows, k, NS, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, phi, d2, damp, nullptr, w.lp, s);
     std::swap(phi, d2);
   }
   // coarse-grid correction
   tb0();
-  k_edge<T><<<ge, 256, 0, s>>>(ne, k, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, phi, res,
-                               EdgeMode::RES, 1, damp);
+  edge_sell<T>(ne, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, phi, res,
+                               EdgeMode::RES, 1, damp, s);
   te0(EdgeMode::RES, 1);
   const int64_t nec = H.L[l + 1].ne, nnc = H.L[l + 1].nn;
-  k_spmv_real<T><<<grid_rows(nec, k), 256, 0, s>>>(nec, k, PT, res, w.r[l + 1], 0, w.lp);
+  spmv_real_sell<T>(nec, w.sell_min_rows, k, PTS, PT, res, w.r[l + 1], 0, nullptr, w.lp, s);
   GP_TRY(vcycle_level<T>(H, w, s, l + 1, nu_pre, nu_post));
-  k_spmv_real<T><<<ge, 256, 0, s>>>(ne, k, P, w.u[l + 1], u, 1, w.lp);
-  k_spmv_real<T><<<gn, 256, 0, s>>>(nn, k, Pn, w.phi[l + 1], phi, 1, w.lp);
+  spmv_real_sell<T>(ne, w.sell_min_rows, k, PS, P, w.u[l + 1], u, 1, nullptr, w.lp, s);
+  spmv_real_sell<T>(nn, w.sell_min_rows, k, PnS, Pn, w.phi[l + 1], phi, 1, nullptr, w.lp, s);
   (void)nnc;
   // post-smoothing: nodal, then edges
   tb0();
-  k_edge<T><<<ge, 256, 0, s>>>(ne, k, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, phi, res,
-                               EdgeMode::RES, 1, damp);
+  edge_sell<T>(ne, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, phi, res,
+                               EdgeMode::RES, 1, damp, s);
   te0(EdgeMode::RES, 1);
-  k_spmv_real<T><<<gn, 256, 0, s>>>(nn, k, GT, res, rn, 0, w.lp);
-  k_node<T><<<gn, 256, 0, s>>>(nn, k, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, nullptr, d1, damp, w.lp);
+  spmv_real_sell<T>(nn, w.sell_min_rows, k, GTS, GT, res, rn, 0, nullptr, w.lp, s);
+  node_sell<T>(nn, w.sell_min_rows, k, NS, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, nullptr, d1, damp, nullptr, w.lp, s);
   for (int it = 1; it < nu_post; ++it) {
-    k_node<T><<<gn, 256, 0, s>>>(nn, k, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, d1, d2, damp, w.lp);
+    node_sell<T>(nn, w.sell_min_rows, k, NS, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, d1, d2, damp, nullptr, w.lp, s);
     std::swap(d1, d2);
   }
   k_add<T><<<gn, 256, 0, s>>>(nn * k, phi, d1);
   for (int it = 0; it < nu_post; ++it) {
     tb0();
-    k_edge<T><<<ge, 256, 0, s>>>(ne, k, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, phi, tmp,
-                                   EdgeMode::JACOBI, 1, damp);
+    edge_sell<T>(ne, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, phi, tmp,
+                                   EdgeMode::JACOBI, 1, damp, s);
     te0(EdgeMode::JACOBI, 1);
     std::swap(u, tmp);
   }
@@ -834,31 +1256,33 @@ static gp_status aux_cycle(const Hierarchy &H, VWork<T> &w, hipStream_t s, int l
   const int64_t n = L.n;
   C2<T> *r = w.ar[l], *x = w.ax[l], *t = w.at[l];
   if (l + 1 == (int)X.L.size()) {
-    k_dense_apply<T><<<k, 256, sizeof(double) * 2 * n, s>>>(n, k, w.fidx, w.tab + 3 * w.cap, r, x);
+    k_dense_apply<T><<<dim3((unsigned)((n + 3) / 4), (unsigned)k), 256, sizeof(double) * 2 * n, s>>>(n, k, w.fidx,
+                                                                                                  w.tab + 3 * w.cap, r, x);
     return GP_OK;
   }
   const MatV<T> N = view<T>(L.N, L.Nf), P = view<T>(L.P, L.Pf), PT = view<T>(L.PT, L.PTf);
+  const SellV<T> NS = sv<T>(L.Ns, L.Nsf), PS = sv<T>(L.Ps, L.Psf), PTS = sv<T>(L.PTs, L.PTsf);
   const T *d0 = arr<T>(L.nd[0], L.ndf[0]), *d1 = arr<T>(L.nd[1], L.ndf[1]);
   const T *l0 = arr<T>(L.nl[0], L.nlf[0]), *l1 = arr<T>(L.nl[1], L.nlf[1]);
   const unsigned gr = grid_rows(n, k);
   const int nu = w.nu_aux;
-  k_node<T><<<gr, 256, 0, s>>>(n, k, N, w.c1, w.c2, d0, d1, l0, l1, r, nullptr, x, (T)1, w.lp);
+  node_sell<T>(n, w.sell_min_rows, k, NS, N, w.c1, w.c2, d0, d1, l0, l1, r, nullptr, x, (T)1, L.N.lidx, w.lp, s);
   for (int it = 1; it < nu; ++it) {
     long_pass<T>(N, k, x, w.lpart, w.lp, s);
-    k_node<T><<<gr, 256, 0, s>>>(n, k, N, w.c1, w.c2, d0, d1, l0, l1, r, x, t, (T)1, w.lp);
+    node_sell<T>(n, w.sell_min_rows, k, NS, N, w.c1, w.c2, d0, d1, l0, l1, r, x, t, (T)1, L.N.lidx, w.lp, s);
     std::swap(x, t);
   }
   long_pass<T>(N, k, x, w.lpart, w.lp, s);
-  k_spmv2<T><<<gr, 256, 0, s>>>(n, k, N, w.c1, w.c2, r, x, t, 0, w.lp);
+  spmv2_sell<T>(n, w.sell_min_rows, k, NS, N, w.c1, w.c2, r, x, t, 0, L.N.lidx, w.lp, s);
   const int64_t nc = X.L[l + 1].n;
   long_pass<T>(PT, k, t, w.lpart, w.lp, s);
-  k_spmv_real<T><<<grid_rows(nc, k), 256, 0, s>>>(nc, k, PT, t, w.ar[l + 1], 0, w.lp);
+  spmv_real_sell<T>(nc, w.sell_min_rows, k, PTS, PT, t, w.ar[l + 1], 0, L.PT.lidx, w.lp, s);
   GP_TRY(aux_cycle<T>(H, w, s, l + 1));
   long_pass<T>(P, k, w.ax[l + 1], w.lpart, w.lp, s);
-  k_spmv_real<T><<<gr, 256, 0, s>>>(n, k, P, w.ax[l + 1], x, 1, w.lp);
+  spmv_real_sell<T>(n, w.sell_min_rows, k, PS, P, w.ax[l + 1], x, 1, L.P.lidx, w.lp, s);
   for (int it = 0; it < nu; ++it) {
     long_pass<T>(N, k, x, w.lpart, w.lp, s);
-    k_node<T><<<gr, 256, 0, s>>>(n, k, N, w.c1, w.c2, d0, d1, l0, l1, r, x, t, (T)1, w.lp);
+    node_sell<T>(n, w.sell_min_rows, k, NS, N, w.c1, w.c2, d0, d1, l0, l1, r, x, t, (T)1, L.N.lidx, w.lp, s);
     std::swap(x, t);
   }
   if (x != w.ax[l]) GP_HIP(hipMemcpyAsync(w.ax[l], x, sizeof(C2<T>) * n * k, hipMemcpyDeviceToDevice, s));
@@ -872,25 +1296,27 @@ gp_status precond(const Hierarchy &H, VWork<T> &w, hipStream_t s, int nu_pre, in
   const int k = w.k;
   const int64_t n = L0.ne, na = X.naux;
   const MatV<T> GxT = view<T>(X.GxT, X.GxTf), Bx = view<T>(X.Bx, X.Bxf);
+  const SellV<T> GxTS = sv<T>(X.GxTs, X.GxTsf), BxS = sv<T>(X.Bxs, X.Bxsf);
   const MatV<T> K = view<T>(L0.K, L0.Kf), B = view<T>(L0.B, L0.Bf);
   const unsigned gn = grid_rows(n, k), ga = grid_rows(na, k);
   // auxiliary correction: a = AMG(Gx^T r), r <- r - Bx a
   k_copy<T><<<blocks_for(n * k), 256, 0, s>>>(n * k, w.r[0], w.r0);
   long_pass<T>(GxT, k, w.r[0], w.lpart, w.lp, s);
-  k_spmv_real<T><<<ga, 256, 0, s>>>(na, k, GxT, w.r[0], w.ar[0], 0, w.lp);
+  spmv_real_sell<T>(na, w.sell_min_rows, k, GxTS, GxT, w.r[0], w.ar[0], 0, X.GxT.lidx, w.lp, s);
   GP_TRY(aux_cycle<T>(H, w, s, 0));
   k_copy<T><<<blocks_for(na * k), 256, 0, s>>>(na * k, w.ax[0], w.a);
-  k_spmv2<T><<<gn, 256, 0, s>>>(n, k, Bx, w.c1, w.c2, nullptr, w.a, w.r[0], 1, w.lp);
+  spmv2_sell<T>(n, w.sell_min_rows, k, BxS, Bx, w.c1, w.c2, nullptr, w.a, w.r[0], 1, nullptr, w.lp, s);
   // edge V-cycle on the remaining residual
   GP_TRY(vcycle<T>(H, w, s, nu_pre, nu_post));
   // auxiliary correction on r0 - K u - B phi - Bx a
   const T *kd0 = arr<T>(L0.kd[0], L0.kdf[0]);
   (void)kd0;
-  k_edge<T><<<gn, 256, 0, s>>>(n, k, K, B, w.c1, w.c2, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, w.r0,
-                               w.u[0], w.phi[0], w.res[0], EdgeMode::RES, 1, (T)1);
-  k_spmv2<T><<<gn, 256, 0, s>>>(n, k, Bx, w.c1, w.c2, nullptr, w.a, w.res[0], 1, w.lp);
+  const SellV<T> KS = sview<T>(L0.Ks, L0.Ksf, L0.K, L0.Kf), BS = sview<T>(L0.Bs, L0.Bsf, L0.B, L0.Bf);
+  edge_sell<T>(n, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, w.r0, w.u[0], w.phi[0],
+               w.res[0], EdgeMode::RES, 1, (T)1, s);
+  spmv2_sell<T>(n, w.sell_min_rows, k, BxS, Bx, w.c1, w.c2, nullptr, w.a, w.res[0], 1, nullptr, w.lp, s);
   long_pass<T>(GxT, k, w.res[0], w.lpart, w.lp, s);
-  k_spmv_real<T><<<ga, 256, 0, s>>>(na, k, GxT, w.res[0], w.ar[0], 0, w.lp);
+  spmv_real_sell<T>(na, w.sell_min_rows, k, GxTS, GxT, w.res[0], w.ar[0], 0, X.GxT.lidx, w.lp, s);
   GP_TRY(aux_cycle<T>(H, w, s, 0));
   k_add<T><<<blocks_for(na * k), 256, 0, s>>>(na * k, w.a, w.ax[0]);
   return GP_OK;

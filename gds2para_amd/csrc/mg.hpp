@@ -22,6 +22,53 @@
 
 namespace gp {
 
+// Workgroups go to the 8 XCDs round-robin (block b to XCD b % 8).  This
+// renumbering gives each XCD one contiguous range of blocks, so the rows one
+// XCD works on are neighbours in memory and their stencil gathers hit its L2.
+__device__ __forceinline__ int64_t xcd_block_id() {
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned per = nb >> 3, rem = nb & 7u, x = b & 7u, q = b >> 3;
+  return (int64_t)(x * per + (x < rem ? x : rem) + q);
+}
+
+// KS consecutive slots of one row, with 16-byte vector loads (the row
+// offset times 8 bytes must be a multiple of 16 for the float4 path: k and the
+// first slot even)
+template <class T, int KS>
+__device__ __forceinline__ void load_slots(const C2<T> *__restrict__ p, C2<T> (&x)[KS]) {
+  if constexpr (sizeof(T) == 4 && KS % 2 == 0) {
+    const float4 *q = reinterpret_cast<const float4 *>(p);
+#pragma unroll
+    for (int h = 0; h < KS / 2; ++h) {
+      const float4 v = q[h];
+      x[2 * h] = {(T)v.x, (T)v.y};
+      x[2 * h + 1] = {(T)v.z, (T)v.w};
+    }
+  } else if constexpr (sizeof(T) == 8) {
+    const double2 *q = reinterpret_cast<const double2 *>(p);
+#pragma unroll
+    for (int h = 0; h < KS; ++h) {
+      const double2 v = q[h];
+      x[h] = {(T)v.x, (T)v.y};
+    }
+  } else {
+#pragma unroll
+    for (int h = 0; h < KS; ++h) x[h] = p[h];
+  }
+}
+
+template <class T, int KS>
+__device__ __forceinline__ void store_slots(C2<T> *__restrict__ p, const C2<T> (&x)[KS]) {
+  if constexpr (sizeof(T) == 4 && KS % 2 == 0) {
+    float4 *q = reinterpret_cast<float4 *>(p);
+#pragma unroll
+    for (int h = 0; h < KS / 2; ++h) q[h] = make_float4(x[2 * h].x, x[2 * h].y, x[2 * h + 1].x, x[2 * h + 1].y);
+  } else {
+#pragma unroll
+    for (int h = 0; h < KS; ++h) p[h] = x[h];
+  }
+}
+
 template <class T>
 struct MatV {  // read-only view of a CSR in precision T
   int64_t nr = 0, nc = 0;
@@ -34,12 +81,26 @@ struct MatV {  // read-only view of a CSR in precision T
   int32_t nchunk = 0;
   const int64_t *lcb = nullptr, *lce = nullptr;
   const int32_t *lqp = nullptr;
+  const T *dg[2] = {nullptr, nullptr};  // diagonal value blocks 1, 2 (then v[1], v[2] unused)
+};
+
+template <class T>
+struct SellV {  // read-only view of a Sell in precision T
+  int64_t nr = 0, nslice = 0;
+  const int64_t *base = nullptr;
+  const int32_t *w = nullptr;
+  const int32_t *ci = nullptr;
+  const T *v[3] = {nullptr, nullptr, nullptr};
+  int nv = 0;
+  const T *dg[2] = {nullptr, nullptr};  // diagonal value blocks 1, 2 (level 0)
 };
 
 struct MgLevel {
   int64_t ne = 0, nn = 0;
   Csr K, B, N, G, GT, P, PT, Pn, PnT;
   CsrF Kf, Bf, Nf, Gf, GTf, Pf, PTf, Pnf, PnTf;
+  Sell Ks, Bs, Ns, GTs, Ps, PTs, Pns;    // SELL copies used by the V-cycle kernels
+  SellF Ksf, Bsf, Nsf, GTsf, Psf, PTsf, Pnsf;
   double *kd[3] = {}, *kl[3] = {}, *nd[2] = {}, *nl[2] = {};
   float *kdf[3] = {}, *klf[3] = {}, *ndf[2] = {}, *nlf[2] = {};
 };
@@ -48,6 +109,8 @@ struct AuxLevel {
   int64_t n = 0;
   Csr N, P, PT;                  // N = [N_eps, N_sig]; P to the next coarser level
   CsrF Nf, Pf, PTf;
+  Sell Ns, Ps, PTs;              // SELL copies (long rows left to the long-row pass)
+  SellF Nsf, Psf, PTsf;
   double *nd[2] = {}, *nl[2] = {};
   float *ndf[2] = {}, *nlf[2] = {};
 };
@@ -57,6 +120,8 @@ struct AuxAMG {
   int64_t naux = 0, ndiel = 0, nL = 0;
   Csr Gx, GxT, Bx;               // Gx: kept edges x naux; Bx = W [eps_r eps0, sigma] Gx
   CsrF Gxf, GxTf, Bxf;
+  Sell GxTs, Bxs;
+  SellF GxTsf, Bxsf;
   std::vector<AuxLevel> L;
   std::vector<double> NL[2];     // coarsest level, dense (host)
   double *NLd[2] = {};           // same, on the device
@@ -68,6 +133,7 @@ struct Hierarchy {
   // S G and S Gx of the rounded FP64 stiffness, formed exactly (double-double):
   // A (u + G phi + Gx a) = S u + SG phi + SGx a + D (u + G phi + Gx a)
   Csr SG, SGx;
+  Sell Ss;                       // SELL copy of S for the outer operator
   bool has_f32 = false;
   // level-0 data for the FP64 outer operator and the V-cycle input scaling
   double *W = nullptr;        // edge dual volume
@@ -120,6 +186,7 @@ struct VWork {
   C2<T> *lp = nullptr;                      // long-row products [nlong][nv][k]
   C2<T> *lpart = nullptr;                   // per-chunk partial products [nchunk][nv][k]
   int nu_aux = 2;
+  int64_t sell_min_rows = int64_t(1) << 20;  // smaller operators use the (row, slot) CSR kernels
   KTimer *kt = nullptr;               // times the level-0 edge smoother launches
   Arena arena;
 };

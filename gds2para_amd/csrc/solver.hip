@@ -77,7 +77,7 @@ template <class TZ>
 __global__ void k_outer(OuterOp op, int k, const double *__restrict__ omega, const C2<TZ> *__restrict__ u,
                         const C2<TZ> *__restrict__ phi, const C2<TZ> *__restrict__ a, const Z2 *__restrict__ b,
                         Z2 *__restrict__ out, int mode, int exact) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t t = xcd_block_id() * blockDim.x + threadIdx.x;
   int64_t i = t / k;
   int j = (int)(t - i * k);
   if (i >= op.n) return;
@@ -119,6 +119,83 @@ __global__ void k_outer(OuterOp op, int k, const double *__restrict__ omega, con
   acc.y += dr * xi.y + di * xi.x;
   if (mode == 1) { const Z2 bb = b[i * k + j]; acc.x = bb.x - acc.x; acc.y = bb.y - acc.y; }
   out[i * k + j] = acc;
+}
+
+// k_outer on the SELL copy of S: one thread per (row, group of KS slots),
+// S read once for the group; G, Gx, S G, S Gx rows are short CSR walks.
+template <class TZ, int KS>
+__global__ void __launch_bounds__(256) k_outer_sell(OuterOp op, SellV<double> SS, int k, const double *__restrict__ omega,
+                                                    const C2<TZ> *__restrict__ u, const C2<TZ> *__restrict__ phi,
+                                                    const C2<TZ> *__restrict__ a, const Z2 *__restrict__ b,
+                                                    Z2 *__restrict__ out, int mode, int exact) {
+  const int64_t i = xcd_block_id() * blockDim.x + threadIdx.x;
+  if (i >= op.n) return;
+  const int j0 = blockIdx.y * KS;
+  const int64_t sl = i >> 6;
+  const int lane = (int)(i & 63);
+  Z2 acc[KS], xi[KS];
+#pragma unroll
+  for (int q = 0; q < KS; ++q) acc[q] = {0., 0.};
+  {
+    const int64_t b0 = SS.base[sl];
+    const int wd = SS.w[sl];
+    for (int t = 0; t < wd; ++t) {
+      const int64_t e = b0 + (int64_t)t * 64 + lane;
+      const double m = SS.v[0][e];
+      C2<TZ> x[KS];
+      load_slots<TZ, KS>(u + (int64_t)SS.ci[e] * k + j0, x);
+#pragma unroll
+      for (int q = 0; q < KS; ++q) { acc[q].x += m * (double)x[q].x; acc[q].y += m * (double)x[q].y; }
+    }
+  }
+  {
+    C2<TZ> x[KS];
+    load_slots<TZ, KS>(u + i * k + j0, x);
+#pragma unroll
+    for (int q = 0; q < KS; ++q) xi[q] = {(double)x[q].x, (double)x[q].y};
+  }
+  for (int64_t p = op.grp[i]; p < op.grp[i + 1]; ++p) {
+    const double g = op.gv[p];
+    C2<TZ> x[KS];
+    load_slots<TZ, KS>(phi + (int64_t)op.gci[p] * k + j0, x);
+#pragma unroll
+    for (int q = 0; q < KS; ++q) { xi[q].x += g * (double)x[q].x; xi[q].y += g * (double)x[q].y; }
+  }
+  for (int64_t p = op.xrp[i]; p < op.xrp[i + 1]; ++p) {
+    const double g = op.xv[p];
+    C2<TZ> x[KS];
+    load_slots<TZ, KS>(a + (int64_t)op.xci[p] * k + j0, x);
+#pragma unroll
+    for (int q = 0; q < KS; ++q) { xi[q].x += g * (double)x[q].x; xi[q].y += g * (double)x[q].y; }
+  }
+  if (exact) {
+    for (int64_t p = op.sgrp[i]; p < op.sgrp[i + 1]; ++p) {
+      const double g = op.sgv[p];
+      C2<TZ> x[KS];
+      load_slots<TZ, KS>(phi + (int64_t)op.sgci[p] * k + j0, x);
+#pragma unroll
+      for (int q = 0; q < KS; ++q) { acc[q].x += g * (double)x[q].x; acc[q].y += g * (double)x[q].y; }
+    }
+    for (int64_t p = op.sxrp[i]; p < op.sxrp[i + 1]; ++p) {
+      const double g = op.sxv[p];
+      C2<TZ> x[KS];
+      load_slots<TZ, KS>(a + (int64_t)op.sxci[p] * k + j0, x);
+#pragma unroll
+      for (int q = 0; q < KS; ++q) { acc[q].x += g * (double)x[q].x; acc[q].y += g * (double)x[q].y; }
+    }
+  }
+  const double sii = op.sdiag[i], er = op.epsr[i], sg = op.sig[i];
+  Z2 bb[KS];
+  if (mode == 1) load_slots<double, KS>(b + i * k + j0, bb);
+#pragma unroll
+  for (int q = 0; q < KS; ++q) {
+    const double w = omega[j0 + q];
+    const double dr = reference_diag_re(sii, w, er), di = w * sg;
+    acc[q].x += dr * xi[q].x - di * xi[q].y;
+    acc[q].y += dr * xi[q].y + di * xi[q].x;
+    if (mode == 1) acc[q] = {bb[q].x - acc[q].x, bb[q].y - acc[q].y};
+  }
+  store_slots<double, KS>(out + i * k + j0, acc);
 }
 
 // Multi-vector dots.  Block (blk, c) handles vectors t in [8c, 8c+8) of V
@@ -289,6 +366,30 @@ OuterOp outer_op(gp_layout *lay, const Hierarchy &H) {
   return op;
 }
 
+// Outer operator launch: the SELL kernel when S is large (sell_min_rows), else
+// the (row, slot) CSR kernel.
+template <class TZ>
+void outer(const OuterOp &op, const Hierarchy &H, int64_t smin, int k, const double *om, const C2<TZ> *u,
+           const C2<TZ> *phi, const C2<TZ> *a, const Z2 *b, Z2 *out, int mode, int exact, hipStream_t s) {
+  if (op.n < smin || H.Ss.nr != op.n) {
+    k_outer<TZ><<<blocks_for(op.n * k), 256, 0, s>>>(op, k, om, u, phi, a, b, out, mode, exact);
+    return;
+  }
+  SellV<double> SS;
+  SS.nr = H.Ss.nr; SS.nslice = H.Ss.nslice; SS.base = H.Ss.base; SS.w = H.Ss.w; SS.ci = H.Ss.ci; SS.nv = 1;
+  SS.v[0] = H.Ss.v[0];
+  const unsigned gx = (unsigned)((op.n + 255) / 256);
+  int ks = k < 16 ? k : 16;
+#define GP_OUTER_SELL(KS) \
+  k_outer_sell<TZ, KS><<<dim3(gx, (unsigned)(k / KS)), 256, 0, s>>>(op, SS, k, om, u, phi, a, b, out, mode, exact)
+  if (ks == 16) GP_OUTER_SELL(16);
+  else if (ks == 8) GP_OUTER_SELL(8);
+  else if (ks == 4) GP_OUTER_SELL(4);
+  else if (ks == 2) GP_OUTER_SELL(2);
+  else GP_OUTER_SELL(1);
+#undef GP_OUTER_SELL
+}
+
 // Column-batched dot products on the device, copied to the host:
 // h[t*k + j] = V_t^H w for t < nvec, and h[nvec*k + j] = |w_j|^2 if self.
 struct Dots {
@@ -360,6 +461,7 @@ gp_status solve_slots(SolveCtx &cx, std::deque<Column> &pending, int k) {
   GP_TRY(vwork_create<T>(lay, H, k, &vw));
   vw.nu_aux = o.smooth_aux > 0 ? std::min(o.smooth_aux, 8) : 2;
   vw.kt = &lay->ktimer;
+  vw.sell_min_rows = o.sell_min_rows > 0 ? o.sell_min_rows : (int64_t(1) << 20);
   DevBuf D;
   Arena &A = D.A;
   Z2 *b, *xu, *xphi, *xa, *V, *yd;
@@ -484,7 +586,7 @@ gp_status solve_slots(SolveCtx &cx, std::deque<Column> &pending, int k) {
   }
   for (;;) {
     // explicit FP64 residual of the split iterate (S G, S Gx included exactly)
-    k_outer<double><<<gnk, 256, 0, s>>>(op, k, om_d, xu, xphi, xa, b, V, 1, 1);
+    outer<double>(op, H, vw.sell_min_rows, k, om_d, xu, xphi, xa, b, V, 1, 1, s);
     GP_TRY(dots.run(s, n, k, V, 0, 0, V, 1, h));
     std::vector<double> relres(k, 0.);
     std::vector<int32_t> done(k, 0);
@@ -501,7 +603,7 @@ gp_status solve_slots(SolveCtx &cx, std::deque<Column> &pending, int k) {
       // refill the finished slots (slots left without a column carry zero vectors)
       GP_TRY(refill(done));
       k_zero_cols<<<gnk, 256, 0, s>>>(n, k, mask_d, V);
-      k_outer<double><<<gnk, 256, 0, s>>>(op, k, om_d, xu, xphi, xa, b, V, 1, 1);
+      outer<double>(op, H, vw.sell_min_rows, k, om_d, xu, xphi, xa, b, V, 1, 1, s);
       GP_TRY(dots.run(s, n, k, V, 0, 0, V, 1, h));
       nactive = 0;
       for (int j = 0; j < k; ++j) {
@@ -527,7 +629,7 @@ gp_status solve_slots(SolveCtx &cx, std::deque<Column> &pending, int k) {
       k_copy_t<T><<<gnnk, 256, 0, s>>>(nn * k, vw.phi[0], zp);
       k_copy_t<T><<<gnak, 256, 0, s>>>(na * k, vw.a, za);
       // w = A z (split form, FP64) written into the next basis slot
-      k_outer<T><<<gnk, 256, 0, s>>>(op, k, om_d, zu, zp, za, nullptr, Vn, 0, 0);
+      outer<T>(op, H, vw.sell_min_rows, k, om_d, zu, zp, za, nullptr, Vn, 0, 0, s);
       // classical Gram-Schmidt with the norm from the same pass; a second pass
       // only when the first one cancelled more than half of |w|^2
       GP_TRY(dots.run(s, n, k, V, n * k, i + 1, Vn, 1, h));
@@ -676,6 +778,7 @@ extern "C" void gp_solve_opts_default(gp_solve_opts *o) {
   o->verbose = 0;
   o->smooth_edge = 0;
   o->smooth_aux = 0;
+  o->sell_min_rows = 0;
 }
 
 extern "C" gp_status gp_prepare_solver(gp_layout *lay, int32_t precision) {
@@ -733,10 +836,15 @@ extern "C" gp_status gp_solve_fullwave_ex(gp_layout *lay, const double *freqs, i
   GP_TRY(coarse_cache_lookup(sv->H, sv->cc, om, &cslot, lay->stream));
   auto t1 = std::chrono::steady_clock::now();
   gp_solve_stats st{};
-  // columns in request order, frequency-major: id = f*np + p
+  // column ids in request order, frequency-major: id = f*np + p.  They enter
+  // the slots highest frequency first: those need the most iterations, and
+  // started early they overlap the quick ones instead of forming the tail.
   const int ncol = nf * np;
   std::deque<Column> pending;
-  for (int32_t f = 0; f < nf; ++f)
+  std::vector<int32_t> forder(nf);
+  for (int32_t f = 0; f < nf; ++f) forder[f] = f;
+  std::stable_sort(forder.begin(), forder.end(), [&](int32_t a, int32_t b) { return fr[a] > fr[b]; });
+  for (int32_t f : forder)
     for (int32_t a = 0; a < np; ++a) pending.push_back({ports[a], f, f * np + a});
   // slot count: requested, else 16 for the mixed path and 1 for plain FP64,
   // bounded by the columns and by half of the free device memory

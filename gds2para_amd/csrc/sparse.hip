@@ -436,4 +436,122 @@ gp_status csr_host_copy(const Csr &a, HostCsr *h, int which) {
   return GP_OK;
 }
 
+namespace {
+
+__global__ void k_sell_width(int64_t nr, int64_t nslice, const int64_t *__restrict__ rp, const int32_t *__restrict__ lidx,
+                             int32_t *__restrict__ w, int64_t *__restrict__ cnt) {
+  int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sl >= nslice) return;
+  int64_t m = 0;
+  for (int64_t i = sl * 64; i < min(nr, sl * 64 + 64); ++i)
+    if (!(lidx && lidx[i] >= 0)) m = max(m, rp[i + 1] - rp[i]);
+  w[sl] = (int32_t)m;
+  cnt[sl] = 64 * m;
+}
+
+__global__ void k_sell_fill(int64_t nr, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci, int nv,
+                            const double *__restrict__ v0, const double *__restrict__ v1, const double *__restrict__ v2,
+                            const int32_t *__restrict__ lidx, const int64_t *__restrict__ base,
+                            const int32_t *__restrict__ w, int32_t *__restrict__ oci, double *__restrict__ o0,
+                            double *__restrict__ o1, double *__restrict__ o2) {
+  // one thread per slot row of every slice, so the lanes past the last row
+  // (never read by the kernels) hold valid padding too
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ((nr + 63) >> 6) << 6) return;
+  const int64_t sl = i >> 6, lane = i & 63;
+  const int64_t b = base[sl];
+  const bool skip = i >= nr || (lidx && lidx[i] >= 0);
+  const int64_t len = skip ? 0 : rp[i + 1] - rp[i];
+  for (int64_t t = 0; t < w[sl]; ++t) {
+    const int64_t e = b + 64 * t + lane;
+    if (t < len) {
+      const int64_t p = rp[i] + t;
+      oci[e] = ci[p];
+      o0[e] = v0[p];
+      if (nv > 1) o1[e] = v1[p];
+      if (nv > 2) o2[e] = v2[p];
+    } else {
+      // padding: value 0 at a column that exists (the row's last one, else
+      // column 0), so the gather it triggers stays inside the input vector
+      oci[e] = len > 0 ? ci[rp[i] + len - 1] : 0;
+      o0[e] = 0.;
+      if (nv > 1) o1[e] = 0.;
+      if (nv > 2) o2[e] = 0.;
+    }
+  }
+}
+
+// bad[0] += 1 for every stored column index outside [0, nc); bad[1], bad[2]:
+// smallest and largest index seen
+__global__ void k_sell_check(int64_t n, int64_t nc, const int32_t *__restrict__ ci, int *__restrict__ bad) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int c = ci[e];
+  if (c < 0 || (int64_t)c >= nc) atomicAdd(bad, 1);
+  atomicMin(bad + 1, c);
+  atomicMax(bad + 2, c);
+}
+
+__global__ void k_d2f_sell(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+}  // namespace
+
+gp_status sell_from_csr(Arena &A, const Csr &a, Sell *out, hipStream_t s, const char *name) {
+  *out = Sell();
+  out->nr = a.nr;
+  out->nv = a.nv;
+  out->nslice = (a.nr + 63) / 64;
+  if (a.nr == 0) return GP_OK;
+  Arena tmp;
+  gp_status st = GP_OK;
+  do {
+    int64_t *cnt = nullptr;
+    if ((st = A.alloc(&out->w, out->nslice)) != GP_OK) break;
+    if ((st = A.alloc(&out->base, out->nslice + 1)) != GP_OK) break;
+    if ((st = tmp.alloc(&cnt, out->nslice)) != GP_OK) break;
+    k_sell_width<<<blocks_for(out->nslice), 256, 0, s>>>(a.nr, out->nslice, a.rp, a.lidx, out->w, cnt);
+    if ((st = exclusive_scan_i64(cnt, out->base, out->nslice, s, tmp)) != GP_OK) break;
+    if (hipMemcpyAsync(&out->nent, out->base + out->nslice, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) { st = fail(GP_ERR_HIP, "sell size"); break; }
+    const int64_t ne = std::max<int64_t>(out->nent, 1);
+    if ((st = A.alloc(&out->ci, ne)) != GP_OK) break;
+    for (int q = 0; q < a.nv; ++q)
+      if ((st = A.alloc(&out->v[q], ne)) != GP_OK) break;
+    if (st != GP_OK) break;
+    k_sell_fill<<<blocks_for(out->nslice * 64), 256, 0, s>>>(a.nr, a.rp, a.ci, a.nv, a.v[0], a.v[1], a.v[2], a.lidx, out->base,
+                                                 out->w, out->ci, out->v[0], out->v[1], out->v[2]);
+    // every column the kernels will gather from must exist
+    int *bad = nullptr, hb[3] = {0, 2147483647, -1};
+    if ((st = tmp.alloc(&bad, 3)) != GP_OK) break;
+    if (hipMemcpyAsync(bad, hb, sizeof(hb), hipMemcpyHostToDevice, s) != hipSuccess) { st = fail(GP_ERR_HIP, "copy"); break; }
+    if (out->nent) k_sell_check<<<blocks_for(out->nent), 256, 0, s>>>(out->nent, a.nc, out->ci, bad);
+    if (hipMemcpyAsync(hb, bad, sizeof(hb), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) { st = fail(GP_ERR_HIP, "sell fill"); break; }
+    if (hb[0]) {
+      st = fail(GP_ERR_STATE, std::string("SELL copy of ") + name + " (" + std::to_string(a.nr) + "x" +
+                                  std::to_string(a.nc) + ", nnz " + std::to_string(a.nnz) + ", " +
+                                  std::to_string(out->nent) + " slots) holds " + std::to_string(hb[0]) +
+                                  " out-of-range column indices, range [" + std::to_string(hb[1]) + ", " +
+                                  std::to_string(hb[2]) + "]");
+      break;
+    }
+  } while (0);
+  tmp.free_all();
+  return st;
+}
+
+gp_status sell_to_float(Arena &A, const Sell &a, SellF *f, hipStream_t s) {
+  *f = SellF();
+  f->nr = a.nr; f->nslice = a.nslice; f->nent = a.nent; f->base = a.base; f->w = a.w; f->ci = a.ci; f->nv = a.nv;
+  for (int q = 0; q < a.nv; ++q) {
+    GP_TRY(A.alloc(&f->v[q], std::max<int64_t>(a.nent, 1)));
+    if (a.nent) k_d2f_sell<<<blocks_for(a.nent), 256, 0, s>>>(a.nent, a.v[q], f->v[q]);
+  }
+  GP_HIP(hipStreamSynchronize(s));
+  return GP_OK;
+}
+
 }  // namespace gp

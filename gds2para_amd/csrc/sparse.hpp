@@ -15,7 +15,7 @@ namespace gp {
 // the row's position in that list or -1.  Row-parallel kernels take those rows'
 // products from a block-per-row reduction instead of walking them serially.
 constexpr int64_t kLongRow = 256;
-constexpr int64_t kLongChunk = 2048;  // entries per block of the long-row reduction
+constexpr int64_t kLongChunk = 256;   // entries per block of the long-row reduction
 
 struct Csr {
   int64_t nr = 0, nc = 0, nnz = 0;
@@ -30,6 +30,8 @@ struct Csr {
   int32_t nchunk = 0;
   int64_t *lcb = nullptr, *lce = nullptr;
   int32_t *lqp = nullptr;
+  // when set, value blocks 1 and 2 are diagonal and live here (one per row)
+  double *dg[2] = {nullptr, nullptr};
 };
 
 // FP32 copy of a Csr's values; row pointers and columns are shared.
@@ -44,7 +46,33 @@ struct CsrF {
   int32_t nchunk = 0;
   const int64_t *lcb = nullptr, *lce = nullptr;
   const int32_t *lqp = nullptr;
+  float *dg[2] = {nullptr, nullptr};
 };
+
+// Sliced ELL with 64-row slices (one wavefront): slice s holds w[s] entry
+// columns; entry t of row 64 s + l sits at base[s] + 64 t + l, so a wavefront
+// reads each entry column of its 64 rows as one coalesced 256-byte line.
+// Rows shorter than the slice width are padded with their own index and value 0.
+struct Sell {
+  int64_t nr = 0, nslice = 0, nent = 0;
+  int64_t *base = nullptr;
+  int32_t *w = nullptr;
+  int32_t *ci = nullptr;
+  int nv = 0;
+  double *v[3] = {nullptr, nullptr, nullptr};
+};
+struct SellF {
+  int64_t nr = 0, nslice = 0, nent = 0;
+  const int64_t *base = nullptr;
+  const int32_t *w = nullptr;
+  const int32_t *ci = nullptr;
+  int nv = 0;
+  float *v[3] = {nullptr, nullptr, nullptr};
+};
+// SELL copy of a CSR (all value arrays); rows listed as long in a (kLongRow)
+// are left empty, their products come from the long-row pass
+gp_status sell_from_csr(Arena &A, const Csr &a, Sell *out, hipStream_t s, const char *name);
+gp_status sell_to_float(Arena &A, const Sell &a, SellF *f, hipStream_t s);
 
 gp_status csr_upload(Arena &A, const HostCsr &h, Csr *out, hipStream_t s);
 gp_status csr_alloc(Arena &A, int64_t nr, int64_t nc, int64_t nnz, int nv, Csr *out);

@@ -1,0 +1,72 @@
+"""Writes the golden fixtures under tests/golden/ from the oracle.
+
+Fixtures (numpy .npz, no pickles):
+  microstrip_60x20x8.npz   S pattern/value digests, V0 sizes, Z at 1 GHz and the
+                           port-edge values of x (refined FP64 direct solve)
+  manhattan_24x20x10.npz   Z at 1e8, 1e9, 1e10 Hz for 3 ports (refined direct
+                           solves) and the port-edge values of every column
+
+The oracle's own checks (tests/test_oracle.py) pin it independently; these
+files freeze its results so the GPU tests compare against recorded vectors
+without re-running the slow refined direct solves.
+
+Run: python tests/golden/make_golden.py
+"""
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import oracle  # noqa: E402
+from gds2para_amd import mesh as M  # noqa: E402
+
+
+def port_rows(m):
+    """Renumbered rows of every port's first side, in Construct_Z_V0_Vh order."""
+    mp, _ = m.edge_map()
+    nsd = int(m.sizes.num_sides)
+    psp = np.ctypeslib.as_array(m.c.port_side_ptr, shape=(m.num_ports + 1,))
+    sep = np.ctypeslib.as_array(m.c.side_edge_ptr, shape=(nsd + 1,))
+    se = np.ctypeslib.as_array(m.c.side_edges, shape=(int(sep[-1]),))
+    out = []
+    for p in range(m.num_ports):
+        s0 = psp[p]
+        out.extend(mp[se[sep[s0]:sep[s0 + 1]]].tolist())
+    return np.array(out, np.int64)
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def main():
+    m = M.Mesh(M.microstrip_spec())
+    S = oracle.stiffness_csr(m)
+    vs = oracle.v0_sizes(m)
+    f = [float(m.freqs()[0])]
+    X, Z = oracle.direct_solve(m, f, S=S, refine=True)
+    rows = port_rows(m)
+    np.savez(os.path.join(HERE, "microstrip_60x20x8.npz"),
+             nnz=np.int64(S.nnz), indptr_sha=np.array(digest(S.indptr.astype(np.int64))),
+             indices_sha=np.array(digest(S.indices.astype(np.int32))), data_sha=np.array(digest(S.data)),
+             data_sum=np.float64(S.data.sum()), data_absmax=np.float64(np.abs(S.data).max()),
+             v0=np.array([vs[k] for k in ("v0d_nnz", "v0d_cols", "ad_nnz", "ad_len", "v0c_nnz", "v0c_cols", "ac_nnz")],
+                         np.int64),
+             freqs=np.array(f), Z=Z, port_rows=rows, x_port=np.array([X[0][0][rows]]))
+    m = M.Mesh(M.manhattan_spec((24, 20, 10), n_ports=3, nfreq=3, f_range=(1e8, 1e10), seed=11))
+    fs = m.freqs()
+    X, Z = oracle.direct_solve(m, fs, refine=True)
+    rows = port_rows(m)
+    xp = np.array([[X[k][p][rows] for p in range(m.num_ports)] for k in range(len(fs))])
+    np.savez(os.path.join(HERE, "manhattan_24x20x10.npz"), freqs=fs, Z=Z, port_rows=rows, x_port=xp)
+    print("written")
+
+
+if __name__ == "__main__":
+    main()
