@@ -57,7 +57,7 @@ def cpu_baseline(cells=(32, 32, 12)):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="manhattan_200x200x24")
     ap.add_argument("--batch", type=int, default=0)
@@ -173,7 +173,7 @@ def main():
         "speedup_vs_fp64_single": value / fp64["value"] if fp64 else None,
         "z_rel_diff_mixed_vs_fp64": zdiff,
         "copy_probe_GBs": copy_gbs,
-        "roofline": {"bound": "hbm", "kernel": "level-0 edge smoother (k_edge<float>)",
+        "roofline": {"bound": "hbm", "kernel": "level-0 edge smoother (k_edge_sell<float, 16>)",
                      "achieved": kbytes / (kms * 1e-3) / 1e9 if kms > 0 else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s",
                      "frac": (kbytes / (kms * 1e-3) / 1e9) / HBM_PEAK_GBS if kms > 0 else None,
