@@ -87,6 +87,15 @@ class GpColumnReport(C.Structure):
     _fields_ = [("iters", i32), ("converged", i32), ("rel_residual", f64), ("start_ms", f64), ("done_ms", f64)]
 
 
+class GpStreamColumn(C.Structure):
+    _fields_ = [("port", i32), ("freq", f64), ("tag", i64)]
+
+
+PULL_FN = C.CFUNCTYPE(i32, C.c_void_p, C.POINTER(GpStreamColumn), i32)
+WAITING_FN = C.CFUNCTYPE(i64, C.c_void_p)
+PUSH_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(GpStreamColumn), C.POINTER(C.c_double), C.POINTER(GpColumnReport))
+
+
 class GpSolveStats(C.Structure):
     _fields_ = [
         ("columns", i32), ("converged", i32), ("max_iters_used", i32), ("batches", i32),
@@ -123,6 +132,9 @@ SIGNATURES = [
     ("gp_solve_fullwave", i32, [C.c_void_p, P(f64), i32, P(i32), i32, P(GpSolveOpts), P(f64), P(f64), P(GpSolveStats)]),
     ("gp_solve_fullwave_ex", i32, [C.c_void_p, P(f64), i32, P(i32), i32, P(GpSolveOpts), P(f64), P(f64),
                                    P(GpSolveStats), P(GpColumnReport)]),
+    ("gp_solve_workspace_bytes", i32, [C.c_void_p, i32, i32, i32, P(i64)]),
+    ("gp_solve_stream", i32, [C.c_void_p, P(GpSolveOpts), i32, PULL_FN, WAITING_FN, PUSH_FN, C.c_void_p,
+                              P(GpSolveStats)]),
     ("gp_extract_z", i32, [P(GpMesh), P(f64), i32, P(f64)]),
     ("gp_solve_real", i32, [P(i64), P(i64), P(f64), i64, i64, P(f64), P(f64), f64, P(i32), P(f64)]),
     ("gp_bench_spmv", i32, [C.c_void_p, i32, i32, i32, P(f64)]),

@@ -415,17 +415,26 @@ struct Dots {
   }
 };
 
-struct Column { int32_t port; int32_t fslot; int32_t id; };
+struct Column { int32_t port; double freq; int64_t id; };
+
+// Where the slots take their columns from and hand the results back to.
+struct ColumnIO {
+  virtual ~ColumnIO() {}
+  // 1: c filled; 0: nothing waiting right now; -1: no further columns.
+  // block: every slot is idle, so the call may wait for work to arrive
+  virtual int next(Column &c, bool block) = 0;
+  virtual int64_t waiting() = 0;
+  virtual bool want_x() const = 0;
+  // xport: port-edge values (Construct_Z_V0_Vh order); x: full solution
+  // (n values at stride xs) or null
+  virtual gp_status done(const Column &c, const std::vector<cd> &xport, const Z2 *x, int64_t xs,
+                         const gp_column_report &r) = 0;
+};
 
 struct SolveCtx {
   gp_layout *lay;
   Solver *sv;
   const gp_solve_opts *o;
-  const std::vector<double> *freqs;        // Hz, per frequency of the request
-  const std::vector<int32_t> *cslot;       // CoarseCache slot per request frequency
-  double *X;                               // optional full solutions, column id order
-  std::vector<std::vector<cd>> *xport;     // port-edge values per column id
-  gp_column_report *rep;                   // per column id
   gp_solve_stats *st;
   std::chrono::steady_clock::time_point t0;
 };
@@ -442,9 +451,9 @@ std::vector<int64_t> port_rows(gp_layout *lay) {
   return idx;
 }
 
-// Solve the pending columns on k slots with preconditioner precision T.
+// Solve the columns io provides on k slots with preconditioner precision T.
 template <class T>
-gp_status solve_slots(SolveCtx &cx, std::deque<Column> &pending, int k) {
+gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
   gp_layout *lay = cx.lay;
   const Hierarchy &H = cx.sv->H;
   const gp_solve_opts &o = *cx.o;
@@ -485,34 +494,42 @@ gp_status solve_slots(SolveCtx &cx, std::deque<Column> &pending, int k) {
   GP_TRY(A.alloc(&prow_d, (int64_t)prow.size())); GP_TRY(A.alloc(&xo, (int64_t)prow.size() * k));
   GP_HIP(hipMemcpyAsync(prow_d, prow.data(), sizeof(int64_t) * prow.size(), hipMemcpyHostToDevice, s));
   Z2 *xfull = nullptr;
-  if (cx.X) GP_TRY(A.alloc(&xfull, n * k));
+  if (io.want_x()) GP_TRY(A.alloc(&xfull, n * k));
 
   const unsigned gnk = blocks_for(n * k), gnnk = blocks_for(nn * k), gnak = blocks_for(na * k);
   // slot state
   std::vector<int32_t> cid(k, -1), its(k, 0), fsl(k, 0);
+  std::vector<Column> ccol(k);
   std::vector<double> t_in(k, 0.);   // when each slot's column entered (ms since the call)
   auto now_ms = [&]() { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - cx.t0).count(); };
-  std::vector<double> bnorm(k, 0.), omega(k, 2. * kPi * (*cx.freqs)[0]), beta(k, 0.);
+  std::vector<double> bnorm(k, 0.), omega(k, 2. * kPi * 1e9), beta(k, 0.);
+  bool closed = false;  // the source will deliver no further columns
   std::vector<cd> Hm((size_t)(m + 1) * m * k), cs((size_t)m * k), sn((size_t)m * k), g((size_t)(m + 1) * k), y((size_t)m * k);
   std::vector<double> sch(k);
   std::vector<cd> h, h2;
   std::vector<Z2> yh((size_t)m * k);
   int cycles = 0;
 
-  // place pending columns into the slots listed in mask (idle when none is left)
-  auto refill = [&](const std::vector<int32_t> &mask) -> gp_status {
+  // place new columns into the slots listed in mask (idle when none comes)
+  auto refill = [&](const std::vector<int32_t> &mask, bool block) -> gp_status {
     std::vector<int64_t> row;
     std::vector<int32_t> col;
     std::vector<double> val;
     for (int j = 0; j < k; ++j) {
       if (!mask[j]) continue;
-      if (pending.empty()) { cid[j] = -1; bnorm[j] = 0.; continue; }
-      Column c = pending.front();
-      pending.pop_front();
-      cid[j] = c.id; its[j] = 0; t_in[j] = now_ms();
-      const double f = (*cx.freqs)[c.fslot];
+      Column c;
+      const int got = closed ? -1 : io.next(c, block);
+      if (got < 0) closed = true;
+      if (got != 1) { cid[j] = -1; bnorm[j] = 0.; continue; }
+      ccol[j] = c;
+      cid[j] = 1; its[j] = 0; t_in[j] = now_ms();
+      const double f = c.freq;
       omega[j] = 2. * kPi * f;
-      fsl[j] = (*cx.cslot)[c.fslot];
+      {
+        std::vector<int32_t> sl;
+        GP_TRY(coarse_cache_lookup(H, cx.sv->cc, std::vector<double>{omega[j]}, &sl, s));
+        fsl[j] = sl[0];
+      }
       double b2 = 0.;
       for (int32_t sd = mesh.port_side_ptr[c.port]; sd < mesh.port_side_ptr[c.port + 1]; ++sd)
         for (int64_t q = mesh.side_edge_ptr[sd]; q < mesh.side_edge_ptr[sd + 1]; ++q) {
@@ -552,26 +569,20 @@ gp_status solve_slots(SolveCtx &cx, std::deque<Column> &pending, int k) {
     std::vector<Z2> xh(prow.size() * k);
     GP_HIP(hipMemcpyAsync(xh.data(), xo, sizeof(Z2) * xh.size(), hipMemcpyDeviceToHost, s));
     std::vector<Z2> xf;
-    if (cx.X) {
+    if (io.want_x()) {
       k_assemble_x<<<gnk, 256, 0, s>>>(n, nullptr, k, G0.rp, G0.ci, G0.v[0], Gx.rp, Gx.ci, Gx.v[0], xu, xphi, xa, xfull);
       xf.resize(n * k);
       GP_HIP(hipMemcpyAsync(xf.data(), xfull, sizeof(Z2) * n * k, hipMemcpyDeviceToHost, s));
     }
     GP_HIP(hipStreamSynchronize(s));
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - cx.t0).count();
+    std::vector<cd> xp(prow.size());
     for (int j = 0; j < k; ++j) {
       if (!mask[j] || cid[j] < 0) continue;
-      const int id = cid[j];
-      auto &xp = (*cx.xport)[id];
-      xp.resize(prow.size());
       for (size_t a = 0; a < prow.size(); ++a) xp[a] = cd(xh[a * k + j].x, xh[a * k + j].y);
-      if (cx.X)
-        for (int64_t i = 0; i < n; ++i) {
-          cx.X[2 * ((int64_t)id * n + i)] = xf[i * k + j].x;
-          cx.X[2 * ((int64_t)id * n + i) + 1] = xf[i * k + j].y;
-        }
       const bool ok = relres[j] <= o.rtol;
-      cx.rep[id] = gp_column_report{its[j], ok ? 1 : 0, relres[j], t_in[j], ms};
+      GP_TRY(io.done(ccol[j], xp, io.want_x() ? xf.data() + j : nullptr, k,
+                     gp_column_report{its[j], ok ? 1 : 0, relres[j], t_in[j], ms}));
       cx.st->columns += 1;
       cx.st->converged += ok ? 1 : 0;
       cx.st->max_iters_used = std::max(cx.st->max_iters_used, its[j]);
@@ -582,9 +593,19 @@ gp_status solve_slots(SolveCtx &cx, std::deque<Column> &pending, int k) {
 
   {  // initial fill
     std::vector<int32_t> all(k, 1);
-    GP_TRY(refill(all));
+    GP_TRY(refill(all, false));
   }
   for (;;) {
+    {  // with every slot idle, wait for the source (its next() may block)
+      bool any = false;
+      for (int j = 0; j < k; ++j) any = any || cid[j] >= 0;
+      while (!any && !closed) {
+        std::vector<int32_t> all(k, 1);
+        GP_TRY(refill(all, true));
+        for (int j = 0; j < k; ++j) any = any || cid[j] >= 0;
+      }
+      if (!any) break;
+    }
     // explicit FP64 residual of the split iterate (S G, S Gx included exactly)
     outer<double>(op, H, vw.sell_min_rows, k, om_d, xu, xphi, xa, b, V, 1, 1, s);
     GP_TRY(dots.run(s, n, k, V, 0, 0, V, 1, h));
@@ -598,10 +619,17 @@ gp_status solve_slots(SolveCtx &cx, std::deque<Column> &pending, int k) {
       if (relres[j] <= o.rtol || its[j] >= o.max_iters) { done[j] = 1; ++ndone; }
       else ++nactive;
     }
-    if (ndone) {
-      GP_TRY(finish(done, relres));
-      // refill the finished slots (slots left without a column carry zero vectors)
-      GP_TRY(refill(done));
+    if (ndone) GP_TRY(finish(done, relres));
+    // refill the finished slots and, while columns may still come, the idle
+    // ones (slots left without a column carry zero vectors)
+    std::vector<int32_t> fill(k, 0);
+    bool anyfill = false;
+    for (int j = 0; j < k; ++j) {
+      fill[j] = done[j] || (cid[j] < 0 && !closed && io.waiting() > 0);
+      anyfill = anyfill || fill[j];
+    }
+    if (anyfill) {
+      GP_TRY(refill(fill, false));
       k_zero_cols<<<gnk, 256, 0, s>>>(n, k, mask_d, V);
       outer<double>(op, H, vw.sell_min_rows, k, om_d, xu, xphi, xa, b, V, 1, 1, s);
       GP_TRY(dots.run(s, n, k, V, 0, 0, V, 1, h));
@@ -611,7 +639,7 @@ gp_status solve_slots(SolveCtx &cx, std::deque<Column> &pending, int k) {
         if (cid[j] >= 0) ++nactive;
       }
     }
-    if (nactive == 0) break;
+    if (nactive == 0) continue;  // every slot idle: the wait at the top decides
     ++cycles;
     for (int j = 0; j < k; ++j) sch[j] = (cid[j] >= 0 && beta[j] > 0) ? 1.0 / beta[j] : 0.0;
     GP_HIP(hipMemcpyAsync(sc_d, sch.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
@@ -693,7 +721,7 @@ gp_status solve_slots(SolveCtx &cx, std::deque<Column> &pending, int k) {
       used = i + 1;
       if (all_in) break;
       // with columns waiting, restart once half of the slots are done
-      if (!pending.empty()) {
+      if (io.waiting() > 0) {
         int conv = 0;
         for (int j = 0; j < k; ++j)
           if (cid[j] < 0 || std::abs(g[(i + 1) * k + j]) <= o.rtol * bnorm[j] || its[j] >= o.max_iters) ++conv;
@@ -719,8 +747,8 @@ gp_status solve_slots(SolveCtx &cx, std::deque<Column> &pending, int k) {
       double est = 0.;
       for (int j = 0; j < k; ++j)
         if (cid[j] >= 0 && bnorm[j] > 0) est = std::max(est, std::abs(g[used * k + j]) / bnorm[j]);
-      std::fprintf(stderr, "[gp] k=%d cycle %d: %d steps, active %d, pending %zu, est %.3e\n", k, cycles, used,
-                   nactive, pending.size(), est);
+      std::fprintf(stderr, "[gp] k=%d cycle %d: %d steps, active %d, waiting %lld, est %.3e\n", k, cycles, used,
+                   nactive, (long long)io.waiting(), est);
     }
   }
   GP_HIP(hipGetLastError());
@@ -728,29 +756,103 @@ gp_status solve_slots(SolveCtx &cx, std::deque<Column> &pending, int k) {
   return GP_OK;
 }
 
-// Z for one finished column (Construct_Z_V0_Vh on the first side of every port,
-// src/fdtd.hpp:4133-4170): Z[f][resp + np*src] = -sum x*leng*dir / area(src)
-void column_z(gp_layout *lay, const Column &c, const std::vector<cd> &xp, double *Z) {
+// Z column of one finished column (Construct_Z_V0_Vh on the first side of
+// every port, src/fdtd.hpp:4133-4170): z[resp] = -sum x*leng*dir / area(src)
+void z_column(gp_layout *lay, int32_t src, const std::vector<cd> &xp, cd *z) {
   const gp_mesh &mesh = lay->hm.view;
-  const int32_t npt = mesh.num_ports, src = c.port, f = c.fslot;
+  const int32_t npt = mesh.num_ports;
   double cur = 0.;
   for (int32_t sd = mesh.port_side_ptr[src]; sd < mesh.port_side_ptr[src + 1]; ++sd) cur += mesh.side_area[sd];
   size_t a = 0;
   for (int32_t p = 0; p < npt; ++p) {
     const int32_t s0 = mesh.port_side_ptr[p];
-    cd z(0., 0.);
+    cd zz(0., 0.);
     for (int64_t q = mesh.side_edge_ptr[s0]; q < mesh.side_edge_ptr[s0 + 1]; ++q, ++a) {
       int dir; int64_t iz, ix, iy;
       lay->g.decode(mesh.side_edges[q], dir, iz, ix, iy);
       double leng = dir == 2 ? mesh.zn[iz + 1] - mesh.zn[iz] : dir == 1 ? mesh.xn[ix + 1] - mesh.xn[ix] : mesh.yn[iy + 1] - mesh.yn[iy];
-      z -= xp[a] * leng * (mesh.side_dir[s0] * 1.0);
+      zz -= xp[a] * leng * (mesh.side_dir[s0] * 1.0);
     }
-    z /= cur;
-    size_t zi = (size_t)f * npt * npt + p + (size_t)npt * src;
-    Z[2 * zi] = z.real();
-    Z[2 * zi + 1] = z.imag();
+    z[p] = zz / cur;
   }
 }
+
+// Columns of one gp_solve_fullwave call: a queue (highest frequency first),
+// results into the caller's Z / X arrays and per-column reports.
+struct QueueIO : ColumnIO {
+  gp_layout *lay;
+  std::deque<Column> q;
+  const int32_t *ports;
+  int32_t np, npt;
+  double *Z, *X;
+  int64_t n;
+  gp_column_report *rep;
+  int next(Column &c, bool) override {
+    if (q.empty()) return -1;
+    c = q.front();
+    q.pop_front();
+    return 1;
+  }
+  int64_t waiting() override { return (int64_t)q.size(); }
+  bool want_x() const override { return X != nullptr; }
+  gp_status done(const Column &c, const std::vector<cd> &xp, const Z2 *x, int64_t xs,
+                 const gp_column_report &r) override {
+    const int64_t id = c.id, f = id / np, a = id % np;
+    rep[id] = r;
+    if (Z) {
+      std::vector<cd> z(npt);
+      z_column(lay, ports[a], xp, z.data());
+      for (int32_t p = 0; p < npt; ++p) {
+        const size_t zi = (size_t)f * npt * npt + p + (size_t)npt * ports[a];
+        Z[2 * zi] = z[p].real();
+        Z[2 * zi + 1] = z[p].imag();
+      }
+    }
+    if (X && x)
+      for (int64_t i = 0; i < n; ++i) {
+        X[2 * (id * n + i)] = x[i * xs].x;
+        X[2 * (id * n + i) + 1] = x[i * xs].y;
+      }
+    return GP_OK;
+  }
+};
+
+// Columns from the caller's callbacks (gp_solve_stream).
+struct CallbackIO : ColumnIO {
+  gp_layout *lay;
+  gp_pull_fn pull;
+  gp_waiting_fn wait_fn;
+  gp_push_fn push;
+  void *user;
+  int32_t npt;
+  std::vector<gp_stream_column> inflight;  // by column id
+  int next(Column &c, bool block) override {
+    gp_stream_column sc{};
+    for (;;) {
+      const int r = pull(user, &sc, block ? 1 : 0);
+      if (r != 1) return r < 0 ? -1 : 0;
+      if (sc.port >= 0 && sc.port < npt && sc.freq > 0) break;
+      // a malformed column comes straight back as a failed result
+      gp_column_report bad{0, 0, -1.0, 0., 0.};
+      push(user, &sc, nullptr, &bad);
+    }
+    c.port = sc.port;
+    c.freq = sc.freq;
+    c.id = (int64_t)inflight.size();
+    inflight.push_back(sc);
+    return 1;
+  }
+  int64_t waiting() override { return wait_fn ? wait_fn(user) : 0; }
+  bool want_x() const override { return false; }
+  gp_status done(const Column &c, const std::vector<cd> &xp, const Z2 *, int64_t, const gp_column_report &r) override {
+    std::vector<cd> z(npt);
+    z_column(lay, c.port, xp, z.data());
+    std::vector<double> zz(2 * (size_t)npt);
+    for (int32_t p = 0; p < npt; ++p) { zz[2 * p] = z[p].real(); zz[2 * p + 1] = z[p].imag(); }
+    push(user, &inflight[c.id], zz.data(), &r);
+    return GP_OK;
+  }
+};
 
 // Device bytes one slot of the batched solve needs (basis, preconditioned
 // basis, iterate, V-cycle vectors), for the choice of the slot count.
@@ -840,12 +942,15 @@ extern "C" gp_status gp_solve_fullwave_ex(gp_layout *lay, const double *freqs, i
   // the slots highest frequency first: those need the most iterations, and
   // started early they overlap the quick ones instead of forming the tail.
   const int ncol = nf * np;
-  std::deque<Column> pending;
+  QueueIO io;
+  io.lay = lay; io.ports = ports; io.np = np; io.npt = mesh.num_ports; io.Z = Z; io.X = X; io.n = lay->n;
+  std::vector<gp_column_report> rep(ncol, gp_column_report{0, 0, 0., 0., 0.});
+  io.rep = rep.data();
   std::vector<int32_t> forder(nf);
   for (int32_t f = 0; f < nf; ++f) forder[f] = f;
   std::stable_sort(forder.begin(), forder.end(), [&](int32_t a, int32_t b) { return fr[a] > fr[b]; });
   for (int32_t f : forder)
-    for (int32_t a = 0; a < np; ++a) pending.push_back({ports[a], f, f * np + a});
+    for (int32_t a = 0; a < np; ++a) io.q.push_back({ports[a], fr[f], (int64_t)f * np + a});
   // slot count: requested, else 16 for the mixed path and 1 for plain FP64,
   // bounded by the columns and by half of the free device memory
   const int m = std::min(std::max(2, o.restart), 64);
@@ -860,20 +965,11 @@ extern "C" gp_status gp_solve_fullwave_ex(gp_layout *lay, const double *freqs, i
     }
   }
   const int k = pow2_at_least(kb);
-  std::vector<std::vector<cd>> xport(ncol);
-  std::vector<gp_column_report> rep(ncol, gp_column_report{0, 0, 0., 0., 0.});
-  SolveCtx cx{lay, sv, &o, &fr, &cslot, X, &xport, rep.data(), &st, t0};
-  gp_status s2 = f32 ? solve_slots<float>(cx, pending, k) : solve_slots<double>(cx, pending, k);
-  if (s2 != GP_OK) return s2;
   const int32_t npt = mesh.num_ports;
-  if (Z) {
-    std::fill(Z, Z + 2 * (size_t)nf * npt * npt, 0.0);
-    for (int32_t f = 0; f < nf; ++f)
-      for (int32_t a = 0; a < np; ++a) {
-        const int id = f * np + a;
-        if (!xport[id].empty()) column_z(lay, Column{ports[a], f, id}, xport[id], Z);
-      }
-  }
+  if (Z) std::fill(Z, Z + 2 * (size_t)nf * npt * npt, 0.0);
+  SolveCtx cx{lay, sv, &o, &st, t0};
+  gp_status s2 = f32 ? solve_slots<float>(cx, io, k) : solve_slots<double>(cx, io, k);
+  if (s2 != GP_OK) return s2;
   if (reports)
     for (int id = 0; id < ncol; ++id) reports[id] = rep[id];
   auto t2 = std::chrono::steady_clock::now();
@@ -881,6 +977,37 @@ extern "C" gp_status gp_solve_fullwave_ex(gp_layout *lay, const double *freqs, i
   st.solve_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
   if (stats) *stats = st;
   return st.converged == st.columns ? GP_OK : fail(GP_ERR_NOCONV, "some columns stopped above the tolerance");
+}
+
+extern "C" gp_status gp_solve_workspace_bytes(gp_layout *lay, int32_t precision, int32_t slots, int32_t restart,
+                                              int64_t *bytes) {
+  if (!lay || !bytes || slots < 1) return fail(GP_ERR_ARG, "bad arguments");
+  GP_TRY(gp_prepare_solver(lay, precision));
+  const int m = std::min(std::max(2, restart), 64);
+  const int k = pow2_at_least(std::min<int32_t>(slots, 64));
+  *bytes = (int64_t)k * bytes_per_slot(lay->solver->H, m, lay->solver->f32 && precision == GP_PREC_MIXED);
+  return GP_OK;
+}
+
+extern "C" gp_status gp_solve_stream(gp_layout *lay, const gp_solve_opts *opts, int32_t slots, gp_pull_fn pull,
+                                     gp_waiting_fn waiting, gp_push_fn push, void *user, gp_solve_stats *stats) {
+  if (!lay || !pull || !push || slots < 1) return fail(GP_ERR_ARG, "bad stream arguments");
+  gp_solve_opts o;
+  if (opts) o = *opts; else gp_solve_opts_default(&o);
+  if (!(o.rtol > 0) || o.max_iters < 1) return fail(GP_ERR_ARG, "rtol and max_iters must be positive");
+  auto t0 = std::chrono::steady_clock::now();
+  GP_TRY(gp_prepare_solver(lay, o.precision));
+  Solver *sv = lay->solver;
+  const bool f32 = sv->f32 && o.precision == GP_PREC_MIXED;
+  CallbackIO io;
+  io.lay = lay; io.pull = pull; io.wait_fn = waiting; io.push = push; io.user = user; io.npt = lay->hm.view.num_ports;
+  gp_solve_stats st{};
+  SolveCtx cx{lay, sv, &o, &st, t0};
+  const int k = pow2_at_least(std::min<int32_t>(slots, 64));
+  gp_status s2 = f32 ? solve_slots<float>(cx, io, k) : solve_slots<double>(cx, io, k);
+  st.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) *stats = st;
+  return s2;
 }
 
 extern "C" gp_status gp_solve_fullwave(gp_layout *lay, const double *freqs, int32_t nf, const int32_t *ports, int32_t np,

@@ -1,0 +1,297 @@
+"""Request-serving loop with continuous batching.
+
+A request asks for the Z columns of some ports of one layout at some
+frequencies.  The server keeps layouts resident on the GPU (stiffness, V0
+basis, multigrid hierarchy, per-frequency coarse inverses) while their memory
+fits a budget, and feeds every resident layout's columns to one long-running
+gp_solve_stream call on that layout's own HIP stream: a column that arrives
+while others are on the GPU takes the next slot that frees up at an FGMRES
+restart, and layouts run side by side on the device.
+
+Admission: a request for a layout that is not resident waits until the
+layout's bytes (hierarchy plus slot workspace) fit the budget, evicting idle
+resident layouts least recently used first.
+
+The reference has no serving path (it solves one layout, one frequency at a
+time); this module is the host side of the gp_solve_stream entry point.
+"""
+from __future__ import annotations
+
+import collections
+import ctypes as C
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _capi as capi
+
+
+@dataclass
+class Request:
+    """Z columns of `ports` at `freqs` (Hz) for layout `layout`."""
+    layout: str
+    ports: list
+    freqs: list
+    rid: int = -1
+    arrival: float = 0.0
+    done: float = 0.0
+    Z: np.ndarray = None           # [nf, np_layout, np_layout] complex, Z[f, resp, src]
+    reports: dict = field(default_factory=dict)
+    remaining: int = 0
+    error: str = ""
+
+    @property
+    def latency(self):
+        return self.done - self.arrival
+
+
+class _Resident:
+    """One layout on the device, its column queue and its solve thread."""
+
+    def __init__(self, key, handle, nbytes):
+        self.key, self.handle, self.nbytes = key, handle, nbytes
+        self.q = collections.deque()       # (port, freq, tag)
+        self.inflight = 0
+        self.closed = False
+        self.cv = threading.Condition()
+        self.last_used = time.monotonic()
+        self.thread = None
+        self.error = None
+
+
+class Server:
+    """Continuous-batching server over a backend (GpuBackend for the library).
+
+    backend.open(key) -> handle; backend.bytes(handle) -> device bytes incl.
+    the slot workspace; backend.stream(handle, pull, waiting, push) runs until
+    pull returns -1; backend.close(handle)."""
+
+    def __init__(self, backend, budget_bytes, clock=time.monotonic):
+        self.backend = backend
+        self.budget = budget_bytes
+        self.clock = clock
+        self.res = {}                      # key -> _Resident
+        self.backlog = collections.deque() # requests waiting for memory
+        self.requests = {}
+        self.tags = {}                     # tag -> (rid, freq index, port)
+        self.lock = threading.RLock()
+        self.all_done = threading.Condition(self.lock)
+        self._next_rid = 0
+        self._next_tag = 0
+
+    # ---- public ----------------------------------------------------------
+    def submit(self, req: Request) -> int:
+        with self.lock:
+            req.rid = self._next_rid
+            self._next_rid += 1
+            req.arrival = self.clock()
+            req.remaining = len(req.ports) * len(req.freqs)
+            self.requests[req.rid] = req
+            self.backlog.append(req)
+            self._admit()
+            return req.rid
+
+    def wait(self, timeout=None):
+        """Blocks until every submitted request has finished."""
+        end = None if timeout is None else self.clock() + timeout
+        with self.lock:
+            while any(r.remaining > 0 for r in self.requests.values()):
+                left = None if end is None else end - self.clock()
+                if left is not None and left <= 0:
+                    raise TimeoutError("requests still running")
+                self.all_done.wait(0.05 if left is None else min(0.05, left))
+                for r in self.res.values():
+                    if r.error:
+                        raise RuntimeError(f"layout {r.key}: {r.error}")
+
+    def close(self):
+        with self.lock:
+            keys = list(self.res)
+        for k in keys:
+            self._evict(k)
+
+    def resident_bytes(self):
+        with self.lock:
+            return sum(r.nbytes for r in self.res.values())
+
+    # ---- admission ---------------------------------------------------------
+    def _admit(self):
+        """Moves backlog requests whose layout is (or can be made) resident into
+        that layout's queue; a request whose layout does not fit yet waits."""
+        waiting = collections.deque()
+        while self.backlog:
+            req = self.backlog.popleft()
+            r = self.res.get(req.layout)
+            if r is None:
+                need = self.backend.estimate(req.layout)
+                me = threading.current_thread()
+                while self.resident_bytes() + need > self.budget:
+                    idle = [x for x in self.res.values()
+                            if not x.q and x.inflight == 0 and x.thread is not me]
+                    if not idle:
+                        break
+                    self._evict(min(idle, key=lambda x: x.last_used).key)
+                if self.resident_bytes() + need > self.budget and self.res:
+                    waiting.append(req)    # retried when a layout goes idle
+                    continue
+                r = self._open(req.layout)
+            nps = self.backend.num_ports(r.handle)
+            req.Z = np.zeros((len(req.freqs), nps, nps), complex)
+            with r.cv:
+                for fi, f in enumerate(req.freqs):
+                    for p in req.ports:
+                        tag = self._next_tag
+                        self._next_tag += 1
+                        self.tags[tag] = (req.rid, fi, p)
+                        r.q.append((p, float(f), tag))
+                r.last_used = self.clock()
+                r.cv.notify_all()
+        self.backlog = waiting
+
+    def _open(self, key):
+        handle = self.backend.open(key)
+        r = _Resident(key, handle, self.backend.bytes(handle))
+        self.res[key] = r
+        r.thread = threading.Thread(target=self._run, args=(r,), daemon=True)
+        r.thread.start()
+        return r
+
+    def _evict(self, key):
+        r = self.res.pop(key, None)
+        if r is None:
+            return
+        with r.cv:
+            r.closed = True
+            r.cv.notify_all()
+        r.thread.join()
+        self.backend.close(r.handle)
+
+    # ---- the stream of one layout -------------------------------------------
+    def _run(self, r: _Resident):
+        def pull(block):
+            with r.cv:
+                while not r.q:
+                    if r.closed:
+                        return None, -1
+                    if not block:
+                        return None, 0
+                    r.cv.wait(0.05)
+                col = r.q.popleft()
+                r.inflight += 1
+                return col, 1
+
+        def waiting():
+            return len(r.q)
+
+        def push(tag, z, rep):
+            with self.lock:
+                rid, fi, p = self.tags.pop(tag)
+                req = self.requests[rid]
+                if z is None:
+                    req.error = "rejected column"
+                else:
+                    req.Z[fi, :, p] = z
+                req.reports[(fi, p)] = rep
+                req.remaining -= 1
+                if req.remaining == 0:
+                    req.done = self.clock()
+                with r.cv:
+                    r.inflight -= 1
+                    r.last_used = self.clock()
+                self._admit()
+                self.all_done.notify_all()
+
+        try:
+            self.backend.stream(r.handle, pull, waiting, push)
+        except Exception as e:  # surfaced by wait()
+            r.error = str(e)
+            with self.lock:
+                self.all_done.notify_all()
+
+
+class GpuBackend:
+    """Backend over libgds2para: one Layout per key, solved with gp_solve_stream."""
+
+    def __init__(self, specs, slots=16, precision=capi.GP_PREC_MIXED, rtol=1e-8, restart=30, max_iters=2000):
+        from . import mesh as M
+        self.M = M
+        self.specs = specs                 # key -> GenSpec
+        self.slots, self.precision, self.rtol = slots, precision, rtol
+        self.restart, self.max_iters = restart, max_iters
+        self.lib = capi.load()
+
+    def estimate(self, key):
+        """Bytes before the layout exists: ~2.3 KB per edge of hierarchy plus
+        the slot workspace (FP64 basis, FP32 preconditioned basis)."""
+        cx, cy, cz = self.specs[key].cells
+        n = 3 * (cx + 1) * (cy + 1) * (cz + 1)
+        ws = self.slots * n * ((self.restart + 1) * 16 + self.restart * 8 * 2 + 256)
+        return int(n * 2300 + ws)
+
+    def open(self, key):
+        from .layout import Layout
+        m = self.M.Mesh(self.specs[key])
+        lay = Layout(m)
+        lay.assemble()
+        lay.prepare(self.precision)
+        return lay
+
+    def bytes(self, lay):
+        ws = C.c_int64()
+        capi.check(self.lib.gp_solve_workspace_bytes(lay.h, self.precision, self.slots, self.restart, C.byref(ws)),
+                   "gp_solve_workspace_bytes")
+        return lay.device_bytes + ws.value
+
+    def num_ports(self, lay):
+        return lay.mesh.num_ports
+
+    def close(self, lay):
+        lay.close()
+
+    def stream(self, lay, pull, waiting, push):
+        npt = lay.mesh.num_ports
+
+        def c_pull(user, colp, block):
+            col, r = pull(bool(block))
+            if r == 1:
+                colp.contents.port, colp.contents.freq, colp.contents.tag = col
+            return r
+
+        def c_waiting(user):
+            return waiting()
+
+        def c_push(user, colp, zp, repp):
+            z = None
+            if zp:
+                a = np.ctypeslib.as_array(zp, shape=(2 * npt,))
+                z = a[0::2] + 1j * a[1::2]
+            rp = repp.contents
+            push(colp.contents.tag, z, dict(iters=rp.iters, converged=rp.converged, rel_residual=rp.rel_residual))
+
+        cb = (capi.PULL_FN(c_pull), capi.WAITING_FN(c_waiting), capi.PUSH_FN(c_push))
+        o = capi.GpSolveOpts()
+        self.lib.gp_solve_opts_default(C.byref(o))
+        o.rtol, o.precision, o.restart, o.max_iters = self.rtol, self.precision, self.restart, self.max_iters
+        st = capi.GpSolveStats()
+        capi.check(self.lib.gp_solve_stream(lay.h, C.byref(o), self.slots, cb[0], cb[1], cb[2], None, C.byref(st)),
+                   "gp_solve_stream")
+
+
+def poisson_arrivals(rate_hz, n, seed=0):
+    """Arrival offsets (s) of a seeded Poisson stream."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate_hz, n))
+
+
+def run_stream(server, requests, offsets, clock=time.monotonic, sleep=time.sleep):
+    """Submits requests at their offsets from now, waits, returns them with latencies."""
+    t0 = clock()
+    for req, off in zip(requests, offsets):
+        dt = t0 + off - clock()
+        if dt > 0:
+            sleep(dt)
+        server.submit(req)
+    server.wait()
+    return requests

@@ -260,6 +260,36 @@ gp_status gp_solve_fullwave_ex(gp_layout *lay, const double *freqs, int32_t nf,
                                double *X, double *Z, gp_solve_stats *stats,
                                gp_column_report *reports);
 
+/* ---- continuous batching (serving) ---------------------------------------
+ * No counterpart in the reference, which solves one frequency at a time with
+ * a direct factorization.  gp_solve_stream keeps `slots` columns on the GPU
+ * and, at every FGMRES restart, replaces each finished column with a new one
+ * from the caller:
+ *   pull(user, &col, block)  1: col filled; 0: nothing waiting; -1: closed.
+ *                            block = 1 when every slot is idle (may wait).
+ *   waiting(user)            columns ready to pull (steers early restarts).
+ *   push(user, &col, z, rep) a finished column: z = its np complex Z entries
+ *                            (Z[resp + np*col.port], resp = 0..np-1), or
+ *                            null with rep->rel_residual < 0 for a rejected
+ *                            column (port out of range, frequency <= 0).
+ * Returns when pull() has returned -1 and every slot is idle.  The callbacks
+ * run on the calling thread. */
+/* Device bytes a solve with `slots` slots and restart length `restart` adds
+ * on top of gp_layout_device_bytes (Krylov bases, iterates, V-cycle vectors);
+ * prepares the solver first if needed.  For admission under a memory budget. */
+gp_status gp_solve_workspace_bytes(gp_layout *lay, int32_t precision, int32_t slots, int32_t restart, int64_t *bytes);
+
+typedef struct gp_stream_column {
+  int32_t port;                   /* source port (0-based)                        */
+  double freq;                    /* Hz                                           */
+  int64_t tag;                    /* caller's id, echoed back                     */
+} gp_stream_column;
+typedef int32_t (*gp_pull_fn)(void *user, gp_stream_column *col, int32_t block);
+typedef int64_t (*gp_waiting_fn)(void *user);
+typedef void (*gp_push_fn)(void *user, const gp_stream_column *col, const double *z, const gp_column_report *rep);
+gp_status gp_solve_stream(gp_layout *lay, const gp_solve_opts *opts, int32_t slots, gp_pull_fn pull,
+                          gp_waiting_fn waiting, gp_push_fn push, void *user, gp_solve_stats *stats);
+
 /* Construct_Z_V0_Vh for one solution vector x (n complex, renumbered space):
  * accumulates column src of the np*np block Zk (complex interleaved). Host only. */
 gp_status gp_extract_z(const gp_mesh *mesh, const double *x, int32_t source_port, double *Zk);

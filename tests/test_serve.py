@@ -1,0 +1,143 @@
+"""Serving loop: continuous batching over gp_solve_stream.
+
+CPU tests drive the scheduler with a mock backend (admission under a memory
+budget, eviction of idle layouts, completion and latency bookkeeping).  The
+GPU test streams overlapping requests over two layouts through the library and
+checks every request's Z against the same request solved alone: within 1e-6
+relative (the columns restart at different points in the mix, so they agree
+to the solve tolerance rather than bit for bit)."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from gds2para_amd import _capi as capi
+from gds2para_amd.serve import Request, Server, poisson_arrivals, run_stream
+
+
+class MockBackend:
+    """Solves a column by sleeping `delay` s; its Z column is a fixed function
+    of (layout, port, freq).  Runs `slots` columns at a time like the library."""
+
+    def __init__(self, sizes, nports=3, slots=2, delay=0.01):
+        self.sizes, self.nports, self.slots, self.delay = sizes, nports, slots, delay
+        self.opened, self.closed, self.live = [], [], set()
+        self.lock = threading.Lock()
+
+    def estimate(self, key):
+        return self.sizes[key]
+
+    def open(self, key):
+        with self.lock:
+            self.opened.append(key)
+            self.live.add(key)
+        return key
+
+    def bytes(self, key):
+        return self.sizes[key]
+
+    def num_ports(self, key):
+        return self.nports
+
+    def close(self, key):
+        with self.lock:
+            self.closed.append(key)
+            self.live.discard(key)
+
+    @staticmethod
+    def zcol(key, port, freq, n):
+        return np.array([hash((key, port, round(freq), r)) % 1000 + 1j for r in range(n)], complex)
+
+    def stream(self, key, pull, waiting, push):
+        active = []
+        closed = False
+        while True:
+            while len(active) < self.slots and not closed:
+                col, r = pull(len(active) == 0)
+                if r == -1:
+                    closed = True
+                if r != 1:
+                    break
+                active.append(col)
+            if not active:
+                if closed:
+                    return
+                continue
+            time.sleep(self.delay)
+            p, f, tag = active.pop(0)
+            push(tag, self.zcol(key, p, f, self.nports), dict(iters=1, converged=1, rel_residual=0.0))
+
+
+def test_serve_completes_and_orders():
+    be = MockBackend({"a": 10, "b": 10})
+    srv = Server(be, budget_bytes=100)
+    reqs = [Request("a" if i % 2 else "b", [i % 3], [1e9, 2e9]) for i in range(8)]
+    run_stream(srv, reqs, poisson_arrivals(200.0, len(reqs), seed=1))
+    for r in reqs:
+        assert r.remaining == 0 and r.latency >= 0 and not r.error
+        for fi, f in enumerate(r.freqs):
+            for p in r.ports:
+                assert np.array_equal(r.Z[fi, :, p], MockBackend.zcol(r.layout, p, f, 3))
+    srv.close()
+    assert sorted(be.closed) == ["a", "b"] and not be.live
+
+
+def test_serve_budget_evicts_idle_layouts():
+    be = MockBackend({"a": 60, "b": 60, "c": 60}, delay=0.002)
+    srv = Server(be, budget_bytes=100)       # one layout at a time
+    reqs = [Request(k, [0, 1], [1e8]) for k in ("a", "b", "c", "a")]
+    run_stream(srv, reqs, [0.0, 0.05, 0.1, 0.15])
+    assert all(r.remaining == 0 for r in reqs)
+    assert be.opened.count("a") == 2          # evicted and reopened
+    assert len(be.live) <= 1
+    srv.close()
+
+
+def test_serve_same_alone_and_in_mix_mock():
+    be = MockBackend({"a": 1, "b": 1}, slots=3)
+    alone = Server(be, 100)
+    r0 = Request("a", [0, 2], [5e8, 1e9])
+    run_stream(alone, [r0], [0.0])
+    alone.close()
+    mix = Server(be, 100)
+    reqs = [Request("b", [1], [1e9]), Request("a", [0, 2], [5e8, 1e9]), Request("a", [1], [2e9])]
+    run_stream(mix, reqs, [0.0, 0.001, 0.002])
+    mix.close()
+    assert np.array_equal(reqs[1].Z, r0.Z)
+
+
+@pytest.mark.gpu
+def test_serve_gpu_same_alone_and_in_mix():
+    from gds2para_amd import mesh as M
+    from gds2para_amd.layout import Layout
+    from gds2para_amd.serve import GpuBackend
+    specs = {"m11": M.manhattan_spec((24, 20, 10), n_ports=3, nfreq=3, f_range=(1e8, 1e10), seed=11),
+             "m12": M.manhattan_spec((20, 24, 10), n_ports=3, nfreq=3, f_range=(1e8, 1e10), seed=12)}
+    rng = np.random.default_rng(5)
+    reqs = []
+    for i in range(10):
+        key = "m11" if i % 3 else "m12"
+        ports = sorted(rng.choice(3, size=int(rng.integers(1, 4)), replace=False).tolist())
+        freqs = sorted(rng.choice([1e8, 1e9, 3e9, 1e10], size=int(rng.integers(1, 3)), replace=False).tolist())
+        reqs.append(Request(key, ports, freqs))
+    print("[serve] streaming 10 requests over 2 layouts, 4 slots", flush=True)
+    srv = Server(GpuBackend(specs, slots=4), budget_bytes=200 << 30)
+    run_stream(srv, reqs, poisson_arrivals(20.0, len(reqs), seed=3))
+    srv.close()
+    lats = [r.latency for r in reqs]
+    print(f"[serve] latency p50 {np.percentile(lats, 50):.3f} s p99 {np.percentile(lats, 99):.3f} s", flush=True)
+    for key, spec in specs.items():
+        m = M.Mesh(spec)
+        lay = Layout(m)
+        for r in (q for q in reqs if q.layout == key):
+            assert not r.error
+            for rep in r.reports.values():
+                assert rep["converged"] == 1 and rep["rel_residual"] <= 1e-8
+            Z, st, _ = lay.solve(r.freqs, ports=r.ports, precision=capi.GP_PREC_MIXED, rtol=1e-8, batch=4)
+            Za = Z.reshape(len(r.freqs), 3, 3)     # [f, src, resp]: index resp + np*src
+            for fi in range(len(r.freqs)):
+                for p in r.ports:
+                    a, b = r.Z[fi, :, p], Za[fi, p, :]
+                    assert np.abs(a - b).max() / np.abs(b).max() < 1e-6, (key, r.freqs[fi], p)
+        lay.close()
