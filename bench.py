@@ -12,6 +12,8 @@ Besides the headline rate the line reports:
   fp64_single     the plain FP64 single-RHS path on a sample of the columns,
                   and the speedup of the mixed batched path over it;
   latency         p50/p99 of the per-column service time in the timed steps;
+  serving         a seeded Poisson stream of requests over three layouts served
+                  with continuous batching: per-request latency p50/p99;
   roofline        achieved algorithmic bandwidth of the dominant kernel (the
                   level-0 edge smoother), timed with HIP events on its stream
                   over the timed region, against 8 TB/s;
@@ -54,6 +56,41 @@ def cpu_baseline(cells=(32, 32, 12)):
                       f"generator ({m.n} edges): oracle assembly + SuperLU FP64 direct solve, {dt:.1f} s"}
 
 
+def serving_leg(n_requests, rate_hz, seed=5):
+    """Seeded Poisson stream of requests over three resident layouts of the
+    same generator (150k, 700k and 2.9M edges), served with continuous
+    batching under a 260 GB budget; per-request latency p50/p99."""
+    from gds2para_amd import mesh as M
+    from gds2para_amd.serve import GpuBackend, Request, Server, poisson_arrivals, run_stream
+    specs = {"manhattan_64x64x12": M.manhattan_spec((64, 64, 12), 8, 10, (1e7, 1e10), 2001),
+             "manhattan_120x120x16": M.manhattan_spec((120, 120, 16), 8, 10, (1e7, 1e10), 2002),
+             "manhattan_200x200x24": M.WORKLOADS["manhattan_200x200x24"]()}
+    keys = list(specs)
+    rng = np.random.default_rng(seed)
+    grid = np.logspace(7, 10, 10)
+    reqs = []
+    for _ in range(n_requests):
+        key = keys[int(rng.choice(3, p=[0.4, 0.35, 0.25]))]
+        ports = sorted(rng.choice(8, size=int(rng.integers(1, 9)), replace=False).tolist())
+        freqs = sorted(rng.choice(grid, size=int(rng.integers(1, 5)), replace=False).tolist())
+        reqs.append(Request(key, ports, freqs))
+    srv = Server(GpuBackend(specs, slots=16), budget_bytes=260 << 30)
+    for k in keys:                       # layouts resident before the stream starts
+        srv.submit(Request(k, [0], [1e9]))
+    srv.wait()
+    t = time.perf_counter()
+    run_stream(srv, reqs, poisson_arrivals(rate_hz, len(reqs), seed=seed))
+    dt = time.perf_counter() - t
+    srv.close()
+    lat = [r.latency for r in reqs]
+    cols = sum(len(r.ports) * len(r.freqs) for r in reqs)
+    conv = sum(1 for r in reqs for rep in r.reports.values() if rep["converged"] == 1)
+    return {"requests": len(reqs), "columns": cols, "converged": f"{conv}/{cols}", "layouts": keys,
+            "arrival_rate_hz": rate_hz, "slots": 16, "latency_s": {"p50": float(np.percentile(lat, 50)),
+                                                                  "p99": float(np.percentile(lat, 99))},
+            "columns_per_s": cols / dt, "wall_s": dt}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +102,8 @@ def main():
     ap.add_argument("--max-iters", type=int, default=1500)
     ap.add_argument("--fp64-sample", type=int, default=3, help="columns of the FP64 single-RHS sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=24, help="requests in the serving leg (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate of the serving leg (1/s)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported by this build "
@@ -182,6 +221,9 @@ def main():
                      "bytes_per_launch": kbytes / launches if launches else None},
     }
     lay.close()
+    if a.serve_requests > 0:
+        print(f"[bench] serving leg: {a.serve_requests} requests", file=sys.stderr, flush=True)
+        line["serving"] = serving_leg(a.serve_requests, a.serve_rate)
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

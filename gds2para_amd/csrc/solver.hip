@@ -495,6 +495,19 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
   GP_HIP(hipMemcpyAsync(prow_d, prow.data(), sizeof(int64_t) * prow.size(), hipMemcpyHostToDevice, s));
   Z2 *xfull = nullptr;
   if (io.want_x()) GP_TRY(A.alloc(&xfull, n * k));
+  // right-hand-side entries of newly placed columns: at most k times the
+  // largest port's edge count, allocated once (a free per refill would
+  // synchronise the device and stall other layouts' streams)
+  int64_t max_port_edges = 1;
+  for (int32_t p = 0; p < mesh.num_ports; ++p) {
+    int64_t c = 0;
+    for (int32_t sd = mesh.port_side_ptr[p]; sd < mesh.port_side_ptr[p + 1]; ++sd)
+      c += mesh.side_edge_ptr[sd + 1] - mesh.side_edge_ptr[sd];
+    max_port_edges = std::max(max_port_edges, c);
+  }
+  int64_t *rhs_row; int32_t *rhs_col; double *rhs_val;
+  GP_TRY(A.alloc(&rhs_row, max_port_edges * k)); GP_TRY(A.alloc(&rhs_col, max_port_edges * k));
+  GP_TRY(A.alloc(&rhs_val, max_port_edges * k));
 
   const unsigned gnk = blocks_for(n * k), gnnk = blocks_for(nn * k), gnak = blocks_for(na * k);
   // slot state
@@ -547,14 +560,11 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
     k_zero_cols<<<gnnk, 256, 0, s>>>(nn, k, mask_d, xphi);
     k_zero_cols<<<gnak, 256, 0, s>>>(na, k, mask_d, xa);
     if (!row.empty()) {
-      DevBuf R;
-      int64_t *drow; int32_t *dcol; double *dval;
-      GP_TRY(R.A.alloc(&drow, (int64_t)row.size())); GP_TRY(R.A.alloc(&dcol, (int64_t)row.size()));
-      GP_TRY(R.A.alloc(&dval, (int64_t)row.size()));
-      GP_HIP(hipMemcpyAsync(drow, row.data(), sizeof(int64_t) * row.size(), hipMemcpyHostToDevice, s));
-      GP_HIP(hipMemcpyAsync(dcol, col.data(), sizeof(int32_t) * row.size(), hipMemcpyHostToDevice, s));
-      GP_HIP(hipMemcpyAsync(dval, val.data(), sizeof(double) * row.size(), hipMemcpyHostToDevice, s));
-      k_rhs<<<blocks_for((int64_t)row.size()), 256, 0, s>>>((int64_t)row.size(), k, drow, dcol, dval, b);
+      if ((int64_t)row.size() > max_port_edges * k) return fail(GP_ERR_STATE, "right-hand side larger than planned");
+      GP_HIP(hipMemcpyAsync(rhs_row, row.data(), sizeof(int64_t) * row.size(), hipMemcpyHostToDevice, s));
+      GP_HIP(hipMemcpyAsync(rhs_col, col.data(), sizeof(int32_t) * row.size(), hipMemcpyHostToDevice, s));
+      GP_HIP(hipMemcpyAsync(rhs_val, val.data(), sizeof(double) * row.size(), hipMemcpyHostToDevice, s));
+      k_rhs<<<blocks_for((int64_t)row.size()), 256, 0, s>>>((int64_t)row.size(), k, rhs_row, rhs_col, rhs_val, b);
       GP_HIP(hipStreamSynchronize(s));
     }
     GP_HIP(hipMemcpyAsync(om_d, omega.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
