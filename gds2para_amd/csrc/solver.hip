@@ -1025,11 +1025,6 @@ extern "C" gp_status gp_solve_fullwave(gp_layout *lay, const double *freqs, int3
   return gp_solve_fullwave_ex(lay, freqs, nf, ports, np, opts, X, Z, stats, nullptr);
 }
 
-extern "C" gp_status gp_solve_real(const int64_t *, const int64_t *, const double *, int64_t, int64_t, const double *,
-                                   double *, double, int32_t *, double *) {
-  return fail(GP_ERR_STATE, "real nodal solve not built yet");
-}
-
 extern "C" gp_status gp_layout_sync(gp_layout *lay) {
   if (!lay) return fail(GP_ERR_ARG, "null layout");
   GP_HIP(hipStreamSynchronize(lay->stream));

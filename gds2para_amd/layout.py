@@ -149,3 +149,21 @@ class Layout:
         ms = C.c_double()
         capi.check(self.lib.gp_bench_spmv(self.h, int(k), int(precision), int(iters), C.byref(ms)), "gp_bench_spmv")
         return ms.value
+
+
+def solve_real(row, col, val, n, b, rtol=0.0):
+    """hypreSolve on the device (gp_solve_real): real A x = b, A as COO triplets.
+
+    rtol = 0 selects the reference's 1e-4.  Returns (x, iterations, relative
+    residual); raises GpError unless the tolerance is met."""
+    lib = capi.load()
+    row = np.ascontiguousarray(row, np.int64)
+    col = np.ascontiguousarray(col, np.int64)
+    val = np.ascontiguousarray(val, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(int(n))
+    it, rr = C.c_int32(), C.c_double()
+    capi.check(lib.gp_solve_real(_p(row, i64p), _p(col, i64p), _p(val, f64p), len(val), int(n), _p(b, f64p),
+                                 _p(x, f64p), float(rtol), C.byref(it), C.byref(rr)), "gp_solve_real")
+    return x, it.value, rr.value
+
