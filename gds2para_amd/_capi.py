@@ -103,6 +103,13 @@ class GpSolveStats(C.Structure):
     ]
 
 
+class GpV0Stats(C.Structure):
+    _fields_ = [
+        ("solves", i32), ("max_iters", i32), ("max_rel_residual", f64), ("setup_ms", f64), ("solve_ms", f64),
+        ("ad_levels", i32), ("ac_levels", i32), ("ad_rows", i64), ("ac_rows", i64),
+    ]
+
+
 # (name, restype, argtypes) for every entry point declared in include/gds2para.h
 SIGNATURES = [
     ("gp_last_error", C.c_char_p, []),
@@ -137,6 +144,8 @@ SIGNATURES = [
                               P(GpSolveStats)]),
     ("gp_extract_z", i32, [P(GpMesh), P(f64), i32, P(f64)]),
     ("gp_solve_real", i32, [P(i64), P(i64), P(f64), i64, i64, P(f64), P(f64), f64, P(i32), P(f64)]),
+    ("gp_solve_v0", i32, [C.c_void_p, f64, P(f64), P(f64), P(GpV0Stats)]),
+    ("gp_z_v0_sweep", i32, [P(GpMesh), P(f64)]),
     ("gp_bench_spmv", i32, [C.c_void_p, i32, i32, i32, P(f64)]),
     ("gp_layout_sync", i32, [C.c_void_p]),
     ("gp_bench_copy", i32, [i64, i32, P(f64)]),

@@ -87,6 +87,9 @@ struct CsrD {            // FP64 CSR, int64 row pointers, int32 columns
 };
 
 struct V0Data;           // defined in v0.hip
+struct V0Path;           // defined in v0path.hip
+void destroy_v0path(V0Path *);
+int64_t v0path_bytes(const V0Path *);
 struct Solver;           // defined in solver.hip
 void destroy_v0(V0Data *);
 void destroy_solver(Solver *);
@@ -142,6 +145,8 @@ struct gp_layout {
   gp::CsrD S;
   // V0 basis
   gp::V0Data *v0 = nullptr;
+  // V0 quasi-static path: V0 operators in CSR and the Ad / Ac hierarchies
+  gp::V0Path *v0path = nullptr;
   // solver hierarchy
   gp::Solver *solver = nullptr;
 };

@@ -181,6 +181,8 @@ extern "C" void gp_layout_destroy(gp_layout *lay) {
   if (lay->stream) (void)hipStreamSynchronize(lay->stream);
   destroy_solver(lay->solver);
   lay->solver = nullptr;
+  destroy_v0path(lay->v0path);
+  lay->v0path = nullptr;
   destroy_v0(lay->v0);
   lay->v0 = nullptr;
   lay->arena.free_all();
@@ -189,5 +191,5 @@ extern "C" void gp_layout_destroy(gp_layout *lay) {
 }
 
 extern "C" int64_t gp_layout_device_bytes(const gp_layout *lay) {
-  return lay ? lay->arena.bytes : 0;
+  return lay ? lay->arena.bytes + v0path_bytes(lay->v0path) : 0;
 }

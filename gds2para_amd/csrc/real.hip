@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "dense.hpp"
+#include "real.hpp"
 #include "sparse.hpp"
 
 namespace gp {
@@ -176,35 +177,95 @@ __global__ void k_r_scale_rows(int64_t n, const int64_t *__restrict__ rp, const 
   for (int64_t p = rp[i]; p < rp[i + 1]; ++p) out[p] = s * v[p];
 }
 
+// Row products split two ways: rows of at most kRLong entries take 8 lanes
+// each (k_r_spmv, k_r_jacobi), longer rows -- the conductor aggregate
+// columns of V0d couple to every dielectric node around a conductor -- take a
+// 256-thread block each (the *_long kernels), listed per matrix at setup.
+constexpr int kRLong = 64;
+constexpr int kRLanes = 8;
+
+__device__ __forceinline__ double row_dot8(int64_t b, int64_t e, int lane, const int32_t *__restrict__ ci,
+                                           const double *__restrict__ v, const double *__restrict__ x) {
+  double s = 0.;
+  for (int64_t p = b + lane; p < e; p += kRLanes) s += v[p] * x[ci[p]];
+  for (int off = kRLanes / 2; off > 0; off >>= 1) s += __shfl_down(s, off, kRLanes);
+  return s;
+}
+
+__device__ __forceinline__ double row_dot_block(int64_t b, int64_t e, const int32_t *__restrict__ ci,
+                                                const double *__restrict__ v, const double *__restrict__ x) {
+  __shared__ double sh[256];
+  double s = 0.;
+  for (int64_t p = b + threadIdx.x; p < e; p += blockDim.x) s += v[p] * x[ci[p]];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
 // y = A x (mode 0), y = r - A x (mode 1), y = y + A x (mode 2)
 __global__ void k_r_spmv(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
                          const double *__restrict__ v, const double *__restrict__ x, const double *__restrict__ r,
                          double *__restrict__ y, int mode) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kRLanes;
+  const int lane = threadIdx.x & (kRLanes - 1);
   if (i >= n) return;
-  double s = 0.;
-  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) s += v[p] * x[ci[p]];
-  y[i] = mode == 0 ? s : mode == 1 ? r[i] - s : y[i] + s;
+  const int64_t b = rp[i], e = rp[i + 1];
+  if (e - b > kRLong) return;
+  const double s = row_dot8(b, e, lane, ci, v, x);
+  if (lane == 0) y[i] = mode == 0 ? s : mode == 1 ? r[i] - s : y[i] + s;
 }
 
-// l1-Jacobi: first sweep xo = r / d; later xo = x + (r - A x) / d
+__global__ void k_r_spmv_long(const int32_t *__restrict__ lrows, const int64_t *__restrict__ rp,
+                              const int32_t *__restrict__ ci, const double *__restrict__ v,
+                              const double *__restrict__ x, const double *__restrict__ r, double *__restrict__ y,
+                              int mode) {
+  const int64_t i = lrows[blockIdx.x];
+  const double s = row_dot_block(rp[i], rp[i + 1], ci, v, x);
+  if (threadIdx.x == 0) y[i] = mode == 0 ? s : mode == 1 ? r[i] - s : y[i] + s;
+}
+
+// l1-Jacobi, first sweep: xo = r / d
+__global__ void k_r_jacobi0(int64_t n, const double *__restrict__ dl1, const double *__restrict__ r,
+                            double *__restrict__ xo) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) xo[i] = 0. + r[i] / dl1[i];
+}
+
+// l1-Jacobi, later sweeps: xo = x + (r - A x) / d
 __global__ void k_r_jacobi(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
                            const double *__restrict__ v, const double *__restrict__ dl1, const double *__restrict__ r,
                            const double *__restrict__ x, double *__restrict__ xo) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kRLanes;
+  const int lane = threadIdx.x & (kRLanes - 1);
   if (i >= n) return;
-  double s = r[i];
-  if (x)
-    for (int64_t p = rp[i]; p < rp[i + 1]; ++p) s -= v[p] * x[ci[p]];
-  xo[i] = (x ? x[i] : 0.) + s / dl1[i];
+  const int64_t b = rp[i], e = rp[i + 1];
+  if (e - b > kRLong) return;
+  const double s = row_dot8(b, e, lane, ci, v, x);
+  if (lane == 0) xo[i] = x[i] + (r[i] - s) / dl1[i];
 }
 
+__global__ void k_r_jacobi_long(const int32_t *__restrict__ lrows, const int64_t *__restrict__ rp,
+                                const int32_t *__restrict__ ci, const double *__restrict__ v,
+                                const double *__restrict__ dl1, const double *__restrict__ r,
+                                const double *__restrict__ x, double *__restrict__ xo) {
+  const int64_t i = lrows[blockIdx.x];
+  const double s = row_dot_block(rp[i], rp[i + 1], ci, v, x);
+  if (threadIdx.x == 0) xo[i] = x[i] + (r[i] - s) / dl1[i];
+}
+
+// x = inv r on the coarsest level, one wavefront per row (coalesced rows)
 __global__ void k_r_dense(int64_t n, const double *__restrict__ inv, const double *__restrict__ r, double *__restrict__ x) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const int lane = threadIdx.x & 63;
   if (i >= n) return;
   double s = 0.;
-  for (int64_t j = 0; j < n; ++j) s += inv[i * n + j] * r[j];
-  x[i] = s;
+  for (int64_t j = lane; j < n; j += 64) s += inv[i * n + j] * r[j];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) x[i] = s;
 }
 
 // partial[t * nb + blk] = sum over the block's rows of V_t . w
@@ -266,17 +327,60 @@ __global__ void k_csr_dense(int64_t n, const int64_t *__restrict__ rp, const int
   for (int64_t p = rp[i]; p < rp[i + 1]; ++p) out[i * n + ci[p]] += v[p];
 }
 
+}  // namespace
+
 struct RLevel {
   Csr A, P, R;
   double *d = nullptr, *dl1 = nullptr;
   double *r = nullptr, *x = nullptr, *t = nullptr;
+  // rows longer than kRLong of A, P and R (device lists) and their counts
+  int32_t *la = nullptr, *lp = nullptr, *lr = nullptr;
+  int64_t nla = 0, nlp = 0, nlr = 0;
 };
 
 struct RealAMG {
   std::vector<RLevel> L;
   double *inv = nullptr;  // coarsest, dense n x n
+  // FGMRES workspace: basis V (m+1 vectors), preconditioned basis Z (m),
+  // block partial sums of the dot products, Hessenberg coefficients
+  double *V = nullptr, *Zb = nullptr, *partial = nullptr, *hd = nullptr;
   Arena A;
 };
+
+namespace {
+
+// rows of M longer than kRLong, as a device list
+gp_status long_rows(Arena &A, const Csr &M, int32_t **out, int64_t *nl, hipStream_t s) {
+  *out = nullptr;
+  *nl = 0;
+  if (M.nr == 0) return GP_OK;
+  std::vector<int64_t> rp((size_t)M.nr + 1);
+  GP_HIP(hipMemcpyAsync(rp.data(), M.rp, sizeof(int64_t) * rp.size(), hipMemcpyDeviceToHost, s));
+  GP_HIP(hipStreamSynchronize(s));
+  std::vector<int32_t> l;
+  for (int64_t i = 0; i < M.nr; ++i)
+    if (rp[i + 1] - rp[i] > kRLong) l.push_back((int32_t)i);
+  if (l.empty()) return GP_OK;
+  GP_TRY(A.alloc(out, (int64_t)l.size()));
+  GP_HIP(hipMemcpyAsync(*out, l.data(), sizeof(int32_t) * l.size(), hipMemcpyHostToDevice, s));
+  GP_HIP(hipStreamSynchronize(s));
+  *nl = (int64_t)l.size();
+  return GP_OK;
+}
+
+void spmv_r(const Csr &M, const int32_t *lr, int64_t nl, const double *x, const double *r, double *y, int mode,
+            hipStream_t s) {
+  if (M.nr == 0) return;
+  k_r_spmv<<<blocks_for(M.nr * kRLanes), 256, 0, s>>>(M.nr, M.rp, M.ci, M.v[0], x, r, y, mode);
+  if (nl) k_r_spmv_long<<<(unsigned)nl, 256, 0, s>>>(lr, M.rp, M.ci, M.v[0], x, r, y, mode);
+}
+
+void jacobi_r(const RLevel &L, const double *x, double *xo, hipStream_t s) {
+  const int64_t n = L.A.nr;
+  if (!x) { k_r_jacobi0<<<blocks_for(n), 256, 0, s>>>(n, L.dl1, L.r, xo); return; }
+  k_r_jacobi<<<blocks_for(n * kRLanes), 256, 0, s>>>(n, L.A.rp, L.A.ci, L.A.v[0], L.dl1, L.r, x, xo);
+  if (L.nla) k_r_jacobi_long<<<(unsigned)L.nla, 256, 0, s>>>(L.la, L.A.rp, L.A.ci, L.A.v[0], L.dl1, L.r, x, xo);
+}
 
 // one pairwise matching pass on M: tentative P (n x ncoarse)
 gp_status match_pass(Arena &A, Arena &tmp, const Csr &M, const double *d, Csr *P, hipStream_t s) {
@@ -355,6 +459,7 @@ gp_status build_real_amg(RealAMG &G, Csr A0, hipStream_t s) {
     GP_TRY(A.alloc(&F.d, n)); GP_TRY(A.alloc(&F.dl1, n));
     k_r_diag<<<blocks_for(n), 256, 0, s>>>(n, F.A.rp, F.A.ci, F.A.v[0], F.d, F.dl1);
     GP_TRY(A.alloc(&F.r, n)); GP_TRY(A.alloc(&F.x, n)); GP_TRY(A.alloc(&F.t, n));
+    GP_TRY(long_rows(A, F.A, &F.la, &F.nla, s));
     if (n <= kCoarsest || G.L.size() >= 24) break;
     Arena tmp;
     gp_status st = GP_OK;
@@ -389,6 +494,8 @@ gp_status build_real_amg(RealAMG &G, Csr A0, hipStream_t s) {
     if (st != GP_OK) return st;
     F.P = P;
     F.R = R;
+    GP_TRY(long_rows(A, F.P, &F.lp, &F.nlp, s));
+    GP_TRY(long_rows(A, F.R, &F.lr, &F.nlr, s));
     RLevel C;
     C.A = Ac;
     G.L.push_back(C);
@@ -419,20 +526,19 @@ gp_status build_real_amg(RealAMG &G, Csr A0, hipStream_t s) {
 void vcycle_real(const RealAMG &G, int l, hipStream_t s) {
   const RLevel &L = G.L[l];
   const int64_t n = L.A.nr;
-  const unsigned gb = blocks_for(n);
   if (l + 1 == (int)G.L.size()) {
-    k_r_dense<<<gb, 256, 0, s>>>(n, G.inv, L.r, L.x);
+    k_r_dense<<<blocks_for(n * 64), 256, 0, s>>>(n, G.inv, L.r, L.x);
     return;
   }
-  k_r_jacobi<<<gb, 256, 0, s>>>(n, L.A.rp, L.A.ci, L.A.v[0], L.dl1, L.r, nullptr, L.x);
-  k_r_jacobi<<<gb, 256, 0, s>>>(n, L.A.rp, L.A.ci, L.A.v[0], L.dl1, L.r, L.x, L.t);
-  k_r_spmv<<<gb, 256, 0, s>>>(n, L.A.rp, L.A.ci, L.A.v[0], L.t, L.r, L.x, 1);      // x <- r - A t
+  jacobi_r(L, nullptr, L.x, s);
+  jacobi_r(L, L.x, L.t, s);
+  spmv_r(L.A, L.la, L.nla, L.t, L.r, L.x, 1, s);      // x <- r - A t
   const RLevel &C = G.L[l + 1];
-  k_r_spmv<<<blocks_for(C.A.nr), 256, 0, s>>>(C.A.nr, L.R.rp, L.R.ci, L.R.v[0], L.x, nullptr, C.r, 0);
+  spmv_r(L.R, L.lr, L.nlr, L.x, nullptr, C.r, 0, s);
   vcycle_real(G, l + 1, s);
-  k_r_spmv<<<gb, 256, 0, s>>>(n, L.P.rp, L.P.ci, L.P.v[0], C.x, nullptr, L.t, 2);   // t += P xc
-  k_r_jacobi<<<gb, 256, 0, s>>>(n, L.A.rp, L.A.ci, L.A.v[0], L.dl1, L.r, L.t, L.x);
-  k_r_jacobi<<<gb, 256, 0, s>>>(n, L.A.rp, L.A.ci, L.A.v[0], L.dl1, L.r, L.x, L.t);
+  spmv_r(L.P, L.lp, L.nlp, C.x, nullptr, L.t, 2, s);  // t += P xc
+  jacobi_r(L, L.t, L.x, s);
+  jacobi_r(L, L.x, L.t, s);
   (void)hipMemcpyAsync(L.x, L.t, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
 }
 
@@ -449,51 +555,47 @@ gp_status rdots(hipStream_t s, int64_t n, const double *V, int64_t vstride, int 
   return GP_OK;
 }
 
+constexpr int kRestart = 30;
+
 }  // namespace
 
-}  // namespace gp
-
-using namespace gp;
-
-extern "C" gp_status gp_solve_real(const int64_t *row, const int64_t *col, const double *val, int64_t nnz, int64_t n,
-                                   const double *b, double *x, double rtol, int32_t *iters_out, double *relres_out) {
-  if (!row || !col || !val || !b || !x || n < 1 || nnz < 1) return fail(GP_ERR_ARG, "bad real solve arguments");
+gp_status real_amg_create(int64_t n, int64_t nnz, const int64_t *rows, const int64_t *cols, const double *vals,
+                          RealAMG **out, hipStream_t s) {
+  *out = nullptr;
+  if (n < 1 || nnz < 1) return fail(GP_ERR_ARG, "empty real matrix");
   if (n > 2147483647LL) return fail(GP_ERR_ARG, "more than 2^31 unknowns");
-  for (int64_t q = 0; q < nnz; ++q)
-    if (row[q] < 0 || row[q] >= n || col[q] < 0 || col[q] >= n) return fail(GP_ERR_ARG, "COO index out of range");
-  GP_TRY(ensure_device());
-  const double tol = rtol > 0 ? rtol : 1e-4;  // the reference's HYPRE tolerance (src/hypreSolver.h:8)
-  hipStream_t s;
-  GP_HIP(hipStreamCreate(&s));
-  struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); } } sg{s};
-  RealAMG G;
-  struct GGuard { RealAMG *g; ~GGuard() { g->A.free_all(); } } gg{&G};
-  Arena &A = G.A;
+  RealAMG *G = new RealAMG();
   Csr A0;
-  {
-    Arena tmp;
-    int64_t *dr, *dc;
-    double *dv;
-    gp_status st = GP_OK;
-    do {
-      if ((st = tmp.alloc(&dr, nnz)) != GP_OK || (st = tmp.alloc(&dc, nnz)) != GP_OK || (st = tmp.alloc(&dv, nnz)) != GP_OK) break;
-      if (hipMemcpyAsync(dr, row, sizeof(int64_t) * nnz, hipMemcpyHostToDevice, s) != hipSuccess ||
-          hipMemcpyAsync(dc, col, sizeof(int64_t) * nnz, hipMemcpyHostToDevice, s) != hipSuccess ||
-          hipMemcpyAsync(dv, val, sizeof(double) * nnz, hipMemcpyHostToDevice, s) != hipSuccess) { st = fail(GP_ERR_HIP, "upload"); break; }
-      const double *vals[1] = {dv};
-      st = csr_from_coo(A, n, n, nnz, dr, dc, vals, 1, &A0, s);
-    } while (0);
-    tmp.free_all();
-    GP_TRY(st);
-  }
-  GP_TRY(build_real_amg(G, A0, s));
-  // flexible GMRES(m), FP64, classical Gram-Schmidt with reorthogonalization
-  const int m = 30, maxit = 1000;
-  double *bd, *xd, *V, *Zb, *w, *partial, *hd;
-  GP_TRY(A.alloc(&bd, n)); GP_TRY(A.alloc(&xd, n)); GP_TRY(A.alloc(&V, (int64_t)(m + 1) * n));
-  GP_TRY(A.alloc(&Zb, (int64_t)m * n)); GP_TRY(A.alloc(&w, n)); GP_TRY(A.alloc(&partial, (int64_t)256 * (m + 1)));
-  GP_TRY(A.alloc(&hd, m + 1));
-  GP_HIP(hipMemcpyAsync(bd, b, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  const double *vv[1] = {vals};
+  gp_status st = csr_from_coo(G->A, n, n, nnz, rows, cols, vv, 1, &A0, s);
+  if (st == GP_OK) st = build_real_amg(*G, A0, s);
+  if (st == GP_OK) st = G->A.alloc(&G->V, (int64_t)(kRestart + 1) * n);
+  if (st == GP_OK) st = G->A.alloc(&G->Zb, (int64_t)kRestart * n);
+  if (st == GP_OK) st = G->A.alloc(&G->partial, (int64_t)256 * (kRestart + 1));
+  if (st == GP_OK) st = G->A.alloc(&G->hd, kRestart + 1);
+  if (st != GP_OK) { real_amg_destroy(G); return st; }
+  *out = G;
+  return GP_OK;
+}
+
+int64_t real_amg_rows(const RealAMG *G) { return G->L[0].A.nr; }
+int32_t real_amg_levels(const RealAMG *G) { return (int32_t)G->L.size(); }
+int64_t real_amg_bytes(const RealAMG *G) { return G->A.bytes; }
+
+void real_amg_destroy(RealAMG *G) {
+  if (!G) return;
+  G->A.free_all();
+  delete G;
+}
+
+// flexible GMRES(m), FP64, classical Gram-Schmidt with one
+// reorthogonalization pass, Givens rotations on the host
+gp_status real_amg_solve(RealAMG *G, const double *bd, double *xd, double tol, int maxit, int32_t *iters_out,
+                         double *relres_out, hipStream_t s) {
+  const int m = kRestart;
+  const RLevel &L0 = G->L[0];
+  const int64_t n = L0.A.nr;
+  double *V = G->V, *Zb = G->Zb, *partial = G->partial, *hd = G->hd;
   GP_HIP(hipMemsetAsync(xd, 0, sizeof(double) * n, s));
   const unsigned gb = blocks_for(n);
   std::vector<double> h, h2;
@@ -501,13 +603,10 @@ extern "C" gp_status gp_solve_real(const int64_t *row, const int64_t *col, const
   const double bn = std::sqrt(h[0]);
   int iters = 0;
   double rel = 0.;
-  if (bn == 0.) {
-    GP_HIP(hipMemsetAsync(xd, 0, sizeof(double) * n, s));
-  } else {
+  if (bn > 0.) {
     std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
-    const RLevel &L0 = G.L[0];
     for (;;) {
-      k_r_spmv<<<gb, 256, 0, s>>>(n, L0.A.rp, L0.A.ci, L0.A.v[0], xd, bd, V, 1);
+      spmv_r(L0.A, L0.la, L0.nla, xd, bd, V, 1, s);
       GP_TRY(rdots(s, n, V, 0, 1, V, partial, h));
       const double beta = std::sqrt(h[0]);
       rel = beta / bn;
@@ -519,9 +618,9 @@ extern "C" gp_status gp_solve_real(const int64_t *row, const int64_t *col, const
       for (int i = 0; i < m && iters < maxit; ++i) {
         double *Vi = V + (int64_t)i * n, *Vn = V + (int64_t)(i + 1) * n, *Zi = Zb + (int64_t)i * n;
         GP_HIP(hipMemcpyAsync(L0.r, Vi, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-        vcycle_real(G, 0, s);
+        vcycle_real(*G, 0, s);
         GP_HIP(hipMemcpyAsync(Zi, L0.x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-        k_r_spmv<<<gb, 256, 0, s>>>(n, L0.A.rp, L0.A.ci, L0.A.v[0], Zi, nullptr, Vn, 0);
+        spmv_r(L0.A, L0.la, L0.nla, Zi, nullptr, Vn, 0, s);
         for (int pass = 0; pass < 2; ++pass) {
           GP_TRY(rdots(s, n, V, n, i + 1, Vn, partial, h2));
           GP_HIP(hipMemcpyAsync(hd, h2.data(), sizeof(double) * (i + 1), hipMemcpyHostToDevice, s));
@@ -558,9 +657,43 @@ extern "C" gp_status gp_solve_real(const int64_t *row, const int64_t *col, const
       k_r_xup<<<gb, 256, 0, s>>>(n, Zb, n, used, hd, xd);
     }
   }
-  GP_HIP(hipMemcpyAsync(x, xd, sizeof(double) * n, hipMemcpyDeviceToHost, s));
   GP_HIP(hipStreamSynchronize(s));
   if (iters_out) *iters_out = iters;
   if (relres_out) *relres_out = rel;
   return rel <= tol ? GP_OK : fail(GP_ERR_NOCONV, "real solve stopped above the tolerance");
+}
+
+}  // namespace gp
+
+using namespace gp;
+
+extern "C" gp_status gp_solve_real(const int64_t *row, const int64_t *col, const double *val, int64_t nnz, int64_t n,
+                                   const double *b, double *x, double rtol, int32_t *iters_out, double *relres_out) {
+  if (!row || !col || !val || !b || !x || n < 1 || nnz < 1) return fail(GP_ERR_ARG, "bad real solve arguments");
+  if (n > 2147483647LL) return fail(GP_ERR_ARG, "more than 2^31 unknowns");
+  for (int64_t q = 0; q < nnz; ++q)
+    if (row[q] < 0 || row[q] >= n || col[q] < 0 || col[q] >= n) return fail(GP_ERR_ARG, "COO index out of range");
+  GP_TRY(ensure_device());
+  const double tol = rtol > 0 ? rtol : 1e-4;  // the reference's HYPRE tolerance (src/hypreSolver.h:8)
+  hipStream_t s;
+  GP_HIP(hipStreamCreate(&s));
+  struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); } } sg{s};
+  Arena tmp;
+  struct TGuard { Arena *a; hipStream_t s; ~TGuard() { (void)hipStreamSynchronize(s); a->free_all(); } } tg{&tmp, s};
+  int64_t *dr, *dc;
+  double *dv, *bd, *xd;
+  GP_TRY(tmp.alloc(&dr, nnz)); GP_TRY(tmp.alloc(&dc, nnz)); GP_TRY(tmp.alloc(&dv, nnz));
+  GP_TRY(tmp.alloc(&bd, n)); GP_TRY(tmp.alloc(&xd, n));
+  GP_HIP(hipMemcpyAsync(dr, row, sizeof(int64_t) * nnz, hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemcpyAsync(dc, col, sizeof(int64_t) * nnz, hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemcpyAsync(dv, val, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemcpyAsync(bd, b, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  RealAMG *G = nullptr;
+  GP_TRY(real_amg_create(n, nnz, dr, dc, dv, &G, s));
+  struct GGuard { RealAMG *g; ~GGuard() { real_amg_destroy(g); } } gg{G};
+  gp_status st = real_amg_solve(G, bd, xd, tol, 1000, iters_out, relres_out, s);
+  if (st != GP_OK && st != GP_ERR_NOCONV) return st;
+  GP_HIP(hipMemcpyAsync(x, xd, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  GP_HIP(hipStreamSynchronize(s));
+  return st;
 }

@@ -104,6 +104,20 @@ class Layout:
         return dict(edge=[(h.ne[i], h.nn[i], h.nnz_k[i]) for i in range(el)],
                     aux=[(h.na[i], h.nnz_n[i], h.nlong[i]) for i in range(al)], naux=h.naux, setup_ms=h.setup_ms)
 
+    def solve_v0(self, rtol=0.0):
+        """paraGenerator's V0 quasi-static solve (gp_solve_v0).
+
+        Returns (Z0 [np*np] complex at freqStart, index resp + np*src;
+        X [nfreq*np*np] complex as print_z_V0 leaves sys->x; stats).  rtol is
+        the real solves' relative residual, 0 = the reference's 1e-4."""
+        npt, nf = self.mesh.num_ports, max(int(self.mesh.c.nfreq), 1)
+        z = np.zeros(2 * npt * npt)
+        x = np.zeros(2 * nf * npt * npt)
+        st = capi.GpV0Stats()
+        capi.check(self.lib.gp_solve_v0(self.h, float(rtol), _p(z, f64p), _p(x, f64p), C.byref(st)), "gp_solve_v0")
+        stats = {k: getattr(st, k) for k, _ in capi.GpV0Stats._fields_}
+        return z[0::2] + 1j * z[1::2], x[0::2] + 1j * x[1::2], stats
+
     def solve(self, freqs, ports=None, precision=capi.GP_PREC_MIXED, rtol=1e-8, batch=0, restart=30,
               max_iters=2000, want_x=False, verbose=0, smooth_edge=0, smooth_aux=0, sell_min_rows=0):
         """Returns (Z[nf, np*np] complex, stats, X or None)."""
@@ -149,6 +163,16 @@ class Layout:
         ms = C.c_double()
         capi.check(self.lib.gp_bench_spmv(self.h, int(k), int(precision), int(iters), C.byref(ms)), "gp_bench_spmv")
         return ms.value
+
+
+def z_v0_sweep(mesh, Z0):
+    """print_z_V0's store on the host (gp_z_v0_sweep): nfreq*np*np complex."""
+    lib = capi.load()
+    npt, nf = mesh.num_ports, max(int(mesh.c.nfreq), 1)
+    x = np.zeros(2 * nf * npt * npt)
+    x[:2 * npt * npt] = np.stack([np.real(Z0), np.imag(Z0)], 1).ravel()
+    capi.check(lib.gp_z_v0_sweep(mesh.ptr, _p(x, f64p)), "gp_z_v0_sweep")
+    return x[0::2] + 1j * x[1::2]
 
 
 def solve_real(row, col, val, n, b, rtol=0.0):

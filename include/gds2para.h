@@ -15,6 +15,9 @@
  *                           void fdtdMesh::reference1(int, int, complex<double>*)   src/fdtd.hpp:3963
  *   gp_solve_real           int hypreSolve(fdtdMesh*, myint*, myint*, double*, myint,
  *                                          double*, myint, double*)  src/hypreSolver.h:14
+ *   gp_solve_v0             paraGenerator's V0 section + Construct_Z_V0   src/matrixCon.cpp:366-560,
+ *                                                                   src/fdtd.hpp:4178
+ *   gp_z_v0_sweep           fdtdMesh::print_z_V0 (its store)        src/fdtd.hpp:4249
  *   gp_extract_z            fdtdMesh::Construct_Z_V0_Vh             src/fdtd.hpp:4133
  *   gp_freq_of              fdtdMesh::freqNo2freq                   src/fdtd.hpp:4319
  *   gp_edge_map             fdtdMesh::setMapEdge                    src/fdtd.hpp:1508
@@ -300,6 +303,33 @@ gp_status gp_extract_z(const gp_mesh *mesh, const double *x, int32_t source_port
 gp_status gp_solve_real(const int64_t *row, const int64_t *col, const double *val, int64_t nnz,
                         int64_t n, const double *b, double *x, double rtol,
                         int32_t *iters_out, double *relres_out);
+
+/* ---- V0 quasi-static path (paraGenerator's V0 section) ------------------ */
+typedef struct gp_v0_stats {
+  int32_t solves;                 /* real solves run (3 per port, 2 without conductors) */
+  int32_t max_iters;              /* largest FGMRES iteration count of those      */
+  double max_rel_residual;        /* largest final relative residual             */
+  double setup_ms;                /* V0 operators + Ad / Ac multigrid setup (once per layout) */
+  double solve_ms;                /* this call's solves and Z extraction          */
+  int32_t ad_levels, ac_levels;   /* multigrid levels of Ad and Ac                */
+  int64_t ad_rows, ac_rows;       /* leng_v0d1, leng_v0c                          */
+} gp_v0_stats;
+
+/* paraGenerator's V0 section (src/matrixCon.cpp:366-560) for every source
+ * port: three real solves (Ad, Ac, Ad; hypreSolve at :392, :511, :544) and
+ * Construct_Z_V0 (src/fdtd.hpp:4178-4215) at freqStart.  Z0 receives the
+ * np*np block (complex interleaved, index resp + np*src).  X, if non-null,
+ * receives sys->x as print_z_V0 leaves it (nfreq*np*np complex, see
+ * gp_z_v0_sweep).  rtol is the real solves' relative residual; 0 selects the
+ * reference's 1e-4.  Builds the V0 basis if the layout has none; the V0
+ * operators and the Ad / Ac hierarchies stay with the layout for later calls. */
+gp_status gp_solve_v0(gp_layout *lay, double rtol, double *Z0, double *X, gp_v0_stats *stats);
+
+/* print_z_V0's store (src/fdtd.hpp:4249-4316), host only: X[0 .. np*np) holds
+ * Z0 on entry; slot k >= 1 receives Re Z0[j + i np] + i Im Z0[j + i np] f0 / f_k
+ * at index k np^2 + i + j np (the reference stores it transposed), with f_k
+ * = freqEnd * unit for the last slot and the linear / log grid between. */
+gp_status gp_z_v0_sweep(const gp_mesh *mesh, double *X);
 
 /* ---- measurement hooks -------------------------------------------------- */
 /* Waits for all work the layout queued on its stream. */

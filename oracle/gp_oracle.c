@@ -616,3 +616,74 @@ double or_freq(const gp_mesh *m, int32_t k) {
   if (m->freq_scale == 1) return (m->freq_start + k * (m->freq_end - m->freq_start) / (m->nfreq - 1)) * m->freq_unit;
   return m->freq_start * m->freq_unit * pow(m->freq_end / m->freq_start, (k * 1.0 / (m->nfreq - 1)));
 }
+
+/* ---- V0 quasi-static path (paraGenerator's V0 section) ------------------- */
+
+/* J of one source port over original edges: J[e] = portDirection of the side,
+ * assigned side by side (src/matrixCon.cpp:367-374); J zeroed by the caller */
+int or_port_j(const gp_mesh *m, int32_t port, double *J) {
+  for (int32_t s = m->port_side_ptr[port]; s < m->port_side_ptr[port + 1]; ++s)
+    for (int64_t k = m->side_edge_ptr[s]; k < m->side_edge_ptr[s + 1]; ++k) J[m->side_edges[k]] = m->side_dir[s];
+  return 0;
+}
+
+/* stackEpsn[e / (N_edge_s + N_edge_v)] * EPSILON0 per original edge
+ * (src/matrixCon.cpp:443, 497) */
+int or_edge_eps(const gp_mesh *m, double *eps) {
+  or_grid g = or_make_grid(m);
+  for (int64_t e = 0; e < g.Ne; ++e) eps[e] = m->stack_epsn[e / g.El] * OR_EPSILON0;
+  return 0;
+}
+
+/* Construct_Z_V0 (src/fdtd.hpp:4178-4215): x over ORIGINAL edge ids (N_edge
+ * complex, interleaved), accumulated into Z0[resp + np * src] */
+int or_extract_z_v0(const gp_mesh *m, const double *x, int32_t src, double *Z0) {
+  or_grid g = or_make_grid(m);
+  const int32_t np = m->num_ports;
+  double cur = 0.;
+  for (int32_t s = m->port_side_ptr[src]; s < m->port_side_ptr[src + 1]; ++s) cur += m->side_area[s];
+  for (int32_t p = 0; p < np; ++p) {
+    double zr = Z0[2 * (p + np * src)], zi = Z0[2 * (p + np * src) + 1];
+    int32_t s0 = m->port_side_ptr[p];
+    for (int64_t k = m->side_edge_ptr[s0]; k < m->side_edge_ptr[s0 + 1]; ++k) {
+      int64_t e = m->side_edges[k];
+      int64_t iz = e / g.El, r = e % g.El;
+      double leng;
+      if (r >= g.Es) leng = m->zn[iz + 1] - m->zn[iz];
+      else if (r >= g.Ny * (g.Nx + 1)) leng = m->xn[(r - g.Ny * (g.Nx + 1)) / (g.Ny + 1) + 1] - m->xn[(r - g.Ny * (g.Nx + 1)) / (g.Ny + 1)];
+      else leng = m->yn[r % g.Ny + 1] - m->yn[r % g.Ny];
+      double dir = m->side_dir[s0] * 1.0;
+      double tr = x[2 * e] * leng, ti = x[2 * e + 1] * leng;
+      tr = tr * dir; ti = ti * dir;
+      zr -= tr; zi -= ti;
+    }
+    zr /= cur; zi /= cur;
+    Z0[2 * (p + np * src)] = zr; Z0[2 * (p + np * src) + 1] = zi;
+  }
+  return 0;
+}
+
+/* print_z_V0's store (src/fdtd.hpp:4249-4316): slot 0 of X (nf * np * np
+ * complex, interleaved) holds Z0; slot id >= 1 receives
+ * Re Z0[j + i np] + i Im Z0[j + i np] f0 / f at index i + j np (transposed),
+ * with f = freqEnd * unit for the last slot and the linear / log grid between */
+int or_print_z_v0(const gp_mesh *m, double *X) {
+  const int32_t np = m->num_ports, nf = m->nfreq;
+  if (nf <= 1) return 0;
+  for (int32_t id = 1; id < nf; ++id) {
+    double f;
+    if (id == nf - 1) f = m->freq_end * m->freq_unit;
+    else if (m->freq_scale == 1) f = (m->freq_start + id * (m->freq_end - m->freq_start) / (nf - 1)) * m->freq_unit;
+    else f = m->freq_start * m->freq_unit * pow(m->freq_end / m->freq_start, (id * 1.0 / (nf - 1)));
+    for (int32_t i = 0; i < np; ++i)
+      for (int32_t j = 0; j < np; ++j) {
+        const double re = X[2 * (j + i * np)], im = X[2 * (j + i * np) + 1];
+        /* x.real() + (1i) * x.imag() * freqStart * freqUnit / freq */
+        const double zi = im * m->freq_start * m->freq_unit / f;
+        const size_t o = (size_t)id * np * np + i + (size_t)j * np;
+        X[2 * o] = re;
+        X[2 * o + 1] = zi;
+      }
+  }
+  return 0;
+}

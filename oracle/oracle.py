@@ -42,6 +42,10 @@ def _load():
         lib.or_extract_z.argtypes = [C.c_void_p, f64p, C.c_int32, f64p]
         lib.or_freq.restype = C.c_double
         lib.or_freq.argtypes = [C.c_void_p, C.c_int32]
+        lib.or_port_j.argtypes = [C.c_void_p, C.c_int32, f64p]
+        lib.or_edge_eps.argtypes = [C.c_void_p, f64p]
+        lib.or_extract_z_v0.argtypes = [C.c_void_p, f64p, C.c_int32, f64p]
+        lib.or_print_z_v0.argtypes = [C.c_void_p, f64p]
         _lib = lib
     return _lib
 
@@ -203,3 +207,86 @@ def direct_solve(mesh, freqs, ports=None, S=None, refine=False):
         X.append(xs)
         Z.append(Zk)
     return X, np.array(Z)
+
+
+# ---- V0 quasi-static path (paraGenerator, src/matrixCon.cpp:48-603) ---------
+
+def port_j(mesh, port):
+    """J over original edges for one source port (src/matrixCon.cpp:367-374)."""
+    J = np.zeros(mesh.n_edge)
+    _load().or_port_j(mesh.ptr, int(port), _p(J, f64p))
+    return J
+
+
+def edge_eps(mesh):
+    """stackEpsn[e / (N_edge_s + N_edge_v)] * EPSILON0 per original edge."""
+    e = np.zeros(mesh.n_edge)
+    _load().or_edge_eps(mesh.ptr, _p(e, f64p))
+    return e
+
+
+def extract_z_v0(mesh, yd, src, Z0):
+    """Construct_Z_V0 (src/fdtd.hpp:4178-4215) into Z0[resp + np*src]."""
+    buf = np.ascontiguousarray(np.stack([Z0.real, Z0.imag], 1).ravel())
+    xi = np.ascontiguousarray(np.stack([yd.real, yd.imag], 1).ravel())
+    _load().or_extract_z_v0(mesh.ptr, _p(xi, f64p), int(src), _p(buf, f64p))
+    return buf[0::2] + 1j * buf[1::2]
+
+
+def print_z_v0(mesh, Z0):
+    """print_z_V0's store (src/fdtd.hpp:4249-4316): the nf*np*np sweep array
+    sys->x, slot 0 = Z0, later slots Re + i Im f0/f stored transposed."""
+    np_, nf = mesh.num_ports, mesh.c.nfreq
+    X = np.zeros(2 * nf * np_ * np_)
+    X[:2 * np_ * np_] = np.stack([Z0.real, Z0.imag], 1).ravel()
+    _load().or_print_z_v0(mesh.ptr, _p(X, f64p))
+    return X[0::2] + 1j * X[1::2]
+
+
+def v0_operators(mesh):
+    """V0d, V0da, V0c, V0ca (N_edge x cols, original edge rows) and Ad, Ac as
+    SciPy CSR, from the restated basis (values before the sqrt(eps) /
+    sqrt(sigma) scaling, which is what the MKL handles of the V0 section hold,
+    src/matrixCon.cpp:104-108, 316-322)."""
+    d, c = v0d(mesh), v0c(mesh)
+    ne = mesh.n_edge
+    nd, ncl = int(d["cols"]), int(c["cols"])
+    Vd = sp.csr_matrix((d["val"], (d["row"], d["col"])), shape=(ne, nd))
+    Vda = sp.csr_matrix((d["aval"], (d["row"], d["col"])), shape=(ne, nd))
+    Vc = sp.csr_matrix((c["val"], (c["row"], c["col"])), shape=(ne, ncl))
+    Vca = sp.csr_matrix((c["aval"], (c["row"], c["col"])), shape=(ne, ncl))
+    k = v0_sizes(mesh)["ad_nnz"]          # kept entries; the trailing zero slots are never read
+    Ad = sp.csr_matrix((d["ad_val"][:k], (d["ad_row"][:k], d["ad_col"][:k])), shape=(nd, nd))
+    Ac = sp.csr_matrix((c["ac_val"], (c["ac_row"], c["ac_col"])), shape=(ncl, ncl))
+    return dict(Vd=Vd, Vda=Vda, Vc=Vc, Vca=Vca, Ad=Ad, Ac=Ac)
+
+
+def v0_path(mesh, ops=None, ports=None):
+    """Z at freqStart from the V0 quasi-static solve of every source port,
+    restating src/matrixCon.cpp:366-560 with exact (SuperLU) solves in place
+    of hypreSolve.  Returns (Z0 [np*np], index resp + np*src; yd per port)."""
+    ops = v0_operators(mesh) if ops is None else ops
+    Vd, Vda, Vc, Vca = ops["Vd"], ops["Vda"], ops["Vc"], ops["Vca"]
+    lud = spla.splu(ops["Ad"].tocsc())
+    luc = spla.splu(ops["Ac"].tocsc())
+    eps = edge_eps(mesh)
+    w0 = 2 * np.pi * mesh.c.freq_start * mesh.c.freq_unit
+    np_ = mesh.num_ports
+    Z0 = np.zeros(np_ * np_, complex)
+    yds = {}
+    for src in (range(np_) if ports is None else ports):
+        J = port_j(mesh, src)
+        y0d = lud.solve(-(Vda.T @ J))                 # :377-392
+        y0d = y0d / w0                                # :401-403
+        yd1 = Vd @ y0d                                # :411 (ydt), :441 (yd1)
+        ydt = yd1 * (-1.0 * w0 * eps)                 # :442
+        rhs_c = -(Vca.T @ J) + Vca.T @ ydt            # :434-460
+        y0c = luc.solve(rhs_c)                        # :511
+        yc = Vc @ y0c                                 # :531
+        yccp = -yc * eps                              # :536-538
+        y0d2 = lud.solve(Vda.T @ yccp)                # :543-544
+        yd2 = Vd @ y0d2                               # :552
+        yd = (yd2 - 1j * yd1) + yc                    # :574-587
+        Z0 = extract_z_v0(mesh, yd, src, Z0)
+        yds[src] = yd
+    return Z0, yds
