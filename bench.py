@@ -104,6 +104,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=24, help="requests in the serving leg (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate of the serving leg (1/s)")
+    ap.add_argument("--sell-min-rows", type=int, default=0,
+                    help="operators with at least this many rows use the sliced-ELL kernels (0: library default)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported by this build "
@@ -127,7 +129,7 @@ def main():
     ports = list(range(m.num_ports))
     ncol = len(freqs) * len(ports)
     kw = dict(ports=ports, precision=capi.GP_PREC_MIXED, rtol=1e-8, batch=a.batch, restart=a.restart,
-              max_iters=a.max_iters)
+              max_iters=a.max_iters, sell_min_rows=a.sell_min_rows)
 
     def step():
         return lay.solve(freqs, **kw)
@@ -212,7 +214,7 @@ def main():
         "speedup_vs_fp64_single": value / fp64["value"] if fp64 else None,
         "z_rel_diff_mixed_vs_fp64": zdiff,
         "copy_probe_GBs": copy_gbs,
-        "roofline": {"bound": "hbm", "kernel": "level-0 edge smoother (k_edge_sell<float, 16>)",
+        "roofline": {"bound": "hbm", "kernel": "level-0 edge smoother (k_edge_sell<float, 2, 8>: 8 lanes per row, 2 slots per lane)",
                      "achieved": kbytes / (kms * 1e-3) / 1e9 if kms > 0 else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s",
                      "frac": (kbytes / (kms * 1e-3) / 1e9) / HBM_PEAK_GBS if kms > 0 else None,

@@ -4,6 +4,8 @@
 #include <cmath>
 #include <complex>
 
+#include <hipcub/hipcub.hpp>
+
 #include "mg.hpp"
 
 namespace gp {
@@ -120,6 +122,46 @@ gp_status dense_host(const Csr &M, int which, int64_t nr, int64_t nc, std::vecto
 
 }  // namespace
 
+// Level-0 slot order of the edge smoother: rows sorted by (16 x 16-cell xy
+// tile, plane, ix, iy, direction).  The reference numbering puts a row's
+// couplings 40k (x-edges of the plane), 80k (z-edges) and 161k (planes above
+// and below) rows away at 200 x 200 x 24, farther than one XCD's L2 holds
+// while it works through that many rows; in tile order the rows in flight on
+// an XCD are one compact block of the mesh and share their neighbours' lines.
+constexpr int kTile = 16;
+
+__global__ void k_tile_keys(int64_t n, const int64_t *__restrict__ map_r, Grid g, uint64_t *__restrict__ key,
+                            int32_t *__restrict__ idx) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int dir; int64_t iz, ix, iy;
+  g.decode(map_r[r], dir, iz, ix, iy);
+  const uint64_t nty = (uint64_t)(g.Ny / kTile + 1);
+  const uint64_t tile = (uint64_t)(ix / kTile) * nty + (uint64_t)(iy / kTile);
+  key[r] = (((tile * (uint64_t)(g.Nz + 1) + (uint64_t)iz) * (uint64_t)(g.Nx + 1) + (uint64_t)ix) *
+                (uint64_t)(g.Ny + 1) + (uint64_t)iy) * 3u + (uint64_t)dir;
+  idx[r] = (int32_t)r;
+}
+
+gp_status level0_order(gp_layout *lay, Arena &A, int32_t **order, hipStream_t s) {
+  const int64_t n = lay->n;
+  *order = nullptr;
+  Arena tmp;
+  uint64_t *k1, *k2;
+  int32_t *i1;
+  GP_TRY(tmp.alloc(&k1, n)); GP_TRY(tmp.alloc(&k2, n)); GP_TRY(tmp.alloc(&i1, n));
+  GP_TRY(A.alloc(order, n));
+  k_tile_keys<<<blocks_for(n), 256, 0, s>>>(n, lay->map_edge_r, lay->g, k1, i1);
+  size_t bytes = 0;
+  GP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k1, k2, i1, *order, (int)n, 0, 64, s));
+  uint8_t *buf;
+  GP_TRY(tmp.alloc(&buf, (int64_t)bytes));
+  GP_HIP(hipcub::DeviceRadixSort::SortPairs(buf, bytes, k1, k2, i1, *order, (int)n, 0, 64, s));
+  GP_HIP(hipStreamSynchronize(s));
+  tmp.free_all();
+  return GP_OK;
+}
+
 gp_status build_hierarchy(gp_layout *lay, Hierarchy *H, bool want_f32, int max_levels) {
   auto t0 = std::chrono::steady_clock::now();
   if (!lay->s_ready) GP_TRY(gp_assemble_stiffness(lay));
@@ -217,10 +259,14 @@ gp_status build_hierarchy(gp_layout *lay, Hierarchy *H, bool want_f32, int max_l
     A.release(L0.K.v[2], L0.K.nnz);
     L0.K.nv = 1;
   }
+  int32_t *order0 = nullptr;
+  Arena otmp;
+  struct OrderGuard { Arena *a; ~OrderGuard() { a->free_all(); } } og{&otmp};  // the SELLs keep their own copy
+  if (H->L.size() > 1) GP_TRY(level0_order(lay, otmp, &order0, s));
   for (size_t l = 0; l + 1 < H->L.size(); ++l) {  // the coarsest level is solved densely
     MgLevel &L = H->L[l];
-    GP_TRY(sell_from_csr(A, L.K, &L.Ks, s, "L.K"));
-    GP_TRY(sell_from_csr(A, L.B, &L.Bs, s, "L.B"));
+    GP_TRY(sell_from_csr(A, L.K, &L.Ks, s, "L.K", l == 0 ? order0 : nullptr));
+    GP_TRY(sell_from_csr(A, L.B, &L.Bs, s, "L.B", l == 0 ? order0 : nullptr));
     GP_TRY(sell_from_csr(A, L.N, &L.Ns, s, "L.N"));
     GP_TRY(sell_from_csr(A, L.GT, &L.GTs, s, "L.GT"));
     GP_TRY(sell_from_csr(A, L.P, &L.Ps, s, "L.P"));
@@ -290,7 +336,7 @@ gp_status build_hierarchy(gp_layout *lay, Hierarchy *H, bool want_f32, int max_l
     Sv.nr = lay->S.n; Sv.nc = lay->S.n; Sv.nnz = lay->S.nnz; Sv.rp = lay->S.rp; Sv.ci = lay->S.ci; Sv.nv = 1; Sv.v[0] = lay->S.v;
     GP_TRY(spgemm_ex(A, Sv, H->L[0].G, &H->SG, s, 1));
     GP_TRY(spgemm_ex(A, Sv, H->aux.Gx, &H->SGx, s, 1));
-    GP_TRY(sell_from_csr(A, Sv, &H->Ss, s, "S"));
+    GP_TRY(sell_from_csr(A, Sv, &H->Ss, s, "S", order0));  // S rows are level 0's rows
   }
   H->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return GP_OK;
@@ -580,7 +626,12 @@ __global__ void k_edge(int64_t nr, int k, MatV<T> K, MatV<T> B, const T *__restr
 // Edge kernel on SELL matrices, one thread per (row, group of KS slots); the
 // same contract as k_edge.  Block = 256 rows (4 slices) of one slot group
 // (blockIdx.y); blocks are numbered per XCD (xcd_block_id).
-template <class T, int KS>
+// Level-0 edge smoother / residual on the sliced-ELL K and B.  LPR adjacent
+// lanes share one row and each holds SPL consecutive slots (one 16-byte load
+// per gathered neighbour row), so one wave-wide gather touches the lines of
+// 64 / LPR rows, not 64: with one thread per row and all slots in registers
+// the L1 tag lookups of the scattered 16-byte pieces, not HBM, set the pace.
+template <class T, int SPL, int LPR>
 __global__ void __launch_bounds__(256) k_edge_sell(int64_t nr, int k, SellV<T> K, SellV<T> B, const T *__restrict__ c1v,
                                                    const T *__restrict__ c2v, const T *__restrict__ kd0,
                                                    const T *__restrict__ kd1, const T *__restrict__ kd2,
@@ -588,11 +639,15 @@ __global__ void __launch_bounds__(256) k_edge_sell(int64_t nr, int k, SellV<T> K
                                                    const T *__restrict__ kl2, const C2<T> *__restrict__ r,
                                                    const C2<T> *__restrict__ u, const C2<T> *__restrict__ phi,
                                                    C2<T> *__restrict__ out, int mode, int use_phi, T damp) {
-  const int64_t i = xcd_block_id() * blockDim.x + threadIdx.x;
-  if (i >= nr) return;
-  const int j0 = blockIdx.y * KS;
-  const int64_t sl = i >> 6;
-  const int lane = (int)(i & 63);
+  constexpr int KS = SPL;
+  const int64_t t = xcd_block_id() * blockDim.x + threadIdx.x;
+  const int64_t pos = t / LPR;
+  if (pos >= nr) return;
+  const int sub = (int)(t - pos * LPR);
+  const int64_t i = K.rows ? (int64_t)K.rows[pos] : pos;  // B shares K's slot order
+  const int j0 = ((int)blockIdx.y * LPR + sub) * SPL;
+  const int64_t sl = pos >> 6;
+  const int lane = (int)(pos & 63);
   T c1[KS], c2[KS];
 #pragma unroll
   for (int q = 0; q < KS; ++q) { c1[q] = c1v[j0 + q]; c2[q] = c2v[j0 + q]; }
@@ -671,16 +726,18 @@ __global__ void __launch_bounds__(256) k_edge_sell(int64_t nr, int k, SellV<T> K
 
 // SELL nodal l1-Jacobi (contract of k_node); long rows (lidx >= 0) take their
 // product N_eps d, N_sig d from lp = [nlong][2][k]
-template <class T, int KS>
+template <class T, int SPL, int LPR>
 __global__ void __launch_bounds__(256) k_node_sell(int64_t nr, int k, SellV<T> N, const T *__restrict__ c1v,
                                                    const T *__restrict__ c2v, const T *__restrict__ nd0,
                                                    const T *__restrict__ nd1, const T *__restrict__ nl0,
                                                    const T *__restrict__ nl1, const C2<T> *__restrict__ rn,
                                                    const C2<T> *__restrict__ d, C2<T> *__restrict__ dout, T damp,
                                                    const int32_t *__restrict__ lidx, const C2<T> *__restrict__ lp) {
-  const int64_t i = xcd_block_id() * blockDim.x + threadIdx.x;
+  constexpr int KS = SPL;  // LPR adjacent lanes share a row, SPL slots each
+  const int64_t t_ = xcd_block_id() * blockDim.x + threadIdx.x;
+  const int64_t i = t_ / LPR;
   if (i >= nr) return;
-  const int j0 = blockIdx.y * KS;
+  const int j0 = ((int)blockIdx.y * LPR + (int)(t_ - i * LPR)) * SPL;
   T c1[KS], c2[KS];
 #pragma unroll
   for (int q = 0; q < KS; ++q) { c1[q] = c1v[j0 + q]; c2[q] = c2v[j0 + q]; }
@@ -731,13 +788,15 @@ __global__ void __launch_bounds__(256) k_node_sell(int64_t nr, int k, SellV<T> N
 }
 
 // SELL y = M x (real M) or y += M x; long rows from lp = [nlong][1][k]
-template <class T, int KS>
+template <class T, int SPL, int LPR>
 __global__ void __launch_bounds__(256) k_spmv_real_sell(int64_t nr, int k, SellV<T> M, const C2<T> *__restrict__ x,
                                                         C2<T> *__restrict__ y, int accumulate,
                                                         const int32_t *__restrict__ lidx, const C2<T> *__restrict__ lp) {
-  const int64_t i = xcd_block_id() * blockDim.x + threadIdx.x;
+  constexpr int KS = SPL;  // LPR adjacent lanes share a row, SPL slots each
+  const int64_t t_ = xcd_block_id() * blockDim.x + threadIdx.x;
+  const int64_t i = t_ / LPR;
   if (i >= nr) return;
-  const int j0 = blockIdx.y * KS;
+  const int j0 = ((int)blockIdx.y * LPR + (int)(t_ - i * LPR)) * SPL;
   C2<T> acc[KS];
   const int32_t lq = lidx ? lidx[i] : -1;
   if (lq >= 0) {
@@ -769,14 +828,16 @@ __global__ void __launch_bounds__(256) k_spmv_real_sell(int64_t nr, int k, SellV
 }
 
 // SELL M2(w) = c1 v0 + i c2 v1: y = r - M2 x (mode 0), y -= M2 x (1), y = M2 x (2)
-template <class T, int KS>
+template <class T, int SPL, int LPR>
 __global__ void __launch_bounds__(256) k_spmv2_sell(int64_t nr, int k, SellV<T> M, const T *__restrict__ c1v,
                                                     const T *__restrict__ c2v, const C2<T> *__restrict__ r,
                                                     const C2<T> *__restrict__ x, C2<T> *__restrict__ y, int mode,
                                                     const int32_t *__restrict__ lidx, const C2<T> *__restrict__ lp) {
-  const int64_t i = xcd_block_id() * blockDim.x + threadIdx.x;
+  constexpr int KS = SPL;  // LPR adjacent lanes share a row, SPL slots each
+  const int64_t t_ = xcd_block_id() * blockDim.x + threadIdx.x;
+  const int64_t i = t_ / LPR;
   if (i >= nr) return;
-  const int j0 = blockIdx.y * KS;
+  const int j0 = ((int)blockIdx.y * LPR + (int)(t_ - i * LPR)) * SPL;
   T c1[KS], c2[KS];
 #pragma unroll
   for (int q = 0; q < KS; ++q) { c1[q] = c1v[j0 + q]; c2[q] = c2v[j0 + q]; }
@@ -1024,6 +1085,7 @@ SellV<double> sview<double>(const Sell &c, const SellF &, const Csr &kc, const C
   SellV<double> v; v.nr = c.nr; v.nslice = c.nslice; v.base = c.base; v.w = c.w; v.ci = c.ci; v.nv = c.nv;
   for (int q = 0; q < c.nv; ++q) v.v[q] = c.v[q];
   v.dg[0] = kc.dg[0]; v.dg[1] = kc.dg[1];
+  v.rows = c.rows;
   return v;
 }
 template <>
@@ -1031,20 +1093,50 @@ SellV<float> sview<float>(const Sell &, const SellF &f, const Csr &, const CsrF 
   SellV<float> v; v.nr = f.nr; v.nslice = f.nslice; v.base = f.base; v.w = f.w; v.ci = f.ci; v.nv = f.nv;
   for (int q = 0; q < f.nv; ++q) v.v[q] = f.v[q];
   v.dg[0] = kf.dg[0]; v.dg[1] = kf.dg[1];
+  v.rows = f.rows;
   return v;
 }
 
-// Kernel choice per operator: SELL with KS slots per thread for the operators
-// with at least sell_min_rows rows (the level-0 edge stencil, ~13 entries per
-// row, where one coalesced matrix read serves all slots of a row), the
-// (row, slot) CSR kernels elsewhere: on the wider Galerkin rows of coarser
-// levels their 16x larger thread count hides the gather latency better
-// (measured at 200x200x24: level-1 edges 340 us CSR vs 609 us SELL).
-inline int pick_ks(int64_t nr, int k) {
-  int ks = k < 16 ? k : 16;
-  while (ks > 1 && nr * (k / ks) < (int64_t(1) << 20)) ks >>= 1;
-  return ks;
+// Kernel choice per operator: the lanes-per-row SELL kernels for operators of
+// at least sell_min_rows rows (default 2^16), the (row, slot) CSR kernels for
+// the small coarse levels, where SELL's padding and slice bookkeeping do not
+// pay off (measured at 200x200x24, 80 columns: 11.46 solves/s at 2^16, 10.61
+// at 2^20, 11.11 with every operator on SELL).
+// Launch shape of the lanes-per-row SELL kernels: SPL slots per lane (two
+// complex FP32 = one 16-byte load, else one), LPR lanes per row (the largest
+// power of two up to 16 that divides k / SPL), k / (SPL LPR) grid rows.
+struct Lanes {
+  int sp = 1, lpr = 1;
+  dim3 grid;
+};
+template <class T>
+Lanes lanes_shape(int64_t nr, int k) {
+  Lanes L;
+  constexpr int spl = sizeof(T) == 4 ? 2 : 1;
+  L.sp = (k % spl == 0) ? spl : 1;
+  while (L.lpr < 16 && (k / L.sp) % (L.lpr * 2) == 0) L.lpr *= 2;
+  L.grid = dim3((unsigned)((nr * L.lpr + 255) / 256), (unsigned)(k / (L.sp * L.lpr)));
+  return L;
 }
+
+#define GP_LANES_DISPATCH(KERNEL, ...)                                                                   \
+  do {                                                                                                   \
+    const Lanes ln_ = lanes_shape<T>(nr, k);                                                             \
+    if (ln_.sp == 2) {                                                                                   \
+      if (ln_.lpr == 16) KERNEL<T, 2, 16><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                         \
+      else if (ln_.lpr == 8) KERNEL<T, 2, 8><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                      \
+      else if (ln_.lpr == 4) KERNEL<T, 2, 4><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                      \
+      else if (ln_.lpr == 2) KERNEL<T, 2, 2><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                      \
+      else KERNEL<T, 2, 1><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                                        \
+    } else {                                                                                             \
+      if (ln_.lpr == 16) KERNEL<T, 1, 16><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                         \
+      else if (ln_.lpr == 8) KERNEL<T, 1, 8><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                      \
+      else if (ln_.lpr == 4) KERNEL<T, 1, 4><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                      \
+      else if (ln_.lpr == 2) KERNEL<T, 1, 2><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                      \
+      else KERNEL<T, 1, 1><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                                        \
+    }                                                                                                    \
+  } while (0)
+
 
 template <class T>
 void edge_sell(int64_t nr, int64_t smin, int k, const SellV<T> &K, const SellV<T> &B, const MatV<T> &Kc, const MatV<T> &Bc,
@@ -1056,17 +1148,7 @@ void edge_sell(int64_t nr, int64_t smin, int k, const SellV<T> &K, const SellV<T
                                                  mode, use_phi, damp);
     return;
   }
-  const unsigned gx = (unsigned)((nr + 255) / 256);
-  const int ks = pick_ks(nr, k);
-#define GP_EDGE_SELL(KS)                                                                                       \
-  k_edge_sell<T, KS><<<dim3(gx, (unsigned)(k / KS)), 256, 0, s>>>(nr, k, K, B, c1, c2, kd0, kd1, kd2, kl0, kl1, kl2, \
-                                                                   r, u, phi, out, mode, use_phi, damp)
-  if (ks == 16) GP_EDGE_SELL(16);
-  else if (ks == 8) GP_EDGE_SELL(8);
-  else if (ks == 4) GP_EDGE_SELL(4);
-  else if (ks == 2) GP_EDGE_SELL(2);
-  else GP_EDGE_SELL(1);
-#undef GP_EDGE_SELL
+  GP_LANES_DISPATCH(k_edge_sell, nr, k, K, B, c1, c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, phi, out, mode, use_phi, damp);
 }
 
 template <class T>
@@ -1084,17 +1166,6 @@ SellV<float> sv<float>(const Sell &, const SellF &f) {
   return v;
 }
 
-#define GP_KS_DISPATCH(KERNEL, ...)                                                          \
-  do {                                                                                       \
-    const unsigned gx_ = (unsigned)((nr + 255) / 256);                                       \
-    const int ks_ = pick_ks(nr, k);                                                          \
-    if (ks_ == 16) KERNEL<T, 16><<<dim3(gx_, (unsigned)(k / 16)), 256, 0, s>>>(__VA_ARGS__); \
-    else if (ks_ == 8) KERNEL<T, 8><<<dim3(gx_, (unsigned)(k / 8)), 256, 0, s>>>(__VA_ARGS__); \
-    else if (ks_ == 4) KERNEL<T, 4><<<dim3(gx_, (unsigned)(k / 4)), 256, 0, s>>>(__VA_ARGS__); \
-    else if (ks_ == 2) KERNEL<T, 2><<<dim3(gx_, (unsigned)(k / 2)), 256, 0, s>>>(__VA_ARGS__); \
-    else KERNEL<T, 1><<<dim3(gx_, (unsigned)k), 256, 0, s>>>(__VA_ARGS__);                  \
-  } while (0)
-
 template <class T>
 void node_sell(int64_t nr, int64_t smin, int k, const SellV<T> &N, const MatV<T> &Nc, const T *c1, const T *c2, const T *nd0,
                const T *nd1, const T *nl0, const T *nl1, const C2<T> *rn, const C2<T> *d, C2<T> *dout, T damp,
@@ -1104,7 +1175,7 @@ void node_sell(int64_t nr, int64_t smin, int k, const SellV<T> &N, const MatV<T>
     k_node<T><<<blocks_for(nr * k), 256, 0, s>>>(nr, k, Nc, c1, c2, nd0, nd1, nl0, nl1, rn, d, dout, damp, lp);
     return;
   }
-  GP_KS_DISPATCH(k_node_sell, nr, k, N, c1, c2, nd0, nd1, nl0, nl1, rn, d, dout, damp, lidx, lp);
+  GP_LANES_DISPATCH(k_node_sell, nr, k, N, c1, c2, nd0, nd1, nl0, nl1, rn, d, dout, damp, lidx, lp);
 }
 
 template <class T>
@@ -1115,7 +1186,7 @@ void spmv_real_sell(int64_t nr, int64_t smin, int k, const SellV<T> &M, const Ma
     k_spmv_real<T><<<blocks_for(nr * k), 256, 0, s>>>(nr, k, Mc, x, y, accumulate, lp);
     return;
   }
-  GP_KS_DISPATCH(k_spmv_real_sell, nr, k, M, x, y, accumulate, lidx, lp);
+  GP_LANES_DISPATCH(k_spmv_real_sell, nr, k, M, x, y, accumulate, lidx, lp);
 }
 
 template <class T>
@@ -1126,7 +1197,7 @@ void spmv2_sell(int64_t nr, int64_t smin, int k, const SellV<T> &M, const MatV<T
     k_spmv2<T><<<blocks_for(nr * k), 256, 0, s>>>(nr, k, Mc, c1, c2, r, x, y, mode, lp);
     return;
   }
-  GP_KS_DISPATCH(k_spmv2_sell, nr, k, M, c1, c2, r, x, y, mode, lidx, lp);
+  GP_LANES_DISPATCH(k_spmv2_sell, nr, k, M, c1, c2, r, x, y, mode, lidx, lp);
 }
 
 template <class T> const T *arr(double *d, float *f);

@@ -93,6 +93,7 @@ struct SellV {  // read-only view of a Sell in precision T
   const T *v[3] = {nullptr, nullptr, nullptr};
   int nv = 0;
   const T *dg[2] = {nullptr, nullptr};  // diagonal value blocks 1, 2 (level 0)
+  const int32_t *rows = nullptr;        // slot position -> row (nullptr: identity)
 };
 
 struct MgLevel {
@@ -186,7 +187,7 @@ struct VWork {
   C2<T> *lp = nullptr;                      // long-row products [nlong][nv][k]
   C2<T> *lpart = nullptr;                   // per-chunk partial products [nchunk][nv][k]
   int nu_aux = 2;
-  int64_t sell_min_rows = int64_t(1) << 20;  // smaller operators use the (row, slot) CSR kernels
+  int64_t sell_min_rows = int64_t(1) << 16;  // smaller operators use the (row, slot) CSR kernels
   KTimer *kt = nullptr;               // times the level-0 edge smoother launches
   Arena arena;
 };

@@ -121,18 +121,22 @@ __global__ void k_outer(OuterOp op, int k, const double *__restrict__ omega, con
   out[i * k + j] = acc;
 }
 
-// k_outer on the SELL copy of S: one thread per (row, group of KS slots),
-// S read once for the group; G, Gx, S G, S Gx rows are short CSR walks.
-template <class TZ, int KS>
+// k_outer on the SELL copy of S: LPR adjacent lanes share a row, each with
+// SPL slots (one 16-byte load of FP32 u per gathered row), S read once per
+// lane group; G, Gx, S G, S Gx rows are short CSR walks.
+template <class TZ, int SPL, int LPR>
 __global__ void __launch_bounds__(256) k_outer_sell(OuterOp op, SellV<double> SS, int k, const double *__restrict__ omega,
                                                     const C2<TZ> *__restrict__ u, const C2<TZ> *__restrict__ phi,
                                                     const C2<TZ> *__restrict__ a, const Z2 *__restrict__ b,
                                                     Z2 *__restrict__ out, int mode, int exact) {
-  const int64_t i = xcd_block_id() * blockDim.x + threadIdx.x;
-  if (i >= op.n) return;
-  const int j0 = blockIdx.y * KS;
-  const int64_t sl = i >> 6;
-  const int lane = (int)(i & 63);
+  constexpr int KS = SPL;
+  const int64_t t_ = xcd_block_id() * blockDim.x + threadIdx.x;
+  const int64_t pos = t_ / LPR;
+  if (pos >= op.n) return;
+  const int j0 = ((int)blockIdx.y * LPR + (int)(t_ - pos * LPR)) * SPL;
+  const int64_t i = SS.rows ? (int64_t)SS.rows[pos] : pos;
+  const int64_t sl = pos >> 6;
+  const int lane = (int)(pos & 63);
   Z2 acc[KS], xi[KS];
 #pragma unroll
   for (int q = 0; q < KS; ++q) acc[q] = {0., 0.};
@@ -378,15 +382,26 @@ void outer(const OuterOp &op, const Hierarchy &H, int64_t smin, int k, const dou
   SellV<double> SS;
   SS.nr = H.Ss.nr; SS.nslice = H.Ss.nslice; SS.base = H.Ss.base; SS.w = H.Ss.w; SS.ci = H.Ss.ci; SS.nv = 1;
   SS.v[0] = H.Ss.v[0];
-  const unsigned gx = (unsigned)((op.n + 255) / 256);
-  int ks = k < 16 ? k : 16;
-#define GP_OUTER_SELL(KS) \
-  k_outer_sell<TZ, KS><<<dim3(gx, (unsigned)(k / KS)), 256, 0, s>>>(op, SS, k, om, u, phi, a, b, out, mode, exact)
-  if (ks == 16) GP_OUTER_SELL(16);
-  else if (ks == 8) GP_OUTER_SELL(8);
-  else if (ks == 4) GP_OUTER_SELL(4);
-  else if (ks == 2) GP_OUTER_SELL(2);
-  else GP_OUTER_SELL(1);
+  SS.rows = H.Ss.rows;
+  const int sp = (sizeof(TZ) == 4 && k % 2 == 0) ? 2 : 1;
+  int lpr = 1;
+  while (lpr < 16 && (k / sp) % (lpr * 2) == 0) lpr *= 2;
+  const dim3 grid((unsigned)((op.n * lpr + 255) / 256), (unsigned)(k / (sp * lpr)));
+#define GP_OUTER_SELL(SP, LP) \
+  k_outer_sell<TZ, SP, LP><<<grid, 256, 0, s>>>(op, SS, k, om, u, phi, a, b, out, mode, exact)
+  if (sp == 2) {
+    if (lpr == 16) GP_OUTER_SELL(2, 16);
+    else if (lpr == 8) GP_OUTER_SELL(2, 8);
+    else if (lpr == 4) GP_OUTER_SELL(2, 4);
+    else if (lpr == 2) GP_OUTER_SELL(2, 2);
+    else GP_OUTER_SELL(2, 1);
+  } else {
+    if (lpr == 16) GP_OUTER_SELL(1, 16);
+    else if (lpr == 8) GP_OUTER_SELL(1, 8);
+    else if (lpr == 4) GP_OUTER_SELL(1, 4);
+    else if (lpr == 2) GP_OUTER_SELL(1, 2);
+    else GP_OUTER_SELL(1, 1);
+  }
 #undef GP_OUTER_SELL
 }
 
@@ -470,7 +485,7 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
   GP_TRY(vwork_create<T>(lay, H, k, &vw));
   vw.nu_aux = o.smooth_aux > 0 ? std::min(o.smooth_aux, 8) : 2;
   vw.kt = &lay->ktimer;
-  vw.sell_min_rows = o.sell_min_rows > 0 ? o.sell_min_rows : (int64_t(1) << 20);
+  vw.sell_min_rows = o.sell_min_rows > 0 ? o.sell_min_rows : (int64_t(1) << 16);
   DevBuf D;
   Arena &A = D.A;
   Z2 *b, *xu, *xphi, *xa, *V, *yd;

@@ -439,28 +439,32 @@ gp_status csr_host_copy(const Csr &a, HostCsr *h, int which) {
 namespace {
 
 __global__ void k_sell_width(int64_t nr, int64_t nslice, const int64_t *__restrict__ rp, const int32_t *__restrict__ lidx,
-                             int32_t *__restrict__ w, int64_t *__restrict__ cnt) {
+                             const int32_t *__restrict__ order, int32_t *__restrict__ w, int64_t *__restrict__ cnt) {
   int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (sl >= nslice) return;
   int64_t m = 0;
-  for (int64_t i = sl * 64; i < min(nr, sl * 64 + 64); ++i)
+  for (int64_t p = sl * 64; p < min(nr, sl * 64 + 64); ++p) {
+    const int64_t i = order ? (int64_t)order[p] : p;
     if (!(lidx && lidx[i] >= 0)) m = max(m, rp[i + 1] - rp[i]);
+  }
   w[sl] = (int32_t)m;
   cnt[sl] = 64 * m;
 }
 
 __global__ void k_sell_fill(int64_t nr, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci, int nv,
                             const double *__restrict__ v0, const double *__restrict__ v1, const double *__restrict__ v2,
-                            const int32_t *__restrict__ lidx, const int64_t *__restrict__ base,
-                            const int32_t *__restrict__ w, int32_t *__restrict__ oci, double *__restrict__ o0,
-                            double *__restrict__ o1, double *__restrict__ o2) {
-  // one thread per slot row of every slice, so the lanes past the last row
-  // (never read by the kernels) hold valid padding too
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ((nr + 63) >> 6) << 6) return;
-  const int64_t sl = i >> 6, lane = i & 63;
+                            const int32_t *__restrict__ lidx, const int32_t *__restrict__ order,
+                            const int64_t *__restrict__ base, const int32_t *__restrict__ w,
+                            int32_t *__restrict__ oci, double *__restrict__ o0, double *__restrict__ o1,
+                            double *__restrict__ o2) {
+  // one thread per slot position of every slice, so the lanes past the last
+  // row (never read by the kernels) hold valid padding too
+  int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= ((nr + 63) >> 6) << 6) return;
+  const int64_t sl = pos >> 6, lane = pos & 63;
   const int64_t b = base[sl];
-  const bool skip = i >= nr || (lidx && lidx[i] >= 0);
+  const int64_t i = pos < nr ? (order ? (int64_t)order[pos] : pos) : pos;
+  const bool skip = pos >= nr || (lidx && lidx[i] >= 0);
   const int64_t len = skip ? 0 : rp[i + 1] - rp[i];
   for (int64_t t = 0; t < w[sl]; ++t) {
     const int64_t e = b + 64 * t + lane;
@@ -499,7 +503,7 @@ __global__ void k_d2f_sell(int64_t n, const double *__restrict__ a, float *__res
 
 }  // namespace
 
-gp_status sell_from_csr(Arena &A, const Csr &a, Sell *out, hipStream_t s, const char *name) {
+gp_status sell_from_csr(Arena &A, const Csr &a, Sell *out, hipStream_t s, const char *name, const int32_t *order) {
   *out = Sell();
   out->nr = a.nr;
   out->nv = a.nv;
@@ -512,7 +516,11 @@ gp_status sell_from_csr(Arena &A, const Csr &a, Sell *out, hipStream_t s, const 
     if ((st = A.alloc(&out->w, out->nslice)) != GP_OK) break;
     if ((st = A.alloc(&out->base, out->nslice + 1)) != GP_OK) break;
     if ((st = tmp.alloc(&cnt, out->nslice)) != GP_OK) break;
-    k_sell_width<<<blocks_for(out->nslice), 256, 0, s>>>(a.nr, out->nslice, a.rp, a.lidx, out->w, cnt);
+    if (order) {
+      if ((st = A.alloc(&out->rows, a.nr)) != GP_OK) break;
+      if (hipMemcpyAsync(out->rows, order, sizeof(int32_t) * a.nr, hipMemcpyDeviceToDevice, s) != hipSuccess) { st = fail(GP_ERR_HIP, "copy"); break; }
+    }
+    k_sell_width<<<blocks_for(out->nslice), 256, 0, s>>>(a.nr, out->nslice, a.rp, a.lidx, out->rows, out->w, cnt);
     if ((st = exclusive_scan_i64(cnt, out->base, out->nslice, s, tmp)) != GP_OK) break;
     if (hipMemcpyAsync(&out->nent, out->base + out->nslice, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) { st = fail(GP_ERR_HIP, "sell size"); break; }
@@ -521,8 +529,9 @@ gp_status sell_from_csr(Arena &A, const Csr &a, Sell *out, hipStream_t s, const 
     for (int q = 0; q < a.nv; ++q)
       if ((st = A.alloc(&out->v[q], ne)) != GP_OK) break;
     if (st != GP_OK) break;
-    k_sell_fill<<<blocks_for(out->nslice * 64), 256, 0, s>>>(a.nr, a.rp, a.ci, a.nv, a.v[0], a.v[1], a.v[2], a.lidx, out->base,
-                                                 out->w, out->ci, out->v[0], out->v[1], out->v[2]);
+    k_sell_fill<<<blocks_for(out->nslice * 64), 256, 0, s>>>(a.nr, a.rp, a.ci, a.nv, a.v[0], a.v[1], a.v[2], a.lidx,
+                                                             out->rows, out->base, out->w, out->ci, out->v[0],
+                                                             out->v[1], out->v[2]);
     // every column the kernels will gather from must exist
     int *bad = nullptr, hb[3] = {0, 2147483647, -1};
     if ((st = tmp.alloc(&bad, 3)) != GP_OK) break;
@@ -546,6 +555,7 @@ gp_status sell_from_csr(Arena &A, const Csr &a, Sell *out, hipStream_t s, const 
 gp_status sell_to_float(Arena &A, const Sell &a, SellF *f, hipStream_t s) {
   *f = SellF();
   f->nr = a.nr; f->nslice = a.nslice; f->nent = a.nent; f->base = a.base; f->w = a.w; f->ci = a.ci; f->nv = a.nv;
+  f->rows = a.rows;
   for (int q = 0; q < a.nv; ++q) {
     GP_TRY(A.alloc(&f->v[q], std::max<int64_t>(a.nent, 1)));
     if (a.nent) k_d2f_sell<<<blocks_for(a.nent), 256, 0, s>>>(a.nent, a.v[q], f->v[q]);

@@ -60,6 +60,7 @@ struct Sell {
   int32_t *ci = nullptr;
   int nv = 0;
   double *v[3] = {nullptr, nullptr, nullptr};
+  int32_t *rows = nullptr;  // slot position -> row (nullptr: position = row)
 };
 struct SellF {
   int64_t nr = 0, nslice = 0, nent = 0;
@@ -68,10 +69,15 @@ struct SellF {
   const int32_t *ci = nullptr;
   int nv = 0;
   float *v[3] = {nullptr, nullptr, nullptr};
+  const int32_t *rows = nullptr;
 };
 // SELL copy of a CSR (all value arrays); rows listed as long in a (kLongRow)
 // are left empty, their products come from the long-row pass
-gp_status sell_from_csr(Arena &A, const Csr &a, Sell *out, hipStream_t s, const char *name);
+// order (device, nr entries, a permutation of the rows, or nullptr): slice
+// position p holds row order[p], so a launch over positions visits the rows in
+// that order; kernels map position -> row through Sell::rows
+gp_status sell_from_csr(Arena &A, const Csr &a, Sell *out, hipStream_t s, const char *name,
+                        const int32_t *order = nullptr);
 gp_status sell_to_float(Arena &A, const Sell &a, SellF *f, hipStream_t s);
 
 gp_status csr_upload(Arena &A, const HostCsr &h, Csr *out, hipStream_t s);

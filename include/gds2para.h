@@ -202,7 +202,7 @@ typedef struct gp_solve_opts {
   int32_t smooth_edge;            /* edge/nodal smoothing sweeps per level (0: 2) */
   int32_t smooth_aux;             /* auxiliary AMG sweeps per level (0: 2)       */
   int64_t sell_min_rows;          /* sliced-ELL kernels on operators with at least
-                                     this many rows (0: 2^20; 1: every operator) */
+                                     this many rows (0: 2^16; 1: every operator) */
 } gp_solve_opts;
 
 typedef struct gp_solve_stats {
