@@ -146,6 +146,9 @@ SIGNATURES = [
     ("gp_solve_real", i32, [P(i64), P(i64), P(f64), i64, i64, P(f64), P(f64), f64, P(i32), P(f64)]),
     ("gp_solve_v0", i32, [C.c_void_p, f64, P(f64), P(f64), P(GpV0Stats)]),
     ("gp_z_v0_sweep", i32, [P(GpMesh), P(f64)]),
+    ("gp_convert_param", i32, [i32, i32, C.c_char, C.c_char, P(f64), P(f64), P(f64)]),
+    ("gp_write_citi", i32, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char, i32, i32, P(f64), P(f64)]),
+    ("gp_write_touchstone", i32, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char, i32, i32, P(f64), P(f64), P(f64)]),
     ("gp_bench_spmv", i32, [C.c_void_p, i32, i32, i32, P(f64)]),
     ("gp_layout_sync", i32, [C.c_void_p]),
     ("gp_bench_copy", i32, [i64, i32, P(f64)]),

@@ -1,0 +1,250 @@
+// netparam.cpp -- network-parameter output of the solved Z (host only):
+//   Parasitics::convertParam        src/solnoutclass.hpp:1456-1600  (Z / Y / S)
+//   SolverDataBase::dumpCITI        src/solnoutclass.hpp:6040-6100
+//   SolverDataBase::dumpTouchstone  src/solnoutclass.hpp:6104-6215
+// Matrices are nf blocks of np x np complex values (interleaved re, im),
+// row-major: mat[(f np + i) np + j] is entry (i, j) at frequency f -- the
+// layout Parasitics::saveNetworkParam reads sys->x with (:1394-1420), so the
+// Z array the solves return is passed as it is.
+#include <cctype>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <ctime>
+#include <string>
+#include <vector>
+
+#include "gds2para.h"
+#include "gp_status.hpp"
+
+namespace {
+
+using cd = std::complex<double>;
+using Mat = std::vector<cd>;  // n x n row-major
+
+Mat eye(int n) {
+  Mat m((size_t)n * n, cd(0., 0.));
+  for (int i = 0; i < n; ++i) m[(size_t)i * n + i] = 1.;
+  return m;
+}
+
+Mat mul(const Mat &a, const Mat &b, int n) {
+  Mat c((size_t)n * n, cd(0., 0.));
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < n; ++k) {
+      const cd aik = a[(size_t)i * n + k];
+      for (int j = 0; j < n; ++j) c[(size_t)i * n + j] += aik * b[(size_t)k * n + j];
+    }
+  return c;
+}
+
+Mat add(const Mat &a, const Mat &b, double sb) {
+  Mat c(a.size());
+  for (size_t q = 0; q < a.size(); ++q) c[q] = a[q] + sb * b[q];
+  return c;
+}
+
+// D a D for the diagonal D = diag(d)
+Mat dscale(const std::vector<double> &d, const Mat &a, int n) {
+  Mat c(a.size());
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) c[(size_t)i * n + j] = d[i] * a[(size_t)i * n + j] * d[j];
+  return c;
+}
+
+// inverse by Gauss-Jordan elimination with partial pivoting; false if singular
+bool inverse(Mat a, int n, Mat *out) {
+  Mat b = eye(n);
+  for (int c = 0; c < n; ++c) {
+    int p = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::abs(a[(size_t)r * n + c]) > std::abs(a[(size_t)p * n + c])) p = r;
+    if (std::abs(a[(size_t)p * n + c]) == 0.) return false;
+    if (p != c)
+      for (int j = 0; j < n; ++j) {
+        std::swap(a[(size_t)p * n + j], a[(size_t)c * n + j]);
+        std::swap(b[(size_t)p * n + j], b[(size_t)c * n + j]);
+      }
+    const cd inv = 1. / a[(size_t)c * n + c];
+    for (int j = 0; j < n; ++j) { a[(size_t)c * n + j] *= inv; b[(size_t)c * n + j] *= inv; }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const cd f = a[(size_t)r * n + c];
+      if (f == 0.) continue;
+      for (int j = 0; j < n; ++j) {
+        a[(size_t)r * n + j] -= f * a[(size_t)c * n + j];
+        b[(size_t)r * n + j] -= f * b[(size_t)c * n + j];
+      }
+    }
+  }
+  *out = b;
+  return true;
+}
+
+std::string fmt_g(double v, int prec) {
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%.*g", prec, v);
+  return buf;
+}
+
+// ostream << setw(w) << setprecision(p) << v with std::left (left = true) or std::right
+std::string field(double v, int prec, int w, bool left) {
+  std::string s = fmt_g(v, prec);
+  if ((int)s.size() >= w) return s;
+  return left ? s + std::string(w - s.size(), ' ') : std::string(w - s.size(), ' ') + s;
+}
+
+std::string stamp(const char *given, const char *pattern) {
+  if (given) return given;
+  char buf[80];
+  std::time_t t = std::time(nullptr);
+  std::strftime(buf, sizeof buf, pattern, std::localtime(&t));
+  return buf;
+}
+
+bool valid_param(char p) { return p == 'Z' || p == 'Y' || p == 'S'; }
+
+}  // namespace
+
+using gp::fail;
+
+extern "C" gp_status gp_convert_param(int32_t np, int32_t nf, char from, char to, const double *zsource,
+                                      const double *in, double *out) {
+  if (np < 1 || nf < 0 || !in || !out || !valid_param(from) || !valid_param(to))
+    return fail(GP_ERR_ARG, "gp_convert_param: bad arguments (parameters are 'Z', 'Y' or 'S')");
+  const bool need_z0 = (from == 'S') != (to == 'S');
+  if (need_z0 && !zsource) return fail(GP_ERR_ARG, "gp_convert_param: S conversions need the port impedances");
+  if (need_z0)
+    for (int i = 0; i < np; ++i)
+      if (!(zsource[i] > 0.)) return fail(GP_ERR_ARG, "gp_convert_param: port impedances must be positive");
+  const size_t nn = (size_t)np * np;
+  std::vector<double> zr(np), yr(np);
+  for (int i = 0; i < np && need_z0; ++i) {
+    zr[i] = std::sqrt(zsource[i]);        // diagPortZ (:1481)
+    yr[i] = std::sqrt(1.0 / zsource[i]);  // diagPortY (:1523)
+  }
+  const Mat I = eye(np);
+  for (int f = 0; f < nf; ++f) {
+    Mat P(nn), R;
+    for (size_t q = 0; q < nn; ++q) P[q] = cd(in[2 * (f * nn + q)], in[2 * (f * nn + q) + 1]);
+    bool ok = true;
+    if (from == to) {
+      R = P;
+    } else if ((from == 'Z' && to == 'Y') || (from == 'Y' && to == 'Z')) {
+      ok = inverse(P, np, &R);                                   // :1467-1472, :1508-1512
+    } else if (from == 'Y' && to == 'S') {                       // :1474-1494
+      const Mat M = dscale(zr, P, np);
+      Mat inv;
+      ok = inverse(add(I, M, 1.), np, &inv);
+      if (ok) R = mul(add(I, M, -1.), inv, np);
+    } else if (from == 'Z' && to == 'S') {                       // :1516-1536
+      const Mat M = dscale(yr, P, np);
+      Mat inv;
+      ok = inverse(add(M, I, 1.), np, &inv);
+      if (ok) R = mul(add(M, I, -1.), inv, np);
+    } else if (from == 'S' && to == 'Y') {                       // :1553-1573
+      Mat inv;
+      ok = inverse(add(I, P, 1.), np, &inv);
+      if (ok) R = dscale(yr, mul(add(I, P, -1.), inv, np), np);
+    } else {                                                     // S -> Z, :1574-1594
+      Mat inv;
+      ok = inverse(add(I, P, -1.), np, &inv);
+      if (ok) R = dscale(zr, mul(add(I, P, 1.), inv, np), np);
+    }
+    if (!ok) return fail(GP_ERR_STATE, "gp_convert_param: singular matrix at frequency index " + std::to_string(f));
+    for (size_t q = 0; q < nn; ++q) {
+      out[2 * (f * nn + q)] = R[q].real();
+      out[2 * (f * nn + q) + 1] = R[q].imag();
+    }
+  }
+  return GP_OK;
+}
+
+extern "C" gp_status gp_write_citi(const char *path, const char *design, const char *timestamp, char param,
+                                   int32_t np, int32_t nf, const double *freqs, const double *mat) {
+  if (!path || !design || np < 1 || nf < 0 || (nf > 0 && (!freqs || !mat)) || !valid_param(param))
+    return fail(GP_ERR_ARG, "gp_write_citi: bad arguments");
+  std::string name(design);
+  for (char &c : name) c = (char)std::toupper((unsigned char)c);
+  std::string o;
+  o += "CITIFILE A.01.01\n";
+  o += "CONSTANT TIME " + stamp(timestamp, "%Y %m %d %H %M %S") + "\n";
+  o += "NAME " + name + "\n";
+  o += "VAR FREQ MAG " + std::to_string(nf) + "\n";
+  for (int i = 0; i < np; ++i)
+    for (int j = 0; j < np; ++j)
+      o += std::string("DATA ") + param + "[" + std::to_string(i + 1) + "," + std::to_string(j + 1) + "] RI\n";
+  o += "VAR_LIST_BEGIN\n";
+  for (int f = 0; f < nf; ++f) o += field(freqs[f], 7, 13, true) + "\n";
+  o += "VAR_LIST_END\n";
+  const size_t nn = (size_t)np * np;
+  for (int i = 0; i < np; ++i)
+    for (int j = 0; j < np; ++j) {
+      o += "BEGIN\n";
+      for (int f = 0; f < nf; ++f) {
+        const size_t q = f * nn + (size_t)i * np + j;
+        o += field(mat[2 * q], 7, 13, false) + "," + field(mat[2 * q + 1], 7, 13, true) + "\n";
+      }
+      o += "END\n";
+    }
+  FILE *fp = std::fopen(path, "w");
+  if (!fp) return fail(GP_ERR_ARG, std::string("gp_write_citi: cannot open ") + path);
+  const bool w = std::fwrite(o.data(), 1, o.size(), fp) == o.size();
+  std::fclose(fp);
+  return w ? GP_OK : fail(GP_ERR_STATE, std::string("gp_write_citi: short write to ") + path);
+}
+
+extern "C" gp_status gp_write_touchstone(const char *path, const char *design, const char *timestamp, char param,
+                                         int32_t np, int32_t nf, const double *freqs, const double *mat,
+                                         const double *zsource) {
+  if (!path || !design || !zsource || np < 1 || nf < 0 || (nf > 0 && (!freqs || !mat)) || !valid_param(param))
+    return fail(GP_ERR_ARG, "gp_write_touchstone: bad arguments");
+  std::string o;
+  const std::string P(1, param);
+  o += "[Version] 2.0\n";
+  o += "# Hz " + P + " RI R " + fmt_g(zsource[0], 6) + "\n";
+  o += "[Number of Ports] " + std::to_string(np) + "\n";
+  if (np == 2) o += "[Two-Port Data Order] 12_21\n";
+  o += "[Number of Frequencies] " + std::to_string(nf) + "\n";
+  o += "[Reference]\n";
+  for (int p = 0; p < np; ++p) o += fmt_g(zsource[p], 6) + " ";
+  o += "\n[Matrix Format] Full\n";
+  o += "! Date/Time " + stamp(timestamp, "%Y-%m-%d %H:%M:%S") + "\n";
+  o += std::string("! Name ") + design + "\n";
+  o += "! Provider: DARPA ERI IDEA/POSH Contributors\n";
+  o += "! Program: Xyce Writer from DARPA ERI\n";
+  o += "! Author: Purdue University\n";
+  o += "[Network Data] \n";
+  o += "!freq         ";
+  for (int i = 0; i < np; ++i) {
+    for (int j = 0; j < np; ++j) {
+      const std::string ij = std::to_string(i + 1) + "," + std::to_string(j + 1);
+      if (np > 2 && i > 0 && j == 0) o += "!Re" + P + ij + std::string(7 - i / 10 - j / 10, ' ');
+      else o += "Re" + P + ij + std::string(8 - i / 10 - j / 10, ' ');
+      o += "Im" + P + ij + std::string(8 - i / 10 - j / 10, ' ');
+      if ((j + 1) % 4 == 0 && j + 1 != np && np > 2) o += "\n!" + std::string(13, ' ');
+    }
+    if (np > 2) o += "\n";
+  }
+  o += "\n";
+  const size_t nn = (size_t)np * np;
+  for (int f = 0; f < nf; ++f) {
+    o += field(freqs[f], 7, 13, true) + " ";
+    for (int i = 0; i < np; ++i) {
+      for (int j = 0; j < np; ++j) {
+        const size_t q = f * nn + (size_t)i * np + j;
+        o += field(mat[2 * q], 7, 13, true) + " ";
+        o += field(mat[2 * q + 1], 7, 13, true) + " ";
+        if ((j + 1) % 4 == 0 && j + 1 != np && np > 2) o += "\n" + std::string(14, ' ');
+      }
+      if (np > 2) o += " ! row " + std::to_string(i + 1) + "\n";
+    }
+    if (np <= 2) o += "\n";
+  }
+  o += "[End]\n";
+  FILE *fp = std::fopen(path, "w");
+  if (!fp) return fail(GP_ERR_ARG, std::string("gp_write_touchstone: cannot open ") + path);
+  const bool w = std::fwrite(o.data(), 1, o.size(), fp) == o.size();
+  std::fclose(fp);
+  return w ? GP_OK : fail(GP_ERR_STATE, std::string("gp_write_touchstone: short write to ") + path);
+}

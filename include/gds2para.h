@@ -19,6 +19,9 @@
  *                                                                   src/fdtd.hpp:4178
  *   gp_z_v0_sweep           fdtdMesh::print_z_V0 (its store)        src/fdtd.hpp:4249
  *   gp_extract_z            fdtdMesh::Construct_Z_V0_Vh             src/fdtd.hpp:4133
+ *   gp_convert_param        Parasitics::convertParam                src/solnoutclass.hpp:1456
+ *   gp_write_citi           SolverDataBase::dumpCITI                src/solnoutclass.hpp:6040
+ *   gp_write_touchstone     SolverDataBase::dumpTouchstone          src/solnoutclass.hpp:6104
  *   gp_freq_of              fdtdMesh::freqNo2freq                   src/fdtd.hpp:4319
  *   gp_edge_map             fdtdMesh::setMapEdge                    src/fdtd.hpp:1508
  */
@@ -330,6 +333,30 @@ gp_status gp_solve_v0(gp_layout *lay, double rtol, double *Z0, double *X, gp_v0_
  * at index k np^2 + i + j np (the reference stores it transposed), with f_k
  * = freqEnd * unit for the last slot and the linear / log grid between. */
 gp_status gp_z_v0_sweep(const gp_mesh *mesh, double *X);
+
+/* ---- network-parameter output (host only) ------------------------------- */
+/* Matrices: nf blocks of np x np complex (interleaved), row-major,
+ * mat[(f*np + i)*np + j] = entry (i, j) at frequency f -- the layout
+ * Parasitics::saveNetworkParam reads sys->x with (src/solnoutclass.hpp:1394),
+ * so a Z array from the solves is passed unchanged. */
+
+/* Parasitics::convertParam (src/solnoutclass.hpp:1456-1600): from / to are
+ * 'Z', 'Y' or 'S'; zsource[np] are the ports' reference impedances (ohm),
+ * needed for conversions to or from S.  in and out may not overlap. */
+gp_status gp_convert_param(int32_t np, int32_t nf, char from, char to, const double *zsource,
+                           const double *in, double *out);
+
+/* SolverDataBase::dumpCITI (src/solnoutclass.hpp:6040-6100).  timestamp
+ * replaces the reference's time stamp (formatted there from an uninitialised
+ * time_t); null writes the current local time in the same format. */
+gp_status gp_write_citi(const char *path, const char *design, const char *timestamp, char param,
+                        int32_t np, int32_t nf, const double *freqs, const double *mat);
+
+/* SolverDataBase::dumpTouchstone (src/solnoutclass.hpp:6104-6215), Touchstone
+ * 2.0 with zsource[np] as the [Reference] line; timestamp as for gp_write_citi. */
+gp_status gp_write_touchstone(const char *path, const char *design, const char *timestamp, char param,
+                              int32_t np, int32_t nf, const double *freqs, const double *mat,
+                              const double *zsource);
 
 /* ---- measurement hooks -------------------------------------------------- */
 /* Waits for all work the layout queued on its stream. */

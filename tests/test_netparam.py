@@ -1,0 +1,101 @@
+"""Network-parameter output (host code of libgds2para, no GPU):
+Parasitics::convertParam (src/solnoutclass.hpp:1456-1600) and the CITIfile /
+Touchstone writers (SolverDataBase::dumpCITI / dumpTouchstone,
+src/solnoutclass.hpp:6040-6215).
+
+Conversions are pinned by mathematics independent of the reference's matrix
+expressions: one-port closed forms (S = (Z - Z0)/(Z + Z0), ...), the
+power-wave definition written another way (S = F^-1 (Z - Z0)(Z + Z0)^-1 F,
+F = diag(sqrt Z0)), unitarity of S for a lossless reciprocal network, and
+round trips.  The writers are pinned by known-answer files worked out by hand
+from the reference's stream operations (sticky setprecision / std::left,
+the 4-column wrap with its 13- and 14-space indents, the '!Re' row starts,
+the blank line after the comment block, ' ! row i' ends)."""
+import numpy as np
+import pytest
+
+from gds2para_amd import _capi as capi
+from gds2para_amd import netparam as N
+
+T1 = '[Version] 2.0\n# Hz Z RI R 50\n[Number of Ports] 1\n[Number of Frequencies] 2\n[Reference]\n50 \n[Matrix Format] Full\n! Date/Time 2026-01-02 03:04:05\n! Name demo\n! Provider: DARPA ERI IDEA/POSH Contributors\n! Program: Xyce Writer from DARPA ERI\n! Author: Purdue University\n[Network Data] \n!freq         ReZ1,1        ImZ1,1        \n1e+09         1.5           -2.25         \n2e+09         0.125         0.001         \n[End]\n'
+C1 = 'CITIFILE A.01.01\nCONSTANT TIME 2026 01 02 03 04 05\nNAME DEMO\nVAR FREQ MAG 2\nDATA Z[1,1] RI\nVAR_LIST_BEGIN\n1e+09        \n2e+09        \nVAR_LIST_END\nBEGIN\n          1.5,-2.25        \n        0.125,0.001        \nEND\n'
+T5 = '[Version] 2.0\n# Hz Z RI R 50\n[Number of Ports] 5\n[Number of Frequencies] 1\n[Reference]\n50 75 50 50 25 \n[Matrix Format] Full\n! Date/Time T\n! Name five\n! Provider: DARPA ERI IDEA/POSH Contributors\n! Program: Xyce Writer from DARPA ERI\n! Author: Purdue University\n[Network Data] \n!freq         ReZ1,1        ImZ1,1        ReZ1,2        ImZ1,2        ReZ1,3        ImZ1,3        ReZ1,4        ImZ1,4        \n!             ReZ1,5        ImZ1,5        \n!ReZ2,1       ImZ2,1        ReZ2,2        ImZ2,2        ReZ2,3        ImZ2,3        ReZ2,4        ImZ2,4        \n!             ReZ2,5        ImZ2,5        \n!ReZ3,1       ImZ3,1        ReZ3,2        ImZ3,2        ReZ3,3        ImZ3,3        ReZ3,4        ImZ3,4        \n!             ReZ3,5        ImZ3,5        \n!ReZ4,1       ImZ4,1        ReZ4,2        ImZ4,2        ReZ4,3        ImZ4,3        ReZ4,4        ImZ4,4        \n!             ReZ4,5        ImZ4,5        \n!ReZ5,1       ImZ5,1        ReZ5,2        ImZ5,2        ReZ5,3        ImZ5,3        ReZ5,4        ImZ5,4        \n!             ReZ5,5        ImZ5,5        \n\n3e+08         1             0.5           2             1             3             1.5           4             2             \n              5             2.5            ! row 1\n6             3             7             3.5           8             4             9             4.5           \n              10            5              ! row 2\n11            5.5           12            6             13            6.5           14            7             \n              15            7.5            ! row 3\n16            8             17            8.5           18            9             19            9.5           \n              20            10             ! row 4\n21            10.5          22            11            23            11.5          24            12            \n              25            12.5           ! row 5\n[End]\n'
+
+
+def _rand(nf, n, seed):
+    g = np.random.default_rng(seed)
+    return g.standard_normal((nf, n, n)) * 30 + 1j * g.standard_normal((nf, n, n)) * 30 + 200 * np.eye(n)
+
+
+def test_one_port_closed_forms():
+    Z = np.array([[[30 - 40j]], [[75 + 5j]], [[1e-3 + 1e3j]]])
+    S = N.convert(Z, "Z", "S", 50.0)
+    assert np.allclose(S[:, 0, 0], (Z[:, 0, 0] - 50) / (Z[:, 0, 0] + 50), rtol=1e-13)
+    Y = 1 / Z
+    assert np.allclose(N.convert(Y, "Y", "S", 50.0)[:, 0, 0], (1 - 50 * Y[:, 0, 0]) / (1 + 50 * Y[:, 0, 0]), rtol=1e-13)
+    assert np.allclose(N.convert(S, "S", "Z", 50.0)[:, 0, 0], 50 * (1 + S[:, 0, 0]) / (1 - S[:, 0, 0]), rtol=1e-12)
+    assert np.allclose(N.convert(S, "S", "Y", 50.0)[:, 0, 0], (1 - S[:, 0, 0]) / (1 + S[:, 0, 0]) / 50, rtol=1e-12)
+
+
+def test_inverse_and_power_wave_definition():
+    Z = _rand(3, 5, 1)
+    z0 = np.array([50., 75., 25., 50., 100.])
+    assert np.allclose(N.convert(Z, "Z", "Y"), np.linalg.inv(Z), rtol=1e-11, atol=1e-15)
+    S = N.convert(Z, "Z", "S", z0)
+    F = np.diag(np.sqrt(z0))
+    Z0 = np.diag(z0)
+    for k in range(3):
+        want = np.linalg.inv(F) @ (Z[k] - Z0) @ np.linalg.inv(Z[k] + Z0) @ F
+        assert np.allclose(S[k], want, rtol=1e-11, atol=1e-13)
+    Y = np.linalg.inv(Z)
+    assert np.allclose(N.convert(Y, "Y", "S", z0), S, rtol=1e-10, atol=1e-12)
+
+
+def test_lossless_network_gives_unitary_s_and_round_trips():
+    g = np.random.default_rng(7)
+    X = g.standard_normal((4, 4)) * 40
+    Z = 1j * (X + X.T)[None]                       # lossless, reciprocal
+    S = N.convert(Z, "Z", "S", 50.0)[0]
+    assert np.allclose(S.conj().T @ S, np.eye(4), atol=1e-12)
+    assert np.allclose(S, S.T, atol=1e-12)
+    Zr = _rand(2, 3, 3)
+    for a, b in (("Z", "S"), ("Y", "S"), ("Z", "Y")):
+        back = N.convert(N.convert(Zr, a, b, [50., 60., 70.]), b, a, [50., 60., 70.])
+        assert np.allclose(back, Zr, rtol=1e-10)
+
+
+def test_convert_errors(lib):
+    import ctypes as C
+    z = np.zeros(8)
+    P = z.ctypes.data_as(C.POINTER(C.c_double))
+    assert lib.gp_convert_param(1, 1, b"Q", b"S", None, P, P) == capi.GP_ERR_ARG
+    assert lib.gp_convert_param(1, 1, b"Z", b"S", None, P, P) == capi.GP_ERR_ARG     # S needs Z0
+    with pytest.raises(capi.GpError):
+        N.convert(np.zeros((1, 2, 2)), "Z", "Y")                                       # singular
+
+
+def test_touchstone_one_port(tmp_path):
+    p = tmp_path / "a.ts"
+    N.write_touchstone(p, "demo", [1e9, 2e9], np.array([[1.5 - 2.25j], [0.125 + 1e-3j]]), timestamp="2026-01-02 03:04:05")
+    assert p.read_text() == T1
+
+
+def test_citi_one_port(tmp_path):
+    p = tmp_path / "a.citi"
+    N.write_citi(p, "demo", [1e9, 2e9], np.array([[1.5 - 2.25j], [0.125 + 1e-3j]]), timestamp="2026 01 02 03 04 05")
+    assert p.read_text() == C1
+
+
+def test_touchstone_five_ports_wraps(tmp_path):
+    p = tmp_path / "b.ts"
+    P = (np.arange(25) + 1) * (1 + 0.5j)
+    N.write_touchstone(p, "five", [3e8], P[None], zsource=[50, 75, 50, 50, 25], timestamp="T")
+    assert p.read_text() == T5
+
+
+def test_two_port_order_line_and_default_time(tmp_path):
+    p = tmp_path / "c.ts"
+    N.write_touchstone(p, "two", [1e9], np.eye(2)[None] * (3 + 1j))
+    lines = p.read_text().splitlines()
+    assert lines[3] == "[Two-Port Data Order] 12_21"
+    assert lines[8].startswith("! Date/Time 20")          # the current local time
