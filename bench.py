@@ -98,7 +98,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="manhattan_200x200x24")
     ap.add_argument("--batch", type=int, default=0)
-    ap.add_argument("--restart", type=int, default=30)
+    ap.add_argument("--restart", type=int, default=16)
     ap.add_argument("--max-iters", type=int, default=1500)
     ap.add_argument("--fp64-sample", type=int, default=3, help="columns of the FP64 single-RHS sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")

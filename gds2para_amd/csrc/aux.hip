@@ -407,7 +407,7 @@ gp_status build_aux(gp_layout *lay, Hierarchy *H, bool want_f32) {
     }
     // SELL copies for the cycle kernels (long rows stay with the long-row pass)
     GP_TRY(sell_from_csr(A, X.GxT, &X.GxTs, s, "X.GxT"));
-    GP_TRY(sell_from_csr(A, X.Bx, &X.Bxs, s, "X.Bx"));
+    GP_TRY(sell_from_csr(A, X.Bx, &X.Bxs, s, "X.Bx", H->ord_e));  // rows: level-0 edges
     for (size_t l = 0; l + 1 < X.L.size(); ++l) {
       AuxLevel &L = X.L[l];
       GP_TRY(sell_from_csr(A, L.N, &L.Ns, s, "L.N"));

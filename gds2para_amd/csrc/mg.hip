@@ -1,5 +1,6 @@
 // mg.hip -- multigrid hierarchy setup (GPU Galerkin products) and the
 // split-form hybrid V-cycle, templated on the working precision.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -162,6 +163,28 @@ gp_status level0_order(gp_layout *lay, Arena &A, int32_t **order, hipStream_t s)
   return GP_OK;
 }
 
+// Level-0 free nodes in the same tile order (keys from the node's plane and
+// (ix, iy)), for the nodal operators N, G^T and Pn of level 0.
+gp_status level0_node_order(const LevelGrid &L, Arena &A, int32_t **order, hipStream_t s) {
+  const Grid &g = L.g;
+  const int64_t nn = L.nn();
+  std::vector<uint64_t> key((size_t)nn);
+  const uint64_t nty = (uint64_t)(g.Ny / kTile + 1);
+  for (int64_t f = 0; f < nn; ++f) {
+    const int64_t v = L.nrev[f], iz = v / g.Nns, r = v - iz * g.Nns, ix = r / (g.Ny + 1), iy = r - ix * (g.Ny + 1);
+    const uint64_t tile = (uint64_t)(ix / kTile) * nty + (uint64_t)(iy / kTile);
+    key[f] = ((tile * (uint64_t)(g.Nz + 1) + (uint64_t)iz) * (uint64_t)(g.Nx + 1) + (uint64_t)ix) * (uint64_t)(g.Ny + 1) +
+             (uint64_t)iy;
+  }
+  std::vector<int32_t> ord((size_t)nn);
+  for (int64_t f = 0; f < nn; ++f) ord[f] = (int32_t)f;
+  std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+  GP_TRY(A.alloc(order, nn));
+  GP_HIP(hipMemcpyAsync(*order, ord.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice, s));
+  GP_HIP(hipStreamSynchronize(s));
+  return GP_OK;
+}
+
 gp_status build_hierarchy(gp_layout *lay, Hierarchy *H, bool want_f32, int max_levels) {
   auto t0 = std::chrono::steady_clock::now();
   if (!lay->s_ready) GP_TRY(gp_assemble_stiffness(lay));
@@ -259,19 +282,20 @@ gp_status build_hierarchy(gp_layout *lay, Hierarchy *H, bool want_f32, int max_l
     A.release(L0.K.v[2], L0.K.nnz);
     L0.K.nv = 1;
   }
-  int32_t *order0 = nullptr;
-  Arena otmp;
-  struct OrderGuard { Arena *a; ~OrderGuard() { a->free_all(); } } og{&otmp};  // the SELLs keep their own copy
-  if (H->L.size() > 1) GP_TRY(level0_order(lay, otmp, &order0, s));
+  if (H->L.size() > 1) {
+    GP_TRY(level0_order(lay, A, &H->ord_e, s));
+    GP_TRY(level0_node_order(grids[0], A, &H->ord_n, s));
+  }
   for (size_t l = 0; l + 1 < H->L.size(); ++l) {  // the coarsest level is solved densely
     MgLevel &L = H->L[l];
-    GP_TRY(sell_from_csr(A, L.K, &L.Ks, s, "L.K", l == 0 ? order0 : nullptr));
-    GP_TRY(sell_from_csr(A, L.B, &L.Bs, s, "L.B", l == 0 ? order0 : nullptr));
-    GP_TRY(sell_from_csr(A, L.N, &L.Ns, s, "L.N"));
-    GP_TRY(sell_from_csr(A, L.GT, &L.GTs, s, "L.GT"));
-    GP_TRY(sell_from_csr(A, L.P, &L.Ps, s, "L.P"));
+    const int32_t *oe = l == 0 ? H->ord_e : nullptr, *on = l == 0 ? H->ord_n : nullptr;
+    GP_TRY(sell_from_csr(A, L.K, &L.Ks, s, "L.K", oe));
+    GP_TRY(sell_from_csr(A, L.B, &L.Bs, s, "L.B", oe));
+    GP_TRY(sell_from_csr(A, L.N, &L.Ns, s, "L.N", on));
+    GP_TRY(sell_from_csr(A, L.GT, &L.GTs, s, "L.GT", on));
+    GP_TRY(sell_from_csr(A, L.P, &L.Ps, s, "L.P", oe));
     GP_TRY(sell_from_csr(A, L.PT, &L.PTs, s, "L.PT"));
-    GP_TRY(sell_from_csr(A, L.Pn, &L.Pns, s, "L.Pn"));
+    GP_TRY(sell_from_csr(A, L.Pn, &L.Pns, s, "L.Pn", on));
   }
   if (want_f32) {
     GP_TRY(to_f32(A, H->W, n, &H->Wf, s));
@@ -336,7 +360,7 @@ gp_status build_hierarchy(gp_layout *lay, Hierarchy *H, bool want_f32, int max_l
     Sv.nr = lay->S.n; Sv.nc = lay->S.n; Sv.nnz = lay->S.nnz; Sv.rp = lay->S.rp; Sv.ci = lay->S.ci; Sv.nv = 1; Sv.v[0] = lay->S.v;
     GP_TRY(spgemm_ex(A, Sv, H->L[0].G, &H->SG, s, 1));
     GP_TRY(spgemm_ex(A, Sv, H->aux.Gx, &H->SGx, s, 1));
-    GP_TRY(sell_from_csr(A, Sv, &H->Ss, s, "S", order0));  // S rows are level 0's rows
+    GP_TRY(sell_from_csr(A, Sv, &H->Ss, s, "S", H->ord_e));  // S rows are level 0's rows
   }
   H->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return GP_OK;
@@ -735,9 +759,10 @@ __global__ void __launch_bounds__(256) k_node_sell(int64_t nr, int k, SellV<T> N
                                                    const int32_t *__restrict__ lidx, const C2<T> *__restrict__ lp) {
   constexpr int KS = SPL;  // LPR adjacent lanes share a row, SPL slots each
   const int64_t t_ = xcd_block_id() * blockDim.x + threadIdx.x;
-  const int64_t i = t_ / LPR;
-  if (i >= nr) return;
-  const int j0 = ((int)blockIdx.y * LPR + (int)(t_ - i * LPR)) * SPL;
+  const int64_t pos = t_ / LPR;
+  if (pos >= nr) return;
+  const int j0 = ((int)blockIdx.y * LPR + (int)(t_ - pos * LPR)) * SPL;
+  const int64_t i = N.rows ? (int64_t)N.rows[pos] : pos;  // slot position -> row
   T c1[KS], c2[KS];
 #pragma unroll
   for (int q = 0; q < KS; ++q) { c1[q] = c1v[j0 + q]; c2[q] = c2v[j0 + q]; }
@@ -755,8 +780,8 @@ __global__ void __launch_bounds__(256) k_node_sell(int64_t nr, int k, SellV<T> N
         acc[q].y -= c1[q] * s0[q].y + c2[q] * s1[q].x;
       }
     } else {
-      const int64_t sl = i >> 6;
-      const int lane = (int)(i & 63);
+      const int64_t sl = pos >> 6;
+      const int lane = (int)(pos & 63);
       const int64_t b0 = N.base[sl];
       const int wd = N.w[sl];
       for (int t = 0; t < wd; ++t) {
@@ -794,9 +819,10 @@ __global__ void __launch_bounds__(256) k_spmv_real_sell(int64_t nr, int k, SellV
                                                         const int32_t *__restrict__ lidx, const C2<T> *__restrict__ lp) {
   constexpr int KS = SPL;  // LPR adjacent lanes share a row, SPL slots each
   const int64_t t_ = xcd_block_id() * blockDim.x + threadIdx.x;
-  const int64_t i = t_ / LPR;
-  if (i >= nr) return;
-  const int j0 = ((int)blockIdx.y * LPR + (int)(t_ - i * LPR)) * SPL;
+  const int64_t pos = t_ / LPR;
+  if (pos >= nr) return;
+  const int j0 = ((int)blockIdx.y * LPR + (int)(t_ - pos * LPR)) * SPL;
+  const int64_t i = M.rows ? (int64_t)M.rows[pos] : pos;  // slot position -> row
   C2<T> acc[KS];
   const int32_t lq = lidx ? lidx[i] : -1;
   if (lq >= 0) {
@@ -804,8 +830,8 @@ __global__ void __launch_bounds__(256) k_spmv_real_sell(int64_t nr, int k, SellV
   } else {
 #pragma unroll
     for (int q = 0; q < KS; ++q) acc[q] = {0, 0};
-    const int64_t sl = i >> 6;
-    const int lane = (int)(i & 63);
+    const int64_t sl = pos >> 6;
+    const int lane = (int)(pos & 63);
     const int64_t b0 = M.base[sl];
     const int wd = M.w[sl];
     for (int t = 0; t < wd; ++t) {
@@ -835,9 +861,10 @@ __global__ void __launch_bounds__(256) k_spmv2_sell(int64_t nr, int k, SellV<T> 
                                                     const int32_t *__restrict__ lidx, const C2<T> *__restrict__ lp) {
   constexpr int KS = SPL;  // LPR adjacent lanes share a row, SPL slots each
   const int64_t t_ = xcd_block_id() * blockDim.x + threadIdx.x;
-  const int64_t i = t_ / LPR;
-  if (i >= nr) return;
-  const int j0 = ((int)blockIdx.y * LPR + (int)(t_ - i * LPR)) * SPL;
+  const int64_t pos = t_ / LPR;
+  if (pos >= nr) return;
+  const int j0 = ((int)blockIdx.y * LPR + (int)(t_ - pos * LPR)) * SPL;
+  const int64_t i = M.rows ? (int64_t)M.rows[pos] : pos;  // slot position -> row
   T c1[KS], c2[KS];
 #pragma unroll
   for (int q = 0; q < KS; ++q) { c1[q] = c1v[j0 + q]; c2[q] = c2v[j0 + q]; }
@@ -852,8 +879,8 @@ __global__ void __launch_bounds__(256) k_spmv2_sell(int64_t nr, int k, SellV<T> 
 #pragma unroll
     for (int q = 0; q < KS; ++q) acc[q] = {c1[q] * s0[q].x - c2[q] * s1[q].y, c1[q] * s0[q].y + c2[q] * s1[q].x};
   } else {
-    const int64_t sl = i >> 6;
-    const int lane = (int)(i & 63);
+    const int64_t sl = pos >> 6;
+    const int lane = (int)(pos & 63);
     const int64_t b0 = M.base[sl];
     const int wd = M.w[sl];
     for (int t = 0; t < wd; ++t) {
@@ -1157,12 +1184,14 @@ template <>
 SellV<double> sv<double>(const Sell &c, const SellF &) {
   SellV<double> v; v.nr = c.nr; v.nslice = c.nslice; v.base = c.base; v.w = c.w; v.ci = c.ci; v.nv = c.nv;
   for (int q = 0; q < c.nv; ++q) v.v[q] = c.v[q];
+  v.rows = c.rows;
   return v;
 }
 template <>
 SellV<float> sv<float>(const Sell &, const SellF &f) {
   SellV<float> v; v.nr = f.nr; v.nslice = f.nslice; v.base = f.base; v.w = f.w; v.ci = f.ci; v.nv = f.nv;
   for (int q = 0; q < f.nv; ++q) v.v[q] = f.v[q];
+  v.rows = f.rows;
   return v;
 }
 

@@ -142,6 +142,9 @@ struct Hierarchy {
   double *epsr = nullptr;     // stack_epsn of each kept edge's layer
   double *sig = nullptr;      // stack_sign if the edge is a conductor edge, else 0
   double *sdiag = nullptr;    // S_ii: the system diagonal is the FP64 sum S_ii + D_ii(w)
+  // level-0 slot orders of the SELL copies whose rows are level-0 edges /
+  // level-0 free nodes: (16 x 16-cell xy tile, plane, ix, iy[, direction])
+  int32_t *ord_e = nullptr, *ord_n = nullptr;
   // coarsest level, dense on the host (row-major), per block
   int64_t neL = 0, nnL = 0;
   std::vector<double> KL[3], BL[2], NL[2], GTL;

@@ -899,7 +899,7 @@ using namespace gp;
 extern "C" void gp_solve_opts_default(gp_solve_opts *o) {
   o->rtol = 1e-8;
   o->max_iters = 500;
-  o->restart = 30;
+  o->restart = 16;
   o->precision = GP_PREC_MIXED;
   o->batch = 0;
   o->verbose = 0;

@@ -118,7 +118,7 @@ class Layout:
         stats = {k: getattr(st, k) for k, _ in capi.GpV0Stats._fields_}
         return z[0::2] + 1j * z[1::2], x[0::2] + 1j * x[1::2], stats
 
-    def solve(self, freqs, ports=None, precision=capi.GP_PREC_MIXED, rtol=1e-8, batch=0, restart=30,
+    def solve(self, freqs, ports=None, precision=capi.GP_PREC_MIXED, rtol=1e-8, batch=0, restart=16,
               max_iters=2000, want_x=False, verbose=0, smooth_edge=0, smooth_aux=0, sell_min_rows=0):
         """Returns (Z[nf, np*np] complex, stats, X or None)."""
         ports = list(range(self.mesh.num_ports)) if ports is None else list(ports)

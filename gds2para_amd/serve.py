@@ -214,7 +214,7 @@ class Server:
 class GpuBackend:
     """Backend over libgds2para: one Layout per key, solved with gp_solve_stream."""
 
-    def __init__(self, specs, slots=16, precision=capi.GP_PREC_MIXED, rtol=1e-8, restart=30, max_iters=2000):
+    def __init__(self, specs, slots=16, precision=capi.GP_PREC_MIXED, rtol=1e-8, restart=16, max_iters=2000):
         from . import mesh as M
         self.M = M
         self.specs = specs                 # key -> GenSpec

@@ -217,7 +217,7 @@ typedef struct gp_solve_stats {
   double setup_ms, solve_ms;      /* device wall time                             */
 } gp_solve_stats;
 
-/* Fills opts with the defaults (rtol 1e-8, restart 30, MIXED, batch 0). */
+/* Fills opts with the defaults (rtol 1e-8, restart 16, MIXED, batch 0). */
 void gp_solve_opts_default(gp_solve_opts *opts);
 
 /* Builds (or re-uses) the multigrid hierarchy for the layout. */
