@@ -655,6 +655,43 @@ __global__ void k_edge(int64_t nr, int k, MatV<T> K, MatV<T> B, const T *__restr
 // per gathered neighbour row), so one wave-wide gather touches the lines of
 // 64 / LPR rows, not 64: with one thread per row and all slots in registers
 // the L1 tag lookups of the scattered 16-byte pieces, not HBM, set the pace.
+constexpr int kGather = 4;  // row gathers in flight per lane in the SELL kernels
+
+// Walks the SELL row at slice base b0 / width wd / lane, kGather entries at a
+// time (indices and values first, then the kGather gathered rows of x), and
+// calls f(a0, a1, a2, xrow) for every entry in order.
+template <class T, int KS, int NV, class F>
+__device__ __forceinline__ void sell_row(const SellV<T> &M, int64_t b0, int wd, int lane, const C2<T> *__restrict__ x,
+                                         int k, int j0, F &&f) {
+  int t0 = 0;
+  for (; t0 + kGather <= wd; t0 += kGather) {
+    int32_t c[kGather];
+    T a[NV][kGather];
+#pragma unroll
+    for (int h = 0; h < kGather; ++h) {
+      const int64_t e = b0 + (int64_t)(t0 + h) * 64 + lane;
+      c[h] = M.ci[e];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) a[v][h] = M.v[v][e];
+    }
+    C2<T> xv[kGather][KS];
+#pragma unroll
+    for (int h = 0; h < kGather; ++h) load_slots<T, KS>(x + (int64_t)c[h] * k + j0, xv[h]);
+#pragma unroll
+    for (int h = 0; h < kGather; ++h) f(a[0][h], NV > 1 ? a[NV > 1 ? 1 : 0][h] : T(0), NV > 2 ? a[NV > 2 ? 2 : 0][h] : T(0), xv[h]);
+  }
+  for (int t = t0; t < wd; ++t) {
+    const int64_t e = b0 + (int64_t)t * 64 + lane;
+    const int32_t c = M.ci[e];
+    T a[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) a[v] = M.v[v][e];
+    C2<T> xv[KS];
+    load_slots<T, KS>(x + (int64_t)c * k + j0, xv);
+    f(a[0], NV > 1 ? a[NV > 1 ? 1 : 0] : T(0), NV > 2 ? a[NV > 2 ? 2 : 0] : T(0), xv);
+  }
+}
+
 template <class T, int SPL, int LPR>
 __global__ void __launch_bounds__(256) k_edge_sell(int64_t nr, int k, SellV<T> K, SellV<T> B, const T *__restrict__ c1v,
                                                    const T *__restrict__ c2v, const T *__restrict__ kd0,
@@ -680,8 +717,30 @@ __global__ void __launch_bounds__(256) k_edge_sell(int64_t nr, int k, SellV<T> K
   if (mode != EdgeMode::JACOBI0) {
     const int64_t b0 = K.base[sl];
     const int wd = K.w[sl];
+    // kGather entries at a time: their column indices and values are read
+    // before any of their rows of u, so kGather row gathers are in flight
+    // per lane (one dependent index -> row chain per entry left the kernel
+    // latency-bound); the accumulation order is unchanged
+    int t0 = 0;
     if (K.nv == 1) {
-      for (int t = 0; t < wd; ++t) {
+      for (; t0 + kGather <= wd; t0 += kGather) {
+        int32_t c[kGather];
+        T a[kGather];
+#pragma unroll
+        for (int h = 0; h < kGather; ++h) {
+          const int64_t e = b0 + (int64_t)(t0 + h) * 64 + lane;
+          c[h] = K.ci[e];
+          a[h] = K.v[0][e];
+        }
+        C2<T> x[kGather][KS];
+#pragma unroll
+        for (int h = 0; h < kGather; ++h) load_slots<T, KS>(u + (int64_t)c[h] * k + j0, x[h]);
+#pragma unroll
+        for (int h = 0; h < kGather; ++h)
+#pragma unroll
+          for (int q = 0; q < KS; ++q) { acc[q].x -= a[h] * x[h][q].x; acc[q].y -= a[h] * x[h][q].y; }
+      }
+      for (int t = t0; t < wd; ++t) {
         const int64_t e = b0 + (int64_t)t * 64 + lane;
         const int32_t c = K.ci[e];
         const T a = K.v[0][e];
@@ -691,7 +750,28 @@ __global__ void __launch_bounds__(256) k_edge_sell(int64_t nr, int k, SellV<T> K
         for (int q = 0; q < KS; ++q) { acc[q].x -= a * x[q].x; acc[q].y -= a * x[q].y; }
       }
     } else {
-      for (int t = 0; t < wd; ++t) {
+      for (; t0 + kGather <= wd; t0 += kGather) {
+        int32_t c[kGather];
+        T a0[kGather], a1[kGather], a2[kGather];
+#pragma unroll
+        for (int h = 0; h < kGather; ++h) {
+          const int64_t e = b0 + (int64_t)(t0 + h) * 64 + lane;
+          c[h] = K.ci[e];
+          a0[h] = K.v[0][e]; a1[h] = K.v[1][e]; a2[h] = K.v[2][e];
+        }
+        C2<T> x[kGather][KS];
+#pragma unroll
+        for (int h = 0; h < kGather; ++h) load_slots<T, KS>(u + (int64_t)c[h] * k + j0, x[h]);
+#pragma unroll
+        for (int h = 0; h < kGather; ++h)
+#pragma unroll
+          for (int q = 0; q < KS; ++q) {
+            const T ar = a0[h] + c1[q] * a1[h], ai = c2[q] * a2[h];
+            acc[q].x -= ar * x[h][q].x - ai * x[h][q].y;
+            acc[q].y -= ar * x[h][q].y + ai * x[h][q].x;
+          }
+      }
+      for (int t = t0; t < wd; ++t) {
         const int64_t e = b0 + (int64_t)t * 64 + lane;
         const int32_t c = K.ci[e];
         const T a0 = K.v[0][e], a1 = K.v[1][e], a2 = K.v[2][e];
@@ -784,19 +864,14 @@ __global__ void __launch_bounds__(256) k_node_sell(int64_t nr, int k, SellV<T> N
       const int lane = (int)(pos & 63);
       const int64_t b0 = N.base[sl];
       const int wd = N.w[sl];
-      for (int t = 0; t < wd; ++t) {
-        const int64_t e = b0 + (int64_t)t * 64 + lane;
-        const int32_t c = N.ci[e];
-        const T a0 = N.v[0][e], a1 = N.v[1][e];
-        C2<T> x[KS];
-        load_slots<T, KS>(d + (int64_t)c * k + j0, x);
+      sell_row<T, KS, 2>(N, b0, wd, lane, d, k, j0, [&](T a0, T a1, T, const C2<T>(&x)[KS]) {
 #pragma unroll
         for (int q = 0; q < KS; ++q) {
           const T ar = c1[q] * a0, ai = c2[q] * a1;
           acc[q].x -= ar * x[q].x - ai * x[q].y;
           acc[q].y -= ar * x[q].y + ai * x[q].x;
         }
-      }
+      });
     }
   }
   const T e0 = nd0[i], e1 = nd1[i], m0 = nl0[i], m1 = nl1[i];
@@ -834,15 +909,10 @@ __global__ void __launch_bounds__(256) k_spmv_real_sell(int64_t nr, int k, SellV
     const int lane = (int)(pos & 63);
     const int64_t b0 = M.base[sl];
     const int wd = M.w[sl];
-    for (int t = 0; t < wd; ++t) {
-      const int64_t e = b0 + (int64_t)t * 64 + lane;
-      const int32_t c = M.ci[e];
-      const T a = M.v[0][e];
-      C2<T> xv[KS];
-      load_slots<T, KS>(x + (int64_t)c * k + j0, xv);
+    sell_row<T, KS, 1>(M, b0, wd, lane, x, k, j0, [&](T a, T, T, const C2<T>(&xv)[KS]) {
 #pragma unroll
       for (int q = 0; q < KS; ++q) { acc[q].x += a * xv[q].x; acc[q].y += a * xv[q].y; }
-    }
+    });
   }
   if (accumulate) {
     C2<T> o[KS];
@@ -883,19 +953,14 @@ __global__ void __launch_bounds__(256) k_spmv2_sell(int64_t nr, int k, SellV<T> 
     const int lane = (int)(pos & 63);
     const int64_t b0 = M.base[sl];
     const int wd = M.w[sl];
-    for (int t = 0; t < wd; ++t) {
-      const int64_t e = b0 + (int64_t)t * 64 + lane;
-      const int32_t c = M.ci[e];
-      const T a0 = M.v[0][e], a1 = M.v[1][e];
-      C2<T> xv[KS];
-      load_slots<T, KS>(x + (int64_t)c * k + j0, xv);
+    sell_row<T, KS, 2>(M, b0, wd, lane, x, k, j0, [&](T a0, T a1, T, const C2<T>(&xv)[KS]) {
 #pragma unroll
       for (int q = 0; q < KS; ++q) {
         const T ar = c1[q] * a0, ai = c2[q] * a1;
         acc[q].x += ar * xv[q].x - ai * xv[q].y;
         acc[q].y += ar * xv[q].y + ai * xv[q].x;
       }
-    }
+    });
   }
   C2<T> o[KS];
   if (mode == 0) load_slots<T, KS>(r + i * k + j0, o);

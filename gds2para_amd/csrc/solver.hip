@@ -143,7 +143,28 @@ __global__ void __launch_bounds__(256) k_outer_sell(OuterOp op, SellV<double> SS
   {
     const int64_t b0 = SS.base[sl];
     const int wd = SS.w[sl];
-    for (int t = 0; t < wd; ++t) {
+    // four entries at a time: indices and values first, then four row
+    // gathers in flight per lane; the accumulation order is unchanged
+    constexpr int G = 4;
+    int t0 = 0;
+    for (; t0 + G <= wd; t0 += G) {
+      int32_t c[G];
+      double m[G];
+#pragma unroll
+      for (int h = 0; h < G; ++h) {
+        const int64_t e = b0 + (int64_t)(t0 + h) * 64 + lane;
+        c[h] = SS.ci[e];
+        m[h] = SS.v[0][e];
+      }
+      C2<TZ> x[G][KS];
+#pragma unroll
+      for (int h = 0; h < G; ++h) load_slots<TZ, KS>(u + (int64_t)c[h] * k + j0, x[h]);
+#pragma unroll
+      for (int h = 0; h < G; ++h)
+#pragma unroll
+        for (int q = 0; q < KS; ++q) { acc[q].x += m[h] * (double)x[h][q].x; acc[q].y += m[h] * (double)x[h][q].y; }
+    }
+    for (int t = t0; t < wd; ++t) {
       const int64_t e = b0 + (int64_t)t * 64 + lane;
       const double m = SS.v[0][e];
       C2<TZ> x[KS];
