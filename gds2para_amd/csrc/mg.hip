@@ -647,49 +647,44 @@ __global__ void k_edge(int64_t nr, int k, MatV<T> K, MatV<T> B, const T *__restr
   out[i * k + j] = {u0.x + damp * q.x, u0.y + damp * q.y};
 }
 
-// Edge kernel on SELL matrices, one thread per (row, group of KS slots); the
-// same contract as k_edge.  Block = 256 rows (4 slices) of one slot group
-// (blockIdx.y); blocks are numbered per XCD (xcd_block_id).
 // Level-0 edge smoother / residual on the sliced-ELL K and B.  LPR adjacent
 // lanes share one row and each holds SPL consecutive slots (one 16-byte load
 // per gathered neighbour row), so one wave-wide gather touches the lines of
 // 64 / LPR rows, not 64: with one thread per row and all slots in registers
 // the L1 tag lookups of the scattered 16-byte pieces, not HBM, set the pace.
-constexpr int kGather = 4;  // row gathers in flight per lane in the SELL kernels
+// Walks the SELL row at slice base b0 / width wd / lane with four row
+// gathers in flight per lane: batches of 4 entries, then single entries
+// (batches of 8 ran slower: 687 vs 536 us for the level-0 edge smoother,
+// register pressure); each batch reads its indices and values first, then its
+// gathered rows of x, and calls f(a0, a1, a2, xrow) for every entry in
+// order (one dependent index -> row chain per entry left these kernels
+// latency-bound).
+template <class T, int KS, int NV, int NB, class F>
+__device__ __forceinline__ void sell_batch(const SellV<T> &M, int64_t b0, int t0, int lane, const C2<T> *__restrict__ x,
+                                           int k, int j0, F &f) {
+  int32_t c[NB];
+  T a[NV][NB];
+#pragma unroll
+  for (int h = 0; h < NB; ++h) {
+    const int64_t e = b0 + (int64_t)(t0 + h) * 64 + lane;
+    c[h] = M.ci[e];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) a[v][h] = M.v[v][e];
+  }
+  C2<T> xv[NB][KS];
+#pragma unroll
+  for (int h = 0; h < NB; ++h) load_slots<T, KS>(x + (int64_t)c[h] * k + j0, xv[h]);
+#pragma unroll
+  for (int h = 0; h < NB; ++h)
+    f(a[0][h], NV > 1 ? a[NV > 1 ? 1 : 0][h] : T(0), NV > 2 ? a[NV > 2 ? 2 : 0][h] : T(0), xv[h]);
+}
 
-// Walks the SELL row at slice base b0 / width wd / lane, kGather entries at a
-// time (indices and values first, then the kGather gathered rows of x), and
-// calls f(a0, a1, a2, xrow) for every entry in order.
 template <class T, int KS, int NV, class F>
 __device__ __forceinline__ void sell_row(const SellV<T> &M, int64_t b0, int wd, int lane, const C2<T> *__restrict__ x,
                                          int k, int j0, F &&f) {
   int t0 = 0;
-  for (; t0 + kGather <= wd; t0 += kGather) {
-    int32_t c[kGather];
-    T a[NV][kGather];
-#pragma unroll
-    for (int h = 0; h < kGather; ++h) {
-      const int64_t e = b0 + (int64_t)(t0 + h) * 64 + lane;
-      c[h] = M.ci[e];
-#pragma unroll
-      for (int v = 0; v < NV; ++v) a[v][h] = M.v[v][e];
-    }
-    C2<T> xv[kGather][KS];
-#pragma unroll
-    for (int h = 0; h < kGather; ++h) load_slots<T, KS>(x + (int64_t)c[h] * k + j0, xv[h]);
-#pragma unroll
-    for (int h = 0; h < kGather; ++h) f(a[0][h], NV > 1 ? a[NV > 1 ? 1 : 0][h] : T(0), NV > 2 ? a[NV > 2 ? 2 : 0][h] : T(0), xv[h]);
-  }
-  for (int t = t0; t < wd; ++t) {
-    const int64_t e = b0 + (int64_t)t * 64 + lane;
-    const int32_t c = M.ci[e];
-    T a[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) a[v] = M.v[v][e];
-    C2<T> xv[KS];
-    load_slots<T, KS>(x + (int64_t)c * k + j0, xv);
-    f(a[0], NV > 1 ? a[NV > 1 ? 1 : 0] : T(0), NV > 2 ? a[NV > 2 ? 2 : 0] : T(0), xv);
-  }
+  for (; t0 + 4 <= wd; t0 += 4) sell_batch<T, KS, NV, 4>(M, b0, t0, lane, x, k, j0, f);
+  for (; t0 < wd; ++t0) sell_batch<T, KS, NV, 1>(M, b0, t0, lane, x, k, j0, f);
 }
 
 template <class T, int SPL, int LPR>
@@ -717,73 +712,20 @@ __global__ void __launch_bounds__(256) k_edge_sell(int64_t nr, int k, SellV<T> K
   if (mode != EdgeMode::JACOBI0) {
     const int64_t b0 = K.base[sl];
     const int wd = K.w[sl];
-    // kGather entries at a time: their column indices and values are read
-    // before any of their rows of u, so kGather row gathers are in flight
-    // per lane (one dependent index -> row chain per entry left the kernel
-    // latency-bound); the accumulation order is unchanged
-    int t0 = 0;
     if (K.nv == 1) {
-      for (; t0 + kGather <= wd; t0 += kGather) {
-        int32_t c[kGather];
-        T a[kGather];
-#pragma unroll
-        for (int h = 0; h < kGather; ++h) {
-          const int64_t e = b0 + (int64_t)(t0 + h) * 64 + lane;
-          c[h] = K.ci[e];
-          a[h] = K.v[0][e];
-        }
-        C2<T> x[kGather][KS];
-#pragma unroll
-        for (int h = 0; h < kGather; ++h) load_slots<T, KS>(u + (int64_t)c[h] * k + j0, x[h]);
-#pragma unroll
-        for (int h = 0; h < kGather; ++h)
-#pragma unroll
-          for (int q = 0; q < KS; ++q) { acc[q].x -= a[h] * x[h][q].x; acc[q].y -= a[h] * x[h][q].y; }
-      }
-      for (int t = t0; t < wd; ++t) {
-        const int64_t e = b0 + (int64_t)t * 64 + lane;
-        const int32_t c = K.ci[e];
-        const T a = K.v[0][e];
-        C2<T> x[KS];
-        load_slots<T, KS>(u + (int64_t)c * k + j0, x);
+      sell_row<T, KS, 1>(K, b0, wd, lane, u, k, j0, [&](T a, T, T, const C2<T>(&x)[KS]) {
 #pragma unroll
         for (int q = 0; q < KS; ++q) { acc[q].x -= a * x[q].x; acc[q].y -= a * x[q].y; }
-      }
+      });
     } else {
-      for (; t0 + kGather <= wd; t0 += kGather) {
-        int32_t c[kGather];
-        T a0[kGather], a1[kGather], a2[kGather];
-#pragma unroll
-        for (int h = 0; h < kGather; ++h) {
-          const int64_t e = b0 + (int64_t)(t0 + h) * 64 + lane;
-          c[h] = K.ci[e];
-          a0[h] = K.v[0][e]; a1[h] = K.v[1][e]; a2[h] = K.v[2][e];
-        }
-        C2<T> x[kGather][KS];
-#pragma unroll
-        for (int h = 0; h < kGather; ++h) load_slots<T, KS>(u + (int64_t)c[h] * k + j0, x[h]);
-#pragma unroll
-        for (int h = 0; h < kGather; ++h)
-#pragma unroll
-          for (int q = 0; q < KS; ++q) {
-            const T ar = a0[h] + c1[q] * a1[h], ai = c2[q] * a2[h];
-            acc[q].x -= ar * x[h][q].x - ai * x[h][q].y;
-            acc[q].y -= ar * x[h][q].y + ai * x[h][q].x;
-          }
-      }
-      for (int t = t0; t < wd; ++t) {
-        const int64_t e = b0 + (int64_t)t * 64 + lane;
-        const int32_t c = K.ci[e];
-        const T a0 = K.v[0][e], a1 = K.v[1][e], a2 = K.v[2][e];
-        C2<T> x[KS];
-        load_slots<T, KS>(u + (int64_t)c * k + j0, x);
+      sell_row<T, KS, 3>(K, b0, wd, lane, u, k, j0, [&](T a0, T a1, T a2, const C2<T>(&x)[KS]) {
 #pragma unroll
         for (int q = 0; q < KS; ++q) {
           const T ar = a0 + c1[q] * a1, ai = c2[q] * a2;
           acc[q].x -= ar * x[q].x - ai * x[q].y;
           acc[q].y -= ar * x[q].y + ai * x[q].x;
         }
-      }
+      });
     }
     if (K.dg[0]) {  // diagonal K_eps, K_sig
       const T de = K.dg[0][i], ds = K.dg[1][i];
@@ -797,21 +739,14 @@ __global__ void __launch_bounds__(256) k_edge_sell(int64_t nr, int k, SellV<T> K
       }
     }
     if (use_phi) {
-      const int64_t bb = B.base[sl];
-      const int wb = B.w[sl];
-      for (int t = 0; t < wb; ++t) {
-        const int64_t e = bb + (int64_t)t * 64 + lane;
-        const int32_t c = B.ci[e];
-        const T a0 = B.v[0][e], a1 = B.v[1][e];
-        C2<T> x[KS];
-        load_slots<T, KS>(phi + (int64_t)c * k + j0, x);
+      sell_row<T, KS, 2>(B, B.base[sl], B.w[sl], lane, phi, k, j0, [&](T a0, T a1, T, const C2<T>(&x)[KS]) {
 #pragma unroll
         for (int q = 0; q < KS; ++q) {
           const T br = c1[q] * a0, bi = c2[q] * a1;
           acc[q].x -= br * x[q].x - bi * x[q].y;
           acc[q].y -= br * x[q].y + bi * x[q].x;
         }
-      }
+      });
     }
   }
   if (mode == EdgeMode::RES) { store_slots<T, KS>(out + i * k + j0, acc); return; }

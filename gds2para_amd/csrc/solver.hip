@@ -121,6 +121,26 @@ __global__ void k_outer(OuterOp op, int k, const double *__restrict__ omega, con
   out[i * k + j] = acc;
 }
 
+template <class TZ, int KS, int NB>
+__device__ __forceinline__ void outer_batch(const SellV<double> &SS, int64_t b0, int t0, int lane,
+                                            const C2<TZ> *__restrict__ u, int k, int j0, Z2 (&acc)[KS]) {
+  int32_t c[NB];
+  double m[NB];
+#pragma unroll
+  for (int h = 0; h < NB; ++h) {
+    const int64_t e = b0 + (int64_t)(t0 + h) * 64 + lane;
+    c[h] = SS.ci[e];
+    m[h] = SS.v[0][e];
+  }
+  C2<TZ> x[NB][KS];
+#pragma unroll
+  for (int h = 0; h < NB; ++h) load_slots<TZ, KS>(u + (int64_t)c[h] * k + j0, x[h]);
+#pragma unroll
+  for (int h = 0; h < NB; ++h)
+#pragma unroll
+    for (int q = 0; q < KS; ++q) { acc[q].x += m[h] * (double)x[h][q].x; acc[q].y += m[h] * (double)x[h][q].y; }
+}
+
 // k_outer on the SELL copy of S: LPR adjacent lanes share a row, each with
 // SPL slots (one 16-byte load of FP32 u per gathered row), S read once per
 // lane group; G, Gx, S G, S Gx rows are short CSR walks.
@@ -143,35 +163,11 @@ __global__ void __launch_bounds__(256) k_outer_sell(OuterOp op, SellV<double> SS
   {
     const int64_t b0 = SS.base[sl];
     const int wd = SS.w[sl];
-    // four entries at a time: indices and values first, then four row
-    // gathers in flight per lane; the accumulation order is unchanged
-    constexpr int G = 4;
+    // batches of 4, then single entries: indices and values first, then the
+    // batch's row gathers in flight together; order unchanged
     int t0 = 0;
-    for (; t0 + G <= wd; t0 += G) {
-      int32_t c[G];
-      double m[G];
-#pragma unroll
-      for (int h = 0; h < G; ++h) {
-        const int64_t e = b0 + (int64_t)(t0 + h) * 64 + lane;
-        c[h] = SS.ci[e];
-        m[h] = SS.v[0][e];
-      }
-      C2<TZ> x[G][KS];
-#pragma unroll
-      for (int h = 0; h < G; ++h) load_slots<TZ, KS>(u + (int64_t)c[h] * k + j0, x[h]);
-#pragma unroll
-      for (int h = 0; h < G; ++h)
-#pragma unroll
-        for (int q = 0; q < KS; ++q) { acc[q].x += m[h] * (double)x[h][q].x; acc[q].y += m[h] * (double)x[h][q].y; }
-    }
-    for (int t = t0; t < wd; ++t) {
-      const int64_t e = b0 + (int64_t)t * 64 + lane;
-      const double m = SS.v[0][e];
-      C2<TZ> x[KS];
-      load_slots<TZ, KS>(u + (int64_t)SS.ci[e] * k + j0, x);
-#pragma unroll
-      for (int q = 0; q < KS; ++q) { acc[q].x += m * (double)x[q].x; acc[q].y += m * (double)x[q].y; }
-    }
+    for (; t0 + 4 <= wd; t0 += 4) outer_batch<TZ, KS, 4>(SS, b0, t0, lane, u, k, j0, acc);
+    for (; t0 < wd; ++t0) outer_batch<TZ, KS, 1>(SS, b0, t0, lane, u, k, j0, acc);
   }
   {
     C2<TZ> x[KS];
