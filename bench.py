@@ -104,6 +104,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=24, help="requests in the serving leg (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate of the serving leg (1/s)")
+    ap.add_argument("--smooth-edge", type=int, default=0, help="edge/nodal smoothing sweeps per level (0: library default)")
+    ap.add_argument("--smooth-aux", type=int, default=0, help="auxiliary AMG sweeps per level (0: library default)")
     ap.add_argument("--sell-min-rows", type=int, default=0,
                     help="operators with at least this many rows use the sliced-ELL kernels (0: library default)")
     a = ap.parse_args()
@@ -129,7 +131,8 @@ def main():
     ports = list(range(m.num_ports))
     ncol = len(freqs) * len(ports)
     kw = dict(ports=ports, precision=capi.GP_PREC_MIXED, rtol=1e-8, batch=a.batch, restart=a.restart,
-              max_iters=a.max_iters, sell_min_rows=a.sell_min_rows)
+              max_iters=a.max_iters, sell_min_rows=a.sell_min_rows, smooth_edge=a.smooth_edge,
+              smooth_aux=a.smooth_aux)
 
     def step():
         return lay.solve(freqs, **kw)
