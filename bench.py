@@ -184,6 +184,18 @@ def main():
         fp64 = {"value": len(fs) / t64, "unit": "solves/s", "columns": len(fs),
                 "freqs_hz": [float(f) for f in fs], "port": 0, "iters": std["iters"],
                 "converged": f"{std['converged']}/{len(fs)}"}
+    # The FP64 complex SpMV y = A(w) x (gp_apply_fullwave's kernel) on resident
+    # vectors, priced on the CSR-equivalent bytes of a plain SpMV:
+    # 12 nnz + 4 (n+1) + k (16 n read + 16 n written) + 16 n (complex diagonal)
+    spmv = {}
+    n_u, nnz_s = m.n, int(m.sizes.nnz_s)
+    for kk in (1, 16):
+        ms = C.c_double()
+        capi.check(lay.lib.gp_bench_spmv(lay.h, kk, capi.GP_PREC_FP64, 20, C.byref(ms)), "gp_bench_spmv")
+        byts = 12.0 * nnz_s + 4.0 * (n_u + 1) + kk * 32.0 * n_u + 16.0 * n_u
+        gbs = byts / (ms.value * 1e-3) / 1e9
+        spmv[f"k{kk}"] = {"avg_us": ms.value * 1e3, "bytes": byts, "achieved_GBs": gbs, "frac": gbs / 8000.0}
+    spmv["kernel"] = "k_amul_sell (FP64 complex A(w) X, S in sliced ELL, tile order)"
     # HBM traffic of the dominant kernel, from the committed counter profile of this command
     traffic, traffic_src = None, None
     tp = os.path.join(ROOT, "profiles", "edge0_traffic.json")
@@ -217,6 +229,7 @@ def main():
         "speedup_vs_fp64_single": value / fp64["value"] if fp64 else None,
         "z_rel_diff_mixed_vs_fp64": zdiff,
         "copy_probe_GBs": copy_gbs,
+        "spmv": spmv,
         "roofline": {"bound": "hbm", "kernel": "level-0 edge smoother (k_edge_sell<float, 2, 8>: 8 lanes per row, 2 slots per lane)",
                      "achieved": kbytes / (kms * 1e-3) / 1e9 if kms > 0 else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s",

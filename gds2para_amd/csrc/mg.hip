@@ -198,13 +198,20 @@ gp_status build_hierarchy(gp_layout *lay, Hierarchy *H, bool want_f32, int max_l
   if (n != lay->n) return fail(GP_ERR_STATE, "level-0 edge count differs from the layout");
   // level-0 per-edge data
   std::vector<double> W, epsr(n), sig(n);
+  std::vector<int16_t> mcode(n);
+  if (m.nz > 16000) return fail(GP_ERR_ARG, "more than 16000 z-planes");
   edge_dual_volume(g0, &W);
   for (int64_t k = 0; k < n; ++k) {
     int64_t e = g0.erev[k];
     int64_t L = e / g0.g.El;
     epsr[k] = m.stack_epsn[L];
     sig[k] = m.mark_edge[e] != 0 ? m.stack_sign[L] : 0.;
+    mcode[k] = (int16_t)(L * 2 + (m.mark_edge[e] != 0 ? 1 : 0));
   }
+  GP_TRY(A.alloc(&H->mcode, n)); GP_TRY(A.alloc(&H->tab_eps, m.nz)); GP_TRY(A.alloc(&H->tab_sig, m.nz));
+  GP_HIP(hipMemcpyAsync(H->mcode, mcode.data(), sizeof(int16_t) * n, hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemcpyAsync(H->tab_eps, m.stack_epsn, sizeof(double) * m.nz, hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemcpyAsync(H->tab_sig, m.stack_sign, sizeof(double) * m.nz, hipMemcpyHostToDevice, s));
   GP_TRY(A.alloc(&H->W, n)); GP_TRY(A.alloc(&H->epsr, n)); GP_TRY(A.alloc(&H->sig, n));
   GP_HIP(hipMemcpyAsync(H->W, W.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
   GP_HIP(hipMemcpyAsync(H->epsr, epsr.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
@@ -667,9 +674,11 @@ __device__ __forceinline__ void sell_batch(const SellV<T> &M, int64_t b0, int t0
 #pragma unroll
   for (int h = 0; h < NB; ++h) {
     const int64_t e = b0 + (int64_t)(t0 + h) * 64 + lane;
-    c[h] = M.ci[e];
+    // the matrix is streamed once per launch: non-temporal, so it does not
+    // push the gathered rows of x out of the caches
+    c[h] = __builtin_nontemporal_load(M.ci + e);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) a[v][h] = M.v[v][e];
+    for (int v = 0; v < NV; ++v) a[v][h] = __builtin_nontemporal_load(M.v[v] + e);
   }
   C2<T> xv[NB][KS];
 #pragma unroll

@@ -141,6 +141,10 @@ struct Hierarchy {
   float *Wf = nullptr;
   double *epsr = nullptr;     // stack_epsn of each kept edge's layer
   double *sig = nullptr;      // stack_sign if the edge is a conductor edge, else 0
+  // the same two values as a 2-byte code per edge, layer * 2 + (conductor edge),
+  // into per-layer tables (for kernels that stream the diagonal per row)
+  int16_t *mcode = nullptr;
+  double *tab_eps = nullptr, *tab_sig = nullptr;
   double *sdiag = nullptr;    // S_ii: the system diagonal is the FP64 sum S_ii + D_ii(w)
   // level-0 slot orders of the SELL copies whose rows are level-0 edges /
   // level-0 free nodes: (16 x 16-cell xy tile, plane, ix, iy[, direction])

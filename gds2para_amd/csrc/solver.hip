@@ -129,8 +129,9 @@ __device__ __forceinline__ void outer_batch(const SellV<double> &SS, int64_t b0,
 #pragma unroll
   for (int h = 0; h < NB; ++h) {
     const int64_t e = b0 + (int64_t)(t0 + h) * 64 + lane;
-    c[h] = SS.ci[e];
-    m[h] = SS.v[0][e];
+    // streamed once per launch: non-temporal, keeping the caches for x
+    c[h] = __builtin_nontemporal_load(SS.ci + e);
+    m[h] = __builtin_nontemporal_load(SS.v[0] + e);
   }
   C2<TZ> x[NB][KS];
 #pragma unroll
@@ -217,6 +218,74 @@ __global__ void __launch_bounds__(256) k_outer_sell(OuterOp op, SellV<double> SS
     if (mode == 1) acc[q] = {bb[q].x - acc[q].x, bb[q].y - acc[q].y};
   }
   store_slots<double, KS>(out + i * k + j0, acc);
+}
+
+// y = A(w) x: the complex matrix reference() assembles, S + diag(w (-w eps_r
+// EPSILON0) + i w sigma) with the diagonal summed in FP64 as the reference
+// does (src/generateStiff.cpp:618-626), applied to k columns interleaved
+// (row i, column j at i*k + j).  S's SELL copy in level-0 tile order; LPR
+// lanes per row, one complex FP64 slot (one 16-byte load) per lane, four row
+// gathers in flight per lane.
+template <int LPR>
+__global__ void __launch_bounds__(256) k_amul_sell(SellV<double> SS, int64_t n, int k, const double *__restrict__ omega,
+                                                   const double *__restrict__ sdiag, const int16_t *__restrict__ mcode,
+                                                   const double *__restrict__ tab_eps, const double *__restrict__ tab_sig,
+                                                   const Z2 *__restrict__ x, Z2 *__restrict__ y) {
+  const int64_t t_ = xcd_block_id() * blockDim.x + threadIdx.x;
+  const int64_t pos = t_ / LPR;
+  if (pos >= n) return;
+  const int j0 = (int)blockIdx.y * LPR + (int)(t_ - pos * LPR);
+  const int64_t i = SS.rows ? (int64_t)SS.rows[pos] : pos;
+  const int64_t sl = pos >> 6;
+  const int lane = (int)(pos & 63);
+  Z2 acc[1] = {{0., 0.}};
+  const int64_t b0 = SS.base[sl];
+  const int wd = SS.w[sl];
+  int t0 = 0;
+  for (; t0 + 8 <= wd; t0 += 8) outer_batch<double, 1, 8>(SS, b0, t0, lane, x, k, j0, acc);
+  for (; t0 + 4 <= wd; t0 += 4) outer_batch<double, 1, 4>(SS, b0, t0, lane, x, k, j0, acc);
+  for (; t0 < wd; ++t0) outer_batch<double, 1, 1>(SS, b0, t0, lane, x, k, j0, acc);
+  const double w = omega[j0];
+  const int mc = mcode[i];  // layer * 2 + (conductor edge)
+  const double er = tab_eps[mc >> 1], sg = (mc & 1) ? tab_sig[mc >> 1] : 0.;
+  const double dr = reference_diag_re(sdiag[i], w, er), di = w * sg;
+  const Z2 xi = x[i * k + j0];
+  acc[0].x += dr * xi.x - di * xi.y;
+  acc[0].y += dr * xi.y + di * xi.x;
+  y[i * k + j0] = acc[0];
+}
+
+void amul(const Hierarchy &H, int64_t n, int k, const double *om, const Z2 *x, Z2 *y, hipStream_t s) {
+  SellV<double> SS;
+  SS.nr = H.Ss.nr; SS.nslice = H.Ss.nslice; SS.base = H.Ss.base; SS.w = H.Ss.w; SS.ci = H.Ss.ci; SS.nv = 1;
+  SS.v[0] = H.Ss.v[0];
+  SS.rows = H.Ss.rows;
+  int lpr = 1;
+  while (lpr < 16 && k % (lpr * 2) == 0) lpr *= 2;
+  const dim3 grid((unsigned)((n * lpr + 255) / 256), (unsigned)(k / lpr));
+#define GP_AMUL(LP) k_amul_sell<LP><<<grid, 256, 0, s>>>(SS, n, k, om, H.sdiag, H.mcode, H.tab_eps, H.tab_sig, x, y)
+  if (lpr == 16) GP_AMUL(16);
+  else if (lpr == 8) GP_AMUL(8);
+  else if (lpr == 4) GP_AMUL(4);
+  else if (lpr == 2) GP_AMUL(2);
+  else GP_AMUL(1);
+#undef GP_AMUL
+}
+
+// column-major host layout (column j at j*n) <-> interleaved rows (i*k + j)
+__global__ void k_cols_to_rows(int64_t n, int k, const Z2 *__restrict__ a, Z2 *__restrict__ b) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * k) return;
+  const int64_t i = t / k;
+  const int j = (int)(t - i * k);
+  b[t] = a[(int64_t)j * n + i];
+}
+__global__ void k_rows_to_cols(int64_t n, int k, const Z2 *__restrict__ a, Z2 *__restrict__ b) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * k) return;
+  const int64_t i = t / k;
+  const int j = (int)(t - i * k);
+  b[(int64_t)j * n + i] = a[t];
 }
 
 // Multi-vector dots.  Block (blk, c) handles vectors t in [8c, 8c+8) of V
@@ -1088,29 +1157,50 @@ extern "C" gp_status gp_kernel_timer_read(gp_layout *lay, int64_t *launches, dou
   return GP_OK;
 }
 
-extern "C" gp_status gp_bench_spmv(gp_layout *lay, int32_t k, int32_t precision, int32_t iters, double *avg_ms) {
-  if (!lay || k < 1 || iters < 1 || !avg_ms) return fail(GP_ERR_ARG, "bad arguments");
+extern "C" gp_status gp_apply_fullwave(gp_layout *lay, const double *freqs, int32_t k, const double *x, double *y) {
+  if (!lay || !freqs || !x || !y || k < 1 || k > 64) return fail(GP_ERR_ARG, "gp_apply_fullwave: bad arguments (1 <= k <= 64)");
+  for (int32_t j = 0; j < k; ++j)
+    if (!(freqs[j] > 0.)) return fail(GP_ERR_ARG, "gp_apply_fullwave: frequencies must be positive");
+  if (!lay->s_ready) return fail(GP_ERR_STATE, "gp_apply_fullwave: stiffness not assembled");
   GP_TRY(gp_prepare_solver(lay, GP_PREC_FP64));
-  Solver *sv = lay->solver;
-  const int64_t n = lay->n, nn = sv->H.L[0].nn, na = sv->H.aux.naux;
+  const Hierarchy &H = lay->solver->H;
+  const int64_t n = lay->n;
   hipStream_t s = lay->stream;
   DevBuf D;
-  Z2 *x, *p, *a, *y;
+  Z2 *xc, *xr, *yr;
+  double *om;
+  GP_TRY(D.A.alloc(&xc, n * k)); GP_TRY(D.A.alloc(&xr, n * k)); GP_TRY(D.A.alloc(&yr, n * k)); GP_TRY(D.A.alloc(&om, k));
+  std::vector<double> omh(k);
+  for (int32_t j = 0; j < k; ++j) omh[j] = 2 * kPi * freqs[j];
+  GP_HIP(hipMemcpyAsync(om, omh.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemcpyAsync(xc, x, sizeof(Z2) * n * k, hipMemcpyHostToDevice, s));
+  k_cols_to_rows<<<blocks_for(n * k), 256, 0, s>>>(n, k, xc, xr);
+  amul(H, n, k, om, xr, yr, s);
+  k_rows_to_cols<<<blocks_for(n * k), 256, 0, s>>>(n, k, yr, xc);
+  GP_HIP(hipMemcpyAsync(y, xc, sizeof(Z2) * n * k, hipMemcpyDeviceToHost, s));
+  GP_HIP(hipStreamSynchronize(s));
+  return GP_OK;
+}
+
+extern "C" gp_status gp_bench_spmv(gp_layout *lay, int32_t k, int32_t precision, int32_t iters, double *avg_ms) {
+  if (!lay || k < 1 || k > 64 || iters < 1 || !avg_ms) return fail(GP_ERR_ARG, "bad arguments");
+  (void)precision;
+  GP_TRY(gp_prepare_solver(lay, GP_PREC_FP64));
+  const Hierarchy &H = lay->solver->H;
+  const int64_t n = lay->n;
+  hipStream_t s = lay->stream;
+  DevBuf D;
+  Z2 *x, *y;
   double *om;
   GP_TRY(D.A.alloc(&x, n * k)); GP_TRY(D.A.alloc(&y, n * k)); GP_TRY(D.A.alloc(&om, k));
-  GP_TRY(D.A.alloc(&p, nn * k)); GP_TRY(D.A.alloc(&a, na * k));
   std::vector<double> omh(k, 2 * kPi * 1e9);
   GP_HIP(hipMemcpyAsync(om, omh.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
   GP_HIP(hipMemsetAsync(x, 0, sizeof(Z2) * n * k, s));
-  GP_HIP(hipMemsetAsync(p, 0, sizeof(Z2) * nn * k, s));
-  GP_HIP(hipMemsetAsync(a, 0, sizeof(Z2) * na * k, s));
-  (void)precision;
-  const OuterOp op = outer_op(lay, sv->H);
   hipEvent_t e0, e1;
   GP_HIP(hipEventCreate(&e0)); GP_HIP(hipEventCreate(&e1));
-  k_outer<double><<<blocks_for(n * k), 256, 0, s>>>(op, k, om, x, p, a, nullptr, y, 0, 0);
+  amul(H, n, k, om, x, y, s);
   GP_HIP(hipEventRecord(e0, s));
-  for (int it = 0; it < iters; ++it) k_outer<double><<<blocks_for(n * k), 256, 0, s>>>(op, k, om, x, p, a, nullptr, y, 0, 0);
+  for (int it = 0; it < iters; ++it) amul(H, n, k, om, x, y, s);
   GP_HIP(hipEventRecord(e1, s));
   GP_HIP(hipEventSynchronize(e1));
   float ms = 0;

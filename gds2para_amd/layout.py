@@ -104,6 +104,18 @@ class Layout:
         return dict(edge=[(h.ne[i], h.nn[i], h.nnz_k[i]) for i in range(el)],
                     aux=[(h.na[i], h.nnz_n[i], h.nlong[i]) for i in range(al)], naux=h.naux, setup_ms=h.setup_ms)
 
+    def apply_fullwave(self, freqs, X):
+        """Y = A(w) X (gp_apply_fullwave): X [k, n] complex, one column per
+        frequency in freqs, in the renumbered space; returns Y [k, n]."""
+        freqs = np.ascontiguousarray(np.atleast_1d(np.asarray(freqs, np.float64)))
+        n = self.mesh.n
+        X = np.asarray(X, np.complex128).reshape(len(freqs), n)
+        xi = np.ascontiguousarray(np.stack([X.real, X.imag], -1).ravel())
+        yi = np.zeros_like(xi)
+        capi.check(self.lib.gp_apply_fullwave(self.h, _p(freqs, f64p), len(freqs), _p(xi, f64p), _p(yi, f64p)),
+                   "gp_apply_fullwave")
+        return (yi[0::2] + 1j * yi[1::2]).reshape(len(freqs), n)
+
     def solve_v0(self, rtol=0.0):
         """paraGenerator's V0 quasi-static solve (gp_solve_v0).
 

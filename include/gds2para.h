@@ -13,6 +13,7 @@
  *                           merge_v0c / generateAc                  src/fdtd.hpp:1823,2778,2866,3481
  *   gp_solve_fullwave       int reference(fdtdMesh*, int, myint*, myint*, double*)  src/generateStiff.cpp:568
  *                           void fdtdMesh::reference1(int, int, complex<double>*)   src/fdtd.hpp:3963
+ *   gp_apply_fullwave       reference()'s A(w), applied             src/generateStiff.cpp:583-626
  *   gp_solve_real           int hypreSolve(fdtdMesh*, myint*, myint*, double*, myint,
  *                                          double*, myint, double*)  src/hypreSolver.h:14
  *   gp_solve_v0             paraGenerator's V0 section + Construct_Z_V0   src/matrixCon.cpp:366-560,
@@ -358,6 +359,14 @@ gp_status gp_write_touchstone(const char *path, const char *design, const char *
                               int32_t np, int32_t nf, const double *freqs, const double *mat,
                               const double *zsource);
 
+/* y = A(w) x for k columns (1 <= k <= 64) at freqs[k] (Hz), with A(w) the
+ * complex matrix reference() hands PARDISO (src/generateStiff.cpp:583-626):
+ * S + diag(w(-w eps_r EPSILON0) + i w sigma), the diagonal summed in FP64 as
+ * the reference sums it.  x and y are host arrays of n*k complex
+ * (interleaved), column j at offset j*n, in the renumbered (mapEdge) space.
+ * Needs the stiffness; builds the solver hierarchy if the layout has none. */
+gp_status gp_apply_fullwave(gp_layout *lay, const double *freqs, int32_t k, const double *x, double *y);
+
 /* ---- measurement hooks -------------------------------------------------- */
 /* Waits for all work the layout queued on its stream. */
 gp_status gp_layout_sync(gp_layout *lay);
@@ -371,8 +380,8 @@ gp_status gp_kernel_timer(gp_layout *lay, int32_t enable);
 gp_status gp_kernel_timer_read(gp_layout *lay, int64_t *launches, double *total_ms, double *total_bytes);
 
 /* Average device time (ms, HIP events on the library stream) of one complex
- * A(w)*X product with k right-hand sides.  precision selects FP64 CSR or the
- * FP32 level-0 operator. */
+ * product y = A(w) x with k columns, the kernel of gp_apply_fullwave on
+ * device-resident vectors.  precision is ignored (FP64). */
 gp_status gp_bench_spmv(gp_layout *lay, int32_t k, int32_t precision, int32_t iters, double *avg_ms);
 
 /* Average device time of a plain copy of `bytes` bytes (one read and one write
