@@ -1134,10 +1134,10 @@ SellV<float> sview<float>(const Sell &, const SellF &f, const Csr &, const CsrF 
 }
 
 // Kernel choice per operator: the lanes-per-row SELL kernels for operators of
-// at least sell_min_rows rows (default 2^16), the (row, slot) CSR kernels for
-// the small coarse levels, where SELL's padding and slice bookkeeping do not
-// pay off (measured at 200x200x24, 80 columns: 11.46 solves/s at 2^16, 10.61
-// at 2^20, 11.11 with every operator on SELL).
+// at least sell_min_rows rows (default 1: every operator), else the (row,
+// slot) CSR kernels.  Measured at 200x200x24, 80 columns, with four gathers
+// in flight per lane: 15.1 solves/s with SELL everywhere, 15.0 from 4096
+// rows, 14.7 from 2^14, 14.2 from 2^16.
 // Launch shape of the lanes-per-row SELL kernels: SPL slots per lane (two
 // complex FP32 = one 16-byte load, else one), LPR lanes per row (the largest
 // power of two up to 16 that divides k / SPL), k / (SPL LPR) grid rows.

@@ -194,7 +194,7 @@ struct VWork {
   C2<T> *lp = nullptr;                      // long-row products [nlong][nv][k]
   C2<T> *lpart = nullptr;                   // per-chunk partial products [nchunk][nv][k]
   int nu_aux = 2;
-  int64_t sell_min_rows = int64_t(1) << 16;  // smaller operators use the (row, slot) CSR kernels
+  int64_t sell_min_rows = 1;  // operators with fewer rows use the (row, slot) CSR kernels
   KTimer *kt = nullptr;               // times the level-0 edge smoother launches
   Arena arena;
 };

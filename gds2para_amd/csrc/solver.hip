@@ -571,7 +571,7 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
   GP_TRY(vwork_create<T>(lay, H, k, &vw));
   vw.nu_aux = o.smooth_aux > 0 ? std::min(o.smooth_aux, 8) : 2;
   vw.kt = &lay->ktimer;
-  vw.sell_min_rows = o.sell_min_rows > 0 ? o.sell_min_rows : (int64_t(1) << 16);
+  vw.sell_min_rows = o.sell_min_rows > 0 ? o.sell_min_rows : 1;
   DevBuf D;
   Arena &A = D.A;
   Z2 *b, *xu, *xphi, *xa, *V, *yd;

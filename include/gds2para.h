@@ -205,8 +205,10 @@ typedef struct gp_solve_opts {
   int32_t verbose;
   int32_t smooth_edge;            /* edge/nodal smoothing sweeps per level (0: 2) */
   int32_t smooth_aux;             /* auxiliary AMG sweeps per level (0: 2)       */
-  int64_t sell_min_rows;          /* sliced-ELL kernels on operators with at least
-                                     this many rows (0: 2^16; 1: every operator) */
+  int64_t sell_min_rows;          /* operators with at least this many rows use the
+                                     sliced-ELL kernels, smaller ones the CSR kernels
+                                     (0: every operator, the default; a value above
+                                     every operator's rows: CSR everywhere) */
 } gp_solve_opts;
 
 typedef struct gp_solve_stats {

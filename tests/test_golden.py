@@ -67,13 +67,16 @@ def test_gpu_stiffness_matches_fixture():
     lay.close()
 
 
+CSR = 1 << 40   # a threshold above every operator's rows: CSR kernels everywhere
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec,batch,sell", [(capi.GP_PREC_FP64, 1, 0), (capi.GP_PREC_MIXED, 4, 0),
-                                             (capi.GP_PREC_FP64, 1, 1), (capi.GP_PREC_MIXED, 16, 1),
-                                             (capi.GP_PREC_MIXED, 2, 1), (capi.GP_PREC_FP64, 4, 1)])
+@pytest.mark.parametrize("prec,batch,sell", [(capi.GP_PREC_FP64, 1, CSR), (capi.GP_PREC_MIXED, 4, CSR),
+                                             (capi.GP_PREC_FP64, 1, 0), (capi.GP_PREC_MIXED, 16, 0),
+                                             (capi.GP_PREC_MIXED, 2, 0), (capi.GP_PREC_FP64, 4, 0)])
 def test_gpu_z_matches_fixture(prec, batch, sell):
-    """sell = 1 runs every level on the sliced-ELL kernels (by default only
-    operators of 2^16+ rows do), so both kernel families meet the fixtures;
+    """sell = 0 (the default) runs every level on the sliced-ELL kernels,
+    sell = CSR on the (row, slot) CSR kernels, so both families meet the fixtures;
     the batches cover the lanes-per-row shapes (FP32: 2 slots per lane and
     1, 2 or 8 lanes per row; FP64: 1 slot per lane and 1 or 4 lanes)."""
     from gds2para_amd.layout import Layout
