@@ -230,7 +230,7 @@ def main():
         "z_rel_diff_mixed_vs_fp64": zdiff,
         "copy_probe_GBs": copy_gbs,
         "spmv": spmv,
-        "roofline": {"bound": "hbm", "kernel": "level-0 edge smoother (k_edge_sell<float, 2, 8>: 8 lanes per row, 2 slots per lane)",
+        "roofline": {"bound": "hbm", "kernel": "level-0 edge smoother (k_edge_sell<float, 4, 4>: 4 lanes per row, 4 slots per lane)",
                      "achieved": kbytes / (kms * 1e-3) / 1e9 if kms > 0 else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s",
                      "frac": (kbytes / (kms * 1e-3) / 1e9) / HBM_PEAK_GBS if kms > 0 else None,

@@ -1138,9 +1138,12 @@ SellV<float> sview<float>(const Sell &, const SellF &f, const Csr &, const CsrF 
 // slot) CSR kernels.  Measured at 200x200x24, 80 columns, with four gathers
 // in flight per lane: 15.1 solves/s with SELL everywhere, 15.0 from 4096
 // rows, 14.7 from 2^14, 14.2 from 2^16.
-// Launch shape of the lanes-per-row SELL kernels: SPL slots per lane (two
-// complex FP32 = one 16-byte load, else one), LPR lanes per row (the largest
-// power of two up to 16 that divides k / SPL), k / (SPL LPR) grid rows.
+// Launch shape of the lanes-per-row SELL kernels: SPL slots per lane (FP32:
+// 4 when k allows -- two 16-byte loads per gathered row --, else 2 or 1;
+// FP64: 1), LPR lanes per row (the largest power of two up to 16 that divides
+// k / SPL), k / (SPL LPR) grid rows.  Measured at 200x200x24, 16 slots, level-0
+// edge smoother: SPL 2 / LPR 8 536 us, SPL 4 / LPR 4 499 us, SPL 8 / LPR 2
+// 534 us (and the whole solve 27 % slower: register pressure).
 struct Lanes {
   int sp = 1, lpr = 1;
   dim3 grid;
@@ -1148,8 +1151,7 @@ struct Lanes {
 template <class T>
 Lanes lanes_shape(int64_t nr, int k) {
   Lanes L;
-  constexpr int spl = sizeof(T) == 4 ? 2 : 1;
-  L.sp = (k % spl == 0) ? spl : 1;
+  if (sizeof(T) == 4) L.sp = (k % 4 == 0) ? 4 : (k % 2 == 0) ? 2 : 1;
   while (L.lpr < 16 && (k / L.sp) % (L.lpr * 2) == 0) L.lpr *= 2;
   L.grid = dim3((unsigned)((nr * L.lpr + 255) / 256), (unsigned)(k / (L.sp * L.lpr)));
   return L;
@@ -1158,7 +1160,13 @@ Lanes lanes_shape(int64_t nr, int k) {
 #define GP_LANES_DISPATCH(KERNEL, ...)                                                                   \
   do {                                                                                                   \
     const Lanes ln_ = lanes_shape<T>(nr, k);                                                             \
-    if (ln_.sp == 2) {                                                                                   \
+    if (ln_.sp == 4) {                                                                            \
+      if (ln_.lpr == 16) KERNEL<T, 4, 16><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                         \
+      else if (ln_.lpr == 8) KERNEL<T, 4, 8><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                      \
+      else if (ln_.lpr == 4) KERNEL<T, 4, 4><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                      \
+      else if (ln_.lpr == 2) KERNEL<T, 4, 2><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                      \
+      else KERNEL<T, 4, 1><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                                        \
+    } else if (ln_.sp == 2) {                                                                            \
       if (ln_.lpr == 16) KERNEL<T, 2, 16><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                         \
       else if (ln_.lpr == 8) KERNEL<T, 2, 8><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                      \
       else if (ln_.lpr == 4) KERNEL<T, 2, 4><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                      \
