@@ -1425,7 +1425,13 @@ gp_status precond(const Hierarchy &H, VWork<T> &w, hipStream_t s, int nu_pre, in
   spmv2_sell<T>(n, w.sell_min_rows, k, BxS, Bx, w.c1, w.c2, nullptr, w.a, w.r[0], 1, nullptr, w.lp, s);
   // edge V-cycle on the remaining residual
   GP_TRY(vcycle<T>(H, w, s, nu_pre, nu_post));
-  // auxiliary correction on r0 - K u - B phi - Bx a
+  // The second auxiliary correction (on r0 - K u - B phi - Bx a) makes the
+  // preconditioner symmetric; FGMRES does not need that, and without it each
+  // application skips a level-0 residual, the Bx / Gx^T products and one
+  // auxiliary cycle.  Measured at 200x200x24, 80 columns: 18.3 solves/s
+  // without it (iterations p50 44, max 126) against 17.2 with it (37, 103),
+  // per-column latency p50 0.69 s / p99 1.99 s against 0.75 s / 2.10 s.
+  if (!w.aux_post) return GP_OK;
   const T *kd0 = arr<T>(L0.kd[0], L0.kdf[0]);
   (void)kd0;
   const SellV<T> KS = sview<T>(L0.Ks, L0.Ksf, L0.K, L0.Kf), BS = sview<T>(L0.Bs, L0.Bsf, L0.B, L0.Bf);

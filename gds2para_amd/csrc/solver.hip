@@ -577,6 +577,7 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
   struct VwGuard { VWork<T> *w; ~VwGuard() { vwork_free(w); } } vg{&vw};
   GP_TRY(vwork_create<T>(lay, H, k, &vw));
   vw.nu_aux = o.smooth_aux > 0 ? std::min(o.smooth_aux, 8) : 2;
+  vw.aux_post = o.aux_post > 0;
   vw.kt = &lay->ktimer;
   vw.sell_min_rows = o.sell_min_rows > 0 ? o.sell_min_rows : 1;
   DevBuf D;
@@ -999,6 +1000,7 @@ extern "C" void gp_solve_opts_default(gp_solve_opts *o) {
   o->smooth_edge = 0;
   o->smooth_aux = 0;
   o->sell_min_rows = 0;
+  o->aux_post = 0;
 }
 
 extern "C" gp_status gp_prepare_solver(gp_layout *lay, int32_t precision) {

@@ -209,6 +209,9 @@ typedef struct gp_solve_opts {
                                      sliced-ELL kernels, smaller ones the CSR kernels
                                      (0: every operator, the default; a value above
                                      every operator's rows: CSR everywhere) */
+  int32_t aux_post;               /* 1: auxiliary correction after the edge V-cycle
+                                     as well (symmetric preconditioner); 0: before
+                                     it only (the default, faster) */
 } gp_solve_opts;
 
 typedef struct gp_solve_stats {

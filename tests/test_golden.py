@@ -73,7 +73,8 @@ CSR = 1 << 40   # a threshold above every operator's rows: CSR kernels everywher
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec,batch,sell", [(capi.GP_PREC_FP64, 1, CSR), (capi.GP_PREC_MIXED, 4, CSR),
                                              (capi.GP_PREC_FP64, 1, 0), (capi.GP_PREC_MIXED, 16, 0),
-                                             (capi.GP_PREC_MIXED, 2, 0), (capi.GP_PREC_FP64, 4, 0)])
+                                             (capi.GP_PREC_MIXED, 2, 0), (capi.GP_PREC_FP64, 4, 0),
+                                             (capi.GP_PREC_MIXED, 8, -1)])
 def test_gpu_z_matches_fixture(prec, batch, sell):
     """sell = 0 (the default) runs every level on the sliced-ELL kernels,
     sell = CSR on the (row, slot) CSR kernels, so both families meet the fixtures;
@@ -87,7 +88,9 @@ def test_gpu_z_matches_fixture(prec, batch, sell):
         m = M.Mesh(spec)
         print(f"[golden] {name} prec={prec} batch={batch} sell={sell}", flush=True)
         lay = Layout(m)
-        Z, st, X = lay.solve(g["freqs"], precision=prec, rtol=1e-8, batch=batch, want_x=True, sell_min_rows=sell)
+        # sell = -1: the symmetric preconditioner (auxiliary correction after the V-cycle too)
+        Z, st, X = lay.solve(g["freqs"], precision=prec, rtol=1e-8, batch=batch, want_x=True,
+                             sell_min_rows=max(sell, 0), aux_post=int(sell < 0))
         assert st["status"] == capi.GP_OK and st["max_rel_residual"] <= 1e-8
         for k in range(len(g["freqs"])):
             assert _rel(Z[k], g["Z"][k]) < ZTOL, (name, k, _rel(Z[k], g["Z"][k]))
