@@ -469,13 +469,19 @@ void outer(const OuterOp &op, const Hierarchy &H, int64_t smin, int k, const dou
   SS.nr = H.Ss.nr; SS.nslice = H.Ss.nslice; SS.base = H.Ss.base; SS.w = H.Ss.w; SS.ci = H.Ss.ci; SS.nv = 1;
   SS.v[0] = H.Ss.v[0];
   SS.rows = H.Ss.rows;
-  const int sp = (sizeof(TZ) == 4 && k % 2 == 0) ? 2 : 1;
+  const int sp = sizeof(TZ) == 4 ? ((k % 4 == 0) ? 4 : (k % 2 == 0) ? 2 : 1) : 1;
   int lpr = 1;
   while (lpr < 16 && (k / sp) % (lpr * 2) == 0) lpr *= 2;
   const dim3 grid((unsigned)((op.n * lpr + 255) / 256), (unsigned)(k / (sp * lpr)));
 #define GP_OUTER_SELL(SP, LP) \
   k_outer_sell<TZ, SP, LP><<<grid, 256, 0, s>>>(op, SS, k, om, u, phi, a, b, out, mode, exact)
-  if (sp == 2) {
+  if (sp == 4) {
+    if (lpr == 16) GP_OUTER_SELL(4, 16);
+    else if (lpr == 8) GP_OUTER_SELL(4, 8);
+    else if (lpr == 4) GP_OUTER_SELL(4, 4);
+    else if (lpr == 2) GP_OUTER_SELL(4, 2);
+    else GP_OUTER_SELL(4, 1);
+  } else if (sp == 2) {
     if (lpr == 16) GP_OUTER_SELL(2, 16);
     else if (lpr == 8) GP_OUTER_SELL(2, 8);
     else if (lpr == 4) GP_OUTER_SELL(2, 4);
@@ -568,7 +574,7 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
   hipStream_t s = lay->stream;
   const int64_t n = lay->n, nn = H.L[0].nn, na = H.aux.naux;
   const int m = std::min(std::max(2, o.restart), 64);
-  const int nu_edge = o.smooth_edge > 0 ? std::min(o.smooth_edge, 8) : 2;
+  const int nu_edge = o.smooth_edge > 0 ? std::min(o.smooth_edge, 8) : 1;
   const gp_mesh &mesh = lay->hm.view;
   const OuterOp op = outer_op(lay, H);
   const Csr &G0 = H.L[0].G, &Gx = H.aux.Gx;
@@ -576,7 +582,7 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
   VWork<T> vw;
   struct VwGuard { VWork<T> *w; ~VwGuard() { vwork_free(w); } } vg{&vw};
   GP_TRY(vwork_create<T>(lay, H, k, &vw));
-  vw.nu_aux = o.smooth_aux > 0 ? std::min(o.smooth_aux, 8) : 2;
+  vw.nu_aux = o.smooth_aux > 0 ? std::min(o.smooth_aux, 8) : 1;
   vw.aux_post = o.aux_post > 0;
   vw.kt = &lay->ktimer;
   vw.sell_min_rows = o.sell_min_rows > 0 ? o.sell_min_rows : 1;
@@ -770,11 +776,14 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
       Z2 *Vi = V + (int64_t)i * n * k, *Vn = V + (int64_t)(i + 1) * n * k;
       // preconditioner: (u, phi, a) = M(W v)
       k_weight_in<T><<<gnk, 256, 0, s>>>(n, k, H.W, Vi, vw.r[0]);
-      GP_TRY(precond<T>(H, vw, s, nu_edge, nu_edge));
+      // the preconditioner writes its level-0 outputs straight into this
+      // step's Z blocks (same n*k interleaved layout), no copies
       C2<T> *zu = Zu + (int64_t)i * n * k, *zp = Zp + (int64_t)i * nn * k, *za = Za + (int64_t)i * na * k;
-      k_copy_t<T><<<gnk, 256, 0, s>>>(n * k, vw.u[0], zu);
-      k_copy_t<T><<<gnnk, 256, 0, s>>>(nn * k, vw.phi[0], zp);
-      k_copy_t<T><<<gnak, 256, 0, s>>>(na * k, vw.a, za);
+      C2<T> *own_u = vw.u[0], *own_phi = vw.phi[0], *own_a = vw.a;
+      vw.u[0] = zu; vw.phi[0] = zp; vw.a = za;
+      const gp_status pst = precond<T>(H, vw, s, nu_edge, nu_edge);
+      vw.u[0] = own_u; vw.phi[0] = own_phi; vw.a = own_a;
+      GP_TRY(pst);
       // w = A z (split form, FP64) written into the next basis slot
       outer<T>(op, H, vw.sell_min_rows, k, om_d, zu, zp, za, nullptr, Vn, 0, 0, s);
       // classical Gram-Schmidt with the norm from the same pass; a second pass

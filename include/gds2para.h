@@ -203,8 +203,8 @@ typedef struct gp_solve_opts {
   int32_t precision;              /* gp_precision                                 */
   int32_t batch;                  /* columns per batch; 0 = autotuned, 1 = single */
   int32_t verbose;
-  int32_t smooth_edge;            /* edge/nodal smoothing sweeps per level (0: 2) */
-  int32_t smooth_aux;             /* auxiliary AMG sweeps per level (0: 2)       */
+  int32_t smooth_edge;            /* edge/nodal smoothing sweeps per level (0: 1) */
+  int32_t smooth_aux;             /* auxiliary AMG sweeps per level (0: 1)       */
   int64_t sell_min_rows;          /* operators with at least this many rows use the
                                      sliced-ELL kernels, smaller ones the CSR kernels
                                      (0: every operator, the default; a value above
