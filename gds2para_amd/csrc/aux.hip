@@ -215,14 +215,18 @@ __global__ void k_scale_rows_inv(int64_t nr, const int64_t *__restrict__ rp, con
 }
 
 // power iteration step for lambda_max(D^-1 A): y = D^-1 A x, returns partial norms
+// y = D^-1 A x, one wavefront per row: the conductor aggregate rows hold up
+// to ~1e5 entries, which one thread per row walked alone (~10 ms a launch)
 __global__ void k_dinv_spmv(int64_t nr, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
                             const double *__restrict__ v, const double *__restrict__ dF, const double *__restrict__ x,
                             double *__restrict__ y) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const int lane = threadIdx.x & 63;
   if (i >= nr) return;
   double s = 0.;
-  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) s += v[p] * x[ci[p]];
-  y[i] = dF[i] > 0 ? s / dF[i] : 0.;
+  for (int64_t p = rp[i] + lane; p < rp[i + 1]; p += 64) s += v[p] * x[ci[p]];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) y[i] = dF[i] > 0 ? s / dF[i] : 0.;
 }
 
 __global__ void k_fill_hash(int64_t n, double *__restrict__ x) {
@@ -377,7 +381,7 @@ gp_status build_aux(gp_layout *lay, Hierarchy *H, bool want_f32) {
       k_fill_hash<<<blocks_for(nf), 256, 0, s>>>(nf, x);
       double lam = 1.0;
       for (int it = 0; it < 12; ++it) {
-        k_dinv_spmv<<<blocks_for(nf), 256, 0, s>>>(nf, AF.rp, AF.ci, AF.v[0], dF, x, y);
+        k_dinv_spmv<<<blocks_for(nf * 64), 256, 0, s>>>(nf, AF.rp, AF.ci, AF.v[0], dF, x, y);
         lam = dev_norm(y, nf, s) / std::max(dev_norm(x, nf, s), 1e-300);
         k_scal<<<blocks_for(nf), 256, 0, s>>>(nf, 1.0 / std::max(dev_norm(y, nf, s), 1e-300), y);
         std::swap(x, y);

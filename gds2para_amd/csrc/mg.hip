@@ -59,17 +59,23 @@ __global__ void k_diag_l1(int64_t nr, const int64_t *__restrict__ rp, const int3
                           const double *__restrict__ v0, const double *__restrict__ v1, const double *__restrict__ v2,
                           double *__restrict__ d0, double *__restrict__ d1, double *__restrict__ d2,
                           double *__restrict__ l0, double *__restrict__ l1, double *__restrict__ l2) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // one wavefront per row (the conductor aggregate rows of the auxiliary
+  // operators hold up to ~1e5 entries)
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const int lane = threadIdx.x & 63;
   if (i >= nr) return;
   const double *v[3] = {v0, v1, v2};
   double d[3] = {0, 0, 0}, l[3] = {0, 0, 0};
-  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) {
+  for (int64_t p = rp[i] + lane; p < rp[i + 1]; p += 64) {
     const bool diag = ci[p] == (int32_t)i;
     for (int k = 0; k < nv; ++k) {
       if (diag) d[k] += v[k][p];
       else l[k] += fabs(v[k][p]);
     }
   }
+  for (int off = 32; off > 0; off >>= 1)
+    for (int k = 0; k < 3; ++k) { d[k] += __shfl_down(d[k], off, 64); l[k] += __shfl_down(l[k], off, 64); }
+  if (lane != 0) return;
   double *dd[3] = {d0, d1, d2}, *ll[3] = {l0, l1, l2};
   for (int k = 0; k < nv; ++k) { dd[k][i] = d[k]; ll[k][i] = l[k]; }
 }
@@ -84,7 +90,7 @@ gp_status smoother_data(Arena &A, const Csr &M, double **d, double **l, hipStrea
     GP_TRY(A.alloc(&d[k], M.nr));
     GP_TRY(A.alloc(&l[k], M.nr));
   }
-  k_diag_l1<<<blocks_for(M.nr), 256, 0, s>>>(M.nr, M.rp, M.ci, M.nv, M.v[0], M.nv > 1 ? M.v[1] : nullptr,
+  k_diag_l1<<<blocks_for(M.nr * 64), 256, 0, s>>>(M.nr, M.rp, M.ci, M.nv, M.v[0], M.nv > 1 ? M.v[1] : nullptr,
                                               M.nv > 2 ? M.v[2] : nullptr, d[0], M.nv > 1 ? d[1] : nullptr,
                                               M.nv > 2 ? d[2] : nullptr, l[0], M.nv > 1 ? l[1] : nullptr,
                                               M.nv > 2 ? l[2] : nullptr);
