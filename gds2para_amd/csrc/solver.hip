@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(256) k_outer_sell(OuterOp op, SellV<double> SS
 // (row i, column j at i*k + j).  S's SELL copy in level-0 tile order; LPR
 // lanes per row, one complex FP64 slot (one 16-byte load) per lane, four row
 // gathers in flight per lane.
-template <int LPR>
+template <int SPL, int LPR>
 __global__ void __launch_bounds__(256) k_amul_sell(SellV<double> SS, int64_t n, int k, const double *__restrict__ omega,
                                                    const double *__restrict__ sdiag, const int16_t *__restrict__ mcode,
                                                    const double *__restrict__ tab_eps, const double *__restrict__ tab_sig,
@@ -234,25 +234,33 @@ __global__ void __launch_bounds__(256) k_amul_sell(SellV<double> SS, int64_t n, 
   const int64_t t_ = xcd_block_id() * blockDim.x + threadIdx.x;
   const int64_t pos = t_ / LPR;
   if (pos >= n) return;
-  const int j0 = (int)blockIdx.y * LPR + (int)(t_ - pos * LPR);
+  const int j0 = ((int)blockIdx.y * LPR + (int)(t_ - pos * LPR)) * SPL;
   const int64_t i = SS.rows ? (int64_t)SS.rows[pos] : pos;
   const int64_t sl = pos >> 6;
   const int lane = (int)(pos & 63);
-  Z2 acc[1] = {{0., 0.}};
+  Z2 acc[SPL];
+#pragma unroll
+  for (int q = 0; q < SPL; ++q) acc[q] = {0., 0.};
   const int64_t b0 = SS.base[sl];
   const int wd = SS.w[sl];
   int t0 = 0;
-  for (; t0 + 8 <= wd; t0 += 8) outer_batch<double, 1, 8>(SS, b0, t0, lane, x, k, j0, acc);
-  for (; t0 + 4 <= wd; t0 += 4) outer_batch<double, 1, 4>(SS, b0, t0, lane, x, k, j0, acc);
-  for (; t0 < wd; ++t0) outer_batch<double, 1, 1>(SS, b0, t0, lane, x, k, j0, acc);
-  const double w = omega[j0];
+  if constexpr (SPL == 1)
+    for (; t0 + 8 <= wd; t0 += 8) outer_batch<double, SPL, 8>(SS, b0, t0, lane, x, k, j0, acc);
+  for (; t0 + 4 <= wd; t0 += 4) outer_batch<double, SPL, 4>(SS, b0, t0, lane, x, k, j0, acc);
+  for (; t0 < wd; ++t0) outer_batch<double, SPL, 1>(SS, b0, t0, lane, x, k, j0, acc);
   const int mc = mcode[i];  // layer * 2 + (conductor edge)
   const double er = tab_eps[mc >> 1], sg = (mc & 1) ? tab_sig[mc >> 1] : 0.;
-  const double dr = reference_diag_re(sdiag[i], w, er), di = w * sg;
-  const Z2 xi = x[i * k + j0];
-  acc[0].x += dr * xi.x - di * xi.y;
-  acc[0].y += dr * xi.y + di * xi.x;
-  y[i * k + j0] = acc[0];
+  const double sii = sdiag[i];
+  Z2 xi[SPL];
+  load_slots<double, SPL>(x + i * k + j0, xi);
+#pragma unroll
+  for (int q = 0; q < SPL; ++q) {
+    const double w = omega[j0 + q];
+    const double dr = reference_diag_re(sii, w, er), di = w * sg;
+    acc[q].x += dr * xi[q].x - di * xi[q].y;
+    acc[q].y += dr * xi[q].y + di * xi[q].x;
+  }
+  store_slots<double, SPL>(y + i * k + j0, acc);
 }
 
 void amul(const Hierarchy &H, int64_t n, int k, const double *om, const Z2 *x, Z2 *y, hipStream_t s) {
@@ -260,15 +268,25 @@ void amul(const Hierarchy &H, int64_t n, int k, const double *om, const Z2 *x, Z
   SS.nr = H.Ss.nr; SS.nslice = H.Ss.nslice; SS.base = H.Ss.base; SS.w = H.Ss.w; SS.ci = H.Ss.ci; SS.nv = 1;
   SS.v[0] = H.Ss.v[0];
   SS.rows = H.Ss.rows;
+  const int sp = (k % 2 == 0) ? 2 : 1;  // two complex FP64 slots = two 16-byte loads per lane
   int lpr = 1;
-  while (lpr < 16 && k % (lpr * 2) == 0) lpr *= 2;
-  const dim3 grid((unsigned)((n * lpr + 255) / 256), (unsigned)(k / lpr));
-#define GP_AMUL(LP) k_amul_sell<LP><<<grid, 256, 0, s>>>(SS, n, k, om, H.sdiag, H.mcode, H.tab_eps, H.tab_sig, x, y)
-  if (lpr == 16) GP_AMUL(16);
-  else if (lpr == 8) GP_AMUL(8);
-  else if (lpr == 4) GP_AMUL(4);
-  else if (lpr == 2) GP_AMUL(2);
-  else GP_AMUL(1);
+  while (lpr < 16 && (k / sp) % (lpr * 2) == 0) lpr *= 2;
+  const dim3 grid((unsigned)((n * lpr + 255) / 256), (unsigned)(k / (sp * lpr)));
+#define GP_AMUL(SP, LP) \
+  k_amul_sell<SP, LP><<<grid, 256, 0, s>>>(SS, n, k, om, H.sdiag, H.mcode, H.tab_eps, H.tab_sig, x, y)
+  if (sp == 2) {
+    if (lpr == 16) GP_AMUL(2, 16);
+    else if (lpr == 8) GP_AMUL(2, 8);
+    else if (lpr == 4) GP_AMUL(2, 4);
+    else if (lpr == 2) GP_AMUL(2, 2);
+    else GP_AMUL(2, 1);
+  } else {
+    if (lpr == 16) GP_AMUL(1, 16);
+    else if (lpr == 8) GP_AMUL(1, 8);
+    else if (lpr == 4) GP_AMUL(1, 4);
+    else if (lpr == 2) GP_AMUL(1, 2);
+    else GP_AMUL(1, 1);
+  }
 #undef GP_AMUL
 }
 
@@ -469,7 +487,7 @@ void outer(const OuterOp &op, const Hierarchy &H, int64_t smin, int k, const dou
   SS.nr = H.Ss.nr; SS.nslice = H.Ss.nslice; SS.base = H.Ss.base; SS.w = H.Ss.w; SS.ci = H.Ss.ci; SS.nv = 1;
   SS.v[0] = H.Ss.v[0];
   SS.rows = H.Ss.rows;
-  const int sp = sizeof(TZ) == 4 ? ((k % 4 == 0) ? 4 : (k % 2 == 0) ? 2 : 1) : 1;
+  const int sp = sizeof(TZ) == 4 ? ((k % 4 == 0) ? 4 : (k % 2 == 0) ? 2 : 1) : ((k % 2 == 0) ? 2 : 1);
   int lpr = 1;
   while (lpr < 16 && (k / sp) % (lpr * 2) == 0) lpr *= 2;
   const dim3 grid((unsigned)((op.n * lpr + 255) / 256), (unsigned)(k / (sp * lpr)));
