@@ -108,6 +108,7 @@ def main():
     ap.add_argument("--smooth-aux", type=int, default=0, help="auxiliary AMG sweeps per level (0: library default)")
     ap.add_argument("--sell-min-rows", type=int, default=0,
                     help="operators with at least this many rows use the sliced-ELL kernels (0: library default)")
+    ap.add_argument("--basis64", action="store_true", help="FP64 Krylov basis on the mixed path (default FP32)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported by this build "
@@ -132,7 +133,7 @@ def main():
     ncol = len(freqs) * len(ports)
     kw = dict(ports=ports, precision=capi.GP_PREC_MIXED, rtol=1e-8, batch=a.batch, restart=a.restart,
               max_iters=a.max_iters, sell_min_rows=a.sell_min_rows, smooth_edge=a.smooth_edge,
-              smooth_aux=a.smooth_aux)
+              smooth_aux=a.smooth_aux, basis=1 if a.basis64 else 0)
 
     def step():
         return lay.solve(freqs, **kw)
@@ -215,11 +216,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "complex fp64 outer / complex fp32 preconditioner",
+        "dtype": "complex fp64 outer (residual, products, iterate) / complex fp32 preconditioner and Krylov basis storage"
+                 if not a.basis64 else "complex fp64 outer / complex fp32 preconditioner",
         "data": "synthetic (seeded Manhattan layout generator)",
         "config": {"workload": a.workload, "cells": list(spec.cells), "edges": int(m.n), "ports": len(ports),
                    "freqs": len(freqs), "f_range_hz": list(spec.f_range), "columns_per_step": ncol,
-                   "batch_slots": a.batch or "auto", "restart": a.restart, "rtol": 1e-8},
+                   "batch_slots": a.batch or "auto", "restart": a.restart, "rtol": 1e-8,
+                   "krylov_basis": "fp64" if a.basis64 else "fp32"},
         "converged": f"{conv}/{a.steps * ncol}",
         "max_rel_residual": res_max,
         "iters": {"p50": float(np.percentile(iters, 50)), "max": int(max(iters))},

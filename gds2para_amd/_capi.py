@@ -71,7 +71,7 @@ class GpSolveOpts(C.Structure):
     _fields_ = [
         ("rtol", f64), ("max_iters", i32), ("restart", i32), ("precision", i32),
         ("batch", i32), ("verbose", i32), ("smooth_edge", i32), ("smooth_aux", i32),
-        ("sell_min_rows", i64), ("aux_post", i32),
+        ("sell_min_rows", i64), ("aux_post", i32), ("basis", i32),
     ]
 
 

@@ -311,7 +311,8 @@ __global__ void k_rows_to_cols(int64_t n, int k, const Z2 *__restrict__ a, Z2 *_
 // partial[(blk * nout + t) * k + j] = sum conj(V_t) w; with self, chunk 0 also
 // writes t = nvec: sum |w|^2.  nout = nvec + self.
 constexpr int kDotChunk = 8;
-__global__ void __launch_bounds__(256) k_mdots(int64_t n, int k, const Z2 *__restrict__ V, int64_t vstride, int nvec,
+template <class TV>
+__global__ void __launch_bounds__(256) k_mdots(int64_t n, int k, const C2<TV> *__restrict__ V, int64_t vstride, int nvec,
                                                const Z2 *__restrict__ w, int self, Z2 *__restrict__ partial) {
   __shared__ Z2 sh[256];
   const int c = blockIdx.y;
@@ -329,7 +330,8 @@ __global__ void __launch_bounds__(256) k_mdots(int64_t n, int k, const Z2 *__res
 #pragma unroll
     for (int q = 0; q < kDotChunk; ++q)
       if (q < nt) {
-        const Z2 a = V[(int64_t)(t0 + q) * vstride + i * k + j];
+        const C2<TV> v = V[(int64_t)(t0 + q) * vstride + i * k + j];
+        const Z2 a = {(double)v.x, (double)v.y};
         acc[q].x += a.x * b.x + a.y * b.y;
         acc[q].y += a.x * b.y - a.y * b.x;
       }
@@ -359,20 +361,22 @@ __global__ void k_mdots_final(int nout, int nb, int k, const Z2 *__restrict__ pa
   h[q] = s;
 }
 
-// w = (w - sum_{t<nvec} V_t h_t) * scale_j   (per column)
-__global__ void k_gs_update(int64_t n, int k, int nvec, const Z2 *__restrict__ V, int64_t vstride,
-                            const Z2 *__restrict__ h, const double *__restrict__ scale, Z2 *__restrict__ w) {
+// out = (w - sum_{t<nvec} V_t h_t) * scale_j   (per column; out may be w)
+template <class TV, class TO>
+__global__ void k_gs_update(int64_t n, int k, int nvec, const C2<TV> *__restrict__ V, int64_t vstride,
+                            const Z2 *__restrict__ h, const double *__restrict__ scale, const Z2 *w, C2<TO> *out) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n * k) return;
   int j = (int)(q % k);
   Z2 acc = w[q];
   for (int t = 0; t < nvec; ++t) {
-    Z2 a = V[(int64_t)t * vstride + q], c = h[t * k + j];
-    acc.x -= a.x * c.x - a.y * c.y;
-    acc.y -= a.x * c.y + a.y * c.x;
+    const C2<TV> a = V[(int64_t)t * vstride + q];
+    const Z2 c = h[t * k + j];
+    acc.x -= (double)a.x * c.x - (double)a.y * c.y;
+    acc.y -= (double)a.x * c.y + (double)a.y * c.x;
   }
   const double sc = scale[j];
-  w[q] = {acc.x * sc, acc.y * sc};
+  out[q] = {(TO)(acc.x * sc), (TO)(acc.y * sc)};
 }
 
 // x += sum_t Z_t y_t (per column)
@@ -392,19 +396,20 @@ __global__ void k_xupdate(int64_t n, int k, int nvec, const C2<TZ> *__restrict__
   x[q] = acc;
 }
 
-__global__ void k_scale_cols(int64_t n, int k, const Z2 *__restrict__ w, const double *__restrict__ inv, Z2 *__restrict__ out) {
+template <class TO>
+__global__ void k_scale_cols(int64_t n, int k, const Z2 *__restrict__ w, const double *__restrict__ inv, C2<TO> *__restrict__ out) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n * k) return;
   const double s = inv[q % k];
-  out[q] = {w[q].x * s, w[q].y * s};
+  out[q] = {(TO)(w[q].x * s), (TO)(w[q].y * s)};
 }
 
-template <class T>
-__global__ void k_weight_in(int64_t n, int k, const double *__restrict__ W, const Z2 *__restrict__ v, C2<T> *__restrict__ out) {
+template <class T, class TV>
+__global__ void k_weight_in(int64_t n, int k, const double *__restrict__ W, const C2<TV> *__restrict__ v, C2<T> *__restrict__ out) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n * k) return;
   const double w = W[q / k];
-  out[q] = {(T)(w * v[q].x), (T)(w * v[q].y)};
+  out[q] = {(T)(w * (double)v[q].x), (T)(w * (double)v[q].y)};
 }
 
 template <class T>
@@ -525,11 +530,12 @@ struct Dots {
     GP_TRY(A.alloc(&hd, (int64_t)(maxv + 1) * k));
     return GP_OK;
   }
-  gp_status run(hipStream_t s, int64_t n, int k, const Z2 *V, int64_t vstride, int nvec, const Z2 *w, int self,
+  template <class TV>
+  gp_status run(hipStream_t s, int64_t n, int k, const C2<TV> *V, int64_t vstride, int nvec, const Z2 *w, int self,
                 std::vector<cd> &h) {
     const int nout = nvec + self;
     const int nch = std::max(1, (nvec + kDotChunk - 1) / kDotChunk);
-    k_mdots<<<dim3(nb, nch), 256, 0, s>>>(n, k, V, vstride, nvec, w, self, partial);
+    k_mdots<TV><<<dim3(nb, nch), 256, 0, s>>>(n, k, V, vstride, nvec, w, self, partial);
     k_mdots_final<<<(nout * k + 255) / 256, 256, 0, s>>>(nout, nb, k, partial, hd);
     std::vector<Z2> tmp((size_t)nout * k);
     GP_HIP(hipMemcpyAsync(tmp.data(), hd, sizeof(Z2) * nout * k, hipMemcpyDeviceToHost, s));
@@ -576,7 +582,13 @@ std::vector<int64_t> port_rows(gp_layout *lay) {
   return idx;
 }
 
-// Solve the columns io provides on k slots with preconditioner precision T.
+// Solve the columns io provides on k slots with preconditioner precision T
+// and the Arnoldi basis stored in TV.  Residuals, operator products, dot
+// products, the Hessenberg matrix and the iterate are FP64 in every case; an
+// FP32 basis (TV = float) halves the bytes the Gram-Schmidt passes read, and
+// every restart begins from the explicit FP64 residual, so the basis
+// precision bounds only what one cycle can gain (restarted GMRES with a
+// low-precision inner solve, Lindquist, Luszczek and Dongarra, 2020).
 // Second Gram-Schmidt pass when a slot's first pass left less than kReorth
 // of |w|^2 (severe cancellation).  The pass reads every basis block for all
 // slots, and the textbook trigger (1/2) fired for some converging slot in
@@ -584,7 +596,7 @@ std::vector<int64_t> port_rows(gp_layout *lay) {
 // 17.2 with 1e-4, at the same iteration counts and residuals.
 constexpr double kReorth = 1e-4;
 
-template <class T>
+template <class T, class TV>
 gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
   gp_layout *lay = cx.lay;
   const Hierarchy &H = cx.sv->H;
@@ -606,12 +618,13 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
   vw.sell_min_rows = o.sell_min_rows > 0 ? o.sell_min_rows : 1;
   DevBuf D;
   Arena &A = D.A;
-  Z2 *b, *xu, *xphi, *xa, *V, *yd;
+  Z2 *b, *xu, *xphi, *xa, *w, *yd;
+  C2<TV> *V;
   C2<T> *Zu, *Zp, *Za;
   double *om_d, *sc_d;
   int32_t *mask_d;
   GP_TRY(A.alloc(&b, n * k)); GP_TRY(A.alloc(&xu, n * k)); GP_TRY(A.alloc(&xphi, nn * k)); GP_TRY(A.alloc(&xa, na * k));
-  GP_TRY(A.alloc(&V, (int64_t)(m + 1) * n * k));
+  GP_TRY(A.alloc(&V, (int64_t)(m + 1) * n * k)); GP_TRY(A.alloc(&w, n * k));
   GP_TRY(A.alloc(&Zu, (int64_t)m * n * k)); GP_TRY(A.alloc(&Zp, (int64_t)m * nn * k)); GP_TRY(A.alloc(&Za, (int64_t)m * na * k));
   GP_TRY(A.alloc(&om_d, k)); GP_TRY(A.alloc(&sc_d, k)); GP_TRY(A.alloc(&mask_d, k)); GP_TRY(A.alloc(&yd, (int64_t)m * k));
   Dots dots;
@@ -750,8 +763,8 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
       if (!any) break;
     }
     // explicit FP64 residual of the split iterate (S G, S Gx included exactly)
-    outer<double>(op, H, vw.sell_min_rows, k, om_d, xu, xphi, xa, b, V, 1, 1, s);
-    GP_TRY(dots.run(s, n, k, V, 0, 0, V, 1, h));
+    outer<double>(op, H, vw.sell_min_rows, k, om_d, xu, xphi, xa, b, w, 1, 1, s);
+    GP_TRY(dots.run(s, n, k, V, 0, 0, w, 1, h));
     std::vector<double> relres(k, 0.);
     std::vector<int32_t> done(k, 0);
     int ndone = 0, nactive = 0;
@@ -773,9 +786,9 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
     }
     if (anyfill) {
       GP_TRY(refill(fill, false));
-      k_zero_cols<<<gnk, 256, 0, s>>>(n, k, mask_d, V);
-      outer<double>(op, H, vw.sell_min_rows, k, om_d, xu, xphi, xa, b, V, 1, 1, s);
-      GP_TRY(dots.run(s, n, k, V, 0, 0, V, 1, h));
+      k_zero_cols<<<gnk, 256, 0, s>>>(n, k, mask_d, w);
+      outer<double>(op, H, vw.sell_min_rows, k, om_d, xu, xphi, xa, b, w, 1, 1, s);
+      GP_TRY(dots.run(s, n, k, V, 0, 0, w, 1, h));
       nactive = 0;
       for (int j = 0; j < k; ++j) {
         beta[j] = cid[j] >= 0 ? std::sqrt(std::max(h[j].real(), 0.0)) : 0.;
@@ -786,14 +799,14 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
     ++cycles;
     for (int j = 0; j < k; ++j) sch[j] = (cid[j] >= 0 && beta[j] > 0) ? 1.0 / beta[j] : 0.0;
     GP_HIP(hipMemcpyAsync(sc_d, sch.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-    k_scale_cols<<<gnk, 256, 0, s>>>(n, k, V, sc_d, V);
+    k_scale_cols<TV><<<gnk, 256, 0, s>>>(n, k, w, sc_d, V);
     std::fill(g.begin(), g.end(), cd(0));
     for (int j = 0; j < k; ++j) g[j] = (cid[j] >= 0) ? beta[j] : 0.;
     int used = 0;
     for (int i = 0; i < m; ++i) {
-      Z2 *Vi = V + (int64_t)i * n * k, *Vn = V + (int64_t)(i + 1) * n * k;
+      C2<TV> *Vi = V + (int64_t)i * n * k, *Vn = V + (int64_t)(i + 1) * n * k;
       // preconditioner: (u, phi, a) = M(W v)
-      k_weight_in<T><<<gnk, 256, 0, s>>>(n, k, H.W, Vi, vw.r[0]);
+      k_weight_in<T, TV><<<gnk, 256, 0, s>>>(n, k, H.W, Vi, vw.r[0]);
       // the preconditioner writes its level-0 outputs straight into this
       // step's Z blocks (same n*k interleaved layout), no copies
       C2<T> *zu = Zu + (int64_t)i * n * k, *zp = Zp + (int64_t)i * nn * k, *za = Za + (int64_t)i * na * k;
@@ -802,11 +815,11 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
       const gp_status pst = precond<T>(H, vw, s, nu_edge, nu_edge);
       vw.u[0] = own_u; vw.phi[0] = own_phi; vw.a = own_a;
       GP_TRY(pst);
-      // w = A z (split form, FP64) written into the next basis slot
-      outer<T>(op, H, vw.sell_min_rows, k, om_d, zu, zp, za, nullptr, Vn, 0, 0, s);
+      // w = A z (split form, FP64)
+      outer<T>(op, H, vw.sell_min_rows, k, om_d, zu, zp, za, nullptr, w, 0, 0, s);
       // classical Gram-Schmidt with the norm from the same pass; a second pass
       // only when the first one cancelled all but kReorth of |w|^2 in a slot
-      GP_TRY(dots.run(s, n, k, V, n * k, i + 1, Vn, 1, h));
+      GP_TRY(dots.run(s, n, k, V, n * k, i + 1, w, 1, h));
       bool reorth = false;
       std::vector<double> rn2(k);
       for (int j = 0; j < k; ++j) {
@@ -820,8 +833,8 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
       if (reorth) {
         std::fill(sch.begin(), sch.end(), 1.0);
         GP_HIP(hipMemcpyAsync(sc_d, sch.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-        k_gs_update<<<gnk, 256, 0, s>>>(n, k, i + 1, V, n * k, dots.hd, sc_d, Vn);
-        GP_TRY(dots.run(s, n, k, V, n * k, i + 1, Vn, 1, h2));
+        k_gs_update<TV, double><<<gnk, 256, 0, s>>>(n, k, i + 1, V, n * k, dots.hd, sc_d, w, w);
+        GP_TRY(dots.run(s, n, k, V, n * k, i + 1, w, 1, h2));
         for (int j = 0; j < k; ++j) {
           double hh = 0.;
           for (int t = 0; t <= i; ++t) hh += std::norm(h2[t * k + j]);
@@ -863,7 +876,7 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
       // V_{i+1} = (w - V h) / |w - V h|, with the coefficients of the last pass
       // (dots.hd holds them: the second pass overwrote the first when it ran)
       GP_HIP(hipMemcpyAsync(sc_d, sch.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-      k_gs_update<<<gnk, 256, 0, s>>>(n, k, i + 1, V, n * k, dots.hd, sc_d, Vn);
+      k_gs_update<TV, TV><<<gnk, 256, 0, s>>>(n, k, i + 1, V, n * k, dots.hd, sc_d, w, Vn);
       used = i + 1;
       if (all_in) break;
       // with columns waiting, restart once half of the slots are done
@@ -1002,13 +1015,20 @@ struct CallbackIO : ColumnIO {
 
 // Device bytes one slot of the batched solve needs (basis, preconditioned
 // basis, iterate, V-cycle vectors), for the choice of the slot count.
-int64_t bytes_per_slot(const Hierarchy &H, int m, bool f32) {
+int64_t bytes_per_slot(const Hierarchy &H, int m, bool f32, bool v32) {
   const int64_t n = H.L[0].ne, nn = H.L[0].nn, na = H.aux.naux;
-  const int64_t tz = f32 ? 8 : 16;
-  int64_t b = (int64_t)(m + 1) * n * 16 + (int64_t)m * (n + nn + na) * tz + (n * 2 + nn + na) * 16;
+  const int64_t tz = f32 ? 8 : 16, tv = v32 ? 8 : 16;
+  int64_t b = (int64_t)(m + 1) * n * tv + (int64_t)m * (n + nn + na) * tz + (n * 3 + nn + na) * 16;
   for (const MgLevel &L : H.L) b += (5 * L.ne + 3 * L.nn) * tz;
   for (const AuxLevel &L : H.aux.L) b += 3 * L.n * tz;
   return b + (n + na) * tz;
+}
+
+// Preconditioner and basis precision from the options: FP32 preconditioner
+// with an FP32 basis unless opts.basis asks for FP64; plain FP64 otherwise.
+gp_status run_slots(SolveCtx &cx, ColumnIO &io, int k, bool f32) {
+  if (!f32) return solve_slots<double, double>(cx, io, k);
+  return cx.o->basis == 1 ? solve_slots<float, double>(cx, io, k) : solve_slots<float, float>(cx, io, k);
 }
 
 }  // namespace
@@ -1028,6 +1048,7 @@ extern "C" void gp_solve_opts_default(gp_solve_opts *o) {
   o->smooth_aux = 0;
   o->sell_min_rows = 0;
   o->aux_post = 0;
+  o->basis = 0;
 }
 
 extern "C" gp_status gp_prepare_solver(gp_layout *lay, int32_t precision) {
@@ -1107,7 +1128,7 @@ extern "C" gp_status gp_solve_fullwave_ex(gp_layout *lay, const double *freqs, i
   {
     size_t fre = 0, tot = 0;
     if (hipMemGetInfo(&fre, &tot) == hipSuccess) {
-      const int64_t per = bytes_per_slot(sv->H, m, f32);
+      const int64_t per = bytes_per_slot(sv->H, m, f32, f32 && o.basis != 1);
       while (kb > 1 && (int64_t)kb * per > (int64_t)(fre / 2)) kb /= 2;
     }
   }
@@ -1115,7 +1136,7 @@ extern "C" gp_status gp_solve_fullwave_ex(gp_layout *lay, const double *freqs, i
   const int32_t npt = mesh.num_ports;
   if (Z) std::fill(Z, Z + 2 * (size_t)nf * npt * npt, 0.0);
   SolveCtx cx{lay, sv, &o, &st, t0};
-  gp_status s2 = f32 ? solve_slots<float>(cx, io, k) : solve_slots<double>(cx, io, k);
+  gp_status s2 = run_slots(cx, io, k, f32);
   if (s2 != GP_OK) return s2;
   if (reports)
     for (int id = 0; id < ncol; ++id) reports[id] = rep[id];
@@ -1132,7 +1153,8 @@ extern "C" gp_status gp_solve_workspace_bytes(gp_layout *lay, int32_t precision,
   GP_TRY(gp_prepare_solver(lay, precision));
   const int m = std::min(std::max(2, restart), 64);
   const int k = pow2_at_least(std::min<int32_t>(slots, 64));
-  *bytes = (int64_t)k * bytes_per_slot(lay->solver->H, m, lay->solver->f32 && precision == GP_PREC_MIXED);
+  const bool f32 = lay->solver->f32 && precision == GP_PREC_MIXED;
+  *bytes = (int64_t)k * bytes_per_slot(lay->solver->H, m, f32, f32);
   return GP_OK;
 }
 
@@ -1151,7 +1173,7 @@ extern "C" gp_status gp_solve_stream(gp_layout *lay, const gp_solve_opts *opts, 
   gp_solve_stats st{};
   SolveCtx cx{lay, sv, &o, &st, t0};
   const int k = pow2_at_least(std::min<int32_t>(slots, 64));
-  gp_status s2 = f32 ? solve_slots<float>(cx, io, k) : solve_slots<double>(cx, io, k);
+  gp_status s2 = run_slots(cx, io, k, f32);
   st.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (stats) *stats = st;
   return s2;

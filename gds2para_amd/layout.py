@@ -132,7 +132,7 @@ class Layout:
 
     def solve(self, freqs, ports=None, precision=capi.GP_PREC_MIXED, rtol=1e-8, batch=0, restart=16,
               max_iters=2000, want_x=False, verbose=0, smooth_edge=0, smooth_aux=0, sell_min_rows=0,
-              aux_post=0):
+              aux_post=0, basis=0):
         """Returns (Z[nf, np*np] complex, stats, X or None)."""
         ports = list(range(self.mesh.num_ports)) if ports is None else list(ports)
         freqs = np.ascontiguousarray(np.asarray(freqs, np.float64))
@@ -142,7 +142,7 @@ class Layout:
         self.lib.gp_solve_opts_default(C.byref(o))
         o.rtol, o.precision, o.batch, o.restart, o.max_iters, o.verbose = rtol, precision, batch, restart, max_iters, verbose
         o.smooth_edge, o.smooth_aux, o.sell_min_rows = smooth_edge, smooth_aux, sell_min_rows
-        o.aux_post = aux_post
+        o.aux_post, o.basis = aux_post, basis
         Z = np.zeros(2 * nf * npt * npt)
         X = np.zeros(2 * self.mesh.n * nf * npp) if want_x else None
         st = capi.GpSolveStats()

@@ -224,7 +224,8 @@ class GpuBackend:
 
     def estimate(self, key):
         """Bytes before the layout exists: ~2.3 KB per edge of hierarchy plus
-        the slot workspace (FP64 basis, FP32 preconditioned basis)."""
+        the slot workspace (basis counted at FP64, an upper bound; FP32
+        preconditioned basis)."""
         cx, cy, cz = self.specs[key].cells
         n = 3 * (cx + 1) * (cy + 1) * (cz + 1)
         ws = self.slots * n * ((self.restart + 1) * 16 + self.restart * 8 * 2 + 256)

@@ -74,12 +74,14 @@ CSR = 1 << 40   # a threshold above every operator's rows: CSR kernels everywher
 @pytest.mark.parametrize("prec,batch,sell", [(capi.GP_PREC_FP64, 1, CSR), (capi.GP_PREC_MIXED, 4, CSR),
                                              (capi.GP_PREC_FP64, 1, 0), (capi.GP_PREC_MIXED, 16, 0),
                                              (capi.GP_PREC_MIXED, 2, 0), (capi.GP_PREC_FP64, 4, 0),
-                                             (capi.GP_PREC_MIXED, 8, -1)])
+                                             (capi.GP_PREC_MIXED, 8, -1), (capi.GP_PREC_MIXED, 16, -2)])
 def test_gpu_z_matches_fixture(prec, batch, sell):
     """sell = 0 (the default) runs every level on the sliced-ELL kernels,
     sell = CSR on the (row, slot) CSR kernels, so both families meet the fixtures;
     the batches cover the lanes-per-row shapes (FP32: 2 slots per lane and
-    1, 2 or 8 lanes per row; FP64: 1 slot per lane and 1 or 4 lanes)."""
+    1, 2 or 8 lanes per row; FP64: 1 slot per lane and 1 or 4 lanes).
+    sell = -1 selects the symmetric preconditioner, sell = -2 the FP64 Krylov
+    basis on the mixed path (FP32 is that path's default)."""
     from gds2para_amd.layout import Layout
     for name, spec in (("microstrip_60x20x8.npz", M.microstrip_spec()),
                        ("manhattan_24x20x10.npz",
@@ -88,9 +90,8 @@ def test_gpu_z_matches_fixture(prec, batch, sell):
         m = M.Mesh(spec)
         print(f"[golden] {name} prec={prec} batch={batch} sell={sell}", flush=True)
         lay = Layout(m)
-        # sell = -1: the symmetric preconditioner (auxiliary correction after the V-cycle too)
         Z, st, X = lay.solve(g["freqs"], precision=prec, rtol=1e-8, batch=batch, want_x=True,
-                             sell_min_rows=max(sell, 0), aux_post=int(sell < 0))
+                             sell_min_rows=max(sell, 0), aux_post=int(sell == -1), basis=int(sell == -2))
         assert st["status"] == capi.GP_OK and st["max_rel_residual"] <= 1e-8
         for k in range(len(g["freqs"])):
             assert _rel(Z[k], g["Z"][k]) < ZTOL, (name, k, _rel(Z[k], g["Z"][k]))
