@@ -1306,6 +1306,10 @@ static gp_status vcycle_level(const Hierarchy &H, VWork<T> &w, hipStream_t s, in
   };
   auto tb0 = [&]() { if (kt) kt->begin(s); };
   auto te0 = [&](int mode, int use_phi) { if (kt) kt->end(s, edge_bytes(mode, use_phi)); };
+  // u and tmp trade places after every Jacobi sweep but the first; start from
+  // the buffer that makes the last sweep write the level's own u (no copy)
+  if (((nu_pre - 1) + nu_post) & 1) std::swap(u, tmp);
+  if ((nu_pre - 1) & 1) std::swap(phi, d2);
   // pre-smoothing on the edges (phi = 0)
   tb0();
   edge_sell<T>(ne, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, nullptr, nullptr, u,
@@ -1389,6 +1393,9 @@ static gp_status aux_cycle(const Hierarchy &H, VWork<T> &w, hipStream_t s, int l
   const T *l0 = arr<T>(L.nl[0], L.nlf[0]), *l1 = arr<T>(L.nl[1], L.nlf[1]);
   const unsigned gr = grid_rows(n, k);
   const int nu = w.nu_aux;
+  // x and t trade places after every sweep but the first (2 nu - 1 swaps):
+  // start from t so that the last sweep writes the level's own x
+  std::swap(x, t);
   node_sell<T>(n, w.sell_min_rows, k, NS, N, w.c1, w.c2, d0, d1, l0, l1, r, nullptr, x, (T)1, L.N.lidx, w.lp, s);
   for (int it = 1; it < nu; ++it) {
     long_pass<T>(N, k, x, w.lpart, w.lp, s);
@@ -1423,11 +1430,20 @@ gp_status precond(const Hierarchy &H, VWork<T> &w, hipStream_t s, int nu_pre, in
   const MatV<T> K = view<T>(L0.K, L0.Kf), B = view<T>(L0.B, L0.Bf);
   const unsigned gn = grid_rows(n, k), ga = grid_rows(na, k);
   // auxiliary correction: a = AMG(Gx^T r), r <- r - Bx a
-  k_copy<T><<<blocks_for(n * k), 256, 0, s>>>(n * k, w.r[0], w.r0);
+  if (w.aux_post) k_copy<T><<<blocks_for(n * k), 256, 0, s>>>(n * k, w.r[0], w.r0);
   long_pass<T>(GxT, k, w.r[0], w.lpart, w.lp, s);
   spmv_real_sell<T>(na, w.sell_min_rows, k, GxTS, GxT, w.r[0], w.ar[0], 0, X.GxT.lidx, w.lp, s);
-  GP_TRY(aux_cycle<T>(H, w, s, 0));
-  k_copy<T><<<blocks_for(na * k), 256, 0, s>>>(na * k, w.ax[0], w.a);
+  if (w.aux_post) {
+    GP_TRY(aux_cycle<T>(H, w, s, 0));
+    k_copy<T><<<blocks_for(na * k), 256, 0, s>>>(na * k, w.ax[0], w.a);
+  } else {
+    // one-sided form: the cycle's level-0 result lands in w.a directly
+    C2<T> *ax0 = w.ax[0];
+    w.ax[0] = w.a;
+    const gp_status ast = aux_cycle<T>(H, w, s, 0);
+    w.ax[0] = ax0;
+    GP_TRY(ast);
+  }
   spmv2_sell<T>(n, w.sell_min_rows, k, BxS, Bx, w.c1, w.c2, nullptr, w.a, w.r[0], 1, nullptr, w.lp, s);
   // edge V-cycle on the remaining residual
   GP_TRY(vcycle<T>(H, w, s, nu_pre, nu_post));

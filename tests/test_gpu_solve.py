@@ -62,3 +62,22 @@ def test_manhattan_z_and_x(manh, prec, batch):
             x, xr = X[k * npt + p], Xr[k][p]
             assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-6
     lay.close()
+
+
+@pytest.mark.parametrize("prec,sweeps", [(capi.GP_PREC_MIXED, (2, 1)), (capi.GP_PREC_MIXED, (3, 2)),
+                                         (capi.GP_PREC_FP64, (2, 3))])
+def test_manhattan_smoothing_sweeps(manh, prec, sweeps):
+    """Edge/nodal and auxiliary sweep counts other than the default one: the
+    V-cycle picks its starting buffers by the parity of the sweeps so that the
+    last sweep writes each level's own vector, which even and odd counts both
+    exercise here."""
+    m, fs, Xr, Zr = manh
+    se, sa = sweeps
+    print(f"[solve] manhattan prec={prec} smooth_edge={se} smooth_aux={sa}", flush=True)
+    lay = Layout(m)
+    Z, st, _ = lay.solve(fs, precision=prec, rtol=1e-8, batch=8 if prec == capi.GP_PREC_MIXED else 1,
+                         smooth_edge=se, smooth_aux=sa)
+    assert st["status"] == capi.GP_OK and st["max_rel_residual"] <= 1e-8
+    for k in range(len(fs)):
+        assert _relerr(Z[k], Zr[k]) < ZTOL, (k, _relerr(Z[k], Zr[k]))
+    lay.close()
