@@ -91,6 +91,21 @@ def serving_leg(n_requests, rate_hz, seed=5):
             "columns_per_s": cols / dt, "wall_s": dt}
 
 
+def serving_leg_guarded(n_requests, rate_hz, limit_s=150):
+    """serving_leg in a child process with a time limit: a stream that stalls
+    is stopped and reported as such instead of holding up the benchmark line."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--serving-only", "--serve-requests", str(n_requests),
+           "--serve-rate", str(rate_hz)]
+    try:
+        p = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=limit_s)
+    except subprocess.TimeoutExpired:
+        return {"error": f"the serving stream did not finish within {limit_s} s and was stopped"}
+    if p.returncode != 0:
+        return {"error": f"the serving process ended with status {p.returncode}"}
+    return json.loads(p.stdout.strip().splitlines()[-1])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,7 +124,11 @@ def main():
     ap.add_argument("--sell-min-rows", type=int, default=0,
                     help="operators with at least this many rows use the sliced-ELL kernels (0: library default)")
     ap.add_argument("--basis64", action="store_true", help="FP64 Krylov basis on the mixed path (default FP32)")
+    ap.add_argument("--serving-only", action="store_true", help=argparse.SUPPRESS)
     a = ap.parse_args()
+    if a.serving_only:                   # the child process of serving_leg_guarded
+        print(json.dumps(serving_leg(a.serve_requests, a.serve_rate)), flush=True)
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported by this build "
               "(the solve path is single-device)", file=sys.stderr)
@@ -244,7 +263,7 @@ def main():
     lay.close()
     if a.serve_requests > 0:
         print(f"[bench] serving leg: {a.serve_requests} requests", file=sys.stderr, flush=True)
-        line["serving"] = serving_leg(a.serve_requests, a.serve_rate)
+        line["serving"] = serving_leg_guarded(a.serve_requests, a.serve_rate)
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
