@@ -672,14 +672,14 @@ __global__ void k_edge(int64_t nr, int k, MatV<T> K, MatV<T> B, const T *__restr
 // gathered rows of x, and calls f(a0, a1, a2, xrow) for every entry in
 // order (one dependent index -> row chain per entry left these kernels
 // latency-bound).
-template <class T, int KS, int NV, int NB, class F>
+template <class T, int KS, int NV, int NB, int ES, class F>
 __device__ __forceinline__ void sell_batch(const SellV<T> &M, int64_t b0, int t0, int lane, const C2<T> *__restrict__ x,
                                            int k, int j0, F &f) {
   int32_t c[NB];
   T a[NV][NB];
 #pragma unroll
   for (int h = 0; h < NB; ++h) {
-    const int64_t e = b0 + (int64_t)(t0 + h) * 64 + lane;
+    const int64_t e = b0 + (int64_t)(t0 + h * ES) * 64 + lane;
     // the matrix is streamed once per launch: non-temporal, so it does not
     // push the gathered rows of x out of the caches
     c[h] = __builtin_nontemporal_load(M.ci + e);
@@ -698,8 +698,31 @@ template <class T, int KS, int NV, class F>
 __device__ __forceinline__ void sell_row(const SellV<T> &M, int64_t b0, int wd, int lane, const C2<T> *__restrict__ x,
                                          int k, int j0, F &&f) {
   int t0 = 0;
-  for (; t0 + 4 <= wd; t0 += 4) sell_batch<T, KS, NV, 4>(M, b0, t0, lane, x, k, j0, f);
-  for (; t0 < wd; ++t0) sell_batch<T, KS, NV, 1>(M, b0, t0, lane, x, k, j0, f);
+  for (; t0 + 4 <= wd; t0 += 4) sell_batch<T, KS, NV, 4, 1>(M, b0, t0, lane, x, k, j0, f);
+  for (; t0 < wd; ++t0) sell_batch<T, KS, NV, 1, 1>(M, b0, t0, lane, x, k, j0, f);
+}
+
+// Entry-split walk: the lane group es of ES groups sharing a row takes the
+// entries es, es + ES, es + 2 ES, ... (rows of the coarse auxiliary operators
+// hold 30-250 entries; one group per row walked them serially and left those
+// launches latency-bound).  es_reduce then sums the groups' partial results:
+// the groups of a row sit LPR lanes apart inside an aligned LPR*ES-lane block
+// of the wavefront, so xor shuffles over LPR, 2 LPR, ... stay inside the row.
+template <class T, int KS, int NV, int ES, class F>
+__device__ __forceinline__ void sell_row_es(const SellV<T> &M, int64_t b0, int wd, int lane, int es,
+                                            const C2<T> *__restrict__ x, int k, int j0, F &&f) {
+  if (ES == 1) { sell_row<T, KS, NV>(M, b0, wd, lane, x, k, j0, f); return; }
+  int t0 = es;
+  for (; t0 + 3 * ES < wd; t0 += 4 * ES) sell_batch<T, KS, NV, 4, ES>(M, b0, t0, lane, x, k, j0, f);
+  for (; t0 < wd; t0 += ES) sell_batch<T, KS, NV, 1, ES>(M, b0, t0, lane, x, k, j0, f);
+}
+
+template <class T, int KS, int LPR, int ES>
+__device__ __forceinline__ void es_reduce(C2<T> (&acc)[KS]) {
+#pragma unroll
+  for (int m = LPR; m < LPR * ES; m <<= 1)
+#pragma unroll
+    for (int q = 0; q < KS; ++q) { acc[q].x += __shfl_xor(acc[q].x, m); acc[q].y += __shfl_xor(acc[q].y, m); }
 }
 
 template <class T, int SPL, int LPR>
@@ -780,27 +803,32 @@ __global__ void __launch_bounds__(256) k_edge_sell(int64_t nr, int k, SellV<T> K
 
 // SELL nodal l1-Jacobi (contract of k_node); long rows (lidx >= 0) take their
 // product N_eps d, N_sig d from lp = [nlong][2][k]
-template <class T, int SPL, int LPR>
+template <class T, int SPL, int LPR, int ES>
 __global__ void __launch_bounds__(256) k_node_sell(int64_t nr, int k, SellV<T> N, const T *__restrict__ c1v,
                                                    const T *__restrict__ c2v, const T *__restrict__ nd0,
                                                    const T *__restrict__ nd1, const T *__restrict__ nl0,
                                                    const T *__restrict__ nl1, const C2<T> *__restrict__ rn,
                                                    const C2<T> *__restrict__ d, C2<T> *__restrict__ dout, T damp,
                                                    const int32_t *__restrict__ lidx, const C2<T> *__restrict__ lp) {
-  constexpr int KS = SPL;  // LPR adjacent lanes share a row, SPL slots each
+  constexpr int KS = SPL;  // LPR adjacent lanes share a row, SPL slots each;
+                           // ES such groups split the row's entries
   const int64_t t_ = xcd_block_id() * blockDim.x + threadIdx.x;
-  const int64_t pos = t_ / LPR;
+  const int64_t pos = t_ / (LPR * ES);
   if (pos >= nr) return;
-  const int j0 = ((int)blockIdx.y * LPR + (int)(t_ - pos * LPR)) * SPL;
+  const int sub = (int)(t_ - pos * (LPR * ES)), es = sub / LPR;
+  const int j0 = ((int)blockIdx.y * LPR + (sub - es * LPR)) * SPL;
   const int64_t i = N.rows ? (int64_t)N.rows[pos] : pos;  // slot position -> row
   T c1[KS], c2[KS];
 #pragma unroll
   for (int q = 0; q < KS; ++q) { c1[q] = c1v[j0 + q]; c2[q] = c2v[j0 + q]; }
   C2<T> acc[KS];
-  load_slots<T, KS>(rn + i * k + j0, acc);
+  if (es == 0) load_slots<T, KS>(rn + i * k + j0, acc);
+  else
+#pragma unroll
+    for (int q = 0; q < KS; ++q) acc[q] = {0, 0};
   if (d) {
     const int32_t lq = lidx ? lidx[i] : -1;
-    if (lq >= 0) {
+    if (lq >= 0 && es == 0) {
       C2<T> s0[KS], s1[KS];
       load_slots<T, KS>(lp + ((int64_t)lq * 2) * k + j0, s0);
       load_slots<T, KS>(lp + ((int64_t)lq * 2 + 1) * k + j0, s1);
@@ -809,12 +837,12 @@ __global__ void __launch_bounds__(256) k_node_sell(int64_t nr, int k, SellV<T> N
         acc[q].x -= c1[q] * s0[q].x - c2[q] * s1[q].y;
         acc[q].y -= c1[q] * s0[q].y + c2[q] * s1[q].x;
       }
-    } else {
+    } else if (lq < 0) {
       const int64_t sl = pos >> 6;
       const int lane = (int)(pos & 63);
       const int64_t b0 = N.base[sl];
       const int wd = N.w[sl];
-      sell_row<T, KS, 2>(N, b0, wd, lane, d, k, j0, [&](T a0, T a1, T, const C2<T>(&x)[KS]) {
+      sell_row_es<T, KS, 2, ES>(N, b0, wd, lane, es, d, k, j0, [&](T a0, T a1, T, const C2<T>(&x)[KS]) {
 #pragma unroll
         for (int q = 0; q < KS; ++q) {
           const T ar = c1[q] * a0, ai = c2[q] * a1;
@@ -824,6 +852,8 @@ __global__ void __launch_bounds__(256) k_node_sell(int64_t nr, int k, SellV<T> N
       });
     }
   }
+  es_reduce<T, KS, LPR, ES>(acc);
+  if (es != 0) return;
   const T e0 = nd0[i], e1 = nd1[i], m0 = nl0[i], m1 = nl1[i];
   C2<T> o[KS];
   if (d) load_slots<T, KS>(d + i * k + j0, o);
@@ -838,32 +868,36 @@ __global__ void __launch_bounds__(256) k_node_sell(int64_t nr, int k, SellV<T> N
 }
 
 // SELL y = M x (real M) or y += M x; long rows from lp = [nlong][1][k]
-template <class T, int SPL, int LPR>
+template <class T, int SPL, int LPR, int ES>
 __global__ void __launch_bounds__(256) k_spmv_real_sell(int64_t nr, int k, SellV<T> M, const C2<T> *__restrict__ x,
                                                         C2<T> *__restrict__ y, int accumulate,
                                                         const int32_t *__restrict__ lidx, const C2<T> *__restrict__ lp) {
-  constexpr int KS = SPL;  // LPR adjacent lanes share a row, SPL slots each
+  constexpr int KS = SPL;  // LPR adjacent lanes share a row, SPL slots each;
+                           // ES such groups split the row's entries
   const int64_t t_ = xcd_block_id() * blockDim.x + threadIdx.x;
-  const int64_t pos = t_ / LPR;
+  const int64_t pos = t_ / (LPR * ES);
   if (pos >= nr) return;
-  const int j0 = ((int)blockIdx.y * LPR + (int)(t_ - pos * LPR)) * SPL;
+  const int sub = (int)(t_ - pos * (LPR * ES)), es = sub / LPR;
+  const int j0 = ((int)blockIdx.y * LPR + (sub - es * LPR)) * SPL;
   const int64_t i = M.rows ? (int64_t)M.rows[pos] : pos;  // slot position -> row
   C2<T> acc[KS];
+#pragma unroll
+  for (int q = 0; q < KS; ++q) acc[q] = {0, 0};
   const int32_t lq = lidx ? lidx[i] : -1;
   if (lq >= 0) {
-    load_slots<T, KS>(lp + (int64_t)lq * k + j0, acc);
+    if (es == 0) load_slots<T, KS>(lp + (int64_t)lq * k + j0, acc);
   } else {
-#pragma unroll
-    for (int q = 0; q < KS; ++q) acc[q] = {0, 0};
     const int64_t sl = pos >> 6;
     const int lane = (int)(pos & 63);
     const int64_t b0 = M.base[sl];
     const int wd = M.w[sl];
-    sell_row<T, KS, 1>(M, b0, wd, lane, x, k, j0, [&](T a, T, T, const C2<T>(&xv)[KS]) {
+    sell_row_es<T, KS, 1, ES>(M, b0, wd, lane, es, x, k, j0, [&](T a, T, T, const C2<T>(&xv)[KS]) {
 #pragma unroll
       for (int q = 0; q < KS; ++q) { acc[q].x += a * xv[q].x; acc[q].y += a * xv[q].y; }
     });
   }
+  es_reduce<T, KS, LPR, ES>(acc);
+  if (es != 0) return;
   if (accumulate) {
     C2<T> o[KS];
     load_slots<T, KS>(y + i * k + j0, o);
@@ -874,16 +908,18 @@ __global__ void __launch_bounds__(256) k_spmv_real_sell(int64_t nr, int k, SellV
 }
 
 // SELL M2(w) = c1 v0 + i c2 v1: y = r - M2 x (mode 0), y -= M2 x (1), y = M2 x (2)
-template <class T, int SPL, int LPR>
+template <class T, int SPL, int LPR, int ES>
 __global__ void __launch_bounds__(256) k_spmv2_sell(int64_t nr, int k, SellV<T> M, const T *__restrict__ c1v,
                                                     const T *__restrict__ c2v, const C2<T> *__restrict__ r,
                                                     const C2<T> *__restrict__ x, C2<T> *__restrict__ y, int mode,
                                                     const int32_t *__restrict__ lidx, const C2<T> *__restrict__ lp) {
-  constexpr int KS = SPL;  // LPR adjacent lanes share a row, SPL slots each
+  constexpr int KS = SPL;  // LPR adjacent lanes share a row, SPL slots each;
+                           // ES such groups split the row's entries
   const int64_t t_ = xcd_block_id() * blockDim.x + threadIdx.x;
-  const int64_t pos = t_ / LPR;
+  const int64_t pos = t_ / (LPR * ES);
   if (pos >= nr) return;
-  const int j0 = ((int)blockIdx.y * LPR + (int)(t_ - pos * LPR)) * SPL;
+  const int sub = (int)(t_ - pos * (LPR * ES)), es = sub / LPR;
+  const int j0 = ((int)blockIdx.y * LPR + (sub - es * LPR)) * SPL;
   const int64_t i = M.rows ? (int64_t)M.rows[pos] : pos;  // slot position -> row
   T c1[KS], c2[KS];
 #pragma unroll
@@ -893,17 +929,19 @@ __global__ void __launch_bounds__(256) k_spmv2_sell(int64_t nr, int k, SellV<T> 
   for (int q = 0; q < KS; ++q) acc[q] = {0, 0};
   const int32_t lq = lidx ? lidx[i] : -1;
   if (lq >= 0) {
-    C2<T> s0[KS], s1[KS];
-    load_slots<T, KS>(lp + ((int64_t)lq * 2) * k + j0, s0);
-    load_slots<T, KS>(lp + ((int64_t)lq * 2 + 1) * k + j0, s1);
+    if (es == 0) {
+      C2<T> s0[KS], s1[KS];
+      load_slots<T, KS>(lp + ((int64_t)lq * 2) * k + j0, s0);
+      load_slots<T, KS>(lp + ((int64_t)lq * 2 + 1) * k + j0, s1);
 #pragma unroll
-    for (int q = 0; q < KS; ++q) acc[q] = {c1[q] * s0[q].x - c2[q] * s1[q].y, c1[q] * s0[q].y + c2[q] * s1[q].x};
+      for (int q = 0; q < KS; ++q) acc[q] = {c1[q] * s0[q].x - c2[q] * s1[q].y, c1[q] * s0[q].y + c2[q] * s1[q].x};
+    }
   } else {
     const int64_t sl = pos >> 6;
     const int lane = (int)(pos & 63);
     const int64_t b0 = M.base[sl];
     const int wd = M.w[sl];
-    sell_row<T, KS, 2>(M, b0, wd, lane, x, k, j0, [&](T a0, T a1, T, const C2<T>(&xv)[KS]) {
+    sell_row_es<T, KS, 2, ES>(M, b0, wd, lane, es, x, k, j0, [&](T a0, T a1, T, const C2<T>(&xv)[KS]) {
 #pragma unroll
       for (int q = 0; q < KS; ++q) {
         const T ar = c1[q] * a0, ai = c2[q] * a1;
@@ -912,6 +950,8 @@ __global__ void __launch_bounds__(256) k_spmv2_sell(int64_t nr, int k, SellV<T> 
       }
     });
   }
+  es_reduce<T, KS, LPR, ES>(acc);
+  if (es != 0) return;
   C2<T> o[KS];
   if (mode == 0) load_slots<T, KS>(r + i * k + j0, o);
   else if (mode == 1) load_slots<T, KS>(y + i * k + j0, o);
@@ -1124,7 +1164,7 @@ template <class T>
 SellV<T> sview(const Sell &c, const SellF &f, const Csr &kc, const CsrF &kf);
 template <>
 SellV<double> sview<double>(const Sell &c, const SellF &, const Csr &kc, const CsrF &) {
-  SellV<double> v; v.nr = c.nr; v.nslice = c.nslice; v.base = c.base; v.w = c.w; v.ci = c.ci; v.nv = c.nv;
+  SellV<double> v; v.nr = c.nr; v.nslice = c.nslice; v.nent = c.nent; v.base = c.base; v.w = c.w; v.ci = c.ci; v.nv = c.nv;
   for (int q = 0; q < c.nv; ++q) v.v[q] = c.v[q];
   v.dg[0] = kc.dg[0]; v.dg[1] = kc.dg[1];
   v.rows = c.rows;
@@ -1132,7 +1172,7 @@ SellV<double> sview<double>(const Sell &c, const SellF &, const Csr &kc, const C
 }
 template <>
 SellV<float> sview<float>(const Sell &, const SellF &f, const Csr &, const CsrF &kf) {
-  SellV<float> v; v.nr = f.nr; v.nslice = f.nslice; v.base = f.base; v.w = f.w; v.ci = f.ci; v.nv = f.nv;
+  SellV<float> v; v.nr = f.nr; v.nslice = f.nslice; v.nent = f.nent; v.base = f.base; v.w = f.w; v.ci = f.ci; v.nv = f.nv;
   for (int q = 0; q < f.nv; ++q) v.v[q] = f.v[q];
   v.dg[0] = kf.dg[0]; v.dg[1] = kf.dg[1];
   v.rows = f.rows;
@@ -1187,6 +1227,49 @@ Lanes lanes_shape(int64_t nr, int k) {
     }                                                                                                    \
   } while (0)
 
+// Entry groups per row for the node / real / two-value SELL kernels, from the
+// operator's stored entries per row (slice widths, padding included).  The
+// coarse auxiliary operators hold 30-250 entries per row.  With one slot (the
+// plain FP64 single-column path) a single lane walked such a row serially;
+// 4 or 16 lanes per row took that path from 3.40 to 3.65 solves/s at
+// 200x200x24.  With 16 slots the same levels are bound by the 128 bytes of
+// slot data each entry gathers, not by the walk: 16 groups per row left those
+// launches as slow as before (45 us), so that shape keeps ES = 1.
+template <class T>
+int entry_split(const SellV<T> &M, const Lanes &L) {
+  if (!(L.sp == 1 && L.lpr == 1) || M.nr == 0) return 1;
+  const double w = (double)M.nent / (double)M.nr;
+  return w >= 128 ? 16 : w >= 24 ? 4 : 1;
+}
+
+#define GP_LANES_ES_DISPATCH(KERNEL, MAT, ...)                                                           \
+  do {                                                                                                   \
+    const Lanes ln_ = lanes_shape<T>(nr, k);                                                             \
+    const int es_ = entry_split<T>(MAT, ln_);                                                            \
+    if (es_ > 1) {                                                                                       \
+      const dim3 g_((unsigned)((nr * es_ + 255) / 256), ln_.grid.y);  /* sp = lpr = 1 */                 \
+      if (es_ == 16) KERNEL<T, 1, 1, 16><<<g_, 256, 0, s>>>(__VA_ARGS__);                                \
+      else KERNEL<T, 1, 1, 4><<<g_, 256, 0, s>>>(__VA_ARGS__);                                           \
+    } else if (ln_.sp == 4) {                                                                            \
+      if (ln_.lpr == 16) KERNEL<T, 4, 16, 1><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                      \
+      else if (ln_.lpr == 8) KERNEL<T, 4, 8, 1><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                   \
+      else if (ln_.lpr == 4) KERNEL<T, 4, 4, 1><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                   \
+      else if (ln_.lpr == 2) KERNEL<T, 4, 2, 1><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                   \
+      else KERNEL<T, 4, 1, 1><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                                     \
+    } else if (ln_.sp == 2) {                                                                            \
+      if (ln_.lpr == 16) KERNEL<T, 2, 16, 1><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                      \
+      else if (ln_.lpr == 8) KERNEL<T, 2, 8, 1><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                   \
+      else if (ln_.lpr == 4) KERNEL<T, 2, 4, 1><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                   \
+      else if (ln_.lpr == 2) KERNEL<T, 2, 2, 1><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                   \
+      else KERNEL<T, 2, 1, 1><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                                     \
+    } else {                                                                                             \
+      if (ln_.lpr == 16) KERNEL<T, 1, 16, 1><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                      \
+      else if (ln_.lpr == 8) KERNEL<T, 1, 8, 1><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                   \
+      else if (ln_.lpr == 4) KERNEL<T, 1, 4, 1><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                   \
+      else if (ln_.lpr == 2) KERNEL<T, 1, 2, 1><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                   \
+      else KERNEL<T, 1, 1, 1><<<ln_.grid, 256, 0, s>>>(__VA_ARGS__);                                     \
+    }                                                                                                    \
+  } while (0)
 
 template <class T>
 void edge_sell(int64_t nr, int64_t smin, int k, const SellV<T> &K, const SellV<T> &B, const MatV<T> &Kc, const MatV<T> &Bc,
@@ -1205,14 +1288,14 @@ template <class T>
 SellV<T> sv(const Sell &c, const SellF &f);
 template <>
 SellV<double> sv<double>(const Sell &c, const SellF &) {
-  SellV<double> v; v.nr = c.nr; v.nslice = c.nslice; v.base = c.base; v.w = c.w; v.ci = c.ci; v.nv = c.nv;
+  SellV<double> v; v.nr = c.nr; v.nslice = c.nslice; v.nent = c.nent; v.base = c.base; v.w = c.w; v.ci = c.ci; v.nv = c.nv;
   for (int q = 0; q < c.nv; ++q) v.v[q] = c.v[q];
   v.rows = c.rows;
   return v;
 }
 template <>
 SellV<float> sv<float>(const Sell &, const SellF &f) {
-  SellV<float> v; v.nr = f.nr; v.nslice = f.nslice; v.base = f.base; v.w = f.w; v.ci = f.ci; v.nv = f.nv;
+  SellV<float> v; v.nr = f.nr; v.nslice = f.nslice; v.nent = f.nent; v.base = f.base; v.w = f.w; v.ci = f.ci; v.nv = f.nv;
   for (int q = 0; q < f.nv; ++q) v.v[q] = f.v[q];
   v.rows = f.rows;
   return v;
@@ -1227,7 +1310,7 @@ void node_sell(int64_t nr, int64_t smin, int k, const SellV<T> &N, const MatV<T>
     k_node<T><<<blocks_for(nr * k), 256, 0, s>>>(nr, k, Nc, c1, c2, nd0, nd1, nl0, nl1, rn, d, dout, damp, lp);
     return;
   }
-  GP_LANES_DISPATCH(k_node_sell, nr, k, N, c1, c2, nd0, nd1, nl0, nl1, rn, d, dout, damp, lidx, lp);
+  GP_LANES_ES_DISPATCH(k_node_sell, N, nr, k, N, c1, c2, nd0, nd1, nl0, nl1, rn, d, dout, damp, lidx, lp);
 }
 
 template <class T>
@@ -1238,7 +1321,7 @@ void spmv_real_sell(int64_t nr, int64_t smin, int k, const SellV<T> &M, const Ma
     k_spmv_real<T><<<blocks_for(nr * k), 256, 0, s>>>(nr, k, Mc, x, y, accumulate, lp);
     return;
   }
-  GP_LANES_DISPATCH(k_spmv_real_sell, nr, k, M, x, y, accumulate, lidx, lp);
+  GP_LANES_ES_DISPATCH(k_spmv_real_sell, M, nr, k, M, x, y, accumulate, lidx, lp);
 }
 
 template <class T>
@@ -1249,7 +1332,7 @@ void spmv2_sell(int64_t nr, int64_t smin, int k, const SellV<T> &M, const MatV<T
     k_spmv2<T><<<blocks_for(nr * k), 256, 0, s>>>(nr, k, Mc, c1, c2, r, x, y, mode, lp);
     return;
   }
-  GP_LANES_DISPATCH(k_spmv2_sell, nr, k, M, c1, c2, r, x, y, mode, lidx, lp);
+  GP_LANES_ES_DISPATCH(k_spmv2_sell, M, nr, k, M, c1, c2, r, x, y, mode, lidx, lp);
 }
 
 template <class T> const T *arr(double *d, float *f);

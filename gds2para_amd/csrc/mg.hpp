@@ -86,7 +86,7 @@ struct MatV {  // read-only view of a CSR in precision T
 
 template <class T>
 struct SellV {  // read-only view of a Sell in precision T
-  int64_t nr = 0, nslice = 0;
+  int64_t nr = 0, nslice = 0, nent = 0;   // nent: stored entries incl. padding
   const int64_t *base = nullptr;
   const int32_t *w = nullptr;
   const int32_t *ci = nullptr;

@@ -122,42 +122,6 @@ __global__ void k_ptent_pairs(int64_t n, const int32_t *__restrict__ partner, co
   v[i] = 1.0;
 }
 
-// truncation of the smoothed prolongation; own(i) is the column of row i of
-// the tentative prolongation (one entry per row)
-__device__ __forceinline__ int32_t own_col(const int64_t *trp, const int32_t *tci, int64_t i) {
-  return trp[i] < trp[i + 1] ? tci[trp[i]] : -1;
-}
-
-__global__ void k_trunc_count(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                              const double *__restrict__ v, const int64_t *__restrict__ trp,
-                              const int32_t *__restrict__ tci, int64_t *__restrict__ cnt) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double mx = 0.;
-  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) mx = fmax(mx, fabs(v[p]));
-  const int32_t own = own_col(trp, tci, i);
-  int64_t c = 0;
-  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) c += (fabs(v[p]) >= kTrunc * mx || ci[p] == own) ? 1 : 0;
-  cnt[i] = c;
-}
-
-__global__ void k_trunc_fill(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                             const double *__restrict__ v, const int64_t *__restrict__ trp,
-                             const int32_t *__restrict__ tci, const int64_t *__restrict__ orp,
-                             int32_t *__restrict__ oci, double *__restrict__ ov) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double mx = 0., rs = 0., rk = 0.;
-  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) { mx = fmax(mx, fabs(v[p])); rs += v[p]; }
-  const int32_t own = own_col(trp, tci, i);
-  for (int64_t p = rp[i]; p < rp[i + 1]; ++p)
-    if (fabs(v[p]) >= kTrunc * mx || ci[p] == own) rk += v[p];
-  const double sc = rk != 0. ? rs / rk : 1.;
-  int64_t q = orp[i];
-  for (int64_t p = rp[i]; p < rp[i + 1]; ++p)
-    if (fabs(v[p]) >= kTrunc * mx || ci[p] == own) { oci[q] = ci[p]; ov[q] = v[p] * sc; ++q; }
-}
-
 __global__ void k_r_dinv_spmv(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
                               const double *__restrict__ v, const double *__restrict__ d, const double *__restrict__ x,
                               double *__restrict__ y) {
@@ -408,22 +372,6 @@ gp_status match_pass(Arena &A, Arena &tmp, const Csr &M, const double *d, Csr *P
   return GP_OK;
 }
 
-gp_status truncate_p(Arena &A, Arena &tmp, const Csr &P, const Csr &Pt, Csr *out, hipStream_t s) {
-  const int64_t n = P.nr;
-  int64_t *cnt, *off;
-  GP_TRY(tmp.alloc(&cnt, n)); GP_TRY(tmp.alloc(&off, n + 1));
-  k_trunc_count<<<blocks_for(n), 256, 0, s>>>(n, P.rp, P.ci, P.v[0], Pt.rp, Pt.ci, cnt);
-  GP_TRY(exclusive_scan_i64(cnt, off, n, s, tmp));
-  int64_t nk = 0;
-  GP_HIP(hipMemcpyAsync(&nk, off + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  GP_HIP(hipStreamSynchronize(s));
-  GP_TRY(csr_alloc(A, n, P.nc, nk, 1, out));
-  GP_HIP(hipMemcpyAsync(out->rp, off, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToDevice, s));
-  k_trunc_fill<<<blocks_for(n), 256, 0, s>>>(n, P.rp, P.ci, P.v[0], Pt.rp, Pt.ci, off, out->ci, out->v[0]);
-  GP_HIP(hipStreamSynchronize(s));
-  return GP_OK;
-}
-
 double power_lambda(Arena &tmp, const Csr &M, const double *d, hipStream_t s) {
   const int64_t n = M.nr;
   double *x = nullptr, *y = nullptr;
@@ -484,7 +432,7 @@ gp_status build_real_amg(RealAMG &G, Csr A0, hipStream_t s) {
       k_r_scale_rows<<<blocks_for(n), 256, 0, s>>>(n, F.A.rp, F.A.v[0], F.d, 4.0 / (3.0 * lam), DA.v[0]);
       if ((st = spgemm(tmp, DA, Pt, &DAP, s)) != GP_OK) break;
       if ((st = csr_add(tmp, Pt, -1.0, DAP, &Ps, s)) != GP_OK) break;
-      if ((st = truncate_p(A, tmp, Ps, Pt, &P, s)) != GP_OK) break;
+      if ((st = truncate_prolongation(A, tmp, Ps, Pt, kTrunc, &P, s)) != GP_OK) break;
       if ((st = transpose(A, P, &R, s)) != GP_OK) break;
       if ((st = spgemm(tmp, F.A, P, &AP, s)) != GP_OK) break;
       if ((st = spgemm(A, R, AP, &Ac, s)) != GP_OK) break;

@@ -564,4 +564,61 @@ gp_status sell_to_float(Arena &A, const Sell &a, SellF *f, hipStream_t s) {
   return GP_OK;
 }
 
+
+namespace {
+// own(i): the column of row i of the tentative prolongation (one entry per row)
+__device__ __forceinline__ int32_t own_col(const int64_t *trp, const int32_t *tci, int64_t i) {
+  return trp[i] < trp[i + 1] ? tci[trp[i]] : -1;
+}
+__device__ __forceinline__ bool keep_entry(double v, int32_t c, double cut, int32_t own) {
+  return fabs(v) >= cut || c == own;
+}
+__global__ void k_trunc_count(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                              const double *__restrict__ v, const int64_t *__restrict__ trp,
+                              const int32_t *__restrict__ tci, double theta, int64_t *__restrict__ cnt) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double mx = 0.;
+  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) mx = fmax(mx, fabs(v[p]));
+  const int32_t own = own_col(trp, tci, i);
+  int64_t c = 0;
+  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) c += keep_entry(v[p], ci[p], theta * mx, own) ? 1 : 0;
+  cnt[i] = c;
+}
+__global__ void k_trunc_fill(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                             const double *__restrict__ v, const int64_t *__restrict__ trp,
+                             const int32_t *__restrict__ tci, double theta, const int64_t *__restrict__ orp,
+                             int32_t *__restrict__ oci, double *__restrict__ ov) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double mx = 0., rs = 0., rk = 0.;
+  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) { mx = fmax(mx, fabs(v[p])); rs += v[p]; }
+  const int32_t own = own_col(trp, tci, i);
+  for (int64_t p = rp[i]; p < rp[i + 1]; ++p)
+    if (keep_entry(v[p], ci[p], theta * mx, own)) rk += v[p];
+  const double sc = rk != 0. ? rs / rk : 1.;
+  int64_t q = orp[i];
+  for (int64_t p = rp[i]; p < rp[i + 1]; ++p)
+    if (keep_entry(v[p], ci[p], theta * mx, own)) { oci[q] = ci[p]; ov[q] = v[p] * sc; ++q; }
+}
+}  // namespace
+
+gp_status truncate_prolongation(Arena &A, Arena &tmp, const Csr &P, const Csr &Pt, double theta, Csr *out,
+                                hipStream_t s) {
+  const int64_t n = P.nr;
+  if (Pt.nr != n) return fail(GP_ERR_ARG, "truncate_prolongation: P and Pt differ in rows");
+  int64_t *cnt, *off;
+  GP_TRY(tmp.alloc(&cnt, std::max<int64_t>(n, 1))); GP_TRY(tmp.alloc(&off, n + 1));
+  k_trunc_count<<<blocks_for(n), 256, 0, s>>>(n, P.rp, P.ci, P.v[0], Pt.rp, Pt.ci, theta, cnt);
+  GP_TRY(exclusive_scan_i64(cnt, off, n, s, tmp));
+  int64_t nk = 0;
+  GP_HIP(hipMemcpyAsync(&nk, off + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  GP_HIP(hipStreamSynchronize(s));
+  GP_TRY(csr_alloc(A, n, P.nc, nk, 1, out));
+  GP_HIP(hipMemcpyAsync(out->rp, off, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToDevice, s));
+  k_trunc_fill<<<blocks_for(n), 256, 0, s>>>(n, P.rp, P.ci, P.v[0], Pt.rp, Pt.ci, theta, off, out->ci, out->v[0]);
+  GP_HIP(hipStreamSynchronize(s));
+  return GP_OK;
+}
+
 }  // namespace gp

@@ -101,6 +101,12 @@ gp_status csr_from_coo(Arena &A, int64_t nr, int64_t nc, int64_t n, const int64_
                        const double *const *vals, int nv, Csr *out, hipStream_t s);
 // C = a + alpha * b (same shape, single value array each)
 gp_status csr_add(Arena &A, const Csr &a, double alpha, const Csr &b, Csr *c, hipStream_t s);
+// Truncated smoothed prolongation: entries of P below theta times the row's
+// largest magnitude are dropped, except the row's own aggregate (the column of
+// the tentative prolongation Pt, one entry per row); the kept entries are
+// scaled so that every row keeps its sum.
+gp_status truncate_prolongation(Arena &A, Arena &tmp, const Csr &P, const Csr &Pt, double theta, Csr *out,
+                                hipStream_t s);
 // host copy of value array `which` of a
 gp_status csr_host_copy(const Csr &a, HostCsr *h, int which);
 
