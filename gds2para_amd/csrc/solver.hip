@@ -669,15 +669,21 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
   std::vector<Z2> yh((size_t)m * k);
   int cycles = 0;
 
-  // place new columns into the slots listed in mask (idle when none comes)
+  // place new columns into the slots listed in mask (idle when none comes).
+  // With block, only the first column is waited for: the other slots take
+  // what is already waiting.  Waiting for every slot left the columns already
+  // placed unsolved until further columns arrived, so the last request to an
+  // idle layout never finished.
   auto refill = [&](const std::vector<int32_t> &mask, bool block) -> gp_status {
     std::vector<int64_t> row;
     std::vector<int32_t> col;
     std::vector<double> val;
+    bool placed = false;
     for (int j = 0; j < k; ++j) {
       if (!mask[j]) continue;
       Column c;
-      const int got = closed ? -1 : io.next(c, block);
+      const int got = closed ? -1 : io.next(c, block && !placed);
+      if (got == 1) placed = true;
       if (got < 0) closed = true;
       if (got != 1) { cid[j] = -1; bnorm[j] = 0.; continue; }
       ccol[j] = c;

@@ -283,7 +283,9 @@ gp_status gp_solve_fullwave_ex(gp_layout *lay, const double *freqs, int32_t nf,
  * and, at every FGMRES restart, replaces each finished column with a new one
  * from the caller:
  *   pull(user, &col, block)  1: col filled; 0: nothing waiting; -1: closed.
- *                            block = 1 when every slot is idle (may wait).
+ *                            block = 1 only for the first column while every
+ *                            slot is idle (may wait); the rest of that refill
+ *                            asks with block = 0.
  *   waiting(user)            columns ready to pull (steers early restarts).
  *   push(user, &col, z, rep) a finished column: z = its np complex Z entries
  *                            (Z[resp + np*col.port], resp = 0..np-1), or

@@ -107,10 +107,86 @@ def test_serve_same_alone_and_in_mix_mock():
     assert np.array_equal(reqs[1].Z, r0.Z)
 
 
+class SlotLoopBackend(MockBackend):
+    """The library's slot loop (solve_slots with CallbackIO): columns enter
+    at restarts; with every slot idle the refill waits for the first column
+    only and takes whatever else is already queued without waiting."""
+
+    def __init__(self, sizes, slots=4, steps=(2, 6), block_every_slot=False):
+        super().__init__(sizes, slots=slots)
+        self.steps, self.block_every_slot = steps, block_every_slot
+
+    def stream(self, key, pull, waiting, push):
+        k = self.slots
+        cid, need = [None] * k, [0] * k
+        closed = False
+        rng = np.random.default_rng(len(key))
+
+        def refill(mask, block):
+            nonlocal closed
+            placed = False
+            for j in range(k):
+                if not mask[j]:
+                    continue
+                col, r = (None, -1) if closed else pull(block and (self.block_every_slot or not placed))
+                closed = closed or r == -1
+                cid[j] = col if r == 1 else None
+                need[j] = int(rng.integers(*self.steps))
+                placed = placed or r == 1
+
+        refill([1] * k, False)
+        while True:
+            while all(c is None for c in cid) and not closed:
+                refill([1] * k, True)
+            if all(c is None for c in cid):
+                return
+            time.sleep(0.002)                       # one restart cycle
+            done = [0] * k
+            for j in range(k):
+                if cid[j] is not None:
+                    need[j] -= 1
+                    if need[j] <= 0:
+                        p, f, tag = cid[j]
+                        push(tag, self.zcol(key, p, f, self.nports), dict(iters=1, converged=1, rel_residual=0.0))
+                        done[j] = 1
+            refill([1 if done[j] or (cid[j] is None and not closed and waiting() > 0) else 0 for j in range(k)],
+                   False)
+
+
+def _serve_idle_layout(be):
+    """A layout goes idle, then gets a request with fewer columns than slots
+    and nothing after it: the request must still finish."""
+    srv = Server(be, 100)
+    first = Request("a", [0, 1], [1e9])
+    srv.submit(first)
+    srv.wait(timeout=10)
+    time.sleep(0.05)                                # every slot idle, the loop waits
+    last = Request("a", [2], [5e8, 1e9])
+    srv.submit(last)
+    try:
+        srv.wait(timeout=5)
+    finally:
+        with srv.lock:
+            stuck = last.remaining
+        for r in list(srv.res.values()):            # unblock the loop either way
+            with r.cv:
+                r.closed = True
+                r.cv.notify_all()
+    return stuck
+
+
+def test_serve_idle_layout_last_request_finishes():
+    assert _serve_idle_layout(SlotLoopBackend({"a": 1})) == 0
+
+
+def test_serve_idle_layout_stalls_when_every_slot_blocks():
+    """The defect the library had: its refill waited for every idle slot, so
+    the columns already placed sat unsolved until more columns arrived."""
+    with pytest.raises(TimeoutError):
+        _serve_idle_layout(SlotLoopBackend({"a": 1}, block_every_slot=True))
+
+
 @pytest.mark.gpu
-@pytest.mark.skip(reason="printed nothing for 180 s on the GPU in one run (with a Gram-Schmidt reduction "
-                         "change since removed; it had passed before that change); no cause found by reading "
-                         "the server, the stream loop or the kernels, so the case stays off the GPU")
 def test_serve_gpu_same_alone_and_in_mix():
     from gds2para_amd import mesh as M
     from gds2para_amd.layout import Layout
