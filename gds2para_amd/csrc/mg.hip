@@ -605,10 +605,13 @@ __global__ void __launch_bounds__(256) k_long_fin(MatV<T> M, int k, const C2<T> 
   }
 }
 
-struct EdgeMode { enum { RES = 0, JACOBI = 1, JACOBI0 = 2 }; };
+// NOK (or-ed into RES or JACOBI): r already holds a residual with the K(w) u
+// term for the current u, so only B(w) phi is subtracted; phi is then the
+// change of the nodal iterate since r was formed.
+struct EdgeMode { enum { RES = 0, JACOBI = 1, JACOBI0 = 2, KIND = 3, NOK = 4 }; };
 
 // Edge kernel, one thread per (row, column):
-//   acc = r - K(w) u - B(w) phi
+//   acc = r - K(w) u - B(w) phi           (without K(w) u under NOK)
 //   RES:     res = acc
 //   JACOBI:  u_out = u + acc / d_l1
 //   JACOBI0: u_out = r / d_l1           (u = phi = 0)
@@ -624,8 +627,10 @@ __global__ void k_edge(int64_t nr, int k, MatV<T> K, MatV<T> B, const T *__restr
   if (i >= nr) return;
   const T c1 = c1v[j], c2 = c2v[j];
   C2<T> acc = r[i * k + j];
-  if (mode != EdgeMode::JACOBI0) {
-    if (K.dg[0]) {  // K_eps and K_sig diagonal (level 0): one value array in the loop
+  const int kind = mode & EdgeMode::KIND;
+  if (kind != EdgeMode::JACOBI0) {
+    if (mode & EdgeMode::NOK) {
+    } else if (K.dg[0]) {  // K_eps and K_sig diagonal (level 0): one value array in the loop
       for (int64_t p = K.rp[i]; p < K.rp[i + 1]; ++p) {
         const T a = K.v[0][p];
         const C2<T> x = u[(int64_t)K.ci[p] * k + j];
@@ -652,10 +657,10 @@ __global__ void k_edge(int64_t nr, int k, MatV<T> K, MatV<T> B, const T *__restr
       }
     }
   }
-  if (mode == EdgeMode::RES) { out[i * k + j] = acc; return; }
+  if (kind == EdgeMode::RES) { out[i * k + j] = acc; return; }
   C2<T> d = l1diag<T>({kd0[i] + c1 * kd1[i], c2 * kd2[i]}, kl0[i] + fabs(c1) * kl1[i] + c2 * kl2[i]);
   C2<T> q = cdiv(acc, d);
-  if (mode == EdgeMode::JACOBI0) { out[i * k + j] = {damp * q.x, damp * q.y}; return; }
+  if (kind == EdgeMode::JACOBI0) { out[i * k + j] = {damp * q.x, damp * q.y}; return; }
   C2<T> u0 = u[i * k + j];
   out[i * k + j] = {u0.x + damp * q.x, u0.y + damp * q.y};
 }
@@ -747,7 +752,8 @@ __global__ void __launch_bounds__(256) k_edge_sell(int64_t nr, int k, SellV<T> K
   for (int q = 0; q < KS; ++q) { c1[q] = c1v[j0 + q]; c2[q] = c2v[j0 + q]; }
   C2<T> acc[KS];
   load_slots<T, KS>(r + i * k + j0, acc);
-  if (mode != EdgeMode::JACOBI0) {
+  const int kind = mode & EdgeMode::KIND;
+  if (kind != EdgeMode::JACOBI0 && !(mode & EdgeMode::NOK)) {
     const int64_t b0 = K.base[sl];
     const int wd = K.w[sl];
     if (K.nv == 1) {
@@ -776,6 +782,8 @@ __global__ void __launch_bounds__(256) k_edge_sell(int64_t nr, int k, SellV<T> K
         acc[q].y -= ar * x[q].y + ai * x[q].x;
       }
     }
+  }
+  if (kind != EdgeMode::JACOBI0) {
     if (use_phi) {
       sell_row<T, KS, 2>(B, B.base[sl], B.w[sl], lane, phi, k, j0, [&](T a0, T a1, T, const C2<T>(&x)[KS]) {
 #pragma unroll
@@ -787,15 +795,15 @@ __global__ void __launch_bounds__(256) k_edge_sell(int64_t nr, int k, SellV<T> K
       });
     }
   }
-  if (mode == EdgeMode::RES) { store_slots<T, KS>(out + i * k + j0, acc); return; }
+  if (kind == EdgeMode::RES) { store_slots<T, KS>(out + i * k + j0, acc); return; }
   const T d0 = kd0[i], d1 = kd1[i], d2 = kd2[i], l0 = kl0[i], l1 = kl1[i], l2 = kl2[i];
   C2<T> u0[KS];
-  if (mode == EdgeMode::JACOBI) load_slots<T, KS>(u + i * k + j0, u0);
+  if (kind == EdgeMode::JACOBI) load_slots<T, KS>(u + i * k + j0, u0);
 #pragma unroll
   for (int q = 0; q < KS; ++q) {
     const C2<T> dq = l1diag<T>({d0 + c1[q] * d1, c2[q] * d2}, l0 + fabs(c1[q]) * l1 + c2[q] * l2);
     const C2<T> z = cdiv(acc[q], dq);
-    if (mode == EdgeMode::JACOBI0) acc[q] = {damp * z.x, damp * z.y};
+    if (kind == EdgeMode::JACOBI0) acc[q] = {damp * z.x, damp * z.y};
     else acc[q] = {u0[q].x + damp * z.x, u0[q].y + damp * z.y};
   }
   store_slots<T, KS>(out + i * k + j0, acc);
@@ -1378,13 +1386,18 @@ static gp_status vcycle_level(const Hierarchy &H, VWork<T> &w, hipStream_t s, in
   KTimer *kt = (l == 0) ? w.kt : nullptr;
   auto edge_bytes = [&](int mode, int use_phi) {
     const double vb = (double)sizeof(C2<T>) * k, tb = (double)sizeof(T);
+    const int kind = mode & EdgeMode::KIND;
+    const bool use_k = !(mode & EdgeMode::NOK);
     double b = vb * ne * 2;                                                     // r in, out
-    if (mode != EdgeMode::JACOBI0) {
-      b += (double)L.K.nnz * (4 + L.K.nv * tb) + 8.0 * (ne + 1) + vb * ne;       // K, u
-      if (L.K.dg[0]) b += 2 * tb * ne;                                          // diagonal K_eps, K_sig
+    if (kind != EdgeMode::JACOBI0) {
+      if (use_k) {
+        b += (double)L.K.nnz * (4 + L.K.nv * tb) + 8.0 * (ne + 1) + vb * ne;     // K, u
+        if (L.K.dg[0]) b += 2 * tb * ne;                                        // diagonal K_eps, K_sig
+      }
       if (use_phi) b += (double)L.B.nnz * (4 + 2 * tb) + 8.0 * (ne + 1) + vb * nn;
     }
-    if (mode != EdgeMode::RES) b += 6 * tb * ne;                                // smoother diagonals
+    if (kind != EdgeMode::RES) b += 6 * tb * ne;                                // smoother diagonals
+    if (kind == EdgeMode::JACOBI && !use_k) b += vb * ne;                       // u for the update
     return b;
   };
   auto tb0 = [&]() { if (kt) kt->begin(s); };
@@ -1416,13 +1429,15 @@ static gp_status vcycle_level(const Hierarchy &H, VWork<T> &w, hipStream_t s, in
     node_sell<T>(nn, w.sell_min_rows, k, NS, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, phi, d2, damp, nullptr, w.lp, s);
     std::swap(phi, d2);
   }
-  // coarse-grid correction
+  // coarse-grid correction.  res holds r - K u for this u already, so only
+  // B phi is subtracted (no second K gather); the result goes to tmp, which
+  // is free until the post-smoothing
   tb0();
-  edge_sell<T>(ne, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, phi, res,
-                               EdgeMode::RES, 1, damp, s);
-  te0(EdgeMode::RES, 1);
+  edge_sell<T>(ne, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, res, u, phi, tmp,
+                               EdgeMode::RES | EdgeMode::NOK, 1, damp, s);
+  te0(EdgeMode::RES | EdgeMode::NOK, 1);
   const int64_t nec = H.L[l + 1].ne, nnc = H.L[l + 1].nn;
-  spmv_real_sell<T>(nec, w.sell_min_rows, k, PTS, PT, res, w.r[l + 1], 0, nullptr, w.lp, s);
+  spmv_real_sell<T>(nec, w.sell_min_rows, k, PTS, PT, tmp, w.r[l + 1], 0, nullptr, w.lp, s);
   GP_TRY(vcycle_level<T>(H, w, s, l + 1, nu_pre, nu_post));
   spmv_real_sell<T>(ne, w.sell_min_rows, k, PS, P, w.u[l + 1], u, 1, nullptr, w.lp, s);
   spmv_real_sell<T>(nn, w.sell_min_rows, k, PnS, Pn, w.phi[l + 1], phi, 1, nullptr, w.lp, s);
@@ -1440,10 +1455,13 @@ static gp_status vcycle_level(const Hierarchy &H, VWork<T> &w, hipStream_t s, in
   }
   k_add<T><<<gn, 256, 0, s>>>(nn * k, phi, d1);
   for (int it = 0; it < nu_post; ++it) {
+    // the first sweep starts from res = r - K u - B phi_old for this u: only
+    // B d1 (the nodal correction just added to phi) is subtracted
+    const int md = it == 0 ? (EdgeMode::JACOBI | EdgeMode::NOK) : EdgeMode::JACOBI;
     tb0();
-    edge_sell<T>(ne, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, phi, tmp,
-                                   EdgeMode::JACOBI, 1, damp, s);
-    te0(EdgeMode::JACOBI, 1);
+    edge_sell<T>(ne, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, it == 0 ? res : r, u,
+                 it == 0 ? d1 : phi, tmp, md, 1, damp, s);
+    te0(md, 1);
     std::swap(u, tmp);
   }
   // results must land in the level's canonical buffers

@@ -306,11 +306,12 @@ __global__ void k_rows_to_cols(int64_t n, int k, const Z2 *__restrict__ a, Z2 *_
   b[(int64_t)j * n + i] = a[t];
 }
 
-// Multi-vector dots.  Block (blk, c) handles vectors t in [8c, 8c+8) of V
+// Multi-vector dots.  Block (blk, c) handles vectors t in [16c, 16c+16) of V
 // (stride vstride) against w over a grid-stride range of rows and writes
 // partial[(blk * nout + t) * k + j] = sum conj(V_t) w; with self, chunk 0 also
-// writes t = nvec: sum |w|^2.  nout = nvec + self.
-constexpr int kDotChunk = 8;
+// writes t = nvec: sum |w|^2.  nout = nvec + self.  Sixteen vectors per chunk
+// cover a whole restart-16 basis, so w is read once per pass.
+constexpr int kDotChunk = 16;
 template <class TV>
 __global__ void __launch_bounds__(256) k_mdots(int64_t n, int k, const C2<TV> *__restrict__ V, int64_t vstride, int nvec,
                                                const Z2 *__restrict__ w, int self, Z2 *__restrict__ partial) {
@@ -353,12 +354,16 @@ __global__ void __launch_bounds__(256) k_mdots(int64_t n, int k, const C2<TV> *_
   }
 }
 
+// h[q] = sum over the nb blocks of partial[b * nout * k + q]: one wavefront
+// per output, lanes over blocks, then a fixed-order shuffle tree.
 __global__ void k_mdots_final(int nout, int nb, int k, const Z2 *__restrict__ partial, Z2 *__restrict__ h) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nout * k) return;
+  const int q = (int)(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
+  const int lane = (int)(threadIdx.x & 63);
+  if (q >= nout * k) return;  // uniform across the wavefront
   Z2 s = {0., 0.};
-  for (int b = 0; b < nb; ++b) { Z2 v = partial[(int64_t)b * nout * k + q]; s.x += v.x; s.y += v.y; }
-  h[q] = s;
+  for (int b = lane; b < nb; b += 64) { const Z2 v = partial[(int64_t)b * nout * k + q]; s.x += v.x; s.y += v.y; }
+  for (int off = 32; off > 0; off >>= 1) { s.x += __shfl_down(s.x, off); s.y += __shfl_down(s.y, off); }
+  if (lane == 0) h[q] = s;
 }
 
 // out = (w - sum_{t<nvec} V_t h_t) * scale_j   (per column; out may be w)
@@ -523,7 +528,7 @@ void outer(const OuterOp &op, const Hierarchy &H, int64_t smin, int k, const dou
 // Column-batched dot products on the device, copied to the host:
 // h[t*k + j] = V_t^H w for t < nvec, and h[nvec*k + j] = |w_j|^2 if self.
 struct Dots {
-  int nb = 512;
+  int nb = 1024;  // 4096 wavefronts: four per SIMD on 256 CUs
   Z2 *partial = nullptr, *hd = nullptr;
   gp_status init(Arena &A, int k, int maxv) {
     GP_TRY(A.alloc(&partial, (int64_t)nb * (maxv + 1) * k));
@@ -536,7 +541,7 @@ struct Dots {
     const int nout = nvec + self;
     const int nch = std::max(1, (nvec + kDotChunk - 1) / kDotChunk);
     k_mdots<TV><<<dim3(nb, nch), 256, 0, s>>>(n, k, V, vstride, nvec, w, self, partial);
-    k_mdots_final<<<(nout * k + 255) / 256, 256, 0, s>>>(nout, nb, k, partial, hd);
+    k_mdots_final<<<(nout * k + 3) / 4, 256, 0, s>>>(nout, nb, k, partial, hd);
     std::vector<Z2> tmp((size_t)nout * k);
     GP_HIP(hipMemcpyAsync(tmp.data(), hd, sizeof(Z2) * nout * k, hipMemcpyDeviceToHost, s));
     GP_HIP(hipStreamSynchronize(s));
