@@ -42,3 +42,44 @@ def test_full_size_reciprocity_and_v0_agreement():
     assert np.abs(Ze - Zf).max() / np.abs(Zf).max() < 1e-4
     assert np.abs(Ze.real - Zf.real).max() / np.abs(Zf.real).max() < 5e-2
     lay.close()
+
+
+@pytest.mark.gpu
+def test_full_size_mixed_matches_fp64_and_independent_residual():
+    """At the benchmark's size: the mixed path (FP32 preconditioner and
+    Krylov basis) and the plain FP64 path give the same Z column to 1e-6 at
+    both ends of the band, and the mixed solution meets the residual bound
+    when re-checked through gp_apply_fullwave, an independent FP64 product
+    (b from the oracle's right-hand side).  That re-check assembles x = u +
+    G phi + Gx a in FP64, whose product has a rounding floor far above 1e-8 at
+    low frequencies (the gradient part dominates there), so it runs at 1 and
+    10 GHz with a 1e-7 bound.  Measured on MI355X: Z within 1.2e-10 / 1.0e-12
+    / 5.0e-12 at 10 MHz / 1 GHz / 10 GHz; re-checked residual 1.2e-8 and
+    9.7e-9."""
+    import oracle
+    from gds2para_amd.layout import Layout
+    m = M.Mesh(M.WORKLOADS["manhattan_200x200x24"]())
+    npt = m.num_ports
+    print("[full] manhattan 200x200x24: setup", flush=True)
+    lay = Layout(m)
+    lay.assemble()
+    lay.build_v0()
+    lay.prepare(capi.GP_PREC_MIXED)
+    X = {}
+    for f in (1e7, 1e9, 1e10):
+        print(f"[full] port 0 at {f:.0e} Hz, mixed and FP64", flush=True)
+        Zm, stm, Xm = lay.solve([f], ports=[0], precision=capi.GP_PREC_MIXED, rtol=1e-8, want_x=True)
+        Zd, std, _ = lay.solve([f], ports=[0], precision=capi.GP_PREC_FP64, rtol=1e-8, batch=1, max_iters=1000)
+        assert stm["status"] == capi.GP_OK and std["status"] == capi.GP_OK
+        zm, zd = Zm[0][:npt], Zd[0][:npt]          # column of source port 0
+        rel = np.abs(zm - zd).max() / np.abs(zd).max()
+        print(f"[full] {f:.0e} Hz: mixed vs FP64 Z {rel:.2e}", flush=True)
+        assert rel < 1e-6
+        X[f] = Xm[0]
+    for f in (1e9, 1e10):
+        b = oracle.rhs(m, f, 0)
+        y = lay.apply_fullwave([f], X[f][None, :])[0]
+        res = np.linalg.norm(y - b) / np.linalg.norm(b)
+        print(f"[full] {f:.0e} Hz: ||A x - b|| / ||b|| via gp_apply_fullwave {res:.2e}", flush=True)
+        assert res < 1e-7
+    lay.close()
