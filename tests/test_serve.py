@@ -220,3 +220,49 @@ def test_serve_gpu_same_alone_and_in_mix():
                     a, b = r.Z[fi, :, p], Za[fi, p, :]
                     assert np.abs(a - b).max() / np.abs(b).max() < 1e-6, (key, r.freqs[fi], p)
         lay.close()
+
+
+@pytest.mark.gpu
+def test_serve_gpu_evicts_under_budget():
+    """Three layouts under a budget that holds one at a time (each layout's
+    estimate is ~94 MB, the budget 110 MB): requests alternate between the
+    layouts, so idle layouts are evicted (their device memory freed while
+    another layout's stream runs) and reopened, requests for a busy layout's
+    rivals wait in the backlog, and every request still finishes with the Z
+    of the same columns solved alone."""
+    from gds2para_amd import mesh as M
+    from gds2para_amd.layout import Layout
+    from gds2para_amd.serve import GpuBackend
+    specs = {"a": M.manhattan_spec((24, 20, 10), n_ports=3, nfreq=3, f_range=(1e8, 1e10), seed=11),
+             "b": M.manhattan_spec((20, 24, 10), n_ports=3, nfreq=3, f_range=(1e8, 1e10), seed=12),
+             "c": M.manhattan_spec((28, 16, 10), n_ports=3, nfreq=3, f_range=(1e8, 1e10), seed=13)}
+
+    class Counting(GpuBackend):
+        opened = []
+
+        def open(self, key):
+            self.opened.append(key)
+            return super().open(key)
+
+    be = Counting(specs, slots=4)
+    keys = ["a", "b", "c", "a", "c", "b", "a", "b"]
+    reqs = [Request(k, [i % 3], [1e9] if i % 2 else [1e8, 1e10]) for i, k in enumerate(keys)]
+    print(f"[serve] {len(reqs)} requests over 3 layouts, budget 110 MB", flush=True)
+    srv = Server(be, budget_bytes=110 << 20)
+    run_stream(srv, reqs, poisson_arrivals(10.0, len(reqs), seed=7))
+    srv.close()
+    print(f"[serve] opened {be.opened}", flush=True)
+    assert len(be.opened) > 3                       # some layout was evicted and reopened
+    for key, spec in specs.items():
+        lay = Layout(M.Mesh(spec))
+        for r in (q for q in reqs if q.layout == key):
+            assert not r.error and r.remaining == 0
+            for rep in r.reports.values():
+                assert rep["converged"] == 1 and rep["rel_residual"] <= 1e-8
+            Z, st, _ = lay.solve(r.freqs, ports=r.ports, precision=capi.GP_PREC_MIXED, rtol=1e-8, batch=4)
+            Za = Z.reshape(len(r.freqs), 3, 3)
+            for fi in range(len(r.freqs)):
+                for p in r.ports:
+                    a, b = r.Z[fi, :, p], Za[fi, p, :]
+                    assert np.abs(a - b).max() / np.abs(b).max() < 1e-6, (key, r.freqs[fi], p)
+        lay.close()
