@@ -417,11 +417,6 @@ __global__ void k_weight_in(int64_t n, int k, const double *__restrict__ W, cons
   out[q] = {(T)(w * (double)v[q].x), (T)(w * (double)v[q].y)};
 }
 
-template <class T>
-__global__ void k_copy_t(int64_t n, const C2<T> *__restrict__ in, C2<T> *__restrict__ out) {
-  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < n) out[q] = in[q];
-}
 
 // v[i*k + j] = 0 for every column j with mask[j]
 __global__ void k_zero_cols(int64_t n, int k, const int32_t *__restrict__ mask, Z2 *__restrict__ v) {
