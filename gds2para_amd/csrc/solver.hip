@@ -264,10 +264,13 @@ __global__ void __launch_bounds__(256) k_amul_sell(SellV<double> SS, int64_t n, 
 }
 
 void amul(const Hierarchy &H, int64_t n, int k, const double *om, const Z2 *x, Z2 *y, hipStream_t s) {
+  // one column: the strip-ordered copy, whose wavefronts read and write whole
+  // lines of the row-indexed vectors
+  const Sell &S = (k == 1 && H.Ss1.nr == H.Ss.nr) ? H.Ss1 : H.Ss;
   SellV<double> SS;
-  SS.nr = H.Ss.nr; SS.nslice = H.Ss.nslice; SS.base = H.Ss.base; SS.w = H.Ss.w; SS.ci = H.Ss.ci; SS.nv = 1;
-  SS.v[0] = H.Ss.v[0];
-  SS.rows = H.Ss.rows;
+  SS.nr = S.nr; SS.nslice = S.nslice; SS.base = S.base; SS.w = S.w; SS.ci = S.ci; SS.nv = 1;
+  SS.v[0] = S.v[0];
+  SS.rows = S.rows;
   const int sp = (k % 2 == 0) ? 2 : 1;  // two complex FP64 slots = two 16-byte loads per lane
   int lpr = 1;
   while (lpr < 16 && (k / sp) % (lpr * 2) == 0) lpr *= 2;
