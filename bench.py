@@ -215,8 +215,8 @@ def main():
         byts = 12.0 * nnz_s + 4.0 * (n_u + 1) + kk * 32.0 * n_u + 16.0 * n_u
         gbs = byts / (ms.value * 1e-3) / 1e9
         spmv[f"k{kk}"] = {"avg_us": ms.value * 1e3, "bytes": byts, "achieved_GBs": gbs, "frac": gbs / 8000.0}
-    spmv["kernel"] = ("k_amul_sell (FP64 complex A(w) X, S in sliced ELL: strip order for one column, "
-                      "16 x 16-cell tile order for 16; average of 50 launches)")
+    spmv["kernel"] = ("k_amul_sell (FP64 complex A(w) X, S in sliced ELL: the reference's row order for one "
+                      "column, 16 x 16-cell tile order for 16; average of 50 launches)")
     # HBM traffic of the dominant kernel, from the committed counter profile of this command
     traffic, traffic_src = None, None
     tp = os.path.join(ROOT, "profiles", "edge0_traffic.json")

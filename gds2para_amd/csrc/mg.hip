@@ -136,36 +136,21 @@ gp_status dense_host(const Csr &M, int which, int64_t nr, int64_t nc, std::vecto
 // while it works through that many rows; in tile order the rows in flight on
 // an XCD are one compact block of the mesh and share their neighbours' lines.
 constexpr int kTile = 16;
-constexpr int kStripX = 8;
 
-// strip = 0: (16 x 16-cell xy tile, plane, ix, iy, direction), the order of
-// the batched kernels.  strip = 1: (strip of 8 cells in x and the whole y
-// range, plane, ix, direction, iy), for single-column products: with one
-// 16-byte value per row, a wavefront's 64 rows are consecutive rows of the
-// reference numbering, so its row-indexed reads and writes are whole lines.
-// Measured at 200x200x24, FP64 A(w) x with one column: 113.6 us in tile
-// order, 107.7 with 4 x 64-cell strips, 108.7 / 107.6 / 105.8 with whole-y
-// strips 4 / 1 / 8 cells wide.
-__global__ void k_tile_keys(int64_t n, const int64_t *__restrict__ map_r, Grid g, int strip, uint64_t *__restrict__ key,
+__global__ void k_tile_keys(int64_t n, const int64_t *__restrict__ map_r, Grid g, uint64_t *__restrict__ key,
                             int32_t *__restrict__ idx) {
   int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   int dir; int64_t iz, ix, iy;
   g.decode(map_r[r], dir, iz, ix, iy);
-  if (strip) {
-    const uint64_t tile = (uint64_t)(ix / kStripX);
-    key[r] = (((tile * (uint64_t)(g.Nz + 1) + (uint64_t)iz) * (uint64_t)(g.Nx + 1) + (uint64_t)ix) * 3u +
-              (uint64_t)dir) * (uint64_t)(g.Ny + 1) + (uint64_t)iy;
-  } else {
-    const uint64_t nty = (uint64_t)(g.Ny / kTile + 1);
-    const uint64_t tile = (uint64_t)(ix / kTile) * nty + (uint64_t)(iy / kTile);
-    key[r] = (((tile * (uint64_t)(g.Nz + 1) + (uint64_t)iz) * (uint64_t)(g.Nx + 1) + (uint64_t)ix) *
-                  (uint64_t)(g.Ny + 1) + (uint64_t)iy) * 3u + (uint64_t)dir;
-  }
+  const uint64_t nty = (uint64_t)(g.Ny / kTile + 1);
+  const uint64_t tile = (uint64_t)(ix / kTile) * nty + (uint64_t)(iy / kTile);
+  key[r] = (((tile * (uint64_t)(g.Nz + 1) + (uint64_t)iz) * (uint64_t)(g.Nx + 1) + (uint64_t)ix) *
+                (uint64_t)(g.Ny + 1) + (uint64_t)iy) * 3u + (uint64_t)dir;
   idx[r] = (int32_t)r;
 }
 
-gp_status level0_order(gp_layout *lay, Arena &A, int32_t **order, hipStream_t s, int strip = 0) {
+gp_status level0_order(gp_layout *lay, Arena &A, int32_t **order, hipStream_t s) {
   const int64_t n = lay->n;
   *order = nullptr;
   Arena tmp;
@@ -173,7 +158,7 @@ gp_status level0_order(gp_layout *lay, Arena &A, int32_t **order, hipStream_t s,
   int32_t *i1;
   GP_TRY(tmp.alloc(&k1, n)); GP_TRY(tmp.alloc(&k2, n)); GP_TRY(tmp.alloc(&i1, n));
   GP_TRY(A.alloc(order, n));
-  k_tile_keys<<<blocks_for(n), 256, 0, s>>>(n, lay->map_edge_r, lay->g, strip, k1, i1);
+  k_tile_keys<<<blocks_for(n), 256, 0, s>>>(n, lay->map_edge_r, lay->g, k1, i1);
   size_t bytes = 0;
   GP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k1, k2, i1, *order, (int)n, 0, 64, s));
   uint8_t *buf;
@@ -389,8 +374,12 @@ gp_status build_hierarchy(gp_layout *lay, Hierarchy *H, bool want_f32, int max_l
     GP_TRY(spgemm_ex(A, Sv, H->L[0].G, &H->SG, s, 1));
     GP_TRY(spgemm_ex(A, Sv, H->aux.Gx, &H->SGx, s, 1));
     GP_TRY(sell_from_csr(A, Sv, &H->Ss, s, "S", H->ord_e));  // S rows are level 0's rows
-    GP_TRY(level0_order(lay, A, &H->ord_s1, s, 1));
-    GP_TRY(sell_from_csr(A, Sv, &H->Ss1, s, "S1", H->ord_s1));
+    // and in the reference's row order for single-column products: there a
+    // wavefront's 64 rows are 64 consecutive rows, so its row-indexed reads
+    // and its 16-byte writes of y are whole lines.  Measured at 200x200x24,
+    // FP64 A(w) x with one column: 113.6 us in tile order, 106-108 in strips
+    // of 8 cells in x with (plane, ix, direction, iy) inside, 105-106 here.
+    GP_TRY(sell_from_csr(A, Sv, &H->Ss1, s, "S1", nullptr));
   }
   H->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return GP_OK;

@@ -135,7 +135,7 @@ struct Hierarchy {
   // A (u + G phi + Gx a) = S u + SG phi + SGx a + D (u + G phi + Gx a)
   Csr SG, SGx;
   Sell Ss;                       // SELL copy of S for the outer operator
-  Sell Ss1;                      // the same in strip order, for single-column products
+  Sell Ss1;                      // the same in the reference's row order, for single-column products
   bool has_f32 = false;
   // level-0 data for the FP64 outer operator and the V-cycle input scaling
   double *W = nullptr;        // edge dual volume
@@ -150,7 +150,6 @@ struct Hierarchy {
   // level-0 slot orders of the SELL copies whose rows are level-0 edges /
   // level-0 free nodes: (16 x 16-cell xy tile, plane, ix, iy[, direction])
   int32_t *ord_e = nullptr, *ord_n = nullptr;
-  int32_t *ord_s1 = nullptr;   // (8-cell x strip, plane, ix, direction, iy)
   // coarsest level, dense on the host (row-major), per block
   int64_t neL = 0, nnL = 0;
   std::vector<double> KL[3], BL[2], NL[2], GTL;

@@ -264,8 +264,8 @@ __global__ void __launch_bounds__(256) k_amul_sell(SellV<double> SS, int64_t n, 
 }
 
 void amul(const Hierarchy &H, int64_t n, int k, const double *om, const Z2 *x, Z2 *y, hipStream_t s) {
-  // one column: the strip-ordered copy, whose wavefronts read and write whole
-  // lines of the row-indexed vectors
+  // one column: the copy in the reference's row order, whose wavefronts read
+  // and write whole lines of the row-indexed vectors
   const Sell &S = (k == 1 && H.Ss1.nr == H.Ss.nr) ? H.Ss1 : H.Ss;
   SellV<double> SS;
   SS.nr = S.nr; SS.nslice = S.nslice; SS.base = S.base; SS.w = S.w; SS.ci = S.ci; SS.nv = 1;
