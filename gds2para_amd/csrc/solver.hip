@@ -421,6 +421,14 @@ __global__ void k_weight_in(int64_t n, int k, const double *__restrict__ W, cons
 }
 
 
+// dst[i*k + j] = src[i*k + j] for every column j with mask[j]
+__global__ void k_copy_cols(int64_t n, int k, const int32_t *__restrict__ mask, const Z2 *__restrict__ src,
+                            Z2 *__restrict__ dst) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n * k) return;
+  if (mask[q % k]) dst[q] = src[q];
+}
+
 // v[i*k + j] = 0 for every column j with mask[j]
 __global__ void k_zero_cols(int64_t n, int k, const int32_t *__restrict__ mask, Z2 *__restrict__ v) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -795,8 +803,10 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
     }
     if (anyfill) {
       GP_TRY(refill(fill, false));
-      k_zero_cols<<<gnk, 256, 0, s>>>(n, k, mask_d, w);
-      outer<double>(op, H, vw.sell_min_rows, k, om_d, xu, xphi, xa, b, w, 1, 1, s);
+      // the refilled columns start from x = 0, so their residual b - A x is b
+      // exactly; the other columns keep the residual formed above (their x
+      // and b are unchanged), so no second FP64 operator pass is needed
+      k_copy_cols<<<gnk, 256, 0, s>>>(n, k, mask_d, b, w);
       GP_TRY(dots.run(s, n, k, V, 0, 0, w, 1, h));
       nactive = 0;
       for (int j = 0; j < k; ++j) {
