@@ -369,10 +369,13 @@ __global__ void k_mdots_final(int nout, int nb, int k, const Z2 *__restrict__ pa
   if (lane == 0) h[q] = s;
 }
 
-// out = (w - sum_{t<nvec} V_t h_t) * scale_j   (per column; out may be w)
-template <class TV, class TO>
+// out = (w - sum_{t<nvec} V_t h_t) * scale_j   (per column; out may be w);
+// with rout, also rout = W out (the next preconditioner input, from the
+// stored value of out)
+template <class TV, class TO, class TR>
 __global__ void k_gs_update(int64_t n, int k, int nvec, const C2<TV> *__restrict__ V, int64_t vstride,
-                            const Z2 *__restrict__ h, const double *__restrict__ scale, const Z2 *w, C2<TO> *out) {
+                            const Z2 *__restrict__ h, const double *__restrict__ scale, const Z2 *w, C2<TO> *out,
+                            const double *__restrict__ W, C2<TR> *__restrict__ rout) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n * k) return;
   int j = (int)(q % k);
@@ -384,7 +387,12 @@ __global__ void k_gs_update(int64_t n, int k, int nvec, const C2<TV> *__restrict
     acc.y -= (double)a.x * c.y + (double)a.y * c.x;
   }
   const double sc = scale[j];
-  out[q] = {(TO)(acc.x * sc), (TO)(acc.y * sc)};
+  const C2<TO> o = {(TO)(acc.x * sc), (TO)(acc.y * sc)};
+  out[q] = o;
+  if (rout) {
+    const double wt = W[q / k];
+    rout[q] = {(TR)(wt * (double)o.x), (TR)(wt * (double)o.y)};
+  }
 }
 
 // x += sum_t Z_t y_t (per column)
@@ -404,21 +412,19 @@ __global__ void k_xupdate(int64_t n, int k, int nvec, const C2<TZ> *__restrict__
   x[q] = acc;
 }
 
-template <class TO>
-__global__ void k_scale_cols(int64_t n, int k, const Z2 *__restrict__ w, const double *__restrict__ inv, C2<TO> *__restrict__ out) {
+// out = w * inv_j per column, and rout = W out (the first preconditioner input)
+template <class TO, class TR>
+__global__ void k_scale_cols(int64_t n, int k, const Z2 *__restrict__ w, const double *__restrict__ inv,
+                             C2<TO> *__restrict__ out, const double *__restrict__ W, C2<TR> *__restrict__ rout) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n * k) return;
   const double s = inv[q % k];
-  out[q] = {(TO)(w[q].x * s), (TO)(w[q].y * s)};
+  const C2<TO> o = {(TO)(w[q].x * s), (TO)(w[q].y * s)};
+  out[q] = o;
+  const double wt = W[q / k];
+  rout[q] = {(TR)(wt * (double)o.x), (TR)(wt * (double)o.y)};
 }
 
-template <class T, class TV>
-__global__ void k_weight_in(int64_t n, int k, const double *__restrict__ W, const C2<TV> *__restrict__ v, C2<T> *__restrict__ out) {
-  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n * k) return;
-  const double w = W[q / k];
-  out[q] = {(T)(w * (double)v[q].x), (T)(w * (double)v[q].y)};
-}
 
 
 // dst[i*k + j] = src[i*k + j] for every column j with mask[j]
@@ -818,14 +824,15 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
     ++cycles;
     for (int j = 0; j < k; ++j) sch[j] = (cid[j] >= 0 && beta[j] > 0) ? 1.0 / beta[j] : 0.0;
     GP_HIP(hipMemcpyAsync(sc_d, sch.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-    k_scale_cols<TV><<<gnk, 256, 0, s>>>(n, k, w, sc_d, V);
+    k_scale_cols<TV, T><<<gnk, 256, 0, s>>>(n, k, w, sc_d, V, H.W, vw.r[0]);
     std::fill(g.begin(), g.end(), cd(0));
     for (int j = 0; j < k; ++j) g[j] = (cid[j] >= 0) ? beta[j] : 0.;
     int used = 0;
     for (int i = 0; i < m; ++i) {
-      C2<TV> *Vi = V + (int64_t)i * n * k, *Vn = V + (int64_t)(i + 1) * n * k;
-      // preconditioner: (u, phi, a) = M(W v)
-      k_weight_in<T, TV><<<gnk, 256, 0, s>>>(n, k, H.W, Vi, vw.r[0]);
+      C2<TV> *Vn = V + (int64_t)(i + 1) * n * k;
+      // preconditioner: (u, phi, a) = M(W v_i).  vw.r[0] already holds W v_i,
+      // written with v_i by k_scale_cols (i = 0) or by the previous step's
+      // k_gs_update
       // the preconditioner writes its level-0 outputs straight into this
       // step's Z blocks (same n*k interleaved layout), no copies
       C2<T> *zu = Zu + (int64_t)i * n * k, *zp = Zp + (int64_t)i * nn * k, *za = Za + (int64_t)i * na * k;
@@ -852,7 +859,7 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
       if (reorth) {
         std::fill(sch.begin(), sch.end(), 1.0);
         GP_HIP(hipMemcpyAsync(sc_d, sch.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-        k_gs_update<TV, double><<<gnk, 256, 0, s>>>(n, k, i + 1, V, n * k, dots.hd, sc_d, w, w);
+        k_gs_update<TV, double, T><<<gnk, 256, 0, s>>>(n, k, i + 1, V, n * k, dots.hd, sc_d, w, w, nullptr, nullptr);
         GP_TRY(dots.run(s, n, k, V, n * k, i + 1, w, 1, h2));
         for (int j = 0; j < k; ++j) {
           double hh = 0.;
@@ -895,7 +902,7 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
       // V_{i+1} = (w - V h) / |w - V h|, with the coefficients of the last pass
       // (dots.hd holds them: the second pass overwrote the first when it ran)
       GP_HIP(hipMemcpyAsync(sc_d, sch.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-      k_gs_update<TV, TV><<<gnk, 256, 0, s>>>(n, k, i + 1, V, n * k, dots.hd, sc_d, w, Vn);
+      k_gs_update<TV, TV, T><<<gnk, 256, 0, s>>>(n, k, i + 1, V, n * k, dots.hd, sc_d, w, Vn, H.W, vw.r[0]);
       used = i + 1;
       if (all_in) break;
       // with columns waiting, restart once half of the slots are done
