@@ -921,12 +921,23 @@ __global__ void __launch_bounds__(256) k_spmv_real_sell(int64_t nr, int k, SellV
   store_slots<T, KS>(y + i * k + j0, acc);
 }
 
+// Optional epilogue of k_spmv2_sell over level-0 edge rows: u = damp y / d_l1,
+// the edge smoother's first sweep from zero, formed from y while it is in
+// registers instead of by a separate pass over y.
+template <class T>
+struct Jac0 {
+  const T *kd0 = nullptr, *kd1 = nullptr, *kd2 = nullptr, *kl0 = nullptr, *kl1 = nullptr, *kl2 = nullptr;
+  C2<T> *u = nullptr;
+  T damp = (T)1;
+};
+
 // SELL M2(w) = c1 v0 + i c2 v1: y = r - M2 x (mode 0), y -= M2 x (1), y = M2 x (2)
 template <class T, int SPL, int LPR, int ES>
 __global__ void __launch_bounds__(256) k_spmv2_sell(int64_t nr, int k, SellV<T> M, const T *__restrict__ c1v,
                                                     const T *__restrict__ c2v, const C2<T> *__restrict__ r,
                                                     const C2<T> *__restrict__ x, C2<T> *__restrict__ y, int mode,
-                                                    const int32_t *__restrict__ lidx, const C2<T> *__restrict__ lp) {
+                                                    const int32_t *__restrict__ lidx, const C2<T> *__restrict__ lp,
+                                                    Jac0<T> jz) {
   constexpr int KS = SPL;  // LPR adjacent lanes share a row, SPL slots each;
                            // ES such groups split the row's entries
   const int64_t t_ = xcd_block_id() * blockDim.x + threadIdx.x;
@@ -975,6 +986,17 @@ __global__ void __launch_bounds__(256) k_spmv2_sell(int64_t nr, int k, SellV<T> 
     else o[q] = {o[q].x - acc[q].x, o[q].y - acc[q].y};
   }
   store_slots<T, KS>(y + i * k + j0, o);
+  if (jz.u) {
+    const T d0 = jz.kd0[i], d1 = jz.kd1[i], d2 = jz.kd2[i], l0 = jz.kl0[i], l1 = jz.kl1[i], l2 = jz.kl2[i];
+    C2<T> uo[KS];
+#pragma unroll
+    for (int q = 0; q < KS; ++q) {
+      const C2<T> dq = l1diag<T>({d0 + c1[q] * d1, c2[q] * d2}, l0 + fabs(c1[q]) * l1 + c2[q] * l2);
+      const C2<T> z = cdiv(o[q], dq);
+      uo[q] = {jz.damp * z.x, jz.damp * z.y};
+    }
+    store_slots<T, KS>(jz.u + i * k + j0, uo);
+  }
 }
 
 // Nodal l1-Jacobi on N(w) = -w^2 N_eps + i w N_sig:
@@ -1338,15 +1360,18 @@ void spmv_real_sell(int64_t nr, int64_t smin, int k, const SellV<T> &M, const Ma
   GP_LANES_ES_DISPATCH(k_spmv_real_sell, M, nr, k, M, x, y, accumulate, lidx, lp);
 }
 
+// Returns whether the optional epilogue jz ran (sliced-ELL path only).
 template <class T>
-void spmv2_sell(int64_t nr, int64_t smin, int k, const SellV<T> &M, const MatV<T> &Mc, const T *c1, const T *c2, const C2<T> *r,
-                const C2<T> *x, C2<T> *y, int mode, const int32_t *lidx, const C2<T> *lp, hipStream_t s) {
-  if (nr == 0) return;
+bool spmv2_sell(int64_t nr, int64_t smin, int k, const SellV<T> &M, const MatV<T> &Mc, const T *c1, const T *c2, const C2<T> *r,
+                const C2<T> *x, C2<T> *y, int mode, const int32_t *lidx, const C2<T> *lp, hipStream_t s,
+                const Jac0<T> &jz = Jac0<T>()) {
+  if (nr == 0) return false;
   if (nr < smin) {
     k_spmv2<T><<<blocks_for(nr * k), 256, 0, s>>>(nr, k, Mc, c1, c2, r, x, y, mode, lp);
-    return;
+    return false;
   }
-  GP_LANES_ES_DISPATCH(k_spmv2_sell, M, nr, k, M, c1, c2, r, x, y, mode, lidx, lp);
+  GP_LANES_ES_DISPATCH(k_spmv2_sell, M, nr, k, M, c1, c2, r, x, y, mode, lidx, lp, jz);
+  return jz.u != nullptr;
 }
 
 template <class T> const T *arr(double *d, float *f);
@@ -1412,11 +1437,16 @@ static gp_status vcycle_level(const Hierarchy &H, VWork<T> &w, hipStream_t s, in
   // the buffer that makes the last sweep write the level's own u (no copy)
   if (((nu_pre - 1) + nu_post) & 1) std::swap(u, tmp);
   if ((nu_pre - 1) & 1) std::swap(phi, d2);
-  // pre-smoothing on the edges (phi = 0)
-  tb0();
-  edge_sell<T>(ne, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, nullptr, nullptr, u,
-                               EdgeMode::JACOBI0, 0, damp, s);
-  te0(EdgeMode::JACOBI0, 0);
+  // pre-smoothing on the edges (phi = 0); at level 0 the first sweep may have
+  // come out of the pass that formed r (precond, Jac0)
+  if (l == 0 && w.u0_ready) {
+    w.u0_ready = false;
+  } else {
+    tb0();
+    edge_sell<T>(ne, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, nullptr, nullptr, u,
+                 EdgeMode::JACOBI0, 0, damp, s);
+    te0(EdgeMode::JACOBI0, 0);
+  }
   for (int it = 1; it < nu_pre; ++it) {
     tb0();
     edge_sell<T>(ne, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, nullptr, tmp,
@@ -1551,7 +1581,15 @@ gp_status precond(const Hierarchy &H, VWork<T> &w, hipStream_t s, int nu_pre, in
     w.ax[0] = ax0;
     GP_TRY(ast);
   }
-  spmv2_sell<T>(n, w.sell_min_rows, k, BxS, Bx, w.c1, w.c2, nullptr, w.a, w.r[0], 1, nullptr, w.lp, s);
+  // the edge V-cycle's first level-0 sweep (u = r / d_l1 from zero) comes out
+  // of the same pass, into the buffer that sweep would write
+  Jac0<T> jz;
+  if (H.L.size() > 1) {
+    jz.kd0 = arr<T>(L0.kd[0], L0.kdf[0]); jz.kd1 = arr<T>(L0.kd[1], L0.kdf[1]); jz.kd2 = arr<T>(L0.kd[2], L0.kdf[2]);
+    jz.kl0 = arr<T>(L0.kl[0], L0.klf[0]); jz.kl1 = arr<T>(L0.kl[1], L0.klf[1]); jz.kl2 = arr<T>(L0.kl[2], L0.klf[2]);
+    jz.u = (((nu_pre - 1) + nu_post) & 1) ? w.tmp[0] : w.u[0];
+  }
+  w.u0_ready = spmv2_sell<T>(n, w.sell_min_rows, k, BxS, Bx, w.c1, w.c2, nullptr, w.a, w.r[0], 1, nullptr, w.lp, s, jz);
   // edge V-cycle on the remaining residual
   GP_TRY(vcycle<T>(H, w, s, nu_pre, nu_post));
   // The second auxiliary correction (on r0 - K u - B phi - Bx a) makes the

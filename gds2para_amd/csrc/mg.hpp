@@ -196,6 +196,7 @@ struct VWork {
   C2<T> *lpart = nullptr;                   // per-chunk partial products [nchunk][nv][k]
   int nu_aux = 2;
   bool aux_post = false;  // auxiliary correction after the edge V-cycle too (symmetric)
+  bool u0_ready = false;  // level 0's first edge sweep already written (set by precond)
   int64_t sell_min_rows = 1;  // operators with fewer rows use the (row, slot) CSR kernels
   KTimer *kt = nullptr;               // times the level-0 edge smoother launches
   Arena arena;
