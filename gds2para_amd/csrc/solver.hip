@@ -16,6 +16,7 @@
 // rounding of S and are formed exactly, which is what lets a 1e-8 relative
 // residual be met at low frequency where the gradient part of x dominates.
 #include <algorithm>
+#include <type_traits>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -73,10 +74,10 @@ struct OuterOp {
 // out = S u [+ SG phi + SGx a] + D(w)(u + G phi + Gx a)   (mode 0)   or b - (...)   (mode 1)
 // D(w) = w * complex(-w * eps_r * EPSILON0, sigma) with the real part rounded as
 // above (src/generateStiff.cpp:620-626).  exact = 1 adds the S G / S Gx terms.
-template <class TZ>
+template <class TZ, class TO>
 __global__ void k_outer(OuterOp op, int k, const double *__restrict__ omega, const C2<TZ> *__restrict__ u,
                         const C2<TZ> *__restrict__ phi, const C2<TZ> *__restrict__ a, const Z2 *__restrict__ b,
-                        Z2 *__restrict__ out, int mode, int exact) {
+                        C2<TO> *__restrict__ out, int mode, int exact) {
   int64_t t = xcd_block_id() * blockDim.x + threadIdx.x;
   int64_t i = t / k;
   int j = (int)(t - i * k);
@@ -118,7 +119,7 @@ __global__ void k_outer(OuterOp op, int k, const double *__restrict__ omega, con
   acc.x += dr * xi.x - di * xi.y;
   acc.y += dr * xi.y + di * xi.x;
   if (mode == 1) { const Z2 bb = b[i * k + j]; acc.x = bb.x - acc.x; acc.y = bb.y - acc.y; }
-  out[i * k + j] = acc;
+  out[i * k + j] = {(TO)acc.x, (TO)acc.y};
 }
 
 template <class TZ, int KS, int NB>
@@ -145,11 +146,11 @@ __device__ __forceinline__ void outer_batch(const SellV<double> &SS, int64_t b0,
 // k_outer on the SELL copy of S: LPR adjacent lanes share a row, each with
 // SPL slots (one 16-byte load of FP32 u per gathered row), S read once per
 // lane group; G, Gx, S G, S Gx rows are short CSR walks.
-template <class TZ, int SPL, int LPR>
+template <class TZ, class TO, int SPL, int LPR>
 __global__ void __launch_bounds__(256) k_outer_sell(OuterOp op, SellV<double> SS, int k, const double *__restrict__ omega,
                                                     const C2<TZ> *__restrict__ u, const C2<TZ> *__restrict__ phi,
                                                     const C2<TZ> *__restrict__ a, const Z2 *__restrict__ b,
-                                                    Z2 *__restrict__ out, int mode, int exact) {
+                                                    C2<TO> *__restrict__ out, int mode, int exact) {
   constexpr int KS = SPL;
   const int64_t t_ = xcd_block_id() * blockDim.x + threadIdx.x;
   const int64_t pos = t_ / LPR;
@@ -217,7 +218,10 @@ __global__ void __launch_bounds__(256) k_outer_sell(OuterOp op, SellV<double> SS
     acc[q].y += dr * xi[q].y + di * xi[q].x;
     if (mode == 1) acc[q] = {bb[q].x - acc[q].x, bb[q].y - acc[q].y};
   }
-  store_slots<double, KS>(out + i * k + j0, acc);
+  C2<TO> o[KS];
+#pragma unroll
+  for (int q = 0; q < KS; ++q) o[q] = {(TO)acc[q].x, (TO)acc[q].y};
+  store_slots<TO, KS>(out + i * k + j0, o);
 }
 
 // y = A(w) x: the complex matrix reference() assembles, S + diag(w (-w eps_r
@@ -315,9 +319,9 @@ __global__ void k_rows_to_cols(int64_t n, int k, const Z2 *__restrict__ a, Z2 *_
 // writes t = nvec: sum |w|^2.  nout = nvec + self.  Sixteen vectors per chunk
 // cover a whole restart-16 basis, so w is read once per pass.
 constexpr int kDotChunk = 16;
-template <class TV>
+template <class TV, class TW>
 __global__ void __launch_bounds__(256) k_mdots(int64_t n, int k, const C2<TV> *__restrict__ V, int64_t vstride, int nvec,
-                                               const Z2 *__restrict__ w, int self, Z2 *__restrict__ partial) {
+                                               const C2<TW> *__restrict__ w, int self, Z2 *__restrict__ partial) {
   __shared__ Z2 sh[256];
   const int c = blockIdx.y;
   const int t0 = c * kDotChunk;
@@ -330,7 +334,8 @@ __global__ void __launch_bounds__(256) k_mdots(int64_t n, int k, const C2<TV> *_
   for (int q = 0; q <= kDotChunk; ++q) acc[q] = {0., 0.};
   const int64_t step = (int64_t)gridDim.x * rows_per;
   for (int64_t i = (int64_t)blockIdx.x * rows_per + threadIdx.x / k; i < n; i += step) {
-    const Z2 b = w[i * k + j];
+    const C2<TW> wv = w[i * k + j];
+    const Z2 b = {(double)wv.x, (double)wv.y};
 #pragma unroll
     for (int q = 0; q < kDotChunk; ++q)
       if (q < nt) {
@@ -372,14 +377,15 @@ __global__ void k_mdots_final(int nout, int nb, int k, const Z2 *__restrict__ pa
 // out = (w - sum_{t<nvec} V_t h_t) * scale_j   (per column; out may be w);
 // with rout, also rout = W out (the next preconditioner input, from the
 // stored value of out)
-template <class TV, class TO, class TR>
+template <class TV, class TW, class TO, class TR>
 __global__ void k_gs_update(int64_t n, int k, int nvec, const C2<TV> *__restrict__ V, int64_t vstride,
-                            const Z2 *__restrict__ h, const double *__restrict__ scale, const Z2 *w, C2<TO> *out,
+                            const Z2 *__restrict__ h, const double *__restrict__ scale, const C2<TW> *w, C2<TO> *out,
                             const double *__restrict__ W, C2<TR> *__restrict__ rout) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n * k) return;
   int j = (int)(q % k);
-  Z2 acc = w[q];
+  const C2<TW> w0 = w[q];
+  Z2 acc = {(double)w0.x, (double)w0.y};
   for (int t = 0; t < nvec; ++t) {
     const C2<TV> a = V[(int64_t)t * vstride + q];
     const Z2 c = h[t * k + j];
@@ -497,12 +503,13 @@ OuterOp outer_op(gp_layout *lay, const Hierarchy &H) {
 }
 
 // Outer operator launch: the SELL kernel when S is large (sell_min_rows), else
-// the (row, slot) CSR kernel.
-template <class TZ>
+// the (row, slot) CSR kernel.  out in TO: FP64 for residuals, the basis
+// precision for the Arnoldi vector w = A z.
+template <class TZ, class TO = double>
 void outer(const OuterOp &op, const Hierarchy &H, int64_t smin, int k, const double *om, const C2<TZ> *u,
-           const C2<TZ> *phi, const C2<TZ> *a, const Z2 *b, Z2 *out, int mode, int exact, hipStream_t s) {
+           const C2<TZ> *phi, const C2<TZ> *a, const Z2 *b, C2<TO> *out, int mode, int exact, hipStream_t s) {
   if (op.n < smin || H.Ss.nr != op.n) {
-    k_outer<TZ><<<blocks_for(op.n * k), 256, 0, s>>>(op, k, om, u, phi, a, b, out, mode, exact);
+    k_outer<TZ, TO><<<blocks_for(op.n * k), 256, 0, s>>>(op, k, om, u, phi, a, b, out, mode, exact);
     return;
   }
   SellV<double> SS;
@@ -514,7 +521,7 @@ void outer(const OuterOp &op, const Hierarchy &H, int64_t smin, int k, const dou
   while (lpr < 16 && (k / sp) % (lpr * 2) == 0) lpr *= 2;
   const dim3 grid((unsigned)((op.n * lpr + 255) / 256), (unsigned)(k / (sp * lpr)));
 #define GP_OUTER_SELL(SP, LP) \
-  k_outer_sell<TZ, SP, LP><<<grid, 256, 0, s>>>(op, SS, k, om, u, phi, a, b, out, mode, exact)
+  k_outer_sell<TZ, TO, SP, LP><<<grid, 256, 0, s>>>(op, SS, k, om, u, phi, a, b, out, mode, exact)
   if (sp == 4) {
     if (lpr == 16) GP_OUTER_SELL(4, 16);
     else if (lpr == 8) GP_OUTER_SELL(4, 8);
@@ -547,12 +554,12 @@ struct Dots {
     GP_TRY(A.alloc(&hd, (int64_t)(maxv + 1) * k));
     return GP_OK;
   }
-  template <class TV>
-  gp_status run(hipStream_t s, int64_t n, int k, const C2<TV> *V, int64_t vstride, int nvec, const Z2 *w, int self,
+  template <class TV, class TW>
+  gp_status run(hipStream_t s, int64_t n, int k, const C2<TV> *V, int64_t vstride, int nvec, const C2<TW> *w, int self,
                 std::vector<cd> &h) {
     const int nout = nvec + self;
     const int nch = std::max(1, (nvec + kDotChunk - 1) / kDotChunk);
-    k_mdots<TV><<<dim3(nb, nch), 256, 0, s>>>(n, k, V, vstride, nvec, w, self, partial);
+    k_mdots<TV, TW><<<dim3(nb, nch), 256, 0, s>>>(n, k, V, vstride, nvec, w, self, partial);
     k_mdots_final<<<(nout * k + 3) / 4, 256, 0, s>>>(nout, nb, k, partial, hd);
     std::vector<Z2> tmp((size_t)nout * k);
     GP_HIP(hipMemcpyAsync(tmp.data(), hd, sizeof(Z2) * nout * k, hipMemcpyDeviceToHost, s));
@@ -642,6 +649,11 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
   int32_t *mask_d;
   GP_TRY(A.alloc(&b, n * k)); GP_TRY(A.alloc(&xu, n * k)); GP_TRY(A.alloc(&xphi, nn * k)); GP_TRY(A.alloc(&xa, na * k));
   GP_TRY(A.alloc(&V, (int64_t)(m + 1) * n * k)); GP_TRY(A.alloc(&w, n * k));
+  // the Arnoldi vector w = A z is stored in the basis precision (its products
+  // and sums are FP64); w itself holds the FP64 restart residuals
+  C2<TV> *wa;
+  if constexpr (std::is_same<TV, double>::value) wa = w;
+  else GP_TRY(A.alloc(&wa, n * k));
   GP_TRY(A.alloc(&Zu, (int64_t)m * n * k)); GP_TRY(A.alloc(&Zp, (int64_t)m * nn * k)); GP_TRY(A.alloc(&Za, (int64_t)m * na * k));
   GP_TRY(A.alloc(&om_d, k)); GP_TRY(A.alloc(&sc_d, k)); GP_TRY(A.alloc(&mask_d, k)); GP_TRY(A.alloc(&yd, (int64_t)m * k));
   Dots dots;
@@ -841,11 +853,11 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
       const gp_status pst = precond<T>(H, vw, s, nu_edge, nu_edge);
       vw.u[0] = own_u; vw.phi[0] = own_phi; vw.a = own_a;
       GP_TRY(pst);
-      // w = A z (split form, FP64)
-      outer<T>(op, H, vw.sell_min_rows, k, om_d, zu, zp, za, nullptr, w, 0, 0, s);
+      // w = A z (split form, FP64 products, stored in the basis precision)
+      outer<T, TV>(op, H, vw.sell_min_rows, k, om_d, zu, zp, za, nullptr, wa, 0, 0, s);
       // classical Gram-Schmidt with the norm from the same pass; a second pass
       // only when the first one cancelled all but kReorth of |w|^2 in a slot
-      GP_TRY(dots.run(s, n, k, V, n * k, i + 1, w, 1, h));
+      GP_TRY(dots.run(s, n, k, V, n * k, i + 1, wa, 1, h));
       bool reorth = false;
       std::vector<double> rn2(k);
       for (int j = 0; j < k; ++j) {
@@ -859,8 +871,8 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
       if (reorth) {
         std::fill(sch.begin(), sch.end(), 1.0);
         GP_HIP(hipMemcpyAsync(sc_d, sch.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-        k_gs_update<TV, double, T><<<gnk, 256, 0, s>>>(n, k, i + 1, V, n * k, dots.hd, sc_d, w, w, nullptr, nullptr);
-        GP_TRY(dots.run(s, n, k, V, n * k, i + 1, w, 1, h2));
+        k_gs_update<TV, TV, TV, T><<<gnk, 256, 0, s>>>(n, k, i + 1, V, n * k, dots.hd, sc_d, wa, wa, nullptr, nullptr);
+        GP_TRY(dots.run(s, n, k, V, n * k, i + 1, wa, 1, h2));
         for (int j = 0; j < k; ++j) {
           double hh = 0.;
           for (int t = 0; t <= i; ++t) hh += std::norm(h2[t * k + j]);
@@ -902,7 +914,7 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
       // V_{i+1} = (w - V h) / |w - V h|, with the coefficients of the last pass
       // (dots.hd holds them: the second pass overwrote the first when it ran)
       GP_HIP(hipMemcpyAsync(sc_d, sch.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-      k_gs_update<TV, TV, T><<<gnk, 256, 0, s>>>(n, k, i + 1, V, n * k, dots.hd, sc_d, w, Vn, H.W, vw.r[0]);
+      k_gs_update<TV, TV, TV, T><<<gnk, 256, 0, s>>>(n, k, i + 1, V, n * k, dots.hd, sc_d, wa, Vn, H.W, vw.r[0]);
       used = i + 1;
       if (all_in) break;
       // with columns waiting, restart once half of the slots are done
@@ -1045,6 +1057,7 @@ int64_t bytes_per_slot(const Hierarchy &H, int m, bool f32, bool v32) {
   const int64_t n = H.L[0].ne, nn = H.L[0].nn, na = H.aux.naux;
   const int64_t tz = f32 ? 8 : 16, tv = v32 ? 8 : 16;
   int64_t b = (int64_t)(m + 1) * n * tv + (int64_t)m * (n + nn + na) * tz + (n * 3 + nn + na) * 16;
+  if (v32) b += n * tv;  // the Arnoldi vector in the basis precision
   for (const MgLevel &L : H.L) b += (5 * L.ne + 3 * L.nn) * tz;
   for (const AuxLevel &L : H.aux.L) b += 3 * L.n * tz;
   return b + (n + na) * tz;

@@ -212,11 +212,12 @@ typedef struct gp_solve_opts {
   int32_t aux_post;               /* 1: auxiliary correction after the edge V-cycle
                                      as well (symmetric preconditioner); 0: before
                                      it only (the default, faster) */
-  int32_t basis;                  /* Krylov basis storage with MIXED: 0 = FP32
-                                     (the default; residuals, products, dot sums
-                                     and the iterate stay FP64, every restart
-                                     starts from the FP64 residual), 1 = FP64.
-                                     FP64 precision always keeps an FP64 basis */
+  int32_t basis;                  /* Krylov basis (and Arnoldi vector) storage
+                                     with MIXED: 0 = FP32 (the default;
+                                     residuals, products, dot sums and the
+                                     iterate stay FP64, every restart starts
+                                     from the FP64 residual), 1 = FP64.  FP64
+                                     precision always keeps an FP64 basis */
 } gp_solve_opts;
 
 typedef struct gp_solve_stats {
