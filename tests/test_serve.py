@@ -6,6 +6,7 @@ GPU test streams overlapping requests over two layouts through the library and
 checks every request's Z against the same request solved alone: within 1e-6
 relative (the columns restart at different points in the mix, so they agree
 to the solve tolerance rather than bit for bit)."""
+import ctypes as C
 import threading
 import time
 
@@ -266,3 +267,53 @@ def test_serve_gpu_evicts_under_budget():
                     a, b = r.Z[fi, :, p], Za[fi, p, :]
                     assert np.abs(a - b).max() / np.abs(b).max() < 1e-6, (key, r.freqs[fi], p)
         lay.close()
+
+
+@pytest.mark.gpu
+def test_stream_rejects_malformed_columns():
+    """gp_solve_stream hands a column whose port is out of range or whose
+    frequency is not positive straight back as rejected (no Z, negative
+    residual in its report) and solves the good columns around it."""
+    from gds2para_amd import mesh as M
+    from gds2para_amd.layout import Layout
+    m = M.Mesh(M.manhattan_spec((24, 20, 10), n_ports=3, nfreq=3, f_range=(1e8, 1e10), seed=11))
+    lay = Layout(m)
+    lay.assemble()
+    lay.prepare(capi.GP_PREC_MIXED)
+    cols = [(0, 1e9, 0), (7, 1e9, 1), (1, -5.0, 2), (2, 1e9, 3)]
+    queue = list(cols)
+    got = {}
+
+    def pull(user, colp, block):
+        if not queue:
+            return -1
+        p, f, tag = queue.pop(0)
+        colp.contents.port, colp.contents.freq, colp.contents.tag = p, f, tag
+        return 1
+
+    def waiting(user):
+        return len(queue)
+
+    def push(user, colp, zp, repp):
+        r = repp.contents
+        z = None if not zp else [zp[2 * q] + 1j * zp[2 * q + 1] for q in range(m.num_ports)]
+        got[colp.contents.tag] = (z, r.converged, r.rel_residual)
+
+    cb = (capi.PULL_FN(pull), capi.WAITING_FN(waiting), capi.PUSH_FN(push))
+    o = capi.GpSolveOpts()
+    lay.lib.gp_solve_opts_default(C.byref(o))
+    st = capi.GpSolveStats()
+    print("[serve] stream with two malformed columns", flush=True)
+    capi.check(lay.lib.gp_solve_stream(lay.h, C.byref(o), 4, cb[0], cb[1], cb[2], None, C.byref(st)),
+               "gp_solve_stream")
+    assert sorted(got) == [0, 1, 2, 3]
+    for tag in (1, 2):
+        z, conv, rr = got[tag]
+        assert z is None and conv == 0 and rr < 0
+    Z, _, _ = lay.solve([1e9], ports=[0, 2], precision=capi.GP_PREC_MIXED, rtol=1e-8, batch=4)
+    Zr = Z[0].reshape(3, 3)
+    for tag, p in ((0, 0), (3, 2)):
+        z, conv, rr = got[tag]
+        assert conv == 1 and rr <= 1e-8
+        assert np.abs(np.array(z) - Zr[p]).max() / np.abs(Zr[p]).max() < 1e-6
+    lay.close()
