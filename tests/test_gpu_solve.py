@@ -81,3 +81,22 @@ def test_manhattan_smoothing_sweeps(manh, prec, sweeps):
     for k in range(len(fs)):
         assert _relerr(Z[k], Zr[k]) < ZTOL, (k, _relerr(Z[k], Zr[k]))
     lay.close()
+
+
+def test_iteration_cap_reports_nonconvergence(manh):
+    """A column cap far below what the solve needs: the call returns
+    GP_ERR_NOCONV, every column's report says not converged with its true
+    residual above the tolerance, and the same layout then solves normally."""
+    m, fs, Xr, Zr = manh
+    print("[solve] manhattan with max_iters 3", flush=True)
+    lay = Layout(m)
+    Z, st, _ = lay.solve(fs, precision=capi.GP_PREC_MIXED, rtol=1e-8, batch=4, max_iters=3)
+    assert st["status"] == capi.GP_ERR_NOCONV
+    assert st["converged"] < st["columns"] == len(fs) * m.num_ports
+    assert st["max_iters"] <= 3 + 16          # stops at the first restart past the cap
+    assert max(st["rel_residual"]) > 1e-8
+    Z, st, _ = lay.solve(fs, precision=capi.GP_PREC_MIXED, rtol=1e-8, batch=4)
+    assert st["status"] == capi.GP_OK
+    for k in range(len(fs)):
+        assert _relerr(Z[k], Zr[k]) < ZTOL
+    lay.close()
