@@ -2,6 +2,8 @@
 //   Parasitics::convertParam        src/solnoutclass.hpp:1456-1600  (Z / Y / S)
 //   SolverDataBase::dumpCITI        src/solnoutclass.hpp:6040-6100
 //   SolverDataBase::dumpTouchstone  src/solnoutclass.hpp:6104-6215
+//   SolverDataBase::dumpXyce        src/solnoutclass.hpp:6032-6036 (via
+//     Parasitics::computeYBusFromParam :1680-1766 and toXyce :2094-2248)
 // Matrices are nf blocks of np x np complex values (interleaved re, im),
 // row-major: mat[(f np + i) np + j] is entry (i, j) at frequency f -- the
 // layout Parasitics::saveNetworkParam reads sys->x with (:1394-1420), so the
@@ -247,4 +249,118 @@ extern "C" gp_status gp_write_touchstone(const char *path, const char *design, c
   const bool w = std::fwrite(o.data(), 1, o.size(), fp) == o.size();
   std::fclose(fp);
   return w ? GP_OK : fail(GP_ERR_STATE, std::string("gp_write_touchstone: short write to ") + path);
+}
+
+namespace {
+
+// Eigen's a.isApprox(b) for dense real matrices at double precision (1e-12):
+// |a - b|_F^2 <= 1e-24 min(|a|_F^2, |b|_F^2)
+bool is_approx_transpose(const std::vector<double> &a, int n) {
+  double d2 = 0., a2 = 0.;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      const double x = a[(size_t)i * n + j], y = a[(size_t)j * n + i];
+      d2 += (x - y) * (x - y);
+      a2 += x * x;
+    }
+  return d2 <= 1e-24 * a2;  // |a^T|_F = |a|_F
+}
+
+// makeGSym / makeCSym (src/solnoutclass.hpp:1325-1335, 1382-1392): replaced
+// by (a + a^T) / 2 unless a already passes isApprox against its transpose
+void make_sym(std::vector<double> &a, int n) {
+  if (is_approx_transpose(a, n)) return;
+  std::vector<double> s(a.size());
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) s[(size_t)i * n + j] = (a[(size_t)i * n + j] + a[(size_t)j * n + i]) * 0.5;
+  a.swap(s);
+}
+
+// getGTotal / getCTotal (:1291-1315, 1348-1372): the upper triangle's sum
+double upper_total(const std::vector<double> &a, int n) {
+  double t = 0.;
+  for (int i = 0; i < n; ++i)
+    for (int j = i; j < n; ++j) t += a[(size_t)i * n + j];
+  return t;
+}
+
+// getGNodeGround / getCNodeGround (:1280-1288, 1337-1345): the row sum
+double row_sum(const std::vector<double> &a, int n, int i) {
+  double t = 0.;
+  for (int j = 0; j < n; ++j) t += a[(size_t)i * n + j];
+  return t;
+}
+
+}  // namespace
+
+// Parasitics::computeYBusFromParam(0) (src/solnoutclass.hpp:1680-1766, the
+// dense branch built by default: EIGEN_SPARSE is commented out at :31) and
+// Parasitics::toXyce (:2094-2248, dense branch), as SolverDataBase::dumpXyce
+// calls them with WRITE_THRESH (:33, :6032-6036).
+extern "C" gp_status gp_write_xyce(const char *path, const char *design, const char *timestamp, char param,
+                                   int32_t np, const char *const *port_names, double f0, const double *mat,
+                                   double save_thresh) {
+  if (!path || !design || np < 1 || !port_names || !mat || !(f0 > 0.)) return fail(GP_ERR_ARG, "bad Xyce arguments");
+  for (int32_t i = 0; i < np; ++i)
+    if (!port_names[i]) return fail(GP_ERR_ARG, "null port name");
+  if (param != 'Z' && param != 'Y')
+    return fail(GP_ERR_ARG, "param must be 'Z' or 'Y' (the reference's 'S' branch leaves its Y matrix empty)");
+  const int n = np;
+  Mat P((size_t)n * n);
+  for (size_t q = 0; q < P.size(); ++q) P[q] = cd(mat[2 * q], mat[2 * q + 1]);
+  // 'Z' is used as Y without inversion and 'Y' is inverted, as the reference
+  // does (:1688-1699)
+  Mat Y;
+  if (param == 'Y') {
+    if (!inverse(P, n, &Y)) return fail(GP_ERR_ARG, "singular Y matrix");
+  } else {
+    Y = P;
+  }
+  const double w0 = 2. * M_PI * f0;
+  std::vector<double> G((size_t)n * n), Cm((size_t)n * n);
+  for (size_t q = 0; q < Y.size(); ++q) {
+    G[q] = Y[q].real();                 // G = Re{Y}
+    Cm[q] = -1. / (Y[q].imag() * w0);   // C = Im{Y} (2 pi f), then -1 / C entrywise
+  }
+  FILE *fp = std::fopen(path, "w");
+  if (!fp) return fail(GP_ERR_ARG, std::string("cannot open ") + path);
+  const std::string when = stamp(timestamp, "%d-%m-%Y %H:%M:%S");
+  std::fprintf(fp, "* File: %s\n* Design: %s\n* Vendor: DARPA ERI Contributors\n", path, design);
+  std::fprintf(fp, "* Program: Xyce Writer from DARPA ERI\n* Author: Purdue University\n* Date: %s\n\n", when.c_str());
+  std::fprintf(fp, ".subckt %s", design);
+  for (int i = 0; i < n; ++i) std::fprintf(fp, " %s", port_names[i]);
+  std::fprintf(fp, "\n");
+  make_sym(Cm, n);
+  make_sym(G, n);
+  const double capTot = upper_total(Cm, n), condTot = upper_total(G, n);
+  int numCap = 1, numInd = 1, numRes = 1;
+  const double w02 = std::pow(w0, 2.);
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < n; ++j) {
+      if (i == j && std::abs(row_sum(Cm, n, i)) >= save_thresh * capTot) {
+        const double cng = row_sum(Cm, n, i);
+        if (cng > 0.) std::fprintf(fp, "C%d %s 0 %s\n", numCap++, port_names[i], fmt_g(cng, 6).c_str());
+        else std::fprintf(fp, "L%d %s 0 %s\n", numInd++, port_names[i], fmt_g(-1. / (cng * w02), 6).c_str());
+      } else if (j > i && std::abs(-Cm[(size_t)i * n + j]) >= save_thresh * capTot) {
+        const double cnn = -1. * Cm[(size_t)i * n + j];
+        if (cnn > 0.)
+          std::fprintf(fp, "C%d %s %s %s\n", numCap++, port_names[i], port_names[j], fmt_g(cnn, 6).c_str());
+        else
+          std::fprintf(fp, "L%d %s %s %s\n", numInd++, port_names[i], port_names[j], fmt_g(-1. / (cnn * w02), 6).c_str());
+      }
+    }
+    for (int j = 0; j < n; ++j) {
+      // the coupling test compares a resistance with the conductance total,
+      // as the reference does (:2207)
+      if (i == j && std::abs(row_sum(G, n, i)) >= save_thresh * condTot) {
+        std::fprintf(fp, "R%d %s 0 %s\n", numRes++, port_names[i], fmt_g(std::abs(1. / row_sum(G, n, i)), 6).c_str());
+      } else if (j > i && std::abs(-1. / G[(size_t)i * n + j]) >= save_thresh * condTot) {
+        std::fprintf(fp, "R%d %s %s %s\n", numRes++, port_names[i], port_names[j],
+                     fmt_g(std::abs(-1. / G[(size_t)i * n + j]), 6).c_str());
+      }
+    }
+  }
+  std::fprintf(fp, ".ends\n");
+  const bool ok = std::fclose(fp) == 0;
+  return ok ? GP_OK : fail(GP_ERR_ARG, std::string("cannot write ") + path);
 }

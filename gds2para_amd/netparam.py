@@ -64,3 +64,19 @@ def write_touchstone(path, design, freqs, P, param="Z", zsource=50.0, timestamp=
                                                None if timestamp is None else timestamp.encode(), param.encode(),
                                                npt, len(fr), fr.ctypes.data_as(f64p), buf.ctypes.data_as(f64p),
                                                z.ctypes.data_as(f64p)), "gp_write_touchstone")
+
+
+def write_xyce(path, design, port_names, f0, P0, param="Z", save_thresh=1e-5, timestamp=None):
+    """SolverDataBase::dumpXyce: SPICE subcircuit from the lowest frequency's
+    np x np block P0 (the Z layout of a solve: index resp + np*src) at f0 Hz
+    (gp_write_xyce)."""
+    P0 = np.asarray(P0, np.complex128).reshape(1, -1)
+    npt = int(round(np.sqrt(P0.shape[1])))
+    if len(port_names) != npt:
+        raise ValueError("one name per port")
+    buf, _ = _interleave(P0)
+    names = (C.c_char_p * npt)(*[str(s).encode() for s in port_names])
+    capi.check(capi.load().gp_write_xyce(str(path).encode(), design.encode(),
+                                         None if timestamp is None else timestamp.encode(), param.encode(), npt,
+                                         names, float(f0), buf.ctypes.data_as(f64p), float(save_thresh)),
+               "gp_write_xyce")

@@ -23,6 +23,7 @@
  *   gp_convert_param        Parasitics::convertParam                src/solnoutclass.hpp:1456
  *   gp_write_citi           SolverDataBase::dumpCITI                src/solnoutclass.hpp:6040
  *   gp_write_touchstone     SolverDataBase::dumpTouchstone          src/solnoutclass.hpp:6104
+ *   gp_write_xyce           SolverDataBase::dumpXyce                src/solnoutclass.hpp:6032
  *   gp_freq_of              fdtdMesh::freqNo2freq                   src/fdtd.hpp:4319
  *   gp_edge_map             fdtdMesh::setMapEdge                    src/fdtd.hpp:1508
  */
@@ -371,6 +372,19 @@ gp_status gp_write_citi(const char *path, const char *design, const char *timest
 gp_status gp_write_touchstone(const char *path, const char *design, const char *timestamp, char param,
                               int32_t np, int32_t nf, const double *freqs, const double *mat,
                               const double *zsource);
+
+/* SolverDataBase::dumpXyce (src/solnoutclass.hpp:6032-6036): the SPICE
+ * subcircuit that Parasitics::computeYBusFromParam(0) and toXyce
+ * (:1680-1766, :2094-2248) write from the lowest frequency's block.  mat is
+ * that np x np block (the layout of gp_convert_param, so Z[0..2 np^2) of a
+ * solve), f0 its frequency (Hz), port_names[np] the node names, save_thresh
+ * the reference's WRITE_THRESH (1e-5).  As in the reference: param 'Z' is
+ * used as Y without inversion, 'Y' is inverted, C = -1 / (Im{Y} 2 pi f0)
+ * entrywise, G = Re{Y}, both symmetrized unless already symmetric to 1e-12.
+ * 'S' is rejected (the reference's branch leaves Y empty).  timestamp as for
+ * gp_write_citi. */
+gp_status gp_write_xyce(const char *path, const char *design, const char *timestamp, char param, int32_t np,
+                        const char *const *port_names, double f0, const double *mat, double save_thresh);
 
 /* y = A(w) x for k columns (1 <= k <= 64) at freqs[k] (Hz), with A(w) the
  * complex matrix reference() hands PARDISO (src/generateStiff.cpp:583-626):

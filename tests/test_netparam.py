@@ -99,3 +99,63 @@ def test_two_port_order_line_and_default_time(tmp_path):
     lines = p.read_text().splitlines()
     assert lines[3] == "[Two-Port Data Order] 12_21"
     assert lines[8].startswith("! Date/Time 20")          # the current local time
+
+
+# ---- Xyce (SolverDataBase::dumpXyce, src/solnoutclass.hpp:6032; Y bus from
+# computeYBusFromParam :1680-1766, writer toXyce :2094-2248) ----------------
+
+XYCE_HEAD = ["* File: {path}", "* Design: d", "* Vendor: DARPA ERI Contributors",
+             "* Program: Xyce Writer from DARPA ERI", "* Author: Purdue University", "* Date: 01-01-2025 00:00:00",
+             "", ".subckt d p0 p1"]
+
+
+def _xyce(tmp_path, P0, **kw):
+    from gds2para_amd.netparam import write_xyce
+    p = tmp_path / "out.cir"
+    write_xyce(p, "d", ["p0", "p1"], 1.0 / (2.0 * np.pi), P0, timestamp="01-01-2025 00:00:00", **kw)
+    return p, p.read_text().splitlines()
+
+
+def test_xyce_known_answer(tmp_path):
+    """Worked by hand from the reference's steps with 2 pi f0 = 1:
+    Z = [[2+4i, 0.5-0.25i], [0.5-0.25i, 1+2i]] is used as Y; G = Re Y =
+    [[2, .5], [.5, 1]]; C = -1 / Im Y = [[-.25, 4], [4, -.5]]; totals (upper
+    triangle) 3.25 and 3.5; node-to-ground row sums C 3.75, 3.5 and G 2.5,
+    1.5; the coupling C is -(4) < 0, so an inductor of -1 / (-4 * 1) = 0.25."""
+    P0 = np.array([2 + 4j, 0.5 - 0.25j, 0.5 - 0.25j, 1 + 2j])
+    p, lines = _xyce(tmp_path, P0)
+    head = [s.format(path=p) for s in XYCE_HEAD]
+    assert lines == head + ["C1 p0 0 3.75", "L1 p0 p1 0.25", "R1 p0 0 0.4", "R2 p0 p1 2",
+                            "C2 p1 0 3.5", "R3 p1 0 0.666667", ".ends"]
+
+
+def test_xyce_symmetrizes_nonreciprocal_input(tmp_path):
+    """G = [[2, .5], [1.5, 1]] and C = [[-.25, 4], [2, -.5]] are not symmetric,
+    so both become (M + M^T) / 2 = [[2, 1], [1, 1]] and [[-.25, 3], [3, -.5]]."""
+    P0 = np.array([2 + 4j, 0.5 - 0.25j, 1.5 - 0.5j, 1 + 2j])
+    p, lines = _xyce(tmp_path, P0)
+    assert lines[8:] == ["C1 p0 0 2.75", "L1 p0 p1 0.333333", "R1 p0 0 0.333333", "R2 p0 p1 1",
+                         "C2 p1 0 2.5", "R3 p1 0 0.5", ".ends"]
+
+
+def test_xyce_threshold_drops_small_elements(tmp_path):
+    """save_thresh 1 against totals 3.25 (C) and 3.5 (G): every C element of
+    the known answer stays (3.75, |-4|, 3.5 >= 3.25), every R goes (row sums
+    2.5, 1.5 and the coupling resistance 2 are below 3.5)."""
+    P0 = np.array([2 + 4j, 0.5 - 0.25j, 0.5 - 0.25j, 1 + 2j])
+    p, lines = _xyce(tmp_path, P0, save_thresh=1.0)
+    assert lines[8:] == ["C1 p0 0 3.75", "L1 p0 p1 0.25", "C2 p1 0 3.5", ".ends"]
+
+
+def test_xyce_y_parameters_are_inverted_first(tmp_path):
+    """'Y' input is inverted before use (the reference's branch), so it writes
+    what 'Z' writes for the inverse matrix."""
+    Y = np.array([[3 + 1j, -1 - 0.5j], [-1 - 0.5j, 2 + 2j]])
+    _, a = _xyce(tmp_path, Y.ravel(), param="Y")
+    _, b = _xyce(tmp_path, np.linalg.inv(Y).ravel(), param="Z")
+    assert a == b and len(a) > 10
+
+
+def test_xyce_rejects_s_parameters(tmp_path):
+    with pytest.raises(capi.GpError):
+        _xyce(tmp_path, np.array([0.1 + 0j, 0, 0, 0.1]), param="S")
