@@ -313,13 +313,14 @@ __global__ void k_rows_to_cols(int64_t n, int k, const Z2 *__restrict__ a, Z2 *_
   b[(int64_t)j * n + i] = a[t];
 }
 
-// Multi-vector dots.  Block (blk, c) handles vectors t in [8c, 8c+8) of V
+// Multi-vector dots.  Block (blk, c) handles vectors t in [4c, 4c+4) of V
 // (stride vstride) against w over a grid-stride range of rows and writes
 // partial[(blk * nout + t) * k + j] = sum conj(V_t) w; with self, chunk 0 also
-// writes t = nvec: sum |w|^2.  nout = nvec + self.  With w in FP32, eight
-// vectors per chunk (twice the wavefronts per SIMD of sixteen, w read once
-// per chunk) ran the 200x200x24 solve 1 % faster than sixteen.
-constexpr int kDotChunk = 8;
+// writes t = nvec: sum |w|^2.  nout = nvec + self.  With w in FP32, small
+// chunks (more wavefronts per SIMD, w read once per chunk) won: at
+// 200x200x24 the solve ran about 1 % faster with 8 than with 16, and about
+// 1.5 % faster again with 4.
+constexpr int kDotChunk = 4;
 template <class TV, class TW>
 __global__ void __launch_bounds__(256) k_mdots(int64_t n, int k, const C2<TV> *__restrict__ V, int64_t vstride, int nvec,
                                                const C2<TW> *__restrict__ w, int self, Z2 *__restrict__ partial) {
