@@ -3,6 +3,13 @@ import sys
 
 import pytest
 
+# Under pytest-xdist every worker would start one BLAS thread per core; on an
+# 8-core host four workers then slow the dense oracle solves many times over.
+# Must run before numpy is imported.
+if "PYTEST_XDIST_WORKER" in os.environ:
+    for _v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+        os.environ.setdefault(_v, "2")
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
