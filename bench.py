@@ -130,7 +130,7 @@ def main():
         print(json.dumps(serving_leg(a.serve_requests, a.serve_rate)), flush=True)
         return
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported by this build "
+        print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported "
               "(the solve path is single-device)", file=sys.stderr)
         sys.exit(2)
 
