@@ -79,7 +79,8 @@ class GpHierarchyInfo(C.Structure):
     _fields_ = [
         ("edge_levels", i32), ("ne", i64 * 16), ("nn", i64 * 16), ("nnz_k", i64 * 16),
         ("aux_levels", i32), ("na", i64 * 16), ("nnz_n", i64 * 16), ("nlong", i64 * 16),
-        ("naux", i64), ("setup_ms", f64),
+        ("naux", i64), ("setup_ms", f64), ("coarse_entry_bytes", i64), ("coarse_entries", i32),
+        ("coarse_evictions", i64), ("workspace_bytes", i64),
     ]
 
 
@@ -124,6 +125,7 @@ SIGNATURES = [
     ("gp_layout_create", i32, [P(GpMesh), P(C.c_void_p)]),
     ("gp_layout_destroy", None, [C.c_void_p]),
     ("gp_layout_device_bytes", i64, [C.c_void_p]),
+    ("gp_layout_trim", i32, [C.c_void_p]),
     ("gp_assemble_stiffness", i32, [C.c_void_p]),
     ("gp_stiffness_copy_csr", i32, [C.c_void_p, P(i64), P(i32), P(f64)]),
     ("gp_stiffness_copy_coo", i32, [C.c_void_p, P(i64), P(i64), P(f64)]),

@@ -266,11 +266,10 @@ gp_status dense_host_aux(const Csr &M, int which, int64_t n, std::vector<double>
 
 }  // namespace
 
-gp_status build_aux(gp_layout *lay, Hierarchy *H, bool want_f32) {
+gp_status build_aux(gp_layout *lay, Arena &A, Hierarchy *H) {
   GP_TRY(build_v0(lay));
   const V0Data &V = *lay->v0;
   hipStream_t s = lay->stream;
-  Arena &A = lay->arena;
   AuxAMG &X = H->aux;
   const int64_t n = lay->n;
   const int64_t nd = V.v0d_cols, nc = V.v0c_cols, naux = nd + nc;
@@ -431,27 +430,6 @@ gp_status build_aux(gp_layout *lay, Hierarchy *H, bool want_f32) {
       GP_TRY(sell_from_csr(A, L.P, &L.Ps, s, "L.P"));
       GP_TRY(sell_from_csr(A, L.PT, &L.PTs, s, "L.PT"));
     }
-    if (want_f32) {
-      GP_TRY(csr_to_float(A, X.Gx, &X.Gxf, s));
-      GP_TRY(csr_to_float(A, X.GxT, &X.GxTf, s));
-      GP_TRY(csr_to_float(A, X.Bx, &X.Bxf, s));
-      GP_TRY(sell_to_float(A, X.GxTs, &X.GxTsf, s));
-      GP_TRY(sell_to_float(A, X.Bxs, &X.Bxsf, s));
-      for (size_t l = 0; l + 1 < X.L.size(); ++l) {
-        AuxLevel &L = X.L[l];
-        GP_TRY(sell_to_float(A, L.Ns, &L.Nsf, s));
-        GP_TRY(sell_to_float(A, L.Ps, &L.Psf, s));
-        GP_TRY(sell_to_float(A, L.PTs, &L.PTsf, s));
-      }
-      for (AuxLevel &L : X.L) {
-        GP_TRY(csr_to_float(A, L.N, &L.Nf, s));
-        if (L.P.nr) { GP_TRY(csr_to_float(A, L.P, &L.Pf, s)); GP_TRY(csr_to_float(A, L.PT, &L.PTf, s)); }
-        for (int k = 0; k < 2; ++k) {
-          GP_TRY(A.alloc(&L.ndf[k], L.n)); GP_TRY(A.alloc(&L.nlf[k], L.n));
-        }
-        for (int k = 0; k < 2; ++k) { GP_TRY(d2f_copy(L.nd[k], L.ndf[k], L.n, s)); GP_TRY(d2f_copy(L.nl[k], L.nlf[k], L.n, s)); }
-      }
-    }
     const AuxLevel &C = X.L.back();
     X.nL = C.n;
     for (int k = 0; k < 2; ++k) GP_TRY(dense_host_aux(C.N, k, C.n, &X.NL[k]));
@@ -460,6 +438,32 @@ gp_status build_aux(gp_layout *lay, Hierarchy *H, bool want_f32) {
   (void)hipStreamSynchronize(s);
   tmp.free_all();
   return st;
+}
+
+
+// FP32 copies of the auxiliary hierarchy's operators (see hierarchy_add_f32).
+gp_status aux_add_f32(Arena &A, Hierarchy *H, hipStream_t s) {
+  AuxAMG &X = H->aux;
+  GP_TRY(csr_to_float(A, X.Gx, &X.Gxf, s));
+  GP_TRY(csr_to_float(A, X.GxT, &X.GxTf, s));
+  GP_TRY(csr_to_float(A, X.Bx, &X.Bxf, s));
+  GP_TRY(sell_to_float(A, X.GxTs, &X.GxTsf, s));
+  GP_TRY(sell_to_float(A, X.Bxs, &X.Bxsf, s));
+  for (size_t l = 0; l + 1 < X.L.size(); ++l) {
+    AuxLevel &L = X.L[l];
+    GP_TRY(sell_to_float(A, L.Ns, &L.Nsf, s));
+    GP_TRY(sell_to_float(A, L.Ps, &L.Psf, s));
+    GP_TRY(sell_to_float(A, L.PTs, &L.PTsf, s));
+  }
+  for (AuxLevel &L : X.L) {
+    GP_TRY(csr_to_float(A, L.N, &L.Nf, s));
+    if (L.P.nr) { GP_TRY(csr_to_float(A, L.P, &L.Pf, s)); GP_TRY(csr_to_float(A, L.PT, &L.PTf, s)); }
+    for (int k = 0; k < 2; ++k) {
+      GP_TRY(A.alloc(&L.ndf[k], L.n)); GP_TRY(A.alloc(&L.nlf[k], L.n));
+    }
+    for (int k = 0; k < 2; ++k) { GP_TRY(d2f_copy(L.nd[k], L.ndf[k], L.n, s)); GP_TRY(d2f_copy(L.nl[k], L.nlf[k], L.n, s)); }
+  }
+  return GP_OK;
 }
 
 }  // namespace gp

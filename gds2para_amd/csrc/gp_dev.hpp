@@ -93,6 +93,7 @@ int64_t v0path_bytes(const V0Path *);
 struct Solver;           // defined in solver.hip
 void destroy_v0(V0Data *);
 void destroy_solver(Solver *);
+int64_t solver_bytes(const Solver *);  // hierarchy, coarse cache and cached slot workspace
 
 }  // namespace gp
 

@@ -191,5 +191,5 @@ extern "C" void gp_layout_destroy(gp_layout *lay) {
 }
 
 extern "C" int64_t gp_layout_device_bytes(const gp_layout *lay) {
-  return lay ? lay->arena.bytes + v0path_bytes(lay->v0path) : 0;
+  return lay ? lay->arena.bytes + v0path_bytes(lay->v0path) + solver_bytes(lay->solver) : 0;
 }

@@ -191,11 +191,52 @@ gp_status level0_node_order(const LevelGrid &L, Arena &A, int32_t **order, hipSt
   return GP_OK;
 }
 
-gp_status build_hierarchy(gp_layout *lay, Hierarchy *H, bool want_f32, int max_levels) {
+// FP32 copies of every operator the mixed-precision V-cycle reads, added to
+// an existing FP64 hierarchy (so a layout solved in FP64 first and in mixed
+// precision afterwards keeps one hierarchy instead of building a second).
+gp_status hierarchy_add_f32(Arena &A, Hierarchy *H, hipStream_t s) {
+  if (H->has_f32) return GP_OK;
+  const int64_t n = H->L[0].ne;
+  GP_TRY(to_f32(A, H->W, n, &H->Wf, s));
+  for (MgLevel &L : H->L) {
+    GP_TRY(csr_to_float(A, L.K, &L.Kf, s));
+    for (int q = 0; q < 2; ++q)
+      if (L.K.dg[q]) GP_TRY(to_f32(A, L.K.dg[q], L.ne, &L.Kf.dg[q], s));
+    if (L.Ks.nr) {
+      GP_TRY(sell_to_float(A, L.Ks, &L.Ksf, s)); GP_TRY(sell_to_float(A, L.Bs, &L.Bsf, s));
+      GP_TRY(sell_to_float(A, L.Ns, &L.Nsf, s)); GP_TRY(sell_to_float(A, L.GTs, &L.GTsf, s));
+      GP_TRY(sell_to_float(A, L.Ps, &L.Psf, s)); GP_TRY(sell_to_float(A, L.PTs, &L.PTsf, s));
+      GP_TRY(sell_to_float(A, L.Pns, &L.Pnsf, s));
+    }
+    GP_TRY(csr_to_float(A, L.B, &L.Bf, s));
+    GP_TRY(csr_to_float(A, L.N, &L.Nf, s));
+    GP_TRY(csr_to_float(A, L.G, &L.Gf, s));
+    GP_TRY(csr_to_float(A, L.GT, &L.GTf, s));
+    if (L.P.nr) {
+      GP_TRY(csr_to_float(A, L.P, &L.Pf, s));
+      GP_TRY(csr_to_float(A, L.PT, &L.PTf, s));
+      GP_TRY(csr_to_float(A, L.Pn, &L.Pnf, s));
+      GP_TRY(csr_to_float(A, L.PnT, &L.PnTf, s));
+    }
+    for (int k = 0; k < 3; ++k) {
+      GP_TRY(to_f32(A, L.kd[k], L.ne, &L.kdf[k], s));
+      GP_TRY(to_f32(A, L.kl[k], L.ne, &L.klf[k], s));
+    }
+    for (int k = 0; k < 2; ++k) {
+      GP_TRY(to_f32(A, L.nd[k], L.nn, &L.ndf[k], s));
+      GP_TRY(to_f32(A, L.nl[k], L.nn, &L.nlf[k], s));
+    }
+  }
+  GP_TRY(aux_add_f32(A, H, s));
+  GP_HIP(hipStreamSynchronize(s));
+  H->has_f32 = true;
+  return GP_OK;
+}
+
+gp_status build_hierarchy(gp_layout *lay, Arena &A, Hierarchy *H, bool want_f32, int max_levels) {
   auto t0 = std::chrono::steady_clock::now();
   if (!lay->s_ready) GP_TRY(gp_assemble_stiffness(lay));
   hipStream_t s = lay->stream;
-  Arena &A = lay->arena;
   const gp_mesh &m = lay->hm.view;
   std::vector<LevelGrid> grids(1);
   level0_grid(m, &grids[0]);
@@ -310,39 +351,6 @@ gp_status build_hierarchy(gp_layout *lay, Hierarchy *H, bool want_f32, int max_l
     GP_TRY(sell_from_csr(A, L.PT, &L.PTs, s, "L.PT"));
     GP_TRY(sell_from_csr(A, L.Pn, &L.Pns, s, "L.Pn", on));
   }
-  if (want_f32) {
-    GP_TRY(to_f32(A, H->W, n, &H->Wf, s));
-    for (MgLevel &L : H->L) {
-      GP_TRY(csr_to_float(A, L.K, &L.Kf, s));
-      for (int q = 0; q < 2; ++q)
-        if (L.K.dg[q]) GP_TRY(to_f32(A, L.K.dg[q], L.ne, &L.Kf.dg[q], s));
-      if (L.Ks.nr) {
-        GP_TRY(sell_to_float(A, L.Ks, &L.Ksf, s)); GP_TRY(sell_to_float(A, L.Bs, &L.Bsf, s));
-        GP_TRY(sell_to_float(A, L.Ns, &L.Nsf, s)); GP_TRY(sell_to_float(A, L.GTs, &L.GTsf, s));
-        GP_TRY(sell_to_float(A, L.Ps, &L.Psf, s)); GP_TRY(sell_to_float(A, L.PTs, &L.PTsf, s));
-        GP_TRY(sell_to_float(A, L.Pns, &L.Pnsf, s));
-      }
-      GP_TRY(csr_to_float(A, L.B, &L.Bf, s));
-      GP_TRY(csr_to_float(A, L.N, &L.Nf, s));
-      GP_TRY(csr_to_float(A, L.G, &L.Gf, s));
-      GP_TRY(csr_to_float(A, L.GT, &L.GTf, s));
-      if (L.P.nr) {
-        GP_TRY(csr_to_float(A, L.P, &L.Pf, s));
-        GP_TRY(csr_to_float(A, L.PT, &L.PTf, s));
-        GP_TRY(csr_to_float(A, L.Pn, &L.Pnf, s));
-        GP_TRY(csr_to_float(A, L.PnT, &L.PnTf, s));
-      }
-      for (int k = 0; k < 3; ++k) {
-        GP_TRY(to_f32(A, L.kd[k], L.ne, &L.kdf[k], s));
-        GP_TRY(to_f32(A, L.kl[k], L.ne, &L.klf[k], s));
-      }
-      for (int k = 0; k < 2; ++k) {
-        GP_TRY(to_f32(A, L.nd[k], L.nn, &L.ndf[k], s));
-        GP_TRY(to_f32(A, L.nl[k], L.nn, &L.nlf[k], s));
-      }
-    }
-    H->has_f32 = true;
-  }
   // coarsest level to the host, dense
   const MgLevel &C = H->L.back();
   H->neL = C.ne;
@@ -352,7 +360,8 @@ gp_status build_hierarchy(gp_layout *lay, Hierarchy *H, bool want_f32, int max_l
   for (int k = 0; k < 2; ++k) GP_TRY(dense_host(C.N, k, C.nn, C.nn, &H->NL[k]));
   GP_TRY(dense_host(C.GT, 0, C.nn, C.ne, &H->GTL));
   GP_HIP(hipStreamSynchronize(s));
-  GP_TRY(build_aux(lay, H, want_f32));
+  GP_TRY(build_aux(lay, A, H));
+  if (want_f32) GP_TRY(hierarchy_add_f32(A, H, s));
   {  // dense coarsest data on the device, for the per-frequency inverses
     auto up = [&](const std::vector<double> &h, double **d) -> gp_status {
       GP_TRY(A.alloc(d, std::max<int64_t>((int64_t)h.size(), 1)));
@@ -385,97 +394,147 @@ gp_status build_hierarchy(gp_layout *lay, Hierarchy *H, bool want_f32, int max_l
   return GP_OK;
 }
 
-void free_hierarchy(gp_layout *lay, Hierarchy *H) {
-  // the layout arena owns every buffer; a rebuilt hierarchy simply adds new ones
-  (void)lay;
-  H->L.clear();
-}
 
 // ---------------------------------------------------------------- coarsest levels per frequency
 
 namespace {
 
-// out[f][i] = c0 * A0[i] + a1_f * A1[i] + i w_f * A2[i]; A0 may be null (zero), n entries per matrix
-__global__ void k_form_freq(int64_t n, int nf, const double *__restrict__ om, const double *__restrict__ A0,
-                            const double *__restrict__ A1, const double *__restrict__ A2, C2<double> *__restrict__ out) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n * nf) return;
-  const int f = (int)(t / n);
-  const int64_t i = t - (int64_t)f * n;
-  const double w = om[f], a1 = -w * w;
-  out[t] = {(A0 ? A0[i] : 0.0) + a1 * A1[i], w * A2[i]};
+// out[i] = A0[i] - w^2 A1[i] + i w A2[i]; A0 may be null (zero), n entries per matrix
+__global__ void k_form_freq(int64_t n, double w, const double *__restrict__ A0, const double *__restrict__ A1,
+                            const double *__restrict__ A2, C2<double> *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a1 = -w * w;
+  out[i] = {(A0 ? A0[i] : 0.0) + a1 * A1[i], w * A2[i]};
+}
+
+}  // namespace
+
+int64_t coarse_entry_bytes(const Hierarchy &H) {
+  const int64_t ne = H.neL, nn = H.nnL, na = H.aux.nL;
+  return (int64_t)sizeof(C2<double>) * (ne * ne + nn * nn + ne * nn + na * na);
+}
+
+namespace {
+
+// (Re)writes the device pointer table, growing it when the slots outgrew it.
+gp_status coarse_table_write(const Hierarchy &H, CoarseCache &C, hipStream_t s) {
+  const int need = (int)C.e.size();
+  if (need > C.cap || !C.tab) {
+    int cap = std::max(16, C.cap);
+    while (cap < need) cap *= 2;
+    // the previous table may still be read by queued kernels: wait for them
+    GP_HIP(hipStreamSynchronize(s));
+    if (C.tab) {
+      GP_HIP(hipFree((void *)C.tab));
+      C.bytes -= (int64_t)sizeof(double *) * 4 * C.cap;
+      C.tab = nullptr;
+    }
+    const double **tab = nullptr;
+    if (hipMalloc((void **)&tab, sizeof(double *) * 4 * (size_t)cap) != hipSuccess) {
+      (void)hipGetLastError();
+      C.cap = 0;
+      return fail(GP_ERR_NOMEM, "coarse cache table allocation failed");
+    }
+    C.tab = tab;
+    C.cap = cap;
+    C.bytes += (int64_t)sizeof(double *) * 4 * cap;
+  }
+  const int64_t ne = H.neL, nn = H.nnL;
+  std::vector<const double *> h(4 * (size_t)C.cap, nullptr);
+  for (int q = 0; q < need; ++q) {
+    if (!C.e[q].mem) continue;
+    C2<double> *b = (C2<double> *)C.e[q].mem;
+    h[q] = (const double *)b;
+    h[C.cap + q] = (const double *)(b + ne * ne);
+    h[2 * C.cap + q] = (const double *)(b + ne * ne + nn * nn);
+    h[3 * C.cap + q] = (const double *)(b + ne * ne + nn * nn + ne * nn);
+  }
+  GP_HIP(hipMemcpyAsync((void *)C.tab, h.data(), sizeof(double *) * h.size(), hipMemcpyHostToDevice, s));
+  GP_HIP(hipStreamSynchronize(s));  // h goes out of scope
+  return GP_OK;
+}
+
+// Frees unpinned entries, least recently used first, until one more entry of
+// `add` bytes fits the bounds (or no unpinned entry is left).
+gp_status coarse_make_room(CoarseCache &C, int64_t add, hipStream_t s) {
+  for (;;) {
+    int live = 0;
+    int64_t b = 0;
+    for (const CoarseEntry &x : C.e)
+      if (x.mem) { ++live; b += x.bytes; }
+    if (live + 1 <= C.max_entries && b + add <= C.max_bytes) return GP_OK;
+    int victim = -1;
+    for (int q = 0; q < (int)C.e.size(); ++q)
+      if (C.e[q].mem && C.e[q].pins == 0 && (victim < 0 || C.e[q].used < C.e[victim].used)) victim = q;
+    if (victim < 0) return GP_OK;   // everything pinned: exceed the bound
+    GP_HIP(hipStreamSynchronize(s));
+    GP_HIP(hipFree(C.e[victim].mem));
+    C.bytes -= C.e[victim].bytes;
+    C.e[victim] = CoarseEntry{};
+    ++C.evictions;
+  }
 }
 
 }  // namespace
 
 gp_status coarse_cache_lookup(const Hierarchy &H, CoarseCache &C, const std::vector<double> &omegas,
-                              std::vector<int32_t> *slots, hipStream_t s) {
+                              std::vector<int32_t> *slots, hipStream_t s, int pin) {
   slots->assign(omegas.size(), -1);
-  std::vector<double> fresh;
+  const int64_t ne = H.neL, nn = H.nnL, na = H.aux.nL;
+  const int64_t eb = coarse_entry_bytes(H);
+  bool fresh = false;
   for (size_t i = 0; i < omegas.size(); ++i) {
-    for (size_t q = 0; q < C.omegas.size(); ++q)
-      if (C.omegas[q] == omegas[i]) { (*slots)[i] = (int32_t)q; break; }
-    if ((*slots)[i] < 0) {
-      bool dup = false;
-      for (double f : fresh) dup = dup || f == omegas[i];
-      if (!dup) fresh.push_back(omegas[i]);
+    int32_t q = -1;
+    for (int32_t t = 0; t < (int32_t)C.e.size(); ++t)
+      if (C.e[t].mem && C.e[t].omega == omegas[i]) { q = t; break; }
+    if (q < 0) {
+      GP_TRY(coarse_make_room(C, eb, s));
+      for (int32_t t = 0; t < (int32_t)C.e.size(); ++t)
+        if (!C.e[t].mem) { q = t; break; }
+      if (q < 0) { q = (int32_t)C.e.size(); C.e.emplace_back(); }
+      CoarseEntry &x = C.e[q];
+      if (hipMalloc(&x.mem, (size_t)std::max<int64_t>(eb, 16)) != hipSuccess) {
+        (void)hipGetLastError();
+        x = CoarseEntry{};
+        return fail(GP_ERR_NOMEM, "coarse cache entry allocation failed");
+      }
+      x.omega = omegas[i];
+      x.bytes = std::max<int64_t>(eb, 16);
+      C.bytes += x.bytes;
+      C2<double> *kb = (C2<double> *)x.mem, *nbuf = kb + ne * ne, *bb = nbuf + nn * nn, *ab = bb + ne * nn;
+      const double w = x.omega;
+      if (ne) k_form_freq<<<blocks_for(ne * ne), 256, 0, s>>>(ne * ne, w, H.KLd[0], H.KLd[1], H.KLd[2], kb);
+      if (nn) {
+        k_form_freq<<<blocks_for(nn * nn), 256, 0, s>>>(nn * nn, w, nullptr, H.NLd[0], H.NLd[1], nbuf);
+        k_form_freq<<<blocks_for(ne * nn), 256, 0, s>>>(ne * nn, w, nullptr, H.BLd[0], H.BLd[1], bb);
+      }
+      if (na) k_form_freq<<<blocks_for(na * na), 256, 0, s>>>(na * na, w, nullptr, H.aux.NLd[0], H.aux.NLd[1], ab);
+      GP_TRY(dense_inverse_batched(kb, (int)ne, 1, s));
+      GP_TRY(dense_inverse_batched(nbuf, (int)nn, 1, s));
+      GP_TRY(dense_inverse_batched(ab, (int)na, 1, s));
+      fresh = true;
     }
+    C.e[q].used = ++C.clock;
+    if (pin) ++C.e[q].pins;
+    (*slots)[i] = q;
   }
-  if (!fresh.empty()) {
-    const int nb = (int)fresh.size();
-    const int64_t ne = H.neL, nn = H.nnL, na = H.aux.nL;
-    Arena &A = C.arena;
-    double *om;
-    C2<double> *kb, *nbuf, *bb, *ab;
-    GP_TRY(A.alloc(&om, nb));
-    GP_TRY(A.alloc(&kb, std::max<int64_t>(ne * ne * nb, 1)));
-    GP_TRY(A.alloc(&nbuf, std::max<int64_t>(nn * nn * nb, 1)));
-    GP_TRY(A.alloc(&bb, std::max<int64_t>(ne * nn * nb, 1)));
-    GP_TRY(A.alloc(&ab, std::max<int64_t>(na * na * nb, 1)));
-    GP_HIP(hipMemcpyAsync(om, fresh.data(), sizeof(double) * nb, hipMemcpyHostToDevice, s));
-    if (ne) k_form_freq<<<blocks_for(ne * ne * nb), 256, 0, s>>>(ne * ne, nb, om, H.KLd[0], H.KLd[1], H.KLd[2], kb);
-    if (nn) {
-      k_form_freq<<<blocks_for(nn * nn * nb), 256, 0, s>>>(nn * nn, nb, om, nullptr, H.NLd[0], H.NLd[1], nbuf);
-      k_form_freq<<<blocks_for(ne * nn * nb), 256, 0, s>>>(ne * nn, nb, om, nullptr, H.BLd[0], H.BLd[1], bb);
-    }
-    if (na) k_form_freq<<<blocks_for(na * na * nb), 256, 0, s>>>(na * na, nb, om, nullptr, H.aux.NLd[0], H.aux.NLd[1], ab);
-    GP_TRY(dense_inverse_batched(kb, (int)ne, nb, s));
-    GP_TRY(dense_inverse_batched(nbuf, (int)nn, nb, s));
-    GP_TRY(dense_inverse_batched(ab, (int)na, nb, s));
-    for (int f = 0; f < nb; ++f) {
-      C.omegas.push_back(fresh[f]);
-      C.kinv.push_back((double *)(kb + (int64_t)f * ne * ne));
-      C.ninv.push_back((double *)(nbuf + (int64_t)f * nn * nn));
-      C.bl.push_back((double *)(bb + (int64_t)f * ne * nn));
-      C.ainv.push_back((double *)(ab + (int64_t)f * na * na));
-    }
-    const int need = (int)C.omegas.size();
-    if (need > C.cap) {
-      int cap = std::max(16, C.cap);
-      while (cap < need) cap *= 2;
-      const double **tab;
-      GP_TRY(A.alloc(&tab, 4 * (int64_t)cap));
-      C.tab = tab;
-      C.cap = cap;
-    }
-    std::vector<const double *> h(4 * (size_t)C.cap, nullptr);
-    for (int q = 0; q < need; ++q) {
-      h[q] = C.kinv[q]; h[C.cap + q] = C.ninv[q]; h[2 * C.cap + q] = C.bl[q]; h[3 * C.cap + q] = C.ainv[q];
-    }
-    GP_HIP(hipMemcpyAsync((void *)C.tab, h.data(), sizeof(double *) * h.size(), hipMemcpyHostToDevice, s));
-    GP_HIP(hipStreamSynchronize(s));
-    for (size_t i = 0; i < omegas.size(); ++i)
-      if ((*slots)[i] < 0)
-        for (size_t q = 0; q < C.omegas.size(); ++q)
-          if (C.omegas[q] == omegas[i]) { (*slots)[i] = (int32_t)q; break; }
-  }
+  if (fresh || !C.tab) GP_TRY(coarse_table_write(H, C, s));
   return GP_OK;
 }
 
+void coarse_cache_unpin(CoarseCache &C, int32_t slot) {
+  if (slot >= 0 && slot < (int32_t)C.e.size() && C.e[slot].pins > 0) --C.e[slot].pins;
+}
+
 void coarse_cache_free(CoarseCache &C) {
-  C.arena.free_all();
-  C.omegas.clear(); C.kinv.clear(); C.ninv.clear(); C.bl.clear(); C.ainv.clear();
-  C.tab = nullptr; C.cap = 0;
+  for (CoarseEntry &x : C.e)
+    if (x.mem) (void)hipFree(x.mem);
+  if (C.tab) (void)hipFree((void *)C.tab);
+  C.e.clear();
+  C.tab = nullptr;
+  C.cap = 0;
+  C.bytes = 0;
 }
 
 template <class T>

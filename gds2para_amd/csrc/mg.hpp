@@ -158,26 +158,48 @@ struct Hierarchy {
 };
 
 // Coarsest-level operators per frequency, formed and inverted on the device
-// once per distinct frequency and kept for the layout's lifetime.  tab holds
-// device pointers [kinv | ninv | bl | ainv] x cap, indexed by frequency slot.
+// once per distinct frequency.  tab holds device pointers
+// [kinv | ninv | bl | ainv] x cap, indexed by an entry's slot.  A solve pins
+// the entry of every device slot's frequency until that slot takes another
+// column or the solve ends; unpinned entries are evicted least recently used
+// first once the cache holds more than max_entries entries or max_bytes bytes
+// (pinned entries are never evicted, so the bound is exceeded only while more
+// distinct frequencies are in flight than it allows).
+struct CoarseEntry {
+  double omega = 0;
+  void *mem = nullptr;           // one allocation: kinv, ninv, bl, ainv
+  int64_t bytes = 0;
+  int pins = 0;
+  uint64_t used = 0;
+};
 struct CoarseCache {
-  std::vector<double> omegas;
-  std::vector<double *> kinv, ninv, bl, ainv;
+  std::vector<CoarseEntry> e;    // index = slot; mem == nullptr marks a free slot
   const double **tab = nullptr;
   int cap = 0;
-  Arena arena;
+  int64_t bytes = 0;             // entries plus the pointer table
+  uint64_t clock = 0;
+  int max_entries = 64;
+  int64_t max_bytes = (int64_t)2 << 30;
+  int64_t evictions = 0;
 };
 
-// Slots of the given angular frequencies, adding (and inverting) new ones.
+// Slots of the given angular frequencies, adding (and inverting) new ones;
+// pin != 0 also pins each returned slot once.
 gp_status coarse_cache_lookup(const Hierarchy &H, CoarseCache &C, const std::vector<double> &omegas,
-                              std::vector<int32_t> *slots, hipStream_t s);
+                              std::vector<int32_t> *slots, hipStream_t s, int pin = 0);
+void coarse_cache_unpin(CoarseCache &C, int32_t slot);
+// Device bytes of one entry of this hierarchy.
+int64_t coarse_entry_bytes(const Hierarchy &H);
 void coarse_cache_free(CoarseCache &C);
 
-gp_status build_hierarchy(gp_layout *lay, Hierarchy *H, bool want_f32, int max_levels);
-gp_status build_aux(gp_layout *lay, Hierarchy *H, bool want_f32);
+// Builds the hierarchy into A (the solver's arena, freed with the solver).
+gp_status build_hierarchy(gp_layout *lay, Arena &A, Hierarchy *H, bool want_f32, int max_levels);
+gp_status build_aux(gp_layout *lay, Arena &A, Hierarchy *H);
+// FP32 copies for the mixed-precision V-cycle, added to an FP64 hierarchy.
+gp_status hierarchy_add_f32(Arena &A, Hierarchy *H, hipStream_t s);
+gp_status aux_add_f32(Arena &A, Hierarchy *H, hipStream_t s);
 gp_status aux_smoother_data(Arena &A, const Csr &M, double **d, double **l, hipStream_t s);
 gp_status d2f_copy(const double *src, float *dst, int64_t n, hipStream_t s);
-void free_hierarchy(gp_layout *lay, Hierarchy *H);
 
 // Per-batch work space: per-slot frequencies and every level's vectors.
 template <class T>

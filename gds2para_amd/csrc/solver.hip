@@ -28,16 +28,41 @@
 
 namespace gp {
 
+// Device buffers of the batched solve (Krylov blocks, iterate, V-cycle
+// vectors), kept with the layout between calls so that repeated solves with
+// the same shape allocate nothing; a call with another shape replaces them.
+struct SlotWorkBase {
+  virtual ~SlotWorkBase() {}
+  int k = 0, m = 0, tsz = 0, tvsz = 0;
+  bool want_x = false;
+  int64_t bytes = 0;
+};
+
 struct Solver {
+  Arena arena;                   // the hierarchy's buffers
   Hierarchy H;
   bool f32 = false;
   CoarseCache cc;
+  SlotWorkBase *ws = nullptr;
 };
+
+void release_workspace(Solver *s) {
+  if (!s || !s->ws) return;
+  delete s->ws;
+  s->ws = nullptr;
+}
 
 void destroy_solver(Solver *s) {
   if (!s) return;
+  release_workspace(s);
   coarse_cache_free(s->cc);
+  s->arena.free_all();
   delete s;
+}
+
+int64_t solver_bytes(const Solver *s) {
+  if (!s) return 0;
+  return s->arena.bytes + s->cc.bytes + (s->ws ? s->ws->bytes : 0);
 }
 
 namespace {
@@ -608,6 +633,87 @@ std::vector<int64_t> port_rows(gp_layout *lay) {
   return idx;
 }
 
+// The batched solve's device buffers for k slots and restart m (see
+// SlotWorkBase): the solution in split form, the right-hand sides, the Krylov
+// basis V (m+1 blocks) and the preconditioned blocks Zu, Zp, Za (m blocks),
+// the FP64 restart residual w and the Arnoldi vector wa, per-slot scalars,
+// the Gram-Schmidt reduction buffers and the port-edge gather lists.
+template <class T, class TV>
+struct SlotWork : SlotWorkBase {
+  VWork<T> vw;
+  Arena A;
+  Z2 *b = nullptr, *xu = nullptr, *xphi = nullptr, *xa = nullptr, *w = nullptr, *yd = nullptr;
+  C2<TV> *V = nullptr, *wa = nullptr;
+  C2<T> *Zu = nullptr, *Zp = nullptr, *Za = nullptr;
+  double *om_d = nullptr, *sc_d = nullptr;
+  int32_t *mask_d = nullptr;
+  Dots dots;
+  std::vector<int64_t> prow, map;
+  int64_t *prow_d = nullptr;
+  Z2 *xo = nullptr, *xfull = nullptr;
+  int64_t max_port_edges = 1;
+  int64_t *rhs_row = nullptr; int32_t *rhs_col = nullptr; double *rhs_val = nullptr;
+  ~SlotWork() override { vwork_free(&vw); A.free_all(); }
+};
+
+template <class T, class TV>
+gp_status slot_work(SolveCtx &cx, int k, int m, bool want_x, SlotWork<T, TV> **out) {
+  Solver *sv = cx.sv;
+  if (sv->ws && sv->ws->k == k && sv->ws->m == m && sv->ws->tsz == (int)sizeof(T) &&
+      sv->ws->tvsz == (int)sizeof(TV) && sv->ws->want_x == want_x) {
+    *out = static_cast<SlotWork<T, TV> *>(sv->ws);
+    return GP_OK;
+  }
+  release_workspace(sv);
+  gp_layout *lay = cx.lay;
+  const Hierarchy &H = sv->H;
+  hipStream_t s = lay->stream;
+  const int64_t n = lay->n, nn = H.L[0].nn, na = H.aux.naux;
+  const gp_mesh &mesh = lay->hm.view;
+  auto *W = new SlotWork<T, TV>();
+  W->k = k; W->m = m; W->tsz = (int)sizeof(T); W->tvsz = (int)sizeof(TV); W->want_x = want_x;
+  auto st = [&]() -> gp_status {
+    GP_TRY(vwork_create<T>(lay, H, k, &W->vw));
+    Arena &A = W->A;
+    GP_TRY(A.alloc(&W->b, n * k)); GP_TRY(A.alloc(&W->xu, n * k)); GP_TRY(A.alloc(&W->xphi, nn * k));
+    GP_TRY(A.alloc(&W->xa, na * k));
+    GP_TRY(A.alloc(&W->V, (int64_t)(m + 1) * n * k)); GP_TRY(A.alloc(&W->w, n * k));
+    // the Arnoldi vector w = A z is stored in the basis precision (its products
+    // and sums are FP64); w itself holds the FP64 restart residuals
+    if constexpr (std::is_same<TV, double>::value) W->wa = W->w;
+    else GP_TRY(A.alloc(&W->wa, n * k));
+    GP_TRY(A.alloc(&W->Zu, (int64_t)m * n * k)); GP_TRY(A.alloc(&W->Zp, (int64_t)m * nn * k));
+    GP_TRY(A.alloc(&W->Za, (int64_t)m * na * k));
+    GP_TRY(A.alloc(&W->om_d, k)); GP_TRY(A.alloc(&W->sc_d, k)); GP_TRY(A.alloc(&W->mask_d, k));
+    GP_TRY(A.alloc(&W->yd, (int64_t)m * k));
+    GP_TRY(W->dots.init(A, k, m + 1));
+    W->prow = port_rows(lay);
+    W->map.resize(lay->g.Ne);
+    gp_edge_map(&mesh, W->map.data(), nullptr);
+    GP_TRY(A.alloc(&W->prow_d, (int64_t)W->prow.size())); GP_TRY(A.alloc(&W->xo, (int64_t)W->prow.size() * k));
+    GP_HIP(hipMemcpyAsync(W->prow_d, W->prow.data(), sizeof(int64_t) * W->prow.size(), hipMemcpyHostToDevice, s));
+    if (want_x) GP_TRY(A.alloc(&W->xfull, n * k));
+    // right-hand-side entries of newly placed columns: at most k times the
+    // largest port's edge count, allocated once (a free per refill would
+    // synchronise the device and stall other layouts' streams)
+    for (int32_t p = 0; p < mesh.num_ports; ++p) {
+      int64_t c = 0;
+      for (int32_t sd = mesh.port_side_ptr[p]; sd < mesh.port_side_ptr[p + 1]; ++sd)
+        c += mesh.side_edge_ptr[sd + 1] - mesh.side_edge_ptr[sd];
+      W->max_port_edges = std::max(W->max_port_edges, c);
+    }
+    GP_TRY(A.alloc(&W->rhs_row, W->max_port_edges * k)); GP_TRY(A.alloc(&W->rhs_col, W->max_port_edges * k));
+    GP_TRY(A.alloc(&W->rhs_val, W->max_port_edges * k));
+    GP_HIP(hipStreamSynchronize(s));
+    return GP_OK;
+  }();
+  if (st != GP_OK) { delete W; return st; }
+  W->bytes = W->A.bytes + W->vw.arena.bytes;
+  sv->ws = W;
+  *out = W;
+  return GP_OK;
+}
+
 // Solve the columns io provides on k slots with preconditioner precision T
 // and the Arnoldi basis stored in TV.  Residuals, operator products, dot
 // products, the Hessenberg matrix and the iterate are FP64 in every case; an
@@ -635,56 +741,35 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
   const OuterOp op = outer_op(lay, H);
   const Csr &G0 = H.L[0].G, &Gx = H.aux.Gx;
 
-  VWork<T> vw;
-  struct VwGuard { VWork<T> *w; ~VwGuard() { vwork_free(w); } } vg{&vw};
-  GP_TRY(vwork_create<T>(lay, H, k, &vw));
+  SlotWork<T, TV> *W = nullptr;
+  GP_TRY((slot_work<T, TV>(cx, k, m, io.want_x(), &W)));
+  VWork<T> &vw = W->vw;
   vw.nu_aux = o.smooth_aux > 0 ? std::min(o.smooth_aux, 8) : 1;
   vw.aux_post = o.aux_post > 0;
   vw.kt = &lay->ktimer;
   vw.sell_min_rows = o.sell_min_rows > 0 ? o.sell_min_rows : 1;
-  DevBuf D;
-  Arena &A = D.A;
-  Z2 *b, *xu, *xphi, *xa, *w, *yd;
-  C2<TV> *V;
-  C2<T> *Zu, *Zp, *Za;
-  double *om_d, *sc_d;
-  int32_t *mask_d;
-  GP_TRY(A.alloc(&b, n * k)); GP_TRY(A.alloc(&xu, n * k)); GP_TRY(A.alloc(&xphi, nn * k)); GP_TRY(A.alloc(&xa, na * k));
-  GP_TRY(A.alloc(&V, (int64_t)(m + 1) * n * k)); GP_TRY(A.alloc(&w, n * k));
-  // the Arnoldi vector w = A z is stored in the basis precision (its products
-  // and sums are FP64); w itself holds the FP64 restart residuals
-  C2<TV> *wa;
-  if constexpr (std::is_same<TV, double>::value) wa = w;
-  else GP_TRY(A.alloc(&wa, n * k));
-  GP_TRY(A.alloc(&Zu, (int64_t)m * n * k)); GP_TRY(A.alloc(&Zp, (int64_t)m * nn * k)); GP_TRY(A.alloc(&Za, (int64_t)m * na * k));
-  GP_TRY(A.alloc(&om_d, k)); GP_TRY(A.alloc(&sc_d, k)); GP_TRY(A.alloc(&mask_d, k)); GP_TRY(A.alloc(&yd, (int64_t)m * k));
-  Dots dots;
-  GP_TRY(dots.init(A, k, m + 1));
+  Z2 *b = W->b, *xu = W->xu, *xphi = W->xphi, *xa = W->xa, *w = W->w, *yd = W->yd;
+  C2<TV> *V = W->V, *wa = W->wa;
+  C2<T> *Zu = W->Zu, *Zp = W->Zp, *Za = W->Za;
+  double *om_d = W->om_d, *sc_d = W->sc_d;
+  int32_t *mask_d = W->mask_d;
+  Dots &dots = W->dots;
   GP_HIP(hipMemsetAsync(b, 0, sizeof(Z2) * n * k, s));
   GP_HIP(hipMemsetAsync(xu, 0, sizeof(Z2) * n * k, s));
   GP_HIP(hipMemsetAsync(xphi, 0, sizeof(Z2) * nn * k, s));
   GP_HIP(hipMemsetAsync(xa, 0, sizeof(Z2) * na * k, s));
-  std::vector<int64_t> prow = port_rows(lay), map(lay->g.Ne);
-  gp_edge_map(&mesh, map.data(), nullptr);
-  int64_t *prow_d;
-  Z2 *xo;
-  GP_TRY(A.alloc(&prow_d, (int64_t)prow.size())); GP_TRY(A.alloc(&xo, (int64_t)prow.size() * k));
-  GP_HIP(hipMemcpyAsync(prow_d, prow.data(), sizeof(int64_t) * prow.size(), hipMemcpyHostToDevice, s));
-  Z2 *xfull = nullptr;
-  if (io.want_x()) GP_TRY(A.alloc(&xfull, n * k));
-  // right-hand-side entries of newly placed columns: at most k times the
-  // largest port's edge count, allocated once (a free per refill would
-  // synchronise the device and stall other layouts' streams)
-  int64_t max_port_edges = 1;
-  for (int32_t p = 0; p < mesh.num_ports; ++p) {
-    int64_t c = 0;
-    for (int32_t sd = mesh.port_side_ptr[p]; sd < mesh.port_side_ptr[p + 1]; ++sd)
-      c += mesh.side_edge_ptr[sd + 1] - mesh.side_edge_ptr[sd];
-    max_port_edges = std::max(max_port_edges, c);
-  }
-  int64_t *rhs_row; int32_t *rhs_col; double *rhs_val;
-  GP_TRY(A.alloc(&rhs_row, max_port_edges * k)); GP_TRY(A.alloc(&rhs_col, max_port_edges * k));
-  GP_TRY(A.alloc(&rhs_val, max_port_edges * k));
+  const std::vector<int64_t> &prow = W->prow, &map = W->map;
+  int64_t *prow_d = W->prow_d;
+  Z2 *xo = W->xo, *xfull = W->xfull;
+  const int64_t max_port_edges = W->max_port_edges;
+  int64_t *rhs_row = W->rhs_row; int32_t *rhs_col = W->rhs_col; double *rhs_val = W->rhs_val;
+  // coarse-cache entries held by the slots (released when a slot takes
+  // another column, and all of them when the call ends)
+  std::vector<int32_t> pinned(k, -1);
+  struct PinGuard {
+    CoarseCache &c; std::vector<int32_t> &p;
+    ~PinGuard() { for (int32_t &q : p) { coarse_cache_unpin(c, q); q = -1; } }
+  } pg{cx.sv->cc, pinned};
 
   const unsigned gnk = blocks_for(n * k), gnnk = blocks_for(nn * k), gnak = blocks_for(na * k);
   // slot state
@@ -723,8 +808,9 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
       omega[j] = 2. * kPi * f;
       {
         std::vector<int32_t> sl;
-        GP_TRY(coarse_cache_lookup(H, cx.sv->cc, std::vector<double>{omega[j]}, &sl, s));
-        fsl[j] = sl[0];
+        GP_TRY(coarse_cache_lookup(H, cx.sv->cc, std::vector<double>{omega[j]}, &sl, s, 1));
+        coarse_cache_unpin(cx.sv->cc, pinned[j]);
+        pinned[j] = fsl[j] = sl[0];
       }
       double b2 = 0.;
       for (int32_t sd = mesh.port_side_ptr[c.port]; sd < mesh.port_side_ptr[c.port + 1]; ++sd)
@@ -1096,12 +1182,29 @@ extern "C" gp_status gp_prepare_solver(gp_layout *lay, int32_t precision) {
   if (!lay) return fail(GP_ERR_ARG, "null layout");
   const bool want32 = precision == GP_PREC_MIXED;
   if (lay->solver && (!want32 || lay->solver->f32)) return GP_OK;
-  if (lay->solver) { destroy_solver(lay->solver); lay->solver = nullptr; }
+  if (lay->solver) {
+    // an FP64 hierarchy gains the FP32 copies in place
+    Solver *sv = lay->solver;
+    GP_TRY(hierarchy_add_f32(sv->arena, &sv->H, lay->stream));
+    sv->f32 = true;
+    return GP_OK;
+  }
   auto *sv = new Solver();
-  gp_status st = build_hierarchy(lay, &sv->H, want32, 12);
-  if (st != GP_OK) { delete sv; return st; }
+  gp_status st = build_hierarchy(lay, sv->arena, &sv->H, want32, 12);
+  if (st != GP_OK) { destroy_solver(sv); return st; }
   sv->f32 = want32;
   lay->solver = sv;
+  return GP_OK;
+}
+
+extern "C" gp_status gp_layout_trim(gp_layout *lay) {
+  if (!lay) return fail(GP_ERR_ARG, "null layout");
+  if (!lay->solver) return GP_OK;
+  GP_HIP(hipStreamSynchronize(lay->stream));
+  release_workspace(lay->solver);
+  for (const CoarseEntry &x : lay->solver->cc.e)
+    if (x.pins) return fail(GP_ERR_STATE, "gp_layout_trim: a solve on this layout is running");
+  coarse_cache_free(lay->solver->cc);
   return GP_OK;
 }
 
@@ -1121,6 +1224,12 @@ extern "C" gp_status gp_hierarchy_info_get(gp_layout *lay, gp_hierarchy_info *ou
   }
   out->naux = H.aux.naux;
   out->setup_ms = H.setup_ms;
+  const CoarseCache &cc = lay->solver->cc;
+  out->coarse_entry_bytes = coarse_entry_bytes(H);
+  out->coarse_entries = 0;
+  for (const CoarseEntry &x : cc.e) out->coarse_entries += x.mem ? 1 : 0;
+  out->coarse_evictions = cc.evictions;
+  out->workspace_bytes = lay->solver->ws ? lay->solver->ws->bytes : 0;
   return GP_OK;
 }
 
@@ -1144,7 +1253,7 @@ extern "C" gp_status gp_solve_fullwave_ex(gp_layout *lay, const double *freqs, i
   std::vector<double> fr(freqs, freqs + nf), om(nf);
   for (int32_t f = 0; f < nf; ++f) om[f] = 2. * kPi * fr[f];
   std::vector<int32_t> cslot;
-  GP_TRY(coarse_cache_lookup(sv->H, sv->cc, om, &cslot, lay->stream));
+  if (nf <= sv->cc.max_entries / 2) GP_TRY(coarse_cache_lookup(sv->H, sv->cc, om, &cslot, lay->stream));
   auto t1 = std::chrono::steady_clock::now();
   gp_solve_stats st{};
   // column ids in request order, frequency-major: id = f*np + p.  They enter
@@ -1195,7 +1304,10 @@ extern "C" gp_status gp_solve_workspace_bytes(gp_layout *lay, int32_t precision,
   const int m = std::min(std::max(2, restart), 64);
   const int k = pow2_at_least(std::min<int32_t>(slots, 64));
   const bool f32 = lay->solver->f32 && precision == GP_PREC_MIXED;
-  *bytes = (int64_t)k * bytes_per_slot(lay->solver->H, m, f32, f32);
+  const CoarseCache &cc = lay->solver->cc;
+  const int64_t coarse = std::min<int64_t>(cc.max_bytes, (int64_t)std::max(cc.max_entries, k) *
+                                                         coarse_entry_bytes(lay->solver->H));
+  *bytes = (int64_t)k * bytes_per_slot(lay->solver->H, m, f32, f32) + coarse;
   return GP_OK;
 }
 

@@ -41,6 +41,10 @@ class Layout:
     def device_bytes(self):
         return int(self.lib.gp_layout_device_bytes(self.h))
 
+    def trim(self):
+        """Releases the cached solve workspace and coarse-level cache."""
+        capi.check(self.lib.gp_layout_trim(self.h), "gp_layout_trim")
+
     # generateStiff --------------------------------------------------------
     def assemble(self):
         capi.check(self.lib.gp_assemble_stiffness(self.h), "gp_assemble_stiffness")
@@ -102,7 +106,9 @@ class Layout:
         capi.check(self.lib.gp_hierarchy_info_get(self.h, C.byref(h)), "gp_hierarchy_info_get")
         el, al = h.edge_levels, h.aux_levels
         return dict(edge=[(h.ne[i], h.nn[i], h.nnz_k[i]) for i in range(el)],
-                    aux=[(h.na[i], h.nnz_n[i], h.nlong[i]) for i in range(al)], naux=h.naux, setup_ms=h.setup_ms)
+                    aux=[(h.na[i], h.nnz_n[i], h.nlong[i]) for i in range(al)], naux=h.naux, setup_ms=h.setup_ms,
+                    coarse_entry_bytes=h.coarse_entry_bytes, coarse_entries=h.coarse_entries,
+                    coarse_evictions=h.coarse_evictions, workspace_bytes=h.workspace_bytes)
 
     def apply_fullwave(self, freqs, X):
         """Y = A(w) X (gp_apply_fullwave): X [k, n] complex, one column per

@@ -73,6 +73,9 @@ WORKLOADS = {
     "manhattan_200x200x24": lambda: manhattan_spec((200, 200, 24), 8, 10, (1e7, 1e10), 2000),
     "manhattan_400x400x32": lambda: manhattan_spec((400, 400, 32), 16, 20, (1e7, 1e10), 3000),
     "manhattan_300x300x32": lambda: manhattan_spec((300, 300, 32), 64, 32, (1e8, 5e10), 4000),
+    # the 200x200x24 and 300x300x32 generators scaled to 48x48x12 cells, 4 ports
+    "manhattan_48x48x12": lambda: manhattan_spec((48, 48, 12), 4, 4, (1e7, 1e10), 2000),
+    "manhattan_48x48x12_hi": lambda: manhattan_spec((48, 48, 12), 4, 4, (1e8, 5e10), 4000),
 }
 
 

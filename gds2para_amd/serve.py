@@ -64,22 +64,37 @@ class _Resident:
 class Server:
     """Continuous-batching server over a backend (GpuBackend for the library).
 
-    backend.open(key) -> handle; backend.bytes(handle) -> device bytes incl.
-    the slot workspace; backend.stream(handle, pull, waiting, push) runs until
-    pull returns -1; backend.close(handle)."""
+    backend.estimate(key) -> bytes before opening; backend.open(key) ->
+    handle; backend.bytes(handle) -> device bytes incl. the slot workspace;
+    backend.stream(handle, pull, waiting, push) runs until pull returns -1;
+    backend.close(handle).
+
+    Admission runs on a thread of its own.  It is woken when a request
+    arrives, when a layout goes idle and when a layout has been opened, and it
+    is the only thread that opens, evicts or queues columns.  Opening a layout
+    (assembly and multigrid setup, seconds at millions of edges) therefore
+    holds neither the server lock nor another layout's stream thread."""
 
     def __init__(self, backend, budget_bytes, clock=time.monotonic):
         self.backend = backend
         self.budget = budget_bytes
         self.clock = clock
         self.res = {}                      # key -> _Resident
-        self.backlog = collections.deque() # requests waiting for memory
+        self.backlog = collections.deque() # requests not yet queued on a layout
         self.requests = {}
         self.tags = {}                     # tag -> (rid, freq index, port)
         self.lock = threading.RLock()
         self.all_done = threading.Condition(self.lock)
+        self._kick = threading.Condition(self.lock)
+        self._dirty = False
+        self._stop = False
+        self._opening = 0                  # bytes reserved for a layout being opened
+        self.evictions = 0
+        self.error = None
         self._next_rid = 0
         self._next_tag = 0
+        self._admitter = threading.Thread(target=self._admission_loop, daemon=True)
+        self._admitter.start()
 
     # ---- public ----------------------------------------------------------
     def submit(self, req: Request) -> int:
@@ -89,8 +104,12 @@ class Server:
             req.arrival = self.clock()
             req.remaining = len(req.ports) * len(req.freqs)
             self.requests[req.rid] = req
-            self.backlog.append(req)
-            self._admit()
+            r = self.res.get(req.layout)
+            if r is not None and not any(q.layout == req.layout for q in self.backlog):
+                self._enqueue(r, req)       # resident: no wait for the admission thread
+            else:
+                self.backlog.append(req)
+                self._wake()
             return req.rid
 
     def wait(self, timeout=None):
@@ -102,11 +121,17 @@ class Server:
                 if left is not None and left <= 0:
                     raise TimeoutError("requests still running")
                 self.all_done.wait(0.05 if left is None else min(0.05, left))
+                if self.error:
+                    raise RuntimeError(f"admission: {self.error}")
                 for r in self.res.values():
                     if r.error:
                         raise RuntimeError(f"layout {r.key}: {r.error}")
 
     def close(self):
+        with self.lock:
+            self._stop = True
+            self._kick.notify_all()
+        self._admitter.join()
         with self.lock:
             keys = list(self.res)
         for k in keys:
@@ -114,59 +139,114 @@ class Server:
 
     def resident_bytes(self):
         with self.lock:
-            return sum(r.nbytes for r in self.res.values())
+            return sum(r.nbytes for r in self.res.values()) + self._opening
 
     # ---- admission ---------------------------------------------------------
-    def _admit(self):
-        """Moves backlog requests whose layout is (or can be made) resident into
-        that layout's queue; a request whose layout does not fit yet waits."""
-        waiting = collections.deque()
-        while self.backlog:
-            req = self.backlog.popleft()
-            r = self.res.get(req.layout)
-            if r is None:
-                need = self.backend.estimate(req.layout)
-                me = threading.current_thread()
-                while self.resident_bytes() + need > self.budget:
-                    idle = [x for x in self.res.values()
-                            if not x.q and x.inflight == 0 and x.thread is not me]
-                    if not idle:
-                        break
-                    self._evict(min(idle, key=lambda x: x.last_used).key)
-                if self.resident_bytes() + need > self.budget and self.res:
-                    waiting.append(req)    # retried when a layout goes idle
+    def _wake(self):
+        """Asks the admission thread for a pass (caller holds the lock)."""
+        self._dirty = True
+        self._kick.notify_all()
+
+    def _admission_loop(self):
+        try:
+            while True:
+                with self.lock:
+                    while not self._dirty and not self._stop:
+                        self._kick.wait()
+                    if self._stop:
+                        return
+                    self._dirty = False
+                    plan = self._plan()
+                if plan is None:
                     continue
-                r = self._open(req.layout)
-            nps = self.backend.num_ports(r.handle)
-            req.Z = np.zeros((len(req.freqs), nps, nps), complex)
-            with r.cv:
-                for fi, f in enumerate(req.freqs):
-                    for p in req.ports:
-                        tag = self._next_tag
-                        self._next_tag += 1
-                        self.tags[tag] = (req.rid, fi, p)
-                        r.q.append((p, float(f), tag))
-                r.last_used = self.clock()
-                r.cv.notify_all()
+                key, victims, need = plan
+                for v in victims:           # already out of self.res: idle, nothing refills them
+                    self._shutdown(v)
+                try:
+                    handle = self.backend.open(key)
+                    nbytes = self.backend.bytes(handle)
+                except Exception as e:     # surfaced by wait()
+                    with self.lock:
+                        self._opening -= need
+                        self.error = f"opening {key}: {e}"
+                        self.all_done.notify_all()
+                    return
+                r = _Resident(key, handle, nbytes)
+                r.thread = threading.Thread(target=self._run, args=(r,), daemon=True)
+                with self.lock:
+                    self._opening -= need
+                    self.res[key] = r
+                    self._wake()            # queue the columns that waited for it
+                r.thread.start()
+        except Exception as e:
+            with self.lock:
+                self.error = str(e)
+                self.all_done.notify_all()
+
+    def _plan(self):
+        """Caller holds the lock.  Queues every backlog request whose layout
+        is resident, then returns (key, victims, bytes) for the oldest request
+        whose layout is not: victims are idle layouts (least recently used
+        first), already taken out of self.res, whose eviction makes it fit;
+        None if nothing is to be opened now (the request waits for a layout to
+        go idle).  A layout larger than the whole budget is opened only when
+        nothing else is resident."""
+        waiting = collections.deque()
+        plan, blocked = None, False
+        for req in self.backlog:
+            r = self.res.get(req.layout)
+            if r is not None:
+                self._enqueue(r, req)
+                continue
+            waiting.append(req)
+            # layouts open one at a time, oldest request first (a large layout
+            # is not starved by smaller ones that keep fitting)
+            if plan is not None or blocked or self._opening:
+                continue
+            need = self.backend.estimate(req.layout)
+            used = sum(x.nbytes for x in self.res.values())
+            idle = sorted((x for x in self.res.values() if not x.q and x.inflight == 0), key=lambda x: x.last_used)
+            victims = []
+            while used + need > self.budget and idle:
+                v = idle.pop(0)
+                victims.append(v)
+                used -= v.nbytes
+            if used + need > self.budget and len(victims) < len(self.res):
+                blocked = True              # retried when a layout goes idle
+                continue
+            for v in victims:
+                del self.res[v.key]
+                self.evictions += 1
+            self._opening = need
+            plan = (req.layout, victims, need)
         self.backlog = waiting
+        return plan
 
-    def _open(self, key):
-        handle = self.backend.open(key)
-        r = _Resident(key, handle, self.backend.bytes(handle))
-        self.res[key] = r
-        r.thread = threading.Thread(target=self._run, args=(r,), daemon=True)
-        r.thread.start()
-        return r
+    def _enqueue(self, r, req):
+        nps = self.backend.num_ports(r.handle)
+        req.Z = np.zeros((len(req.freqs), nps, nps), complex)
+        with r.cv:
+            for fi, f in enumerate(req.freqs):
+                for p in req.ports:
+                    tag = self._next_tag
+                    self._next_tag += 1
+                    self.tags[tag] = (req.rid, fi, p)
+                    r.q.append((p, float(f), tag))
+            r.last_used = self.clock()
+            r.cv.notify_all()
 
-    def _evict(self, key):
-        r = self.res.pop(key, None)
-        if r is None:
-            return
+    def _shutdown(self, r):
         with r.cv:
             r.closed = True
             r.cv.notify_all()
         r.thread.join()
         self.backend.close(r.handle)
+
+    def _evict(self, key):
+        with self.lock:
+            r = self.res.pop(key, None)
+        if r is not None:
+            self._shutdown(r)
 
     # ---- the stream of one layout -------------------------------------------
     def _run(self, r: _Resident):
@@ -200,7 +280,9 @@ class Server:
                 with r.cv:
                     r.inflight -= 1
                     r.last_used = self.clock()
-                self._admit()
+                    idle = not r.q and r.inflight == 0
+                if idle and self.backlog:
+                    self._wake()            # a waiting request may now evict this layout
                 self.all_done.notify_all()
 
         try:

@@ -160,8 +160,14 @@ typedef struct gp_layout gp_layout;
 gp_status gp_layout_create(const gp_mesh *mesh, gp_layout **out);
 void gp_layout_destroy(gp_layout *lay);
 
-/* Device bytes currently held by the layout (matrices, hierarchy, workspaces). */
+/* Device bytes currently held by the layout (matrices, hierarchy, the cached
+ * solve workspace and coarse-level cache). */
 int64_t gp_layout_device_bytes(const gp_layout *lay);
+
+/* Releases the solve workspace and the per-frequency coarse-level cache that
+ * solves keep with the layout (the matrices and hierarchy stay).  No
+ * counterpart in the reference, which frees its PARDISO factors per call. */
+gp_status gp_layout_trim(gp_layout *lay);
 
 /* ---- stiffness (generateStiff + mklMatrixMulti_nt) ---------------------- */
 /* Assembles S on the device: CSR, int64 row pointers, int32 columns in the
@@ -202,7 +208,8 @@ typedef struct gp_solve_opts {
   int32_t max_iters;              /* total Krylov iterations per column           */
   int32_t restart;                /* FGMRES restart length m                      */
   int32_t precision;              /* gp_precision                                 */
-  int32_t batch;                  /* columns per batch; 0 = autotuned, 1 = single */
+  int32_t batch;                  /* device slots (columns solved side by side);
+                                     0 = 16 with MIXED, 1 with FP64; 1 = single RHS */
   int32_t verbose;
   int32_t smooth_edge;            /* edge/nodal smoothing sweeps per level (0: 1) */
   int32_t smooth_aux;             /* auxiliary AMG sweeps per level (0: 1)       */
@@ -246,6 +253,10 @@ typedef struct {
   int64_t na[GP_MAX_LEVELS], nnz_n[GP_MAX_LEVELS], nlong[GP_MAX_LEVELS];
   int64_t naux;                   /* columns of [V0d|V0c]                          */
   double setup_ms;
+  int64_t coarse_entry_bytes;     /* device bytes of one frequency's coarse inverses */
+  int32_t coarse_entries;         /* frequencies held in the coarse-level cache    */
+  int64_t coarse_evictions;       /* entries evicted so far (least recently used)  */
+  int64_t workspace_bytes;        /* cached solve workspace (0 if none)            */
 } gp_hierarchy_info;
 gp_status gp_hierarchy_info_get(gp_layout *lay, gp_hierarchy_info *out);
 
@@ -296,7 +307,11 @@ gp_status gp_solve_fullwave_ex(gp_layout *lay, const double *freqs, int32_t nf,
  * Returns when pull() has returned -1 and every slot is idle.  The callbacks
  * run on the calling thread. */
 /* Device bytes a solve with `slots` slots and restart length `restart` adds
- * on top of gp_layout_device_bytes (Krylov bases, iterates, V-cycle vectors);
+ * on top of gp_layout_device_bytes of a layout that holds no solve workspace
+ * yet: the Krylov bases, iterates and V-cycle vectors, plus the bound of the
+ * per-frequency coarse-level cache (64 frequencies or 2 GiB, whichever is
+ * less; entries in use by a running solve are never evicted).  A solve keeps
+ * both with the layout for the next call (gp_layout_trim releases them);
  * prepares the solver first if needed.  For admission under a memory budget. */
 gp_status gp_solve_workspace_bytes(gp_layout *lay, int32_t precision, int32_t slots, int32_t restart, int64_t *bytes);
 

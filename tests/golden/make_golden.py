@@ -5,12 +5,19 @@ Fixtures (numpy .npz, no pickles):
                            port-edge values of x (refined FP64 direct solve)
   manhattan_24x20x10.npz   Z at 1e8, 1e9, 1e10 Hz for 3 ports (refined direct
                            solves) and the port-edge values of every column
+  manhattan_48x48x12_lo.npz   the 200x200x24 generator (seed 2000) scaled to
+                           48x48x12 cells (85,260 unknowns), 4 ports, at the
+                           band edges 1e7, 1e8, 1e9, 1e10 Hz
+  manhattan_48x48x12_hi.npz   the 300x300x32 generator (seed 4000) scaled to
+                           48x48x12 cells, 4 ports, at 1e8, 1e10, 2e10, 5e10 Hz
 
 The oracle's own checks (tests/test_oracle.py) pin it independently; these
 files freeze its results so the GPU tests compare against recorded vectors
 without re-running the slow refined direct solves.
 
-Run: python tests/golden/make_golden.py
+Run: python tests/golden/make_golden.py [--band-edges]
+(--band-edges writes only the two 48x48x12 files: one SuperLU factorization
+per frequency, about 75 s each on one host core)
 """
 import hashlib
 import os
@@ -45,7 +52,36 @@ def digest(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
+# The band-edge instances: the generators of the 200x200x24 and 300x300x32
+# workloads at 48x48x12 cells, 4 ports, at the edges of their bands.
+BAND_EDGE = {
+    "manhattan_48x48x12_lo.npz": dict(seed=2000, freqs=[1e7, 1e8, 1e9, 1e10], f_range=(1e7, 1e10)),
+    "manhattan_48x48x12_hi.npz": dict(seed=4000, freqs=[1e8, 1e10, 2e10, 5e10], f_range=(1e8, 5e10)),
+}
+
+
+def band_edge_spec(name):
+    b = BAND_EDGE[name]
+    return M.manhattan_spec((48, 48, 12), n_ports=4, nfreq=len(b["freqs"]), f_range=b["f_range"], seed=b["seed"])
+
+
+def band_edges():
+    for name, b in BAND_EDGE.items():
+        m = M.Mesh(band_edge_spec(name))
+        fs = np.array(b["freqs"])
+        X, Z = oracle.direct_solve(m, fs, refine=True)
+        rows = port_rows(m)
+        xp = np.array([[X[k][p][rows] for p in range(m.num_ports)] for k in range(len(fs))])
+        xn = np.array([[np.linalg.norm(X[k][p]) for p in range(m.num_ports)] for k in range(len(fs))])
+        np.savez(os.path.join(HERE, name), freqs=fs, Z=Z, port_rows=rows, x_port=xp, x_norm=xn,
+                 n=np.int64(m.n))
+        print("written", name, flush=True)
+
+
 def main():
+    if "--band-edges" in sys.argv:
+        band_edges()
+        return
     m = M.Mesh(M.microstrip_spec())
     S = oracle.stiffness_csr(m)
     vs = oracle.v0_sizes(m)

@@ -101,3 +101,43 @@ def test_gpu_z_matches_fixture(prec, batch, sell):
             for p in range(m.num_ports):
                 assert _rel(X[k * m.num_ports + p][rows], xp[k, p]) < ZTOL
         lay.close()
+
+
+# Band edges: the 200x200x24 generator (seed 2000) and the 300x300x32
+# generator (seed 4000) at 48x48x12 cells (85,260 unknowns), 4 ports, at the
+# ends of their bands (10 MHz ... 50 GHz).
+BAND = {"manhattan_48x48x12_lo.npz": "manhattan_48x48x12", "manhattan_48x48x12_hi.npz": "manhattan_48x48x12_hi"}
+
+
+@pytest.mark.parametrize("name", sorted(BAND))
+def test_band_edge_fixture_shapes(name):
+    g = _load(name)
+    nf = len(g["freqs"])
+    assert g["Z"].shape == (nf, 16) and g["x_port"].shape[:2] == (nf, 4)
+    m = M.Mesh(M.WORKLOADS[BAND[name]]())
+    assert int(g["n"]) == m.n == 85260
+    for k in range(nf):
+        Zk = g["Z"][k].reshape(4, 4)
+        assert np.abs(Zk - Zk.T).max() / np.abs(Zk).max() < 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,batch", [(capi.GP_PREC_FP64, 1), (capi.GP_PREC_FP64, 16),
+                                        (capi.GP_PREC_MIXED, 1), (capi.GP_PREC_MIXED, 16)])
+def test_gpu_band_edges_match_fixture(prec, batch):
+    from gds2para_amd.layout import Layout
+    for name in sorted(BAND):
+        g = _load(name)
+        m = M.Mesh(M.WORKLOADS[BAND[name]]())
+        print(f"[golden] {name} prec={prec} batch={batch}", flush=True)
+        lay = Layout(m)
+        Z, st, X = lay.solve(g["freqs"], precision=prec, rtol=1e-8, batch=batch, want_x=True, max_iters=1500)
+        assert st["status"] == capi.GP_OK and st["max_rel_residual"] <= 1e-8
+        rows = g["port_rows"]
+        for k, f in enumerate(g["freqs"]):
+            ez = _rel(Z[k], g["Z"][k])
+            ex = max(_rel(X[k * 4 + p][rows], g["x_port"][k, p]) for p in range(4))
+            its = max(st["iters"][k * 4:(k + 1) * 4])
+            print(f"[golden] {name} f={f:.0e} Z err {ez:.2e} x err {ex:.2e} iters {its}", flush=True)
+            assert ez < ZTOL and ex < ZTOL, (name, f, ez, ex)
+        lay.close()

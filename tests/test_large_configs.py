@@ -82,3 +82,25 @@ def test_batching_config_300x300x32():
     print(f"[large] port 0: mixed batched vs FP64 {rel:.2e}", flush=True)
     assert rel < 1e-6
     lay.close()
+
+
+def test_batching_config_300x300x32_upper_band():
+    """The top of the 300x300x32 band (1e10 and 5e10 Hz), where the
+    -w^2 eps term is largest: 16 ports per frequency on 16 slots, the 16x16
+    Z block among them reciprocal, and port 0 matching the plain FP64 solve."""
+    m, lay = _setup("manhattan_300x300x32")
+    npt = m.num_ports
+    sub = list(range(16))
+    for f in (1e10, 5e10):
+        print(f"[large] 300x300x32 ports 0-15 at {f:.0e} Hz on 16 slots", flush=True)
+        Z, st, _ = lay.solve([f], ports=sub, precision=capi.GP_PREC_MIXED, rtol=1e-8, batch=16, max_iters=1500)
+        print(f"[large] converged {st['converged']}/16, iterations max {st['max_iters']}, "
+              f"residual {st['max_rel_residual']:.2e}, {st['solve_ms']:.0f} ms", flush=True)
+        assert st["status"] == capi.GP_OK and st["converged"] == 16 and st["max_rel_residual"] <= 1e-8
+        Zm = Z[0].reshape(npt, npt)                    # [src, resp]
+        blk = Zm[np.ix_(sub, sub)]
+        asym = _rel(blk, blk.T)
+        rel = _rel(Zm[0], _fp64_column(lay, f, 0))
+        print(f"[large] {f:.0e} Hz: reciprocity {asym:.2e}, port 0 mixed vs FP64 {rel:.2e}", flush=True)
+        assert asym < 1e-8 and rel < 1e-6
+    lay.close()

@@ -95,6 +95,55 @@ def test_serve_budget_evicts_idle_layouts():
     srv.close()
 
 
+def test_serve_backlog_admitted_when_last_layout_goes_idle():
+    """Budget for one layout: a slow request for a, then at once one for b.
+    b can only be admitted once a is idle, and a goes idle inside its own
+    stream's last push; admission must still run then (it once ran only on
+    the stream thread, which could not evict its own layout, and nothing
+    retried the backlog afterwards)."""
+    be = MockBackend({"a": 60, "b": 60}, slots=1, delay=0.05)
+    srv = Server(be, budget_bytes=100)
+    a = Request("a", [0], [1e9])
+    b = Request("b", [1], [1e9])
+    srv.submit(a)
+    srv.submit(b)
+    try:
+        srv.wait(timeout=10)
+    finally:
+        srv.close()
+    assert a.remaining == 0 and b.remaining == 0
+    assert be.opened == ["a", "b"] and be.closed[0] == "a"
+    assert srv.evictions == 1
+
+
+def test_serve_open_runs_off_the_lock():
+    """Opening a layout (seconds on the GPU) holds neither the server lock nor
+    another layout's stream: requests on a resident layout complete while a
+    second layout is still being opened."""
+    class SlowOpen(MockBackend):
+        def open(self, key):
+            if key == "slow":
+                time.sleep(1.0)
+            return super().open(key)
+
+    be = SlowOpen({"fast": 1, "slow": 1}, slots=2, delay=0.005)
+    srv = Server(be, budget_bytes=100)
+    warm = Request("fast", [0], [1e9])
+    srv.submit(warm)
+    srv.wait(timeout=5)
+    srv.submit(Request("slow", [0], [1e9]))
+    time.sleep(0.05)                                 # the slow open is under way
+    t = time.monotonic()
+    quick = Request("fast", [1, 2], [1e9, 2e9])
+    srv.submit(quick)
+    while quick.remaining and time.monotonic() - t < 5:
+        time.sleep(0.005)
+    dt = time.monotonic() - t
+    srv.wait(timeout=10)
+    srv.close()
+    assert dt < 0.5, dt
+
+
 def test_serve_same_alone_and_in_mix_mock():
     be = MockBackend({"a": 1, "b": 1}, slots=3)
     alone = Server(be, 100)
@@ -317,3 +366,57 @@ def test_stream_rejects_malformed_columns():
         assert conv == 1 and rr <= 1e-8
         assert np.abs(np.array(z) - Zr[p]).max() / np.abs(Zr[p]).max() < 1e-6
     lay.close()
+
+
+@pytest.mark.gpu
+def test_serve_gpu_stream_generator():
+    """The serving workload's generator (gds2para_amd.stream) at a size a test
+    can run: 6 layouts of 3e4-3e5 edges with 8 ports each, 16 requests of 1-8
+    ports x 1-6 frequencies at 8 requests/s, under a budget that holds about
+    two layouts.  Every column converges, layouts are evicted and reopened
+    (admission from the backlog), and a sample of requests solved alone on a
+    fresh layout gives the same Z to 1e-6."""
+    from gds2para_amd import stream as ST
+    from gds2para_amd.layout import Layout
+    from gds2para_amd import mesh as M
+    from gds2para_amd.serve import GpuBackend
+    sp = ST.StreamSpec(n_layouts=6, edges=(3e4, 3e5), max_ports=8, max_freqs=6, n_requests=16, rate_hz=8.0,
+                       seed=77)
+    specs = ST.layouts(sp)
+    raw, offs = ST.requests(sp)
+
+    class Counting(GpuBackend):
+        opened = []
+
+        def open(self, key):
+            self.opened.append(key)
+            return super().open(key)
+
+    be = Counting(specs, slots=8)
+    est = sorted(be.estimate(k) for k in specs)
+    budget = est[-1] + est[-2]
+    reqs = [Request(k, p, f) for k, p, f in raw]
+    print(f"[serve] stream generator: {len(reqs)} requests over {len({k for k, _, _ in raw})} layouts, "
+          f"budget {budget >> 20} MB", flush=True)
+    srv = Server(be, budget_bytes=budget)
+    run_stream(srv, reqs, offs)
+    ev = srv.evictions
+    srv.close()
+    lats = [r.latency for r in reqs]
+    print(f"[serve] opened {len(be.opened)}, evictions {ev}, latency p50 {np.percentile(lats, 50):.3f} s "
+          f"p99 {np.percentile(lats, 99):.3f} s", flush=True)
+    assert ev >= 1 and len(be.opened) > len(set(be.opened))
+    for r in reqs:
+        assert not r.error and r.remaining == 0
+        for rep in r.reports.values():
+            assert rep["converged"] == 1 and rep["rel_residual"] <= 1e-8
+    for r in reqs[:3]:
+        lay = Layout(M.Mesh(specs[r.layout]))
+        Z, st, _ = lay.solve(r.freqs, ports=r.ports, precision=capi.GP_PREC_MIXED, rtol=1e-8, batch=8)
+        lay.close()
+        npt = specs[r.layout].n_ports
+        Za = Z.reshape(len(r.freqs), npt, npt)
+        for fi in range(len(r.freqs)):
+            for p in r.ports:
+                a, b = r.Z[fi, :, p], Za[fi, p, :]
+                assert np.abs(a - b).max() / np.abs(b).max() < 1e-6, (r.layout, r.freqs[fi], p)
