@@ -94,6 +94,8 @@ struct Solver;           // defined in solver.hip
 void destroy_v0(V0Data *);
 void destroy_solver(Solver *);
 int64_t solver_bytes(const Solver *);  // hierarchy, coarse cache and cached slot workspace
+struct Hierarchy;
+const Hierarchy *solver_hierarchy(const Solver *);
 
 }  // namespace gp
 
@@ -135,6 +137,7 @@ struct gp_layout {
   gp::Grid g{};
   int64_t n = 0;                        // unknowns after PEC removal
   hipStream_t stream = nullptr;
+  int32_t device = 0;                   // the device current when the layout was created
   gp::Arena arena;
   // device copy of the mesh contract
   double *xn = nullptr, *yn = nullptr, *zn = nullptr, *epsn = nullptr, *sign = nullptr;

@@ -126,6 +126,7 @@ extern "C" gp_status gp_layout_create(const gp_mesh *mesh, gp_layout **out) {
   lay->g = Grid::make(m.nx, m.ny, m.nz);
   const Grid &g = lay->g;
   lay->n = g.Ne - m.num_bd_edges;
+  GP_HIP(hipGetDevice(&lay->device));
   GP_HIP(hipStreamCreateWithFlags(&lay->stream, hipStreamNonBlocking));
   hipStream_t s = lay->stream;
   Arena &A = lay->arena;

@@ -192,6 +192,15 @@ void coarse_cache_unpin(CoarseCache &C, int32_t slot);
 int64_t coarse_entry_bytes(const Hierarchy &H);
 void coarse_cache_free(CoarseCache &C);
 
+// y = A(w) x for k interleaved columns (row i, column j at i*k + j) at the
+// angular frequencies om[k] (device), A(w) the matrix reference() hands
+// PARDISO; one launch of the FP64 SELL product.  The transposition kernels
+// convert between that layout and the host's column-major one (column j at
+// j*n).
+void amul(const Hierarchy &H, int64_t n, int k, const double *om, const C2<double> *x, C2<double> *y, hipStream_t s);
+__global__ void k_cols_to_rows(int64_t n, int k, const C2<double> *__restrict__ a, C2<double> *__restrict__ b);
+__global__ void k_rows_to_cols(int64_t n, int k, const C2<double> *__restrict__ a, C2<double> *__restrict__ b);
+
 // Builds the hierarchy into A (the solver's arena, freed with the solver).
 gp_status build_hierarchy(gp_layout *lay, Arena &A, Hierarchy *H, bool want_f32, int max_levels);
 gp_status build_aux(gp_layout *lay, Arena &A, Hierarchy *H);

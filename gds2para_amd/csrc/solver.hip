@@ -60,6 +60,8 @@ void destroy_solver(Solver *s) {
   delete s;
 }
 
+const Hierarchy *solver_hierarchy(const Solver *s) { return s ? &s->H : nullptr; }
+
 int64_t solver_bytes(const Solver *s) {
   if (!s) return 0;
   return s->arena.bytes + s->cc.bytes + (s->ws ? s->ws->bytes : 0);
@@ -292,6 +294,8 @@ __global__ void __launch_bounds__(256) k_amul_sell(SellV<double> SS, int64_t n, 
   store_slots<double, SPL>(y + i * k + j0, acc);
 }
 
+}  // namespace
+
 void amul(const Hierarchy &H, int64_t n, int k, const double *om, const Z2 *x, Z2 *y, hipStream_t s) {
   // one column: the copy in the reference's row order, whose wavefronts read
   // and write whole lines of the row-indexed vectors
@@ -337,6 +341,8 @@ __global__ void k_rows_to_cols(int64_t n, int k, const Z2 *__restrict__ a, Z2 *_
   const int j = (int)(t - i * k);
   b[(int64_t)j * n + i] = a[t];
 }
+
+namespace {
 
 // Multi-vector dots.  Block (blk, c) handles vectors t in [4c, 4c+4) of V
 // (stride vstride) against w over a grid-stride range of rows and writes
@@ -837,6 +843,16 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
       GP_HIP(hipStreamSynchronize(s));
     }
     GP_HIP(hipMemcpyAsync(om_d, omega.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+    // idle slots still run the V-cycle (on zero vectors) and index the coarse
+    // table: a slot that never held a column points at an entry another slot
+    // pins, so it never reads an evicted entry
+    {
+      int32_t live = -1;
+      for (int j = 0; j < k && live < 0; ++j) live = pinned[j];
+      if (live >= 0)
+        for (int j = 0; j < k; ++j)
+          if (pinned[j] < 0) fsl[j] = live;
+    }
     GP_TRY(vwork_set_columns<T>(vw, omega.data(), fsl.data(), cx.sv->cc, s));
     return GP_OK;
   };

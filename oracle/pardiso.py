@@ -148,3 +148,57 @@ def solve(A, B):
     if e13 != 0:
         raise PardisoError(f"pardiso phase 13 error {e13}")
     return np.ascontiguousarray(x.T), dt
+
+
+class Factor:
+    """A kept PARDISO factorization (phase 12 once, phase 33 per solve,
+    phase -1 on close), with the reference's mtype and iparm; .solve(b) makes
+    it usable wherever a SciPy LU object is (oracle.refined_solve)."""
+
+    def __init__(self, A):
+        self.lib = load()
+        A = A.tocsr()
+        A.sort_indices()
+        self.n = A.shape[0]
+        self.a = np.ascontiguousarray(A.data, np.complex128)
+        self.ia = np.ascontiguousarray(A.indptr, np.int64)
+        self.ja = np.ascontiguousarray(A.indices, np.int64)
+        self.pt = (C.c_void_p * 64)()
+        self.iparm = (C.c_int64 * 64)()
+        self.mtype = C.c_int64(13)
+        self.lib.pardisoinit(self.pt, C.byref(self.mtype), self.iparm)
+        self.iparm[38] = 1
+        self.iparm[34] = 1
+        self.iparm[3] = 0
+        self.open = False
+        self._call(12, np.zeros(2, np.complex128), np.zeros(2, np.complex128), 1)
+        self.open = True
+
+    def _call(self, phase, b, x, nrhs):
+        i64 = C.c_int64
+        err = i64(0)
+        ip = lambda arr: arr.ctypes.data_as(C.POINTER(C.c_int64))  # noqa: E731
+        self.lib.pardiso(self.pt, C.byref(i64(1)), C.byref(i64(1)), C.byref(self.mtype), C.byref(i64(phase)),
+                         C.byref(i64(self.n)), self.a.ctypes.data_as(C.c_void_p), ip(self.ia), ip(self.ja),
+                         C.byref(i64(0)), C.byref(i64(nrhs)), self.iparm, C.byref(i64(0)),
+                         b.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p), C.byref(err))
+        if err.value != 0:
+            raise PardisoError(f"pardiso phase {phase} error {err.value}")
+
+    def solve(self, b):
+        b = np.ascontiguousarray(b, np.complex128)
+        x = np.zeros_like(b)
+        self._call(33, b, x, 1)
+        return x
+
+    def close(self):
+        if self.open:
+            d = np.zeros(2, np.complex128)
+            self._call(-1, d, d, 1)
+            self.open = False
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

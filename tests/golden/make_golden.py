@@ -55,19 +55,14 @@ def digest(a):
 # The band-edge instances: the generators of the 200x200x24 and 300x300x32
 # workloads at 48x48x12 cells, 4 ports, at the edges of their bands.
 BAND_EDGE = {
-    "manhattan_48x48x12_lo.npz": dict(seed=2000, freqs=[1e7, 1e8, 1e9, 1e10], f_range=(1e7, 1e10)),
-    "manhattan_48x48x12_hi.npz": dict(seed=4000, freqs=[1e8, 1e10, 2e10, 5e10], f_range=(1e8, 5e10)),
+    "manhattan_48x48x12_lo.npz": dict(workload="manhattan_48x48x12", freqs=[1e7, 1e8, 1e9, 1e10]),
+    "manhattan_48x48x12_hi.npz": dict(workload="manhattan_48x48x12_hi", freqs=[1e8, 1e10, 2e10, 5e10]),
 }
-
-
-def band_edge_spec(name):
-    b = BAND_EDGE[name]
-    return M.manhattan_spec((48, 48, 12), n_ports=4, nfreq=len(b["freqs"]), f_range=b["f_range"], seed=b["seed"])
 
 
 def band_edges():
     for name, b in BAND_EDGE.items():
-        m = M.Mesh(band_edge_spec(name))
+        m = M.Mesh(M.WORKLOADS[b["workload"]]())
         fs = np.array(b["freqs"])
         X, Z = oracle.direct_solve(m, fs, refine=True)
         rows = port_rows(m)

@@ -121,6 +121,15 @@ def test_band_edge_fixture_shapes(name):
         assert np.abs(Zk - Zk.T).max() / np.abs(Zk).max() < 1e-6
 
 
+def _xtol(f):
+    """Port-edge solution tolerance at a 1e-8 relative residual: 1e-6, except
+    3e-6 at 10 MHz, where the gradient part of x dominates and the residual
+    bound leaves about 1e-6 in the port-edge values (measured 1.04e-6 in FP64;
+    Z, the product of the solve, stays at 1.6e-7).  The same column solved to
+    a 1e-10 residual meets 1e-7 (test_gpu_band_edge_low_tight)."""
+    return 3e-6 if f <= 1e7 else ZTOL
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec,batch", [(capi.GP_PREC_FP64, 1), (capi.GP_PREC_FP64, 16),
                                         (capi.GP_PREC_MIXED, 1), (capi.GP_PREC_MIXED, 16)])
@@ -139,5 +148,28 @@ def test_gpu_band_edges_match_fixture(prec, batch):
             ex = max(_rel(X[k * 4 + p][rows], g["x_port"][k, p]) for p in range(4))
             its = max(st["iters"][k * 4:(k + 1) * 4])
             print(f"[golden] {name} f={f:.0e} Z err {ez:.2e} x err {ex:.2e} iters {its}", flush=True)
-            assert ez < ZTOL and ex < ZTOL, (name, f, ez, ex)
+            assert ez < ZTOL and ex < _xtol(f), (name, f, ez, ex)
         lay.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [capi.GP_PREC_FP64, capi.GP_PREC_MIXED])
+def test_gpu_band_edge_low_tight(prec):
+    """10 MHz on the 48x48x12 instance of the 200x200x24 generator solved to a
+    1e-10 residual: Z within 1e-8 and the port-edge values within 1e-7 of the
+    refined direct solve, on both precisions (every restart starts from the
+    exact FP64 residual, so the FP32 preconditioner and basis do not bound the
+    attainable accuracy)."""
+    from gds2para_amd.layout import Layout
+    g = _load("manhattan_48x48x12_lo.npz")
+    m = M.Mesh(M.WORKLOADS["manhattan_48x48x12"]())
+    print(f"[golden] 48x48x12 at 10 MHz, rtol 1e-10, prec={prec}", flush=True)
+    lay = Layout(m)
+    Z, st, X = lay.solve(g["freqs"][:1], precision=prec, rtol=1e-10, batch=4, want_x=True, max_iters=3000)
+    assert st["status"] == capi.GP_OK and st["max_rel_residual"] <= 1e-10
+    rows = g["port_rows"]
+    ez = _rel(Z[0], g["Z"][0])
+    ex = max(_rel(X[p][rows], g["x_port"][0, p]) for p in range(4))
+    print(f"[golden] 10 MHz rtol 1e-10: Z err {ez:.2e} x err {ex:.2e} iters {max(st['iters'])}", flush=True)
+    assert ez < 1e-8 and ex < 1e-7
+    lay.close()
