@@ -153,6 +153,7 @@ SIGNATURES = [
     ("gp_write_touchstone", i32, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char, i32, i32, P(f64), P(f64), P(f64)]),
     ("gp_write_xyce", i32, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char, i32, P(C.c_char_p), f64, P(f64), f64]),
     ("gp_apply_fullwave", i32, [C.c_void_p, P(f64), i32, P(f64), P(f64)]),
+    ("gp_apply_fullwave_static", i32, [C.c_void_p, P(f64), i32, P(f64), P(f64)]),
     ("gp_bench_spmv", i32, [C.c_void_p, i32, i32, i32, P(f64)]),
     ("gp_layout_sync", i32, [C.c_void_p]),
     ("gp_bench_copy", i32, [i64, i32, P(f64)]),

@@ -409,6 +409,18 @@ gp_status gp_write_xyce(const char *path, const char *design, const char *timest
  * Needs the stiffness; builds the solver hierarchy if the layout has none. */
 gp_status gp_apply_fullwave(gp_layout *lay, const double *freqs, int32_t k, const double *x, double *y);
 
+/* gp_apply_fullwave without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results (the same bits) as gp_apply_fullwave, but the
+ * layout must already hold the stiffness and a prepared solver
+ * (gp_assemble_stiffness, gp_prepare_solver; GP_ERR_STATE otherwise), and x,
+ * y and the frequencies are staged in one static 96 MiB device buffer per GPU,
+ * so n*k <= 2,097,146 (GP_ERR_ARG beyond).  Every call on one GPU uses that
+ * GPU's buffer: calls that overlap on one GPU race and must be serialised by
+ * the caller. */
+gp_status gp_apply_fullwave_static(gp_layout *lay, const double *freqs, int32_t k, const double *x, double *y);
+
 /* ---- measurement hooks -------------------------------------------------- */
 /* Waits for all work the layout queued on its stream. */
 gp_status gp_layout_sync(gp_layout *lay);
