@@ -232,6 +232,7 @@ def measure(workload, steps, warmup, a, per_frequency):
     elapsed = time.perf_counter() - t
     launches, kms, kbytes = lay.kernel_timer_read()
     lay.kernel_timer(False)
+    hinfo = lay.hierarchy_info()
     ncol = npt * (1 if per_frequency else nf)
     value = steps * ncol / elapsed
 
@@ -285,6 +286,10 @@ def measure(workload, steps, warmup, a, per_frequency):
                   "per_freq_max": {f: int(max(v)) for f, v in it_f.items()}},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "setup_s": setup_s,
+        "coarse": {"form_ms_total": hinfo["coarse_form_ms"], "frequencies": hinfo["coarse_entries"],
+                   "aux_rows": hinfo["aux"][-1][0] if hinfo["aux"] else 0,
+                   "aux_blocks": hinfo["aux_coarse_blocks"], "aux_max_block": hinfo["aux_coarse_max_block"],
+                   "edge_rows": hinfo["edge"][-1][0] if hinfo["edge"] else 0},
         "fp64_single": fp64,
         "speedup_vs_fp64_single": value / fp64["value"] if fp64 else None,
         "z_rel_diff_mixed_vs_fp64": zdiff,

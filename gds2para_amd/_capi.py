@@ -80,7 +80,8 @@ class GpHierarchyInfo(C.Structure):
         ("edge_levels", i32), ("ne", i64 * 16), ("nn", i64 * 16), ("nnz_k", i64 * 16),
         ("aux_levels", i32), ("na", i64 * 16), ("nnz_n", i64 * 16), ("nlong", i64 * 16),
         ("naux", i64), ("setup_ms", f64), ("coarse_entry_bytes", i64), ("coarse_entries", i32),
-        ("coarse_evictions", i64), ("workspace_bytes", i64),
+        ("coarse_evictions", i64), ("workspace_bytes", i64), ("aux_coarse_blocks", i32),
+        ("aux_coarse_max_block", i32), ("coarse_form_ms", f64), ("aux_coarse_eigen", i32),
     ]
 
 
@@ -138,6 +139,8 @@ SIGNATURES = [
     ("gp_solve_opts_default", None, [P(GpSolveOpts)]),
     ("gp_prepare_solver", i32, [C.c_void_p, i32]),
     ("gp_hierarchy_info_get", i32, [C.c_void_p, P(GpHierarchyInfo)]),
+    ("gp_coarse_aux_inverse", i32, [C.c_void_p, f64, P(i64), P(f64), P(f64)]),
+    ("gp_coarse_aux_apply", i32, [C.c_void_p, f64, i32, i32, P(f64), P(f64)]),
     ("gp_solve_fullwave", i32, [C.c_void_p, P(f64), i32, P(i32), i32, P(GpSolveOpts), P(f64), P(f64), P(GpSolveStats)]),
     ("gp_solve_fullwave_ex", i32, [C.c_void_p, P(f64), i32, P(i32), i32, P(GpSolveOpts), P(f64), P(f64),
                                    P(GpSolveStats), P(GpColumnReport)]),

@@ -191,6 +191,74 @@ gp_status level0_node_order(const LevelGrid &L, Arena &A, int32_t **order, hipSt
   return GP_OK;
 }
 
+// Connected components of the auxiliary coarsest level (union-find on the
+// pattern of NL[0] | NL[1]): the positions of each block in ascending coarse
+// index, blocks by ascending size.  Gauss-Jordan on the whole matrix with
+// partial pivoting keeps every row inside its own block (off-block entries
+// are exact zeros, and x - 0 * v = x), so inverting each block on its own
+// with the columns in the same order gives the same bits in far less work.
+gp_status aux_coarse_blocks(Arena &A, AuxAMG &X, hipStream_t s) {
+  const int64_t n = X.nL;
+  if (n <= 0) return GP_OK;
+  std::vector<int32_t> par(n);
+  for (int64_t i = 0; i < n; ++i) par[i] = (int32_t)i;
+  auto find = [&](int32_t a) {
+    while (par[a] != a) { par[a] = par[par[a]]; a = par[a]; }
+    return a;
+  };
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t j = i + 1; j < n; ++j)
+      if (X.NL[0][i * n + j] != 0.0 || X.NL[1][i * n + j] != 0.0 || X.NL[0][j * n + i] != 0.0 ||
+          X.NL[1][j * n + i] != 0.0) {
+        const int32_t a = find((int32_t)i), b = find((int32_t)j);
+        if (a != b) par[std::max(a, b)] = std::min(a, b);
+      }
+  std::vector<int32_t> root(n), size(n, 0);
+  for (int64_t i = 0; i < n; ++i) { root[i] = find((int32_t)i); ++size[root[i]]; }
+  std::vector<int32_t> order(n);
+  for (int64_t i = 0; i < n; ++i) order[i] = (int32_t)i;
+  // by block size, then by block (its smallest index), then ascending index
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+    if (size[root[a]] != size[root[b]]) return size[root[a]] < size[root[b]];
+    if (root[a] != root[b]) return root[a] < root[b];
+    return a < b;
+  });
+  X.perm_h = order;
+  X.boff_h.assign(1, 0);
+  std::vector<int32_t> rowblk(n);
+  std::vector<int64_t> ioff, aoff;
+  X.inv_len = X.aug_len = 0;
+  X.max_blk = 0;
+  X.small_h.clear();
+  X.big_h.clear();
+  for (int64_t p = 0; p < n;) {
+    const int32_t r = root[order[p]], b = size[r];
+    const int32_t blk = (int32_t)X.boff_h.size() - 1;
+    for (int32_t q = 0; q < b; ++q) rowblk[p + q] = blk;
+    ioff.push_back(X.inv_len);
+    aoff.push_back(X.aug_len);
+    X.inv_len += (int64_t)b * b;
+    X.aug_len += 2 * (int64_t)b * b;
+    (b <= kBlockGJMax ? X.small_h : X.big_h).push_back(blk);
+    X.max_blk = std::max(X.max_blk, b);
+    p += b;
+    X.boff_h.push_back((int32_t)p);
+  }
+  X.nblk = (int32_t)X.boff_h.size() - 1;
+  GP_TRY(A.alloc(&X.perm, n)); GP_TRY(A.alloc(&X.rowblk, n)); GP_TRY(A.alloc(&X.boff, X.nblk + 1));
+  GP_TRY(A.alloc(&X.ioff, X.nblk)); GP_TRY(A.alloc(&X.aoff, X.nblk));
+  GP_TRY(A.alloc(&X.small, std::max<int64_t>((int64_t)X.small_h.size(), 1)));
+  GP_HIP(hipMemcpyAsync(X.perm, X.perm_h.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemcpyAsync(X.rowblk, rowblk.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemcpyAsync(X.boff, X.boff_h.data(), sizeof(int32_t) * (X.nblk + 1), hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemcpyAsync(X.ioff, ioff.data(), sizeof(int64_t) * X.nblk, hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemcpyAsync(X.aoff, aoff.data(), sizeof(int64_t) * X.nblk, hipMemcpyHostToDevice, s));
+  if (!X.small_h.empty())
+    GP_HIP(hipMemcpyAsync(X.small, X.small_h.data(), sizeof(int32_t) * X.small_h.size(), hipMemcpyHostToDevice, s));
+  GP_HIP(hipStreamSynchronize(s));  // the host vectors above go out of scope
+  return GP_OK;
+}
+
 // FP32 copies of every operator the mixed-precision V-cycle reads, added to
 // an existing FP64 hierarchy (so a layout solved in FP64 first and in mixed
 // precision afterwards keeps one hierarchy instead of building a second).
@@ -361,6 +429,7 @@ gp_status build_hierarchy(gp_layout *lay, Arena &A, Hierarchy *H, bool want_f32,
   GP_TRY(dense_host(C.GT, 0, C.nn, C.ne, &H->GTL));
   GP_HIP(hipStreamSynchronize(s));
   GP_TRY(build_aux(lay, A, H));
+  GP_TRY(aux_coarse_blocks(A, H->aux, s));
   if (want_f32) GP_TRY(hierarchy_add_f32(A, H, s));
   {  // dense coarsest data on the device, for the per-frequency inverses
     auto up = [&](const std::vector<double> &h, double **d) -> gp_status {
@@ -377,6 +446,7 @@ gp_status build_hierarchy(gp_layout *lay, Arena &A, Hierarchy *H, bool want_f32,
     GP_TRY(up(H->GTL, &H->GTLd));
     GP_HIP(hipStreamSynchronize(s));
   }
+  GP_TRY(aux_coarse_eigen(A, H->aux, s));
   {
     Csr Sv;
     Sv.nr = lay->S.n; Sv.nc = lay->S.n; Sv.nnz = lay->S.nnz; Sv.rp = lay->S.rp; Sv.ci = lay->S.ci; Sv.nv = 1; Sv.v[0] = lay->S.v;
@@ -408,11 +478,59 @@ __global__ void k_form_freq(int64_t n, double w, const double *__restrict__ A0, 
   out[i] = {(A0 ? A0[i] : 0.0) + a1 * A1[i], w * A2[i]};
 }
 
+// Augmented [A_b | I] of every block b of the auxiliary coarsest level at
+// angular frequency w, A_b = -w^2 NL0 + i w NL1 restricted to the block's
+// coarse indices (perm), row-major b x 2b at aoff[b].  One workgroup per block.
+__global__ void __launch_bounds__(256) k_blk_form(const int32_t *__restrict__ perm, const int32_t *__restrict__ boff,
+                                                  const int64_t *__restrict__ aoff, int64_t na, double w,
+                                                  const double *__restrict__ N0, const double *__restrict__ N1,
+                                                  C2<double> *__restrict__ aug) {
+  const int blk = blockIdx.x;
+  const int o = boff[blk], b = boff[blk + 1] - o;
+  C2<double> *A = aug + aoff[blk];
+  const double a1 = -w * w;
+  for (int64_t t = threadIdx.x; t < 2 * (int64_t)b * b; t += blockDim.x) {
+    const int r = (int)(t / (2 * b)), c = (int)(t - (int64_t)r * 2 * b);
+    C2<double> v;
+    if (c < b) {
+      const int64_t q = (int64_t)perm[o + r] * na + perm[o + c];
+      v = {0.0 + a1 * N0[q], w * N1[q]};
+    } else {
+      v = {c - b == r ? 1.0 : 0.0, 0.0};
+    }
+    A[t] = v;
+  }
+}
+
+// The inverse halves of the augmented blocks in `list` into the packed
+// inverse (block b row-major b x b at ioff[b]).  One workgroup per block.
+__global__ void __launch_bounds__(256) k_blk_extract(const int32_t *__restrict__ list, const int32_t *__restrict__ boff,
+                                                     const int64_t *__restrict__ aoff, const int64_t *__restrict__ ioff,
+                                                     const C2<double> *__restrict__ aug, C2<double> *__restrict__ inv) {
+  const int blk = list[blockIdx.x];
+  const int b = boff[blk + 1] - boff[blk];
+  const C2<double> *A = aug + aoff[blk];
+  C2<double> *I = inv + ioff[blk];
+  for (int64_t t = threadIdx.x; t < (int64_t)b * b; t += blockDim.x) {
+    const int r = (int)(t / b), c = (int)(t - (int64_t)r * b);
+    I[t] = A[(int64_t)r * 2 * b + b + c];
+  }
+}
+
+// one block's augmented matrix <-> a dense [A | I] for dense_inverse_batched,
+// which takes the n x n matrix and keeps the augmentation to itself
+__global__ void k_blk_to_dense(int b, const C2<double> *__restrict__ aug, C2<double> *__restrict__ a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)b * b) return;
+  const int64_t r = t / b, c = t - r * b;
+  a[t] = aug[r * 2 * b + c];
+}
+
 }  // namespace
 
 int64_t coarse_entry_bytes(const Hierarchy &H) {
-  const int64_t ne = H.neL, nn = H.nnL, na = H.aux.nL;
-  return (int64_t)sizeof(C2<double>) * (ne * ne + nn * nn + ne * nn + na * na);
+  const int64_t ne = H.neL, nn = H.nnL;
+  return (int64_t)sizeof(C2<double>) * (ne * ne + nn * nn + ne * nn + (H.aux.eig ? 0 : H.aux.inv_len));
 }
 
 namespace {
@@ -478,11 +596,89 @@ gp_status coarse_make_room(CoarseCache &C, int64_t add, hipStream_t s) {
 
 }  // namespace
 
+// The auxiliary coarsest level's inverse at angular frequency w, block by
+// block (AuxAMG::nblk), packed into inv (block b at offset ioff[b]).
+gp_status aux_coarse_inverse(const AuxAMG &X, double w, C2<double> *inv, hipStream_t s) {
+  Arena T;
+  struct Free { Arena &a; ~Free() { a.free_all(); } } fr{T};
+  C2<double> *aug = nullptr;
+  int *bad = nullptr;
+  GP_TRY(T.alloc(&aug, std::max<int64_t>(X.aug_len, 1)));
+  GP_TRY(T.alloc(&bad, 1));
+  GP_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+  k_blk_form<<<X.nblk, 256, 0, s>>>(X.perm, X.boff, X.aoff, X.nL, w, X.NLd[0], X.NLd[1], aug);
+  const unsigned nsmall = (unsigned)X.small_h.size();
+  if (nsmall) {
+    GP_TRY(gj_workgroup(X.small, X.boff, X.aoff, 0, (int)nsmall, aug, bad, s));
+    k_blk_extract<<<nsmall, 256, 0, s>>>(X.small, X.boff, X.aoff, X.ioff, aug, inv);
+  }
+  for (int32_t blk : X.big_h) {
+    const int b = X.boff_h[blk + 1] - X.boff_h[blk];
+    int64_t ao = 0, io = 0;
+    for (int32_t q = 0; q < blk; ++q) {
+      const int64_t bq = X.boff_h[q + 1] - X.boff_h[q];
+      ao += 2 * bq * bq;
+      io += bq * bq;
+    }
+    C2<double> *dst = inv + io;
+    k_blk_to_dense<<<blocks_for((int64_t)b * b), 256, 0, s>>>(b, aug + ao, dst);
+    GP_TRY(dense_inverse_batched(dst, b, 1, s));
+  }
+  int hbad = 0;
+  GP_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  GP_HIP(hipStreamSynchronize(s));
+  if (hbad) return fail(GP_ERR_STATE, "singular block in the auxiliary coarsest level");
+  return GP_OK;
+}
+
+gp_status aux_coarse_dense(const Hierarchy &H, double w, std::vector<std::complex<double>> *mat,
+                           std::vector<std::complex<double>> *inv, hipStream_t s) {
+  const AuxAMG &X = H.aux;
+  const int64_t n = X.nL;
+  mat->assign(n * n, 0.0);
+  inv->assign(n * n, 0.0);
+  if (n == 0) return GP_OK;
+  for (int64_t q = 0; q < n * n; ++q) (*mat)[q] = {0.0 + (-w * w) * X.NL[0][q], w * X.NL[1][q]};
+  if (X.eig) {
+    std::vector<double> V(n * n), lam(n);
+    GP_HIP(hipMemcpyAsync(V.data(), X.V, sizeof(double) * n * n, hipMemcpyDeviceToHost, s));
+    GP_HIP(hipMemcpyAsync(lam.data(), X.lam, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    GP_HIP(hipStreamSynchronize(s));
+    std::vector<std::complex<double>> d(n);
+    for (int64_t m = 0; m < n; ++m) d[m] = 1.0 / std::complex<double>(-w * w, w * lam[m]);
+    for (int64_t i = 0; i < n; ++i)
+      for (int64_t j = 0; j < n; ++j) {
+        std::complex<double> acc = 0;
+        for (int64_t m = 0; m < n; ++m) acc += V[i * n + m] * d[m] * V[j * n + m];
+        (*inv)[i * n + j] = acc;
+      }
+    return GP_OK;
+  }
+  Arena T;
+  struct Free { Arena &a; ~Free() { a.free_all(); } } fr{T};
+  C2<double> *d = nullptr;
+  GP_TRY(T.alloc(&d, std::max<int64_t>(X.inv_len, 1)));
+  GP_TRY(aux_coarse_inverse(X, w, d, s));
+  std::vector<std::complex<double>> h(X.inv_len);
+  GP_HIP(hipMemcpyAsync(h.data(), d, sizeof(C2<double>) * X.inv_len, hipMemcpyDeviceToHost, s));
+  GP_HIP(hipStreamSynchronize(s));
+  int64_t io = 0;
+  for (int32_t B = 0; B < X.nblk; ++B) {
+    const int o = X.boff_h[B], b = X.boff_h[B + 1] - o;
+    for (int r = 0; r < b; ++r)
+      for (int c = 0; c < b; ++c) (*inv)[(int64_t)X.perm_h[o + r] * n + X.perm_h[o + c]] = h[io + (int64_t)r * b + c];
+    io += (int64_t)b * b;
+  }
+  return GP_OK;
+}
+
 gp_status coarse_cache_lookup(const Hierarchy &H, CoarseCache &C, const std::vector<double> &omegas,
                               std::vector<int32_t> *slots, hipStream_t s, int pin) {
+  const auto t0 = std::chrono::steady_clock::now();
   slots->assign(omegas.size(), -1);
   const int64_t ne = H.neL, nn = H.nnL, na = H.aux.nL;
   const int64_t eb = coarse_entry_bytes(H);
+  if (na > 0 && H.aux.nblk == 0) return fail(GP_ERR_STATE, "auxiliary coarsest level without its block structure");
   bool fresh = false;
   for (size_t i = 0; i < omegas.size(); ++i) {
     int32_t q = -1;
@@ -509,10 +705,9 @@ gp_status coarse_cache_lookup(const Hierarchy &H, CoarseCache &C, const std::vec
         k_form_freq<<<blocks_for(nn * nn), 256, 0, s>>>(nn * nn, w, nullptr, H.NLd[0], H.NLd[1], nbuf);
         k_form_freq<<<blocks_for(ne * nn), 256, 0, s>>>(ne * nn, w, nullptr, H.BLd[0], H.BLd[1], bb);
       }
-      if (na) k_form_freq<<<blocks_for(na * na), 256, 0, s>>>(na * na, w, nullptr, H.aux.NLd[0], H.aux.NLd[1], ab);
       GP_TRY(dense_inverse_batched(kb, (int)ne, 1, s));
       GP_TRY(dense_inverse_batched(nbuf, (int)nn, 1, s));
-      GP_TRY(dense_inverse_batched(ab, (int)na, 1, s));
+      if (na && !H.aux.eig) GP_TRY(aux_coarse_inverse(H.aux, w, ab, s));
       fresh = true;
     }
     C.e[q].used = ++C.clock;
@@ -520,6 +715,7 @@ gp_status coarse_cache_lookup(const Hierarchy &H, CoarseCache &C, const std::vec
     (*slots)[i] = q;
   }
   if (fresh || !C.tab) GP_TRY(coarse_table_write(H, C, s));
+  if (fresh) C.form_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return GP_OK;
 }
 
@@ -559,6 +755,7 @@ gp_status vwork_create(gp_layout *lay, const Hierarchy &H, int k, VWork<T> *w) {
     GP_TRY(A.alloc(&w->ar[l], X.L[l].n * k)); GP_TRY(A.alloc(&w->ax[l], X.L[l].n * k)); GP_TRY(A.alloc(&w->at[l], X.L[l].n * k));
   }
   GP_TRY(A.alloc(&w->a, X.naux * k));
+  if (X.eig) GP_TRY(A.alloc(&w->eig_y, X.nL * k));
   GP_TRY(A.alloc(&w->r0, H.L[0].ne * k));
   int64_t nlp = (int64_t)X.GxT.nlong * k, ncp = (int64_t)X.GxT.nchunk * k;
   for (const AuxLevel &L : X.L) {
@@ -1201,31 +1398,38 @@ __global__ void k_spmv2(int64_t nr, int k, MatV<T> M, const T *__restrict__ c1v,
   else y[i * k + j] = acc;
 }
 
-// x = A_f^-1 r for each column (dense complex FP64 inverse per frequency):
-// grid (rows/4, columns), one wavefront per matrix row, the column staged in LDS
+// x = A_f^-1 r at the auxiliary coarsest level, A_f block diagonal
+// (AuxAMG::nblk) with the block inverses packed per frequency: position i
+// (coarse index perm[i]) of block B sums over B's columns only.  Grid
+// (positions / 4, columns): one wavefront per position, products in FP64.
 template <class T>
-__global__ void __launch_bounds__(256) k_dense_apply(int64_t n, int k, const int32_t *__restrict__ fidx,
-                                                     const double *const *__restrict__ tab,
-                                                     const C2<T> *__restrict__ r, C2<T> *__restrict__ x) {
-  extern __shared__ double sr[];
+__global__ void __launch_bounds__(256) k_blkdiag_apply(int64_t n, int k, const int32_t *__restrict__ perm,
+                                                       const int32_t *__restrict__ rowblk,
+                                                       const int32_t *__restrict__ boff,
+                                                       const int64_t *__restrict__ ioff,
+                                                       const int32_t *__restrict__ fidx,
+                                                       const double *const *__restrict__ tab,
+                                                       const C2<T> *__restrict__ r, C2<T> *__restrict__ x) {
   const int j = blockIdx.y;
-  const double *M = tab[fidx[j]];
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) { sr[2 * i] = (double)r[i * k + j].x; sr[2 * i + 1] = (double)r[i * k + j].y; }
-  __syncthreads();
+  const C2<double> *M = (const C2<double> *)tab[fidx[j]];
   const int lane = threadIdx.x & 63;
   const int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (i >= n) return;
+  const int B = rowblk[i];
+  const int o = boff[B], b = boff[B + 1] - o;
+  const C2<double> *R = M + ioff[B] + (int64_t)(i - o) * b;
   double ax = 0, ay = 0;
-  for (int64_t b = lane; b < n; b += 64) {
-    const double mx = M[2 * (i * n + b)], my = M[2 * (i * n + b) + 1];
-    ax += mx * sr[2 * b] - my * sr[2 * b + 1];
-    ay += mx * sr[2 * b + 1] + my * sr[2 * b];
+  for (int c = lane; c < b; c += 64) {
+    const C2<double> m = R[c];
+    const C2<T> v = r[(int64_t)perm[o + c] * k + j];
+    ax += m.x * (double)v.x - m.y * (double)v.y;
+    ay += m.x * (double)v.y + m.y * (double)v.x;
   }
   for (int off = 32; off > 0; off >>= 1) {
     ax += __shfl_xor(ax, off);
     ay += __shfl_xor(ay, off);
   }
-  if (lane == 0) x[i * k + j] = {(T)ax, (T)ay};
+  if (lane == 0) x[(int64_t)perm[i] * k + j] = {(T)ax, (T)ay};
 }
 
 template <class T>
@@ -1579,8 +1783,9 @@ static gp_status aux_cycle(const Hierarchy &H, VWork<T> &w, hipStream_t s, int l
   const int64_t n = L.n;
   C2<T> *r = w.ar[l], *x = w.ax[l], *t = w.at[l];
   if (l + 1 == (int)X.L.size()) {
-    k_dense_apply<T><<<dim3((unsigned)((n + 3) / 4), (unsigned)k), 256, sizeof(double) * 2 * n, s>>>(n, k, w.fidx,
-                                                                                                  w.tab + 3 * w.cap, r, x);
+    if (X.eig) return aux_eig_apply<T>(X, k, w.om64, r, x, w.eig_y, s);
+    k_blkdiag_apply<T><<<dim3((unsigned)((n + 3) / 4), (unsigned)k), 256, 0, s>>>(n, k, X.perm, X.rowblk, X.boff,
+                                                                                 X.ioff, w.fidx, w.tab + 3 * w.cap, r, x);
     return GP_OK;
   }
   const MatV<T> N = view<T>(L.N, L.Nf), P = view<T>(L.P, L.Pf), PT = view<T>(L.PT, L.PTf);

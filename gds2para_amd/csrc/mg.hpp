@@ -14,6 +14,7 @@
 // auxiliary problem, both with per-column frequencies.  The coarsest level is
 // solved exactly in FP64 (dense inverses of N_L(w) and K_L(w) per frequency).
 #pragma once
+#include <complex>
 #include <vector>
 
 #include "dense.hpp"
@@ -116,6 +117,11 @@ struct AuxLevel {
   float *ndf[2] = {}, *nlf[2] = {};
 };
 
+// Blocks of the auxiliary coarsest level up to this many rows are inverted by
+// one workgroup each (one launch for all of them); larger ones by the batched
+// multi-launch Gauss-Jordan of dense.hip.
+constexpr int32_t kBlockGJMax = kGJWorkgroupMax;
+
 // Auxiliary nodal space [V0d | V0c] and its class-aware smoothed-aggregation hierarchy.
 struct AuxAMG {
   int64_t naux = 0, ndiel = 0, nL = 0;
@@ -126,6 +132,22 @@ struct AuxAMG {
   std::vector<AuxLevel> L;
   std::vector<double> NL[2];     // coarsest level, dense (host)
   double *NLd[2] = {};           // same, on the device
+  // The coarsest level is block diagonal: V0d lives on dielectric edges and
+  // V0c on conductor edges, and conductors share no edge, so the dielectric
+  // aggregates couple only among themselves and each conductor's only within
+  // that conductor.  Its connected components (of the pattern NL[0] | NL[1])
+  // are found once; per frequency each block is inverted on its own.
+  int32_t nblk = 0, max_blk = 0;
+  int64_t inv_len = 0, aug_len = 0;        // sum of b^2 and of 2 b^2 over the blocks
+  std::vector<int32_t> perm_h, boff_h;     // position -> coarse index (ascending within a block); offsets [nblk+1]
+  std::vector<int32_t> small_h, big_h;     // blocks inverted by one workgroup each / by the batched launches
+  int32_t *perm = nullptr, *rowblk = nullptr, *boff = nullptr, *small = nullptr;  // device copies
+  int64_t *ioff = nullptr, *aoff = nullptr;  // per block: offset of its packed inverse / augmented matrix
+  // Frequency-independent form of the coarsest inverse, when the pencil
+  // NL[1] v = lambda NL[0] v could be solved (coarse_eig.hip):
+  // N(w)^-1 = V diag(1 / (-w^2 + i w lambda)) V^T.  V and V^T row-major.
+  bool eig = false;
+  double *V = nullptr, *VT = nullptr, *lam = nullptr;
 };
 
 struct Hierarchy {
@@ -181,6 +203,7 @@ struct CoarseCache {
   int max_entries = 64;
   int64_t max_bytes = (int64_t)2 << 30;
   int64_t evictions = 0;
+  double form_ms = 0;            // host wall time spent forming and inverting entries
 };
 
 // Slots of the given angular frequencies, adding (and inverting) new ones;
@@ -188,6 +211,11 @@ struct CoarseCache {
 gp_status coarse_cache_lookup(const Hierarchy &H, CoarseCache &C, const std::vector<double> &omegas,
                               std::vector<int32_t> *slots, hipStream_t s, int pin = 0);
 void coarse_cache_unpin(CoarseCache &C, int32_t slot);
+// The auxiliary coarsest level at angular frequency w as a dense matrix and
+// its inverse assembled from the per-block inverses (n x n row-major, host);
+// for checking the block inverse.
+gp_status aux_coarse_dense(const Hierarchy &H, double w, std::vector<std::complex<double>> *mat,
+                           std::vector<std::complex<double>> *inv, hipStream_t s);
 // Device bytes of one entry of this hierarchy.
 int64_t coarse_entry_bytes(const Hierarchy &H);
 void coarse_cache_free(CoarseCache &C);
@@ -204,6 +232,16 @@ __global__ void k_rows_to_cols(int64_t n, int k, const C2<double> *__restrict__ 
 // Builds the hierarchy into A (the solver's arena, freed with the solver).
 gp_status build_hierarchy(gp_layout *lay, Arena &A, Hierarchy *H, bool want_f32, int max_levels);
 gp_status build_aux(gp_layout *lay, Arena &A, Hierarchy *H);
+// Block structure of the auxiliary coarsest level (from its host dense copy).
+gp_status aux_coarse_blocks(Arena &A, AuxAMG &X, hipStream_t s);
+// Solves the auxiliary coarsest level's pencil once (sets X.eig on success).
+gp_status aux_coarse_eigen(Arena &A, AuxAMG &X, hipStream_t s);
+// x = N(w_j)^-1 r for k columns through the pencil's eigenvectors (X.eig);
+// om[k] the columns' angular frequencies (device, FP64), yp an nL x k FP64
+// work block.  Asynchronous on s.
+template <class T>
+gp_status aux_eig_apply(const AuxAMG &X, int k, const double *om, const C2<T> *r, C2<T> *x, C2<double> *yp,
+                        hipStream_t s);
 // FP32 copies for the mixed-precision V-cycle, added to an FP64 hierarchy.
 gp_status hierarchy_add_f32(Arena &A, Hierarchy *H, hipStream_t s);
 gp_status aux_add_f32(Arena &A, Hierarchy *H, hipStream_t s);
@@ -222,6 +260,8 @@ struct VWork {
   std::vector<C2<T> *> r, u, phi, res, tmp, rn, dn, dn2;
   // auxiliary AMG: per-level rhs / iterate / scratch, level-0 output, saved input
   std::vector<C2<T> *> ar, ax, at;
+  const double *om64 = nullptr;       // per slot: angular frequency in FP64 (device)
+  C2<double> *eig_y = nullptr;        // coarsest auxiliary projection (eigen form)
   C2<T> *a = nullptr, *r0 = nullptr;
   C2<T> *lp = nullptr;                      // long-row products [nlong][nv][k]
   C2<T> *lpart = nullptr;                   // per-chunk partial products [nchunk][nv][k]

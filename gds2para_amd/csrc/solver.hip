@@ -21,6 +21,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstring>
 #include <deque>
 #include <map>
 
@@ -750,6 +751,7 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
   SlotWork<T, TV> *W = nullptr;
   GP_TRY((slot_work<T, TV>(cx, k, m, io.want_x(), &W)));
   VWork<T> &vw = W->vw;
+  vw.om64 = W->om_d;
   vw.nu_aux = o.smooth_aux > 0 ? std::min(o.smooth_aux, 8) : 1;
   vw.aux_post = o.aux_post > 0;
   vw.kt = &lay->ktimer;
@@ -1213,6 +1215,53 @@ extern "C" gp_status gp_prepare_solver(gp_layout *lay, int32_t precision) {
   return GP_OK;
 }
 
+extern "C" gp_status gp_coarse_aux_inverse(gp_layout *lay, double freq, int64_t *n, double *mat, double *inv) {
+  if (!lay || !n) return fail(GP_ERR_ARG, "null argument");
+  if (!lay->solver) return fail(GP_ERR_STATE, "solver not prepared");
+  const Hierarchy &H = lay->solver->H;
+  *n = H.aux.nL;
+  if (!mat && !inv) return GP_OK;
+  if (!(freq > 0)) return fail(GP_ERR_ARG, "frequency must be positive");
+  std::vector<cd> m, v;
+  GP_TRY(aux_coarse_dense(H, 2 * kPi * freq, &m, &v, lay->stream));
+  if (mat) std::memcpy(mat, m.data(), sizeof(cd) * m.size());
+  if (inv) std::memcpy(inv, v.data(), sizeof(cd) * v.size());
+  return GP_OK;
+}
+
+extern "C" gp_status gp_coarse_aux_apply(gp_layout *lay, double freq, int32_t k, int32_t precision, const double *r,
+                                         double *x) {
+  if (!lay || !r || !x || k < 1 || k > 64 || !(freq > 0)) return fail(GP_ERR_ARG, "bad arguments (1 <= k <= 64)");
+  if (!lay->solver) return fail(GP_ERR_STATE, "solver not prepared");
+  const AuxAMG &X = lay->solver->H.aux;
+  if (!X.eig) return fail(GP_ERR_STATE, "the auxiliary coarsest level is not in eigenvector form");
+  const int64_t n = X.nL;
+  hipStream_t s = lay->stream;
+  DevBuf D;
+  double *om = nullptr;
+  Z2 *rd = nullptr, *xd = nullptr, *yp = nullptr;
+  GP_TRY(D.A.alloc(&om, k)); GP_TRY(D.A.alloc(&rd, n * k)); GP_TRY(D.A.alloc(&xd, n * k)); GP_TRY(D.A.alloc(&yp, n * k));
+  std::vector<double> omh(k, 2 * kPi * freq);
+  GP_HIP(hipMemcpyAsync(om, omh.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+  GP_HIP(hipMemcpyAsync(rd, r, sizeof(Z2) * n * k, hipMemcpyHostToDevice, s));
+  if (precision == GP_PREC_MIXED) {
+    std::vector<C2<float>> rf(n * k), xf(n * k);
+    for (int64_t q = 0; q < n * k; ++q) rf[q] = {(float)r[2 * q], (float)r[2 * q + 1]};
+    C2<float> *rfd = (C2<float> *)rd, *xfd = (C2<float> *)xd;
+    GP_HIP(hipMemcpyAsync(rfd, rf.data(), sizeof(C2<float>) * n * k, hipMemcpyHostToDevice, s));
+    GP_TRY(aux_eig_apply<float>(X, k, om, rfd, xfd, yp, s));
+    GP_HIP(hipMemcpyAsync(xf.data(), xfd, sizeof(C2<float>) * n * k, hipMemcpyDeviceToHost, s));
+    GP_HIP(hipStreamSynchronize(s));
+    for (int64_t q = 0; q < n * k; ++q) { x[2 * q] = xf[q].x; x[2 * q + 1] = xf[q].y; }
+  } else {
+    GP_TRY(aux_eig_apply<double>(X, k, om, rd, xd, yp, s));
+    GP_HIP(hipMemcpyAsync(x, xd, sizeof(Z2) * n * k, hipMemcpyDeviceToHost, s));
+    GP_HIP(hipStreamSynchronize(s));
+  }
+  GP_HIP(hipGetLastError());
+  return GP_OK;
+}
+
 extern "C" gp_status gp_layout_trim(gp_layout *lay) {
   if (!lay) return fail(GP_ERR_ARG, "null layout");
   if (!lay->solver) return GP_OK;
@@ -1246,6 +1295,10 @@ extern "C" gp_status gp_hierarchy_info_get(gp_layout *lay, gp_hierarchy_info *ou
   for (const CoarseEntry &x : cc.e) out->coarse_entries += x.mem ? 1 : 0;
   out->coarse_evictions = cc.evictions;
   out->workspace_bytes = lay->solver->ws ? lay->solver->ws->bytes : 0;
+  out->aux_coarse_blocks = H.aux.nblk;
+  out->aux_coarse_max_block = H.aux.max_blk;
+  out->aux_coarse_eigen = H.aux.eig ? 1 : 0;
+  out->coarse_form_ms = cc.form_ms;
   return GP_OK;
 }
 

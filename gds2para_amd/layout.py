@@ -108,7 +108,9 @@ class Layout:
         return dict(edge=[(h.ne[i], h.nn[i], h.nnz_k[i]) for i in range(el)],
                     aux=[(h.na[i], h.nnz_n[i], h.nlong[i]) for i in range(al)], naux=h.naux, setup_ms=h.setup_ms,
                     coarse_entry_bytes=h.coarse_entry_bytes, coarse_entries=h.coarse_entries,
-                    coarse_evictions=h.coarse_evictions, workspace_bytes=h.workspace_bytes)
+                    coarse_evictions=h.coarse_evictions, workspace_bytes=h.workspace_bytes,
+                    aux_coarse_blocks=h.aux_coarse_blocks, aux_coarse_max_block=h.aux_coarse_max_block,
+                    coarse_form_ms=h.coarse_form_ms, aux_coarse_eigen=h.aux_coarse_eigen)
 
     def apply_fullwave(self, freqs, X):
         """Y = A(w) X (gp_apply_fullwave): X [k, n] complex, one column per

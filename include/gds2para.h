@@ -257,8 +257,28 @@ typedef struct {
   int32_t coarse_entries;         /* frequencies held in the coarse-level cache    */
   int64_t coarse_evictions;       /* entries evicted so far (least recently used)  */
   int64_t workspace_bytes;        /* cached solve workspace (0 if none)            */
+  int32_t aux_coarse_blocks;      /* diagonal blocks of the auxiliary coarsest level */
+  int32_t aux_coarse_max_block;   /* rows of its largest block                     */
+  double coarse_form_ms;          /* time spent forming and inverting coarse entries */
+  int32_t aux_coarse_eigen;       /* 1: the auxiliary coarsest level is solved through
+                                     its pencil's eigenvectors (no per-frequency
+                                     inverse); 0: per-frequency Gauss-Jordan       */
 } gp_hierarchy_info;
 gp_status gp_hierarchy_info_get(gp_layout *lay, gp_hierarchy_info *out);
+
+/* Diagnostics (no counterpart in the reference): the auxiliary coarsest-level
+ * matrix at freq (Hz) and the inverse the solve uses, assembled from its
+ * per-block inverses or from its eigenvector form (O(n^3) host work: for
+ * checks on small layouts); both n x n complex interleaved, row-major.  *n
+ * receives the size; with mat and inv null nothing else is done. */
+gp_status gp_coarse_aux_inverse(gp_layout *lay, double freq, int64_t *n, double *mat, double *inv);
+
+/* Diagnostics: x = N(w)^-1 r at the auxiliary coarsest level through its
+ * eigenvector form (the V-cycle's kernels; GP_ERR_STATE when the level has no
+ * eigenvector form), for k <= 64 columns at one frequency; r and x are n x k
+ * complex interleaved, row i column j at i*k + j.  precision GP_PREC_MIXED
+ * passes r and x through FP32 as the mixed V-cycle does. */
+gp_status gp_coarse_aux_apply(gp_layout *lay, double freq, int32_t k, int32_t precision, const double *r, double *x);
 
 /* Solves (S + diag(-w^2 eps + i w sigma)) x = -i w J for every (port, freq)
  * pair, exactly the system reference() hands to PARDISO.  freqs in Hz.

@@ -179,10 +179,110 @@ def refined_solve(lu, A, b, steps=4):
     return xr, xi
 
 
-def direct_solve(mesh, freqs, ports=None, S=None, refine=False):
+# ---- double-double residuals (error-free transformations) -----------------
+# refined_solve keeps x and the residual in 80-bit long double, which leaves a
+# floor of about cond * 5e-20: at 10 MHz on a 48x48x12 mesh (cond ~ 1e13-1e15)
+# the refinement stalls near 1e-7 of |Z| and then drifts.  refined_solve_dd
+# carries x as an unevaluated sum of two doubles and forms b - A x exactly up
+# to ~1e-32 relative (Dekker's product split, Knuth's TwoSum), so each step
+# contracts the error by cond * 1e-16 until the FP64 system is solved exactly.
+
+def _split(a):
+    c = 134217729.0 * a                      # 2^27 + 1
+    hi = c - (c - a)
+    return hi, a - hi
+
+
+def _two_prod(a, b):
+    p = a * b
+    ah, al = _split(a)
+    bh, bl = _split(b)
+    return p, ((ah * bh - p) + ah * bl + al * bh) + al * bl
+
+
+def _two_sum(a, b):
+    s = a + b
+    bb = s - a
+    return s, (a - (s - bb)) + (b - bb)
+
+
+def _dd_add(ah, al, bh, bl):
+    s, e = _two_sum(ah, bh)
+    e = e + (al + bl)
+    h = s + e
+    return h, e - (h - s)
+
+
+def _dd_mul(xh, xl, a):
+    """(xh + xl) * a for a double a."""
+    p, e = _two_prod(xh, a)
+    e = e + xl * a
+    h = p + e
+    return h, e - (h - p)
+
+
+def _padded(A):
+    """CSR as dense (n, w) column / value arrays, padded with zero values."""
+    A = A.tocsr()
+    n = A.shape[0]
+    cnt = np.diff(A.indptr)
+    w = int(cnt.max()) if n else 0
+    col = np.zeros((n, w), np.int64)
+    vr = np.zeros((n, w))
+    vi = np.zeros((n, w))
+    pos = np.arange(A.nnz) - np.repeat(A.indptr[:-1], cnt)
+    rows = np.repeat(np.arange(n), cnt)
+    col[rows, pos] = A.indices
+    vr[rows, pos] = A.data.real
+    vi[rows, pos] = A.data.imag
+    return col, vr, vi
+
+
+def residual_dd(P, b, xr, xi):
+    """b - A x in double-double; x = (xr_hi, xr_lo) + i (xi_hi, xi_lo)."""
+    col, vr, vi = P
+    n, w = col.shape
+    rh, rl = b.real.copy(), np.zeros(n)
+    ih, il = b.imag.copy(), np.zeros(n)
+    for t in range(w):
+        c = col[:, t]
+        ar, ai = vr[:, t], vi[:, t]
+        xrh, xrl, xih, xil = xr[0][c], xr[1][c], xi[0][c], xi[1][c]
+        for (yh, yl, a, sg, re) in ((xrh, xrl, ar, -1.0, True), (xih, xil, ai, 1.0, True),
+                                    (xih, xil, ar, -1.0, False), (xrh, xrl, ai, -1.0, False)):
+            ph, pl = _dd_mul(yh, yl, a)
+            if re:
+                rh, rl = _dd_add(rh, rl, sg * ph, sg * pl)
+            else:
+                ih, il = _dd_add(ih, il, sg * ph, sg * pl)
+    return rh + 1j * ih, (rh, rl), (ih, il)
+
+
+def refined_solve_dd(lu, A, b, steps=60, tol=1e-18):
+    """FP64 factorization (anything with .solve) plus iterative refinement
+    with double-double iterate and residual.  Returns (xr, xi) as
+    (hi, lo) pairs; stops once a correction is below tol of |x|."""
+    P = _padded(A)
+    n = A.shape[0]
+    x = lu.solve(b)
+    xr, xi = (x.real.copy(), np.zeros(n)), (x.imag.copy(), np.zeros(n))
+    for _ in range(steps):
+        r, _, _ = residual_dd(P, b, xr, xi)
+        d = lu.solve(r)
+        xr = _dd_add(xr[0], xr[1], d.real, np.zeros(n))
+        xi = _dd_add(xi[0], xi[1], d.imag, np.zeros(n))
+        if np.abs(d).max() <= tol * np.abs(xr[0] + 1j * xi[0]).max():
+            break
+    return xr, xi
+
+
+def direct_solve(mesh, freqs, ports=None, S=None, refine=False, factor=None):
     """FP64 direct solve of every (port, freq) column; returns (X, Z).
 
-    refine=True adds long-double iterative refinement (refined_solve).
+    refine=True adds long-double iterative refinement (refined_solve),
+    refine="dd" double-double refinement (refined_solve_dd), which solves the
+    FP64 system exactly even where cond * 5e-20 is not small.  factor(A), if
+    given, makes the factorization (default SuperLU with COLAMD ordering).
 
     X[k][p] is the solution vector, Z[k] the np x np block laid out as
     Construct_Z_V0_Vh writes it (index resp + np*src)."""
@@ -191,13 +291,16 @@ def direct_solve(mesh, freqs, ports=None, S=None, refine=False):
     np_ = mesh.num_ports
     X, Z = [], []
     for f in freqs:
-        lu = spla.splu(system(mesh, f, S), permc_spec="COLAMD")
+        A = system(mesh, f, S)
+        lu = factor(A) if factor is not None else spla.splu(A, permc_spec="COLAMD")
         xs = []
         Zk = np.zeros(np_ * np_, np.complex128)
-        A = system(mesh, f, S) if refine else None
         for p in ports:
             b = rhs(mesh, f, p)
-            if refine:
+            if refine == "dd":
+                xr, xi = refined_solve_dd(lu, A, b)
+                x = (xr[0] + xr[1]) + 1j * (xi[0] + xi[1])
+            elif refine:
                 xr, xi = refined_solve(lu, A, b)
                 x = xr.astype(np.float64) + 1j * xi.astype(np.float64)
             else:

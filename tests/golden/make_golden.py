@@ -15,9 +15,13 @@ The oracle's own checks (tests/test_oracle.py) pin it independently; these
 files freeze its results so the GPU tests compare against recorded vectors
 without re-running the slow refined direct solves.
 
+Every solve is the FP64 direct solve refined with double-double residuals
+(oracle.refined_solve_dd), i.e. the exact solution of the reference's FP64
+system; the factorization is MKL PARDISO's when the MKL runtime is present
+(oracle/pardiso.py), else SuperLU's (about 75 s per 48x48x12 frequency).
+
 Run: python tests/golden/make_golden.py [--band-edges]
-(--band-edges writes only the two 48x48x12 files: one SuperLU factorization
-per frequency, about 75 s each on one host core)
+(--band-edges writes only the two 48x48x12 files)
 """
 import hashlib
 import os
@@ -32,6 +36,14 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 import oracle  # noqa: E402
 from gds2para_amd import mesh as M  # noqa: E402
+
+try:
+    import pardiso  # noqa: E402
+    pardiso.load()
+    pardiso.set_threads(os.cpu_count() or 1)
+    FACTOR = pardiso.Factor
+except OSError:
+    FACTOR = None
 
 
 def port_rows(m):
@@ -64,7 +76,7 @@ def band_edges():
     for name, b in BAND_EDGE.items():
         m = M.Mesh(M.WORKLOADS[b["workload"]]())
         fs = np.array(b["freqs"])
-        X, Z = oracle.direct_solve(m, fs, refine=True)
+        X, Z = oracle.direct_solve(m, fs, refine="dd", factor=FACTOR)
         rows = port_rows(m)
         xp = np.array([[X[k][p][rows] for p in range(m.num_ports)] for k in range(len(fs))])
         xn = np.array([[np.linalg.norm(X[k][p]) for p in range(m.num_ports)] for k in range(len(fs))])
@@ -81,7 +93,7 @@ def main():
     S = oracle.stiffness_csr(m)
     vs = oracle.v0_sizes(m)
     f = [float(m.freqs()[0])]
-    X, Z = oracle.direct_solve(m, f, S=S, refine=True)
+    X, Z = oracle.direct_solve(m, f, S=S, refine="dd", factor=FACTOR)
     rows = port_rows(m)
     np.savez(os.path.join(HERE, "microstrip_60x20x8.npz"),
              nnz=np.int64(S.nnz), indptr_sha=np.array(digest(S.indptr.astype(np.int64))),
@@ -92,7 +104,7 @@ def main():
              freqs=np.array(f), Z=Z, port_rows=rows, x_port=np.array([X[0][0][rows]]))
     m = M.Mesh(M.manhattan_spec((24, 20, 10), n_ports=3, nfreq=3, f_range=(1e8, 1e10), seed=11))
     fs = m.freqs()
-    X, Z = oracle.direct_solve(m, fs, refine=True)
+    X, Z = oracle.direct_solve(m, fs, refine="dd", factor=FACTOR)
     rows = port_rows(m)
     xp = np.array([[X[k][p][rows] for p in range(m.num_ports)] for k in range(len(fs))])
     np.savez(os.path.join(HERE, "manhattan_24x20x10.npz"), freqs=fs, Z=Z, port_rows=rows, x_port=xp)

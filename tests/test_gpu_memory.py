@@ -87,3 +87,68 @@ def test_coarse_cache_bounded():
     for a, k in enumerate(pick):
         col, ref = Z[k].reshape(npt, npt)[0], Zf[a].reshape(npt, npt)[0]   # source port 0: index resp + np*0
         assert _relerr(col, ref) < 1e-6, (k, _relerr(col, ref))
+
+
+@pytest.mark.parametrize("f", [1e7, 1e9, 5e10])
+def test_aux_coarse_block_inverse(f):
+    """The auxiliary coarsest level is inverted block by block (its connected
+    components); assembled, the inverse must be the inverse of the whole
+    matrix: A A^-1 = I to 1e-10 and the same as numpy's dense inverse."""
+    import ctypes as C
+    m = M.Mesh(M.WORKLOADS["manhattan_48x48x12"]())
+    lay = Layout(m)
+    lay.prepare(capi.GP_PREC_MIXED)
+    info = lay.hierarchy_info()
+    n = C.c_int64()
+    capi.check(lay.lib.gp_coarse_aux_inverse(lay.h, f, C.byref(n), None, None), "gp_coarse_aux_inverse")
+    n = n.value
+    mat, inv = np.zeros(2 * n * n), np.zeros(2 * n * n)
+    f64p = C.POINTER(C.c_double)
+    capi.check(lay.lib.gp_coarse_aux_inverse(lay.h, f, C.byref(C.c_int64()), mat.ctypes.data_as(f64p),
+                                             inv.ctypes.data_as(f64p)), "gp_coarse_aux_inverse")
+    lay.close()
+    A = (mat[0::2] + 1j * mat[1::2]).reshape(n, n)
+    Ai = (inv[0::2] + 1j * inv[1::2]).reshape(n, n)
+    print(f"[mem] aux coarsest {n} rows, {info['aux_coarse_blocks']} blocks, largest "
+          f"{info['aux_coarse_max_block']}, eigenvector form {info['aux_coarse_eigen']}", flush=True)
+    assert info["aux_coarse_blocks"] >= 1 and 1 <= info["aux_coarse_max_block"] <= n
+    assert np.abs(A @ Ai - np.eye(n)).max() < 1e-10
+    ref = np.linalg.inv(A)
+    assert np.abs(Ai - ref).max() / np.abs(ref).max() < 1e-10
+
+
+@pytest.mark.parametrize("prec,k", [(capi.GP_PREC_FP64, 16), (capi.GP_PREC_FP64, 1), (capi.GP_PREC_MIXED, 8)])
+def test_aux_coarse_eigen_apply(prec, k):
+    """The auxiliary coarsest solve through the pencil's eigenvectors, on the
+    FP64 matrix cores, against numpy's dense solve of the same matrix at
+    1 GHz: 1e-10 relative in FP64, 1e-6 with the FP32 input and output of the
+    mixed V-cycle.  k = 1 and 16 columns cover partial and full MFMA column
+    tiles; the columns carry unrelated random data."""
+    import ctypes as C
+    m = M.Mesh(M.WORKLOADS["manhattan_48x48x12"]())
+    lay = Layout(m)
+    lay.prepare(capi.GP_PREC_MIXED)
+    assert lay.hierarchy_info()["aux_coarse_eigen"] == 1
+    f64p = C.POINTER(C.c_double)
+    n = C.c_int64()
+    capi.check(lay.lib.gp_coarse_aux_inverse(lay.h, 1e9, C.byref(n), None, None), "gp_coarse_aux_inverse")
+    n = n.value
+    mat = np.zeros(2 * n * n)
+    capi.check(lay.lib.gp_coarse_aux_inverse(lay.h, 1e9, C.byref(C.c_int64()), mat.ctypes.data_as(f64p), None),
+               "gp_coarse_aux_inverse")
+    A = (mat[0::2] + 1j * mat[1::2]).reshape(n, n)
+    rng = np.random.default_rng(7)
+    R = rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k))
+    r = np.ascontiguousarray(np.stack([R.real, R.imag], -1).ravel())
+    x = np.zeros_like(r)
+    print(f"[mem] eigen-form coarse apply n={n} k={k} prec={prec}", flush=True)
+    capi.check(lay.lib.gp_coarse_aux_apply(lay.h, 1e9, k, prec, r.ctypes.data_as(f64p), x.ctypes.data_as(f64p)),
+               "gp_coarse_aux_apply")
+    lay.close()
+    X = (x[0::2] + 1j * x[1::2]).reshape(n, k)
+    if prec == capi.GP_PREC_MIXED:               # the kernel sees R rounded to FP32
+        R = R.astype(np.complex64).astype(np.complex128)
+    ref = np.linalg.solve(A, R)
+    err = np.abs(X - ref).max() / np.abs(ref).max()
+    print(f"[mem] relative error {err:.2e}", flush=True)
+    assert err < (1e-10 if prec == capi.GP_PREC_FP64 else 1e-6)
