@@ -16,7 +16,7 @@ i32, i64, f64 = C.c_int32, C.c_int64, C.c_double
 P = C.POINTER
 
 GP_OK, GP_ERR_ARG, GP_ERR_HIP, GP_ERR_NOMEM, GP_ERR_NOCONV, GP_ERR_STATE, GP_ERR_NODEVICE = range(7)
-GP_GEN_MICROSTRIP, GP_GEN_MANHATTAN = 0, 1
+GP_GEN_MICROSTRIP, GP_GEN_MANHATTAN, GP_GEN_PLATE = 0, 1, 2
 GP_PREC_FP64, GP_PREC_MIXED = 0, 1
 
 
@@ -155,6 +155,8 @@ SIGNATURES = [
     ("gp_write_citi", i32, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char, i32, i32, P(f64), P(f64)]),
     ("gp_write_touchstone", i32, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char, i32, i32, P(f64), P(f64), P(f64)]),
     ("gp_write_xyce", i32, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char, i32, P(C.c_char_p), f64, P(f64), f64]),
+    ("gp_write_spef", i32, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char, i32, P(C.c_char_p), C.c_char_p, f64,
+                            P(f64), f64]),
     ("gp_apply_fullwave", i32, [C.c_void_p, P(f64), i32, P(f64), P(f64)]),
     ("gp_apply_fullwave_static", i32, [C.c_void_p, P(f64), i32, P(f64), P(f64)]),
     ("gp_bench_spmv", i32, [C.c_void_p, i32, i32, i32, P(f64)]),

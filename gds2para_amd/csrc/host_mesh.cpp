@@ -335,6 +335,36 @@ gp_status gen_microstrip(const gp_gen_params &p, gp_owned_mesh &M) {
   return GP_OK;
 }
 
+// Parallel plate: a conductor plate one cell thick over the whole domain at
+// cell zp = Nz / 2 above the PEC ground plane, two dielectric layers between
+// them (eps_lo for cells [0, zp / 2), eps_hi for [zp / 2, zp); eps_hi above
+// the plate), one z-port from the ground to the plate at the domain centre.
+// With no fringing (the plate spans the lateral walls, which are natural
+// boundaries) its capacitance is eps0 * area / sum(d_i / eps_i) exactly, for
+// the discrete operator as for the continuum.
+gp_status gen_plate(const gp_gen_params &p, gp_owned_mesh &M) {
+  int64_t Nx = p.nx_cells, Ny = p.ny_cells, Nz = p.nz_cells;
+  if (Nx < 2 || Ny < 2 || Nz < 6) return gp::fail(GP_ERR_ARG, "parallel plate needs at least 2x2x6 cells");
+  M.xn = axis(Nx, p.dx, p.grade); M.yn = axis(Ny, p.dy, p.grade * 0.5); M.zn = axis(Nz, p.dz, 0.0);
+  Grid g = Grid::make(Nx + 1, Ny + 1, Nz + 1);
+  const int64_t zp = Nz / 2, z1 = zp / 2;
+  std::vector<Box> boxes{{0, Nx, 0, Ny, zp, zp + 1, 0}};
+  mark_conductors(M, g, boxes);
+  M.epsn.resize(Nz + 1);
+  for (int64_t iz = 0; iz <= Nz; ++iz) M.epsn[iz] = iz < z1 ? p.eps_lo : p.eps_hi;
+  M.sign.assign(Nz + 1, p.sigma);
+  const int64_t xc = Nx / 2, yc = Ny / 2;
+  int32_t plate = M.mark_node[g.node(zp, xc, yc)], ground = M.mark_node[0];
+  M.cdt_mark_port[ground - 1] = -1;
+  M.cdt_mark_port[plate - 1] = 1;
+  M.port_side_ptr = {0, 1};
+  M.side_edge_ptr = {0};
+  add_z_port(M, g, xc, yc, 0, zp, +1);
+  boundary_sets(M, g);
+  finish_view(M, p);
+  return GP_OK;
+}
+
 gp_status gen_manhattan(const gp_gen_params &p, gp_owned_mesh &M) {
   int64_t Nx = p.nx_cells, Ny = p.ny_cells, Nz = p.nz_cells;
   int nm = p.n_metal;
@@ -433,6 +463,7 @@ extern "C" gp_status gp_generate(const gp_gen_params *p, gp_owned_mesh **out) {
   auto *M = new gp_owned_mesh();
   gp_status st = p->kind == GP_GEN_MICROSTRIP ? gen_microstrip(*p, *M)
                : p->kind == GP_GEN_MANHATTAN ? gen_manhattan(*p, *M)
+               : p->kind == GP_GEN_PLATE ? gen_plate(*p, *M)
                : gp::fail(GP_ERR_ARG, "unknown generator kind");
   if (st != GP_OK) { delete M; return st; }
   *out = M;

@@ -364,3 +364,87 @@ extern "C" gp_status gp_write_xyce(const char *path, const char *design, const c
   const bool ok = std::fclose(fp) == 0;
   return ok ? GP_OK : fail(GP_ERR_ARG, std::string("cannot write ") + path);
 }
+
+// Parasitics::toSPEF + SolverDataBase::dumpSPEF (src/solnoutclass.hpp:1972-2092,
+// 6008-6030): the elements of the Xyce reduction (computeYBusFromParam at the
+// lowest frequency, C and G symmetrized unless already symmetric) as one
+// IEEE 1481-1998 net "all": a ground capacitance per port (its C row sum)
+// when |row sum| >= save_thresh * C total, a coupling capacitance -C_ij for
+// j > i when |C_ij| >= save_thresh * C total, a ground resistance
+// |1 / (G row sum)| and a coupling resistance |1 / G_ij| under the same rule
+// against the G total (the reference compares the coupling resistance, not
+// the conductance, with that total).  Unlike toXyce, negative capacitances
+// stay capacitances.  The reference prints the record through Parser-SPEF's
+// dump (absent here); the text follows the standard's grammar, with a
+// node-to-ground resistance written with its one node as the reference
+// records it.
+extern "C" gp_status gp_write_spef(const char *path, const char *design, const char *timestamp, char param,
+                                   int32_t np, const char *const *port_names, const char *port_dirs, double f0,
+                                   const double *mat, double save_thresh) {
+  if (!path || !design || np < 1 || !port_names || !mat || !(f0 > 0.)) return fail(GP_ERR_ARG, "bad SPEF arguments");
+  for (int32_t i = 0; i < np; ++i)
+    if (!port_names[i]) return fail(GP_ERR_ARG, "null port name");
+  if (param != 'Z' && param != 'Y')
+    return fail(GP_ERR_ARG, "param must be 'Z' or 'Y' (the reference's 'S' branch leaves its Y matrix empty)");
+  const int n = np;
+  Mat P((size_t)n * n);
+  for (size_t q = 0; q < P.size(); ++q) P[q] = cd(mat[2 * q], mat[2 * q + 1]);
+  Mat Y;
+  if (param == 'Y') {
+    if (!inverse(P, n, &Y)) return fail(GP_ERR_ARG, "singular Y matrix");
+  } else {
+    Y = P;
+  }
+  const double w0 = 2. * M_PI * f0;
+  std::vector<double> G((size_t)n * n), Cm((size_t)n * n);
+  for (size_t q = 0; q < Y.size(); ++q) {
+    G[q] = Y[q].real();
+    Cm[q] = -1. / (Y[q].imag() * w0);
+  }
+  make_sym(Cm, n);
+  make_sym(G, n);
+  const double capTot = upper_total(Cm, n), condTot = upper_total(G, n);
+  auto dir_of = [&](int i) -> const char * {
+    const char d = port_dirs ? port_dirs[i] : 'B';
+    return d == 'O' ? "O" : d == 'I' ? "I" : "B";   // unclear directions are bidirectional (:2014)
+  };
+  FILE *fp = std::fopen(path, "w");
+  if (!fp) return fail(GP_ERR_ARG, std::string("cannot open ") + path);
+  const std::string when = stamp(timestamp, "%d-%m-%Y %H:%M:%S");
+  std::fprintf(fp, "*SPEF \"IEEE 1481-1998\"\n*DESIGN \"%s\"\n*DATE \"%s\"\n", design, when.c_str());
+  std::fprintf(fp, "*VENDOR \"DARPA ERI Contributors\"\n*PROGRAM \"SPEF Writer from DARPA ERI\"\n");
+  std::fprintf(fp, "*VERSION \"1.0\"\n*DESIGN_FLOW \"NETLIST_TYPE_VERILOG\"\n*DIVIDER /\n*DELIMITER :\n");
+  std::fprintf(fp, "*BUS_DELIMITER [ ]\n*T_UNIT 1 S\n*C_UNIT 1 F\n*R_UNIT 1 OHM\n*L_UNIT 1 H\n\n*NAME_MAP\n");
+  for (int i = 0; i < n; ++i) std::fprintf(fp, "*%d %s\n", i + 1, port_names[i]);
+  std::fprintf(fp, "\n*PORTS\n");
+  for (int i = 0; i < n; ++i) std::fprintf(fp, "*%d %s\n", i + 1, dir_of(i));
+  std::fprintf(fp, "\n*D_NET all %s\n*CONN\n", fmt_g(capTot, 6).c_str());
+  for (int i = 0; i < n; ++i) std::fprintf(fp, "*P %s %s\n", port_names[i], dir_of(i));
+  std::string caps, ress;
+  int nc = 0, nr = 0;
+  char buf[256];
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < n; ++j) {
+      if (i == j && std::abs(row_sum(Cm, n, i)) >= save_thresh * capTot) {
+        std::snprintf(buf, sizeof buf, "%d *%d %s\n", ++nc, i + 1, fmt_g(row_sum(Cm, n, i), 6).c_str());
+        caps += buf;
+      } else if (j > i && std::abs(-Cm[(size_t)i * n + j]) >= save_thresh * capTot) {
+        std::snprintf(buf, sizeof buf, "%d *%d *%d %s\n", ++nc, i + 1, j + 1, fmt_g(-Cm[(size_t)i * n + j], 6).c_str());
+        caps += buf;
+      }
+    }
+    for (int j = 0; j < n; ++j) {
+      if (i == j && std::abs(row_sum(G, n, i)) >= save_thresh * condTot) {
+        std::snprintf(buf, sizeof buf, "%d *%d %s\n", ++nr, i + 1, fmt_g(std::abs(1. / row_sum(G, n, i)), 6).c_str());
+        ress += buf;
+      } else if (j > i && std::abs(-1. / G[(size_t)i * n + j]) >= save_thresh * condTot) {
+        std::snprintf(buf, sizeof buf, "%d *%d *%d %s\n", ++nr, i + 1, j + 1,
+                      fmt_g(std::abs(-1. / G[(size_t)i * n + j]), 6).c_str());
+        ress += buf;
+      }
+    }
+  }
+  std::fprintf(fp, "*CAP\n%s*RES\n%s*END\n", caps.c_str(), ress.c_str());
+  const bool ok = std::fclose(fp) == 0;
+  return ok ? GP_OK : fail(GP_ERR_ARG, std::string("cannot write ") + path);
+}

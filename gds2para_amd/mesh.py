@@ -64,6 +64,13 @@ def microstrip_spec() -> GenSpec:
                    grade=0.0, eps=(3.9, 4.4), n_ports=1, nfreq=1, f_range=(1e9, 1e9), seed=1000)
 
 
+def plate_spec(cells=(24, 20, 12), eps=(3.9, 4.4), f=1e7) -> GenSpec:
+    """A plate over the whole domain above two dielectric layers (eps[0] for
+    the lower, eps[1] for the upper), one z-port; uniform 1 x 1 x 0.5 um cells."""
+    return GenSpec(kind=capi.GP_GEN_PLATE, cells=cells, h=(1 * UM, 1 * UM, 0.5 * UM), grade=0.0, eps=eps,
+                   n_ports=1, nfreq=1, f_range=(f, f), seed=1)
+
+
 def manhattan_spec(cells=(200, 200, 24), n_ports=8, nfreq=10, f_range=(1e7, 1e10), seed=2000) -> GenSpec:
     return GenSpec(cells=cells, n_ports=n_ports, nfreq=nfreq, f_range=f_range, seed=seed)
 

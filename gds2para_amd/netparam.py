@@ -80,3 +80,20 @@ def write_xyce(path, design, port_names, f0, P0, param="Z", save_thresh=1e-5, ti
                                          None if timestamp is None else timestamp.encode(), param.encode(), npt,
                                          names, float(f0), buf.ctypes.data_as(f64p), float(save_thresh)),
                "gp_write_xyce")
+
+
+def write_spef(path, design, port_names, f0, P0, param="Z", port_dirs=None, save_thresh=1e-5, timestamp=None):
+    """SolverDataBase::dumpSPEF: the same reduction as write_xyce written as an
+    IEEE 1481-1998 SPEF net (gp_write_spef); port_dirs is a string of 'I',
+    'O' or 'B', one per port (default all 'B')."""
+    P0 = np.asarray(P0, np.complex128).reshape(1, -1)
+    npt = int(round(np.sqrt(P0.shape[1])))
+    if len(port_names) != npt or (port_dirs is not None and len(port_dirs) != npt):
+        raise ValueError("one name and one direction per port")
+    buf, _ = _interleave(P0)
+    names = (C.c_char_p * npt)(*[str(s).encode() for s in port_names])
+    capi.check(capi.load().gp_write_spef(str(path).encode(), design.encode(),
+                                         None if timestamp is None else timestamp.encode(), param.encode(), npt,
+                                         names, None if port_dirs is None else port_dirs.encode(), float(f0),
+                                         buf.ctypes.data_as(f64p), float(save_thresh)),
+               "gp_write_spef")

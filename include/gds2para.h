@@ -24,6 +24,7 @@
  *   gp_write_citi           SolverDataBase::dumpCITI                src/solnoutclass.hpp:6040
  *   gp_write_touchstone     SolverDataBase::dumpTouchstone          src/solnoutclass.hpp:6104
  *   gp_write_xyce           SolverDataBase::dumpXyce                src/solnoutclass.hpp:6032
+ *   gp_write_spef           SolverDataBase::dumpSPEF, Parasitics::toSPEF  src/solnoutclass.hpp:6008,1972
  *   gp_freq_of              fdtdMesh::freqNo2freq                   src/fdtd.hpp:4319
  *   gp_edge_map             fdtdMesh::setMapEdge                    src/fdtd.hpp:1508
  */
@@ -113,7 +114,8 @@ double gp_freq_of(const gp_mesh *mesh, int32_t freq_no);
 /* ------------------------------------------------------------------------ */
 typedef enum gp_gen_kind {
   GP_GEN_MICROSTRIP = 0,          /* one strip over a ground plane, one z-port    */
-  GP_GEN_MANHATTAN = 1            /* random Manhattan metal layers with vias      */
+  GP_GEN_MANHATTAN = 1,           /* random Manhattan metal layers with vias      */
+  GP_GEN_PLATE = 2                /* plate over the ground plane, two dielectrics */
 } gp_gen_kind;
 
 typedef struct gp_gen_params {
@@ -420,6 +422,18 @@ gp_status gp_write_touchstone(const char *path, const char *design, const char *
  * gp_write_citi. */
 gp_status gp_write_xyce(const char *path, const char *design, const char *timestamp, char param, int32_t np,
                         const char *const *port_names, double f0, const double *mat, double save_thresh);
+
+/* Parasitics::toSPEF + SolverDataBase::dumpSPEF (src/solnoutclass.hpp:1972,
+ * :6008): the elements of gp_write_xyce's reduction (same param, f0, mat and
+ * save_thresh rules; negative capacitances stay capacitances) as one IEEE
+ * 1481-1998 net "all" with *NAME_MAP, *PORTS, *CONN, *CAP and *RES sections.
+ * port_dirs[np] holds 'I', 'O' or 'B' per port (null: all 'B').  The
+ * reference prints through Parser-SPEF, which is absent here, so only the
+ * record's content, not its spacing, follows the reference.  timestamp as for
+ * gp_write_citi. */
+gp_status gp_write_spef(const char *path, const char *design, const char *timestamp, char param, int32_t np,
+                        const char *const *port_names, const char *port_dirs, double f0, const double *mat,
+                        double save_thresh);
 
 /* y = A(w) x for k columns (1 <= k <= 64) at freqs[k] (Hz), with A(w) the
  * complex matrix reference() hands PARDISO (src/generateStiff.cpp:583-626):

@@ -100,3 +100,22 @@ def test_iteration_cap_reports_nonconvergence(manh):
     for k in range(len(fs)):
         assert _relerr(Z[k], Zr[k]) < ZTOL
     lay.close()
+
+
+@pytest.mark.parametrize("prec", [capi.GP_PREC_FP64, capi.GP_PREC_MIXED])
+def test_parallel_plate_closed_form_gpu(prec):
+    """The plate over two dielectric layers on the device: Z at 1 GHz is
+    1 / (i w C) with C = eps0 area / (d1 / eps1 + d2 / eps2) to 2e-6 (the
+    area from the reference's dual widths), in both precisions."""
+    from test_oracle import _plate_capacitance
+    cells = (24, 20, 12)
+    m = M.Mesh(M.plate_spec(cells))
+    C = _plate_capacitance(m, cells)
+    print(f"[solve] parallel plate {cells} prec={prec}", flush=True)
+    lay = Layout(m)
+    Z, st, _ = lay.solve([1e9], precision=prec, rtol=1e-8)
+    lay.close()
+    zc = 1 / (1j * 2 * np.pi * 1e9 * C)
+    err = abs(Z[0][0].imag - zc.imag) / abs(zc)
+    print(f"[solve] Z {Z[0][0]:.8g} closed form {zc:.8g} rel {err:.2e}", flush=True)
+    assert st["status"] == capi.GP_OK and err < 2e-6

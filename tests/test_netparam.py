@@ -159,3 +159,49 @@ def test_xyce_y_parameters_are_inverted_first(tmp_path):
 def test_xyce_rejects_s_parameters(tmp_path):
     with pytest.raises(capi.GpError):
         _xyce(tmp_path, np.array([0.1 + 0j, 0, 0, 0.1]), param="S")
+
+
+# ---- SPEF (SolverDataBase::dumpSPEF, src/solnoutclass.hpp:6008; Parasitics::
+# toSPEF :1972-2092).  The reference prints through Parser-SPEF, which is not
+# in this image and whose text no fixture of the reference records: the
+# element selection and values below are pinned by hand; the spacing of the
+# text is parity unpinned. ----------------------------------------------------
+
+SPEF_HEAD = ['*SPEF "IEEE 1481-1998"', '*DESIGN "d"', '*DATE "01-01-2025 00:00:00"',
+             '*VENDOR "DARPA ERI Contributors"', '*PROGRAM "SPEF Writer from DARPA ERI"', '*VERSION "1.0"',
+             '*DESIGN_FLOW "NETLIST_TYPE_VERILOG"', "*DIVIDER /", "*DELIMITER :", "*BUS_DELIMITER [ ]",
+             "*T_UNIT 1 S", "*C_UNIT 1 F", "*R_UNIT 1 OHM", "*L_UNIT 1 H", "", "*NAME_MAP", "*1 p0", "*2 p1", "",
+             "*PORTS", "*1 I", "*2 B", ""]
+
+
+def _spef(tmp_path, P0, **kw):
+    from gds2para_amd.netparam import write_spef
+    p = tmp_path / "out.spef"
+    write_spef(p, "d", ["p0", "p1"], 1.0 / (2.0 * np.pi), P0, port_dirs="IX", timestamp="01-01-2025 00:00:00", **kw)
+    return p.read_text().splitlines()
+
+
+def test_spef_known_answer(tmp_path):
+    """The Xyce known answer as SPEF: C total 3.25 on the net, ground caps
+    3.75 and 3.5 (row sums), the coupling -C_01 = -4 kept as a (negative)
+    capacitance, ground resistances |1/2.5| and |1/1.5|, the coupling
+    resistance |-1/0.5| = 2; an unknown direction ('X') is written as 'B'."""
+    lines = _spef(tmp_path, np.array([2 + 4j, 0.5 - 0.25j, 0.5 - 0.25j, 1 + 2j]))
+    assert lines == SPEF_HEAD + ["*D_NET all 3.25", "*CONN", "*P p0 I", "*P p1 B", "*CAP", "1 *1 3.75",
+                                 "2 *1 *2 -4", "3 *2 3.5", "*RES", "1 *1 0.4", "2 *1 *2 2", "3 *2 0.666667",
+                                 "*END"]
+
+
+def test_spef_threshold_and_symmetrization(tmp_path):
+    """A non-reciprocal input is symmetrized first: C = [[-.25, 3], [3, -.5]]
+    (total 2.25, row sums 2.75 and 2.5), G = [[2, 1], [1, 1]] (total 4, row
+    sums 3 and 2).  With save_thresh 1 every C element stays (2.75, |-3|,
+    2.5 >= 2.25) and every R goes (3, 2 and the coupling resistance 1 < 4)."""
+    lines = _spef(tmp_path, np.array([2 + 4j, 0.5 - 0.25j, 1.5 - 0.5j, 1 + 2j]), save_thresh=1.0)
+    assert lines[len(SPEF_HEAD):] == ["*D_NET all 2.25", "*CONN", "*P p0 I", "*P p1 B", "*CAP", "1 *1 2.75",
+                                      "2 *1 *2 -3", "3 *2 2.5", "*RES", "*END"]
+
+
+def test_spef_rejects_s_parameters(tmp_path):
+    with pytest.raises(capi.GpError):
+        _spef(tmp_path, np.array([0.1 + 0j, 0, 0, 0.1]), param="S")

@@ -378,3 +378,40 @@ def test_refined_dd_solves_the_fp64_system():
     z1 = oracle.extract_z(m, (xr[0] + xr[1]) + 1j * (xi[0] + xi[1]), 0)
     z2 = oracle.extract_z(m, lr.astype(np.float64) + 1j * li.astype(np.float64), 0)
     assert np.abs(z1 - z2).max() / np.abs(z1).max() < 1e-8
+
+
+# ---- parallel plate: a closed form for the eps layers and Z normalisation ---
+
+def _plate_capacitance(m, cells, eps=(3.9, 4.4)):
+    """eps0 * area / (d1 / eps1 + d2 / eps2) for the plate generator, with the
+    area the reference's discrete one: the sum of its dual widths, whose end
+    cells are whole cells (src/generateStiff.cpp:253-258)."""
+    xn, yn, zn = m.coords()
+    zp = cells[2] // 2
+    z1 = zp // 2
+    d1, d2 = zn[z1] - zn[0], zn[zp] - zn[z1]
+    dxa = np.r_[xn[1] - xn[0], (xn[2:] - xn[:-2]) / 2, xn[-1] - xn[-2]]
+    dya = np.r_[yn[1] - yn[0], (yn[2:] - yn[:-2]) / 2, yn[-1] - yn[-2]]
+    eps0 = 1.0 / (299792458.0 ** 2 * 4e-7 * np.pi)
+    return eps0 * dxa.sum() * dya.sum() / (d1 / eps[0] + d2 / eps[1])
+
+
+@pytest.mark.parametrize("cells", [(16, 16, 8), (24, 20, 12)])
+def test_parallel_plate_closed_form(cells):
+    """A plate over the whole domain above two dielectric layers (eps_r 3.9 and
+    4.4): at 1 GHz the full-wave Z is 1 / (i w C) with C from the layer stack
+    to 2e-6 (measured 3.8e-7 and 6.6e-7; at 100 MHz the reference's FP64
+    diagonal S_ii - w^2 eps rounds the eps term by ~1e-5, at 10 GHz the plate
+    inductance adds ~5e-5).  This pins the eps layer of every z-edge (the layer
+    below the upper node: taking the layer above moves C by 6 %), the port-area
+    normalisation of Z and the reference's whole-cell boundary dual widths
+    (half cells would move C by 9-13 %)."""
+    m = M.Mesh(M.plate_spec(cells))
+    C = _plate_capacitance(m, cells)
+    f = 1e9
+    _, Z = oracle.direct_solve(m, [f], refine="dd")
+    zc = 1 / (1j * 2 * np.pi * f * C)
+    assert abs(Z[0][0].imag - zc.imag) / abs(zc) < 2e-6
+    assert abs(Z[0][0].real) < 1e-6 * abs(zc)
+    # the same formula with both layers at eps 4.4 is far off (the pin is sharp)
+    assert abs(1 / (1j * 2 * np.pi * f * _plate_capacitance(m, cells, (4.4, 4.4))) - zc) / abs(zc) > 0.05
