@@ -207,6 +207,11 @@ def measure(workload, steps, warmup, a, per_frequency):
         fs = freqs[[t % nf]] if per_frequency else freqs
         return fs, lay.solve(fs, **kw)
 
+    tuned = None
+    if a.batch == 0:
+        tuned = lay.autotune(a.restart)
+        print(f"[bench] {workload} autotuned: {tuned['slots']} slots, lanes cap {tuned['lanes_cap']} "
+              f"({tuned['tune_ms']:.0f} ms)", file=sys.stderr, flush=True)
     for w in range(warmup):
         print(f"[bench] {workload} warmup step {w + 1}/{warmup}", file=sys.stderr, flush=True)
         step(w)
@@ -279,13 +284,17 @@ def measure(workload, steps, warmup, a, per_frequency):
                    "f_range_hz": list(spec.f_range),
                    "step": (f"{npt} ports at one frequency; timed step t takes frequency t mod {nf}"
                             if per_frequency else f"all {npt * nf} columns"),
-                   "columns_per_step": ncol, "batch_slots": a.batch or 16, "restart": a.restart, "rtol": 1e-8,
+                   "columns_per_step": ncol,
+                   "batch_slots": a.batch if a.batch else f"autotuned: {tuned['slots']} (of 8/16/32), "
+                                                          f"lanes cap {tuned['lanes_cap']}",
+                   "restart": a.restart, "rtol": 1e-8,
                    "krylov_basis": "fp64" if a.basis64 else "fp32"},
         "converged": f"{conv}/{steps * ncol}", "max_rel_residual": res_max,
         "iters": {"p50": float(np.percentile(iters, 50)), "max": int(max(iters)),
                   "per_freq_max": {f: int(max(v)) for f, v in it_f.items()}},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "setup_s": setup_s,
+        "autotune": tuned,
         "coarse": {"form_ms_total": hinfo["coarse_form_ms"], "frequencies": hinfo["coarse_entries"],
                    "aux_rows": hinfo["aux"][-1][0] if hinfo["aux"] else 0,
                    "aux_blocks": hinfo["aux_coarse_blocks"], "aux_max_block": hinfo["aux_coarse_max_block"],

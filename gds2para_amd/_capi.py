@@ -85,6 +85,10 @@ class GpHierarchyInfo(C.Structure):
     ]
 
 
+class GpAutotuneInfo(C.Structure):
+    _fields_ = [("slots", i32), ("lanes_cap", i32), ("slot_us", f64 * 3), ("lanes_us", f64 * 3), ("tune_ms", f64)]
+
+
 class GpColumnReport(C.Structure):
     _fields_ = [("iters", i32), ("converged", i32), ("rel_residual", f64), ("start_ms", f64), ("done_ms", f64)]
 
@@ -139,6 +143,7 @@ SIGNATURES = [
     ("gp_solve_opts_default", None, [P(GpSolveOpts)]),
     ("gp_prepare_solver", i32, [C.c_void_p, i32]),
     ("gp_hierarchy_info_get", i32, [C.c_void_p, P(GpHierarchyInfo)]),
+    ("gp_autotune", i32, [C.c_void_p, i32, P(GpAutotuneInfo)]),
     ("gp_coarse_aux_inverse", i32, [C.c_void_p, f64, P(i64), P(f64), P(f64)]),
     ("gp_coarse_aux_apply", i32, [C.c_void_p, f64, i32, i32, P(f64), P(f64)]),
     ("gp_solve_fullwave", i32, [C.c_void_p, P(f64), i32, P(i32), i32, P(GpSolveOpts), P(f64), P(f64), P(GpSolveStats)]),

@@ -1493,11 +1493,13 @@ struct Lanes {
   int sp = 1, lpr = 1;
   dim3 grid;
 };
+thread_local int t_lanes_cap = 16;
+
 template <class T>
 Lanes lanes_shape(int64_t nr, int k) {
   Lanes L;
   if (sizeof(T) == 4) L.sp = (k % 4 == 0) ? 4 : (k % 2 == 0) ? 2 : 1;
-  while (L.lpr < 16 && (k / L.sp) % (L.lpr * 2) == 0) L.lpr *= 2;
+  while (L.lpr < t_lanes_cap && (k / L.sp) % (L.lpr * 2) == 0) L.lpr *= 2;
   L.grid = dim3((unsigned)((nr * L.lpr + 255) / 256), (unsigned)(k / (L.sp * L.lpr)));
   return L;
 }
@@ -1887,5 +1889,11 @@ template void vwork_free<float>(VWork<float> *);
 template void vwork_free<double>(VWork<double> *);
 template gp_status vcycle<float>(const Hierarchy &, VWork<float> &, hipStream_t, int, int);
 template gp_status vcycle<double>(const Hierarchy &, VWork<double> &, hipStream_t, int, int);
+
+int set_lanes_cap(int cap) {
+  const int prev = t_lanes_cap;
+  t_lanes_cap = cap >= 1 ? cap : 16;
+  return prev;
+}
 
 }  // namespace gp

@@ -282,6 +282,10 @@ gp_status vwork_set_columns(VWork<T> &w, const double *omega_cols, const int32_t
 template <class T>
 void vwork_free(VWork<T> *w);
 
+// Largest lanes-per-row the V-cycle's sliced-ELL kernels use on the calling
+// thread (16 by default; the autotuner may lower it); returns the previous cap.
+int set_lanes_cap(int cap);
+
 // One V-cycle: input w->r[0] (K-form residual, n x k), output w->u[0], w->phi[0].
 template <class T>
 gp_status vcycle(const Hierarchy &H, VWork<T> &w, hipStream_t s, int nu_pre, int nu_post);

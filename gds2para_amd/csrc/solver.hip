@@ -39,12 +39,23 @@ struct SlotWorkBase {
   int64_t bytes = 0;
 };
 
+// The autotuner's choice for the mixed-precision path (gp_autotune).
+struct Tuning {
+  bool done = false;
+  int k = 16;                    // device slots (block width)
+  int lanes = 16;                // lanes-per-row cap of the V-cycle's SELL kernels
+  double us[3] = {-1, -1, -1};   // per column-iteration at 8, 16, 32 slots
+  double lanes_us[3] = {-1, -1, -1};  // the chosen width at caps 16, 4, 2
+  double ms = 0;
+};
+
 struct Solver {
   Arena arena;                   // the hierarchy's buffers
   Hierarchy H;
   bool f32 = false;
   CoarseCache cc;
   SlotWorkBase *ws = nullptr;
+  Tuning tune;
 };
 
 void release_workspace(Solver *s) {
@@ -1176,11 +1187,98 @@ gp_status run_slots(SolveCtx &cx, ColumnIO &io, int k, bool f32) {
   return cx.o->basis == 1 ? solve_slots<float, double>(cx, io, k) : solve_slots<float, float>(cx, io, k);
 }
 
+// Block width and lanes-per-row cap for the mixed path, measured: one
+// FGMRES cycle of the real solve loop (restart m, the preconditioner, the
+// outer operator and Gram-Schmidt) on k synthetic columns at the middle of the
+// layout's frequency grid, for k = 8, 16 and 32 while the workspace fits half
+// of the device memory, then the lanes-per-row caps 16, 4 and 2 at the best
+// width.  The choice with the least time per column-iteration is kept with the
+// solver.  The workspace and the coarse entry it leaves are reused by the
+// solves that follow.
+gp_status autotune(gp_layout *lay, const gp_solve_opts &o) {
+  Solver *sv = lay->solver;
+  Tuning &T = sv->tune;
+  const auto t0 = std::chrono::steady_clock::now();
+  const gp_mesh &mesh = lay->hm.view;
+  const int m = std::min(std::max(2, o.restart), 64);
+  const double fmid = gp_freq_of(&mesh, std::max(0, mesh.nfreq / 2));
+  gp_solve_opts to = o;
+  to.rtol = 1e-300;              // run the cycle to its end
+  to.max_iters = m;
+  to.verbose = 0;
+  to.basis = 0;
+  std::vector<int32_t> ports(1, 0);
+  auto run = [&](int k, int lanes, double *us) -> gp_status {
+    QueueIO io;
+    io.lay = lay; io.ports = ports.data(); io.np = 1; io.npt = mesh.num_ports; io.Z = nullptr; io.X = nullptr;
+    io.n = lay->n;
+    std::vector<gp_column_report> rep(k);
+    io.rep = rep.data();
+    for (int j = 0; j < k; ++j) io.q.push_back({(int32_t)(j % std::max(1, mesh.num_ports)), fmid, (int64_t)0});
+    gp_solve_stats st{};
+    SolveCtx cx{lay, sv, &to, &st, std::chrono::steady_clock::now()};
+    const int prev = set_lanes_cap(lanes);
+    GP_HIP(hipStreamSynchronize(lay->stream));
+    const auto a = std::chrono::steady_clock::now();
+    const gp_status r = solve_slots<float, float>(cx, io, k);
+    const gp_status q = hipStreamSynchronize(lay->stream) == hipSuccess ? GP_OK : fail(GP_ERR_HIP, "autotune sync");
+    set_lanes_cap(prev);
+    GP_TRY(r);
+    GP_TRY(q);
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
+    if (us) *us = 1e3 * ms / ((double)k * m);
+    return GP_OK;
+  };
+  size_t fre = 0, tot = 0;
+  GP_HIP(hipMemGetInfo(&fre, &tot));
+  const int64_t per = bytes_per_slot(sv->H, m, true, true);
+  auto fits = [&](int k) { return (int64_t)k * per <= ((int64_t)fre + (sv->ws ? sv->ws->bytes : 0)) / 2; };
+  // first launches of every kernel instance load code; keep them out of the timing
+  GP_TRY(run(fits(8) ? 8 : 1, 16, nullptr));
+  const int ks[3] = {8, 16, 32};
+  double best = 1e300;
+  T.k = 1;
+  for (int q = 0; q < 3; ++q) {
+    if (!fits(ks[q])) continue;
+    GP_TRY(run(ks[q], 16, &T.us[q]));           // the workspace is allocated in a first pass
+    GP_TRY(run(ks[q], 16, &T.us[q]));
+    if (T.us[q] < best) { best = T.us[q]; T.k = ks[q]; }
+  }
+  const int caps[3] = {16, 4, 2};
+  T.lanes = 16;
+  T.lanes_us[0] = best;
+  for (int q = 1; q < 3; ++q) {
+    GP_TRY(run(T.k, caps[q], &T.lanes_us[q]));
+    if (T.lanes_us[q] < best) { best = T.lanes_us[q]; T.lanes = caps[q]; }
+  }
+  T.done = true;
+  T.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return GP_OK;
+}
+
 }  // namespace
 
 }  // namespace gp
 
 using namespace gp;
+
+extern "C" gp_status gp_autotune(gp_layout *lay, int32_t restart, gp_autotune_info *out) {
+  if (!lay) return fail(GP_ERR_ARG, "null layout");
+  GP_TRY(gp_prepare_solver(lay, GP_PREC_MIXED));
+  gp_solve_opts o;
+  gp_solve_opts_default(&o);
+  if (restart > 0) o.restart = restart;
+  lay->solver->tune = Tuning{};
+  GP_TRY(autotune(lay, o));
+  if (out) {
+    const Tuning &T = lay->solver->tune;
+    out->slots = T.k;
+    out->lanes_cap = T.lanes;
+    for (int q = 0; q < 3; ++q) { out->slot_us[q] = T.us[q]; out->lanes_us[q] = T.lanes_us[q]; }
+    out->tune_ms = T.ms;
+  }
+  return GP_OK;
+}
 
 extern "C" void gp_solve_opts_default(gp_solve_opts *o) {
   o->rtol = 1e-8;
@@ -1341,21 +1439,27 @@ extern "C" gp_status gp_solve_fullwave_ex(gp_layout *lay, const double *freqs, i
   // slot count: requested, else 16 for the mixed path and 1 for plain FP64,
   // bounded by the columns and by half of the free device memory
   const int m = std::min(std::max(2, o.restart), 64);
-  int kb = o.batch > 0 ? o.batch : (f32 ? 16 : 1);
+  if (o.batch <= 0 && f32 && !sv->tune.done) GP_TRY(autotune(lay, o));
+  int kb = o.batch > 0 ? o.batch : (f32 ? sv->tune.k : 1);
   kb = std::min(kb, 64);
   kb = std::min(kb, pow2_at_least(ncol));
   {
+    // a workspace kept from an earlier call is reused or released, so its
+    // bytes count as available
     size_t fre = 0, tot = 0;
     if (hipMemGetInfo(&fre, &tot) == hipSuccess) {
+      const int64_t avail = (int64_t)fre + (sv->ws ? sv->ws->bytes : 0);
       const int64_t per = bytes_per_slot(sv->H, m, f32, f32 && o.basis != 1);
-      while (kb > 1 && (int64_t)kb * per > (int64_t)(fre / 2)) kb /= 2;
+      while (kb > 1 && (int64_t)kb * per > avail / 2) kb /= 2;
     }
   }
   const int k = pow2_at_least(kb);
   const int32_t npt = mesh.num_ports;
   if (Z) std::fill(Z, Z + 2 * (size_t)nf * npt * npt, 0.0);
   SolveCtx cx{lay, sv, &o, &st, t0};
+  const int cap_prev = set_lanes_cap(f32 && o.batch <= 0 ? sv->tune.lanes : 16);
   gp_status s2 = run_slots(cx, io, k, f32);
+  set_lanes_cap(cap_prev);
   if (s2 != GP_OK) return s2;
   if (reports)
     for (int id = 0; id < ncol; ++id) reports[id] = rep[id];

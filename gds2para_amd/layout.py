@@ -112,6 +112,13 @@ class Layout:
                     aux_coarse_blocks=h.aux_coarse_blocks, aux_coarse_max_block=h.aux_coarse_max_block,
                     coarse_form_ms=h.coarse_form_ms, aux_coarse_eigen=h.aux_coarse_eigen)
 
+    def autotune(self, restart=16):
+        """gp_autotune: the mixed path's block width and lanes-per-row cap."""
+        info = capi.GpAutotuneInfo()
+        capi.check(self.lib.gp_autotune(self.h, int(restart), C.byref(info)), "gp_autotune")
+        return dict(slots=info.slots, lanes_cap=info.lanes_cap, slot_us=list(info.slot_us),
+                    lanes_us=list(info.lanes_us), tune_ms=info.tune_ms)
+
     def apply_fullwave(self, freqs, X):
         """Y = A(w) X (gp_apply_fullwave): X [k, n] complex, one column per
         frequency in freqs, in the renumbered space; returns Y [k, n]."""

@@ -211,7 +211,9 @@ typedef struct gp_solve_opts {
   int32_t restart;                /* FGMRES restart length m                      */
   int32_t precision;              /* gp_precision                                 */
   int32_t batch;                  /* device slots (columns solved side by side);
-                                     0 = 16 with MIXED, 1 with FP64; 1 = single RHS */
+                                     0 = autotuned with MIXED (gp_autotune, run on
+                                     the layout's first such solve), 1 with FP64;
+                                     1 = single RHS */
   int32_t verbose;
   int32_t smooth_edge;            /* edge/nodal smoothing sweeps per level (0: 1) */
   int32_t smooth_aux;             /* auxiliary AMG sweeps per level (0: 1)       */
@@ -281,6 +283,23 @@ gp_status gp_coarse_aux_inverse(gp_layout *lay, double freq, int64_t *n, double 
  * complex interleaved, row i column j at i*k + j.  precision GP_PREC_MIXED
  * passes r and x through FP32 as the mixed V-cycle does. */
 gp_status gp_coarse_aux_apply(gp_layout *lay, double freq, int32_t k, int32_t precision, const double *r, double *x);
+
+/* Autotuner (no counterpart in the reference): picks the mixed path's block
+ * width among 8, 16 and 32 device slots and the lanes-per-row cap (16, 4 or
+ * 2) of the V-cycle's sliced-ELL kernels by timing one FGMRES cycle of the
+ * real solve on synthetic columns (restart length `restart`, 0 = 16) and
+ * keeping the least time per column-iteration; widths whose workspace would
+ * exceed half of the device memory are skipped.  Solves with batch 0 use the
+ * choice.  slot_us / lanes_us: measured microseconds per column-iteration (-1
+ * where skipped). */
+typedef struct gp_autotune_info {
+  int32_t slots;
+  int32_t lanes_cap;
+  double slot_us[3];              /* at 8, 16, 32 slots, lanes cap 16            */
+  double lanes_us[3];             /* at the chosen width, caps 16, 4, 2          */
+  double tune_ms;
+} gp_autotune_info;
+gp_status gp_autotune(gp_layout *lay, int32_t restart, gp_autotune_info *out);
 
 /* Solves (S + diag(-w^2 eps + i w sigma)) x = -i w J for every (port, freq)
  * pair, exactly the system reference() hands to PARDISO.  freqs in Hz.

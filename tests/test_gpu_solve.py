@@ -119,3 +119,23 @@ def test_parallel_plate_closed_form_gpu(prec):
     err = abs(Z[0][0].imag - zc.imag) / abs(zc)
     print(f"[solve] Z {Z[0][0]:.8g} closed form {zc:.8g} rel {err:.2e}", flush=True)
     assert st["status"] == capi.GP_OK and err < 2e-6
+
+
+def test_autotune_then_solve(manh):
+    """gp_autotune measures the mixed path at 8, 16 and 32 slots and at three
+    lanes-per-row caps, keeps the fastest per column-iteration, and solves
+    with batch 0 then use that choice and still meet the fixtures."""
+    m, fs, Xr, Zr = manh
+    lay = Layout(m)
+    print("[solve] autotune on the 24x20x10 Manhattan", flush=True)
+    info = lay.autotune()
+    print(f"[solve] autotune {info}", flush=True)
+    assert info["slots"] in (8, 16, 32) and info["lanes_cap"] in (16, 4, 2)
+    q = (8, 16, 32).index(info["slots"])
+    assert info["slot_us"][q] > 0 and min(u for u in info["slot_us"] if u > 0) == info["slot_us"][q]
+    assert min(info["lanes_us"]) > 0
+    Z, st, _ = lay.solve(fs, precision=capi.GP_PREC_MIXED, rtol=1e-8, batch=0)
+    assert st["status"] == capi.GP_OK and st["max_rel_residual"] <= 1e-8
+    for k in range(len(fs)):
+        assert _relerr(Z[k], Zr[k]) < ZTOL
+    lay.close()
