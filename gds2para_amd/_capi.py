@@ -71,7 +71,7 @@ class GpSolveOpts(C.Structure):
     _fields_ = [
         ("rtol", f64), ("max_iters", i32), ("restart", i32), ("precision", i32),
         ("batch", i32), ("verbose", i32), ("smooth_edge", i32), ("smooth_aux", i32),
-        ("sell_min_rows", i64), ("aux_post", i32), ("basis", i32),
+        ("sell_min_rows", i64), ("aux_post", i32), ("basis", i32), ("graphs", i32),
     ]
 
 
@@ -82,11 +82,16 @@ class GpHierarchyInfo(C.Structure):
         ("naux", i64), ("setup_ms", f64), ("coarse_entry_bytes", i64), ("coarse_entries", i32),
         ("coarse_evictions", i64), ("workspace_bytes", i64), ("aux_coarse_blocks", i32),
         ("aux_coarse_max_block", i32), ("coarse_form_ms", f64), ("aux_coarse_eigen", i32),
+        ("graph_captures", i64), ("graph_replays", i64), ("graph_failed", i32),
     ]
 
 
 class GpAutotuneInfo(C.Structure):
     _fields_ = [("slots", i32), ("lanes_cap", i32), ("slot_us", f64 * 3), ("lanes_us", f64 * 3), ("tune_ms", f64)]
+
+
+class GpVhInfo(C.Structure):
+    _fields_ = [("k", i32), ("leng_vh", i32), ("h_next", f64), ("arnoldi_ms", f64), ("eig_ms", f64), ("vh_ms", f64)]
 
 
 class GpColumnReport(C.Structure):
@@ -156,6 +161,9 @@ SIGNATURES = [
     ("gp_solve_real", i32, [P(i64), P(i64), P(f64), i64, i64, P(f64), P(f64), f64, P(i32), P(f64)]),
     ("gp_solve_v0", i32, [C.c_void_p, f64, P(f64), P(f64), P(GpV0Stats)]),
     ("gp_z_v0_sweep", i32, [P(GpMesh), P(f64)]),
+    ("gp_find_vh", i32, [C.c_void_p, i32, f64, f64, P(f64), P(f64), i64, P(f64), P(GpVhInfo)]),
+    ("gp_hessenberg_eigenvalues", i32, [i32, P(f64), P(f64)]),
+    ("gp_hessenberg_eigenvector", i32, [i32, P(f64), f64, f64, P(f64)]),
     ("gp_convert_param", i32, [i32, i32, C.c_char, C.c_char, P(f64), P(f64), P(f64)]),
     ("gp_write_citi", i32, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char, i32, i32, P(f64), P(f64)]),
     ("gp_write_touchstone", i32, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char, i32, i32, P(f64), P(f64), P(f64)]),

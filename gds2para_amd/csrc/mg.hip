@@ -1892,7 +1892,7 @@ template gp_status vcycle<double>(const Hierarchy &, VWork<double> &, hipStream_
 
 int set_lanes_cap(int cap) {
   const int prev = t_lanes_cap;
-  t_lanes_cap = cap >= 1 ? cap : 16;
+  if (cap >= 1) t_lanes_cap = cap;
   return prev;
 }
 

@@ -283,7 +283,8 @@ template <class T>
 void vwork_free(VWork<T> *w);
 
 // Largest lanes-per-row the V-cycle's sliced-ELL kernels use on the calling
-// thread (16 by default; the autotuner may lower it); returns the previous cap.
+// thread (16 by default; the autotuner may lower it); returns the previous
+// cap; cap < 1 only reads it.
 int set_lanes_cap(int cap);
 
 // One V-cycle: input w->r[0] (K-form residual, n x k), output w->u[0], w->phi[0].

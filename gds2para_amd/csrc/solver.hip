@@ -56,6 +56,8 @@ struct Solver {
   CoarseCache cc;
   SlotWorkBase *ws = nullptr;
   Tuning tune;
+  int64_t graph_captures = 0, graph_replays = 0;
+  bool graph_failed = false;
 };
 
 void release_workspace(Solver *s) {
@@ -671,8 +673,28 @@ struct SlotWork : SlotWorkBase {
   Z2 *xo = nullptr, *xfull = nullptr;
   int64_t max_port_edges = 1;
   int64_t *rhs_row = nullptr; int32_t *rhs_col = nullptr; double *rhs_val = nullptr;
-  ~SlotWork() override { vwork_free(&vw); A.free_all(); }
+  // the preconditioner of Krylov step i captured in a HIP graph (built on the
+  // step's first use); gkey identifies what the captured launches depend on
+  std::vector<hipGraphExec_t> gex;
+  uint64_t gkey = 0;
+  bool graph_off = false;        // capture failed once: launch directly
+  void drop_graphs() {
+    for (hipGraphExec_t &g : gex)
+      if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+  }
+  ~SlotWork() override { drop_graphs(); vwork_free(&vw); A.free_all(); }
 };
+
+// What a captured preconditioner depends on besides the workspace: the
+// smoothing options, the lanes cap and the coarse-level table (re-allocated
+// when the cache grows).
+uint64_t graph_key(const gp_solve_opts &o, const void *tab, int cap, int lanes) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+  mix((uint64_t)o.smooth_edge); mix((uint64_t)o.smooth_aux); mix((uint64_t)o.aux_post);
+  mix((uint64_t)o.sell_min_rows); mix((uint64_t)(uintptr_t)tab); mix((uint64_t)cap); mix((uint64_t)lanes);
+  return h;
+}
 
 template <class T, class TV>
 gp_status slot_work(SolveCtx &cx, int k, int m, bool want_x, SlotWork<T, TV> **out) {
@@ -763,6 +785,12 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
   GP_TRY((slot_work<T, TV>(cx, k, m, io.want_x(), &W)));
   VWork<T> &vw = W->vw;
   vw.om64 = W->om_d;
+  // the preconditioner of each Krylov step replays a captured graph, except
+  // while the kernel timer records per-launch events (replays would overwrite
+  // them)
+  const bool use_graphs = o.graphs != 0 && !lay->ktimer.on;
+  const int lanes_now = set_lanes_cap(0);
+  set_lanes_cap(lanes_now);
   vw.nu_aux = o.smooth_aux > 0 ? std::min(o.smooth_aux, 8) : 1;
   vw.aux_post = o.aux_post > 0;
   vw.kt = &lay->ktimer;
@@ -967,7 +995,43 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
       C2<T> *zu = Zu + (int64_t)i * n * k, *zp = Zp + (int64_t)i * nn * k, *za = Za + (int64_t)i * na * k;
       C2<T> *own_u = vw.u[0], *own_phi = vw.phi[0], *own_a = vw.a;
       vw.u[0] = zu; vw.phi[0] = zp; vw.a = za;
-      const gp_status pst = precond<T>(H, vw, s, nu_edge, nu_edge);
+      gp_status pst = GP_OK;
+      if (use_graphs && !W->graph_off) {
+        const uint64_t key = graph_key(o, vw.tab, vw.cap, lanes_now);
+        if (key != W->gkey || (int)W->gex.size() != m) {
+          W->drop_graphs();
+          W->gex.assign(m, nullptr);
+          W->gkey = key;
+        }
+        if (!W->gex[i]) {
+          hipGraph_t g = nullptr;
+          hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
+          if (e == hipSuccess) {
+            pst = precond<T>(H, vw, s, nu_edge, nu_edge);
+            e = hipStreamEndCapture(s, &g);
+          }
+          if (e == hipSuccess && pst == GP_OK) e = hipGraphInstantiate(&W->gex[i], g, nullptr, nullptr, 0);
+          if (g) (void)hipGraphDestroy(g);
+          if (e != hipSuccess || pst != GP_OK) {   // no graph for this workspace: launch directly
+            (void)hipGetLastError();
+            W->gex[i] = nullptr;
+            W->graph_off = true;
+            lay->solver->graph_failed = true;
+            pst = GP_OK;
+          } else {
+            ++lay->solver->graph_captures;
+          }
+        }
+        if (W->gex[i]) {
+          const hipError_t e = hipGraphLaunch(W->gex[i], s);
+          if (e != hipSuccess) pst = fail(GP_ERR_HIP, "hipGraphLaunch of the preconditioner failed");
+          ++lay->solver->graph_replays;
+        } else {
+          pst = precond<T>(H, vw, s, nu_edge, nu_edge);
+        }
+      } else {
+        pst = precond<T>(H, vw, s, nu_edge, nu_edge);
+      }
       vw.u[0] = own_u; vw.phi[0] = own_phi; vw.a = own_a;
       GP_TRY(pst);
       // w = A z (split form, FP64 products, stored in the basis precision)
@@ -1292,6 +1356,7 @@ extern "C" void gp_solve_opts_default(gp_solve_opts *o) {
   o->sell_min_rows = 0;
   o->aux_post = 0;
   o->basis = 0;
+  o->graphs = 1;
 }
 
 extern "C" gp_status gp_prepare_solver(gp_layout *lay, int32_t precision) {
@@ -1397,6 +1462,9 @@ extern "C" gp_status gp_hierarchy_info_get(gp_layout *lay, gp_hierarchy_info *ou
   out->aux_coarse_max_block = H.aux.max_blk;
   out->aux_coarse_eigen = H.aux.eig ? 1 : 0;
   out->coarse_form_ms = cc.form_ms;
+  out->graph_captures = lay->solver->graph_captures;
+  out->graph_replays = lay->solver->graph_replays;
+  out->graph_failed = lay->solver->graph_failed ? 1 : 0;
   return GP_OK;
 }
 

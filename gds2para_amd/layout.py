@@ -110,7 +110,8 @@ class Layout:
                     coarse_entry_bytes=h.coarse_entry_bytes, coarse_entries=h.coarse_entries,
                     coarse_evictions=h.coarse_evictions, workspace_bytes=h.workspace_bytes,
                     aux_coarse_blocks=h.aux_coarse_blocks, aux_coarse_max_block=h.aux_coarse_max_block,
-                    coarse_form_ms=h.coarse_form_ms, aux_coarse_eigen=h.aux_coarse_eigen)
+                    coarse_form_ms=h.coarse_form_ms, aux_coarse_eigen=h.aux_coarse_eigen,
+                    graph_captures=h.graph_captures, graph_replays=h.graph_replays, graph_failed=h.graph_failed)
 
     def autotune(self, restart=16):
         """gp_autotune: the mixed path's block width and lanes-per-row cap."""
@@ -145,9 +146,28 @@ class Layout:
         stats = {k: getattr(st, k) for k, _ in capi.GpV0Stats._fields_}
         return z[0::2] + 1j * z[1::2], x[0::2] + 1j * x[1::2], stats
 
+    def find_vh(self, k, scale=1e14, eps=1e-6, max_cols=None):
+        """fdtdMesh::find_Vh(k) (src/fdtd.hpp:3577-3859) through gp_find_vh.
+
+        Returns (Vh [n, leng_Vh] complex in the renumbered space, Ritz values
+        lambda [k] complex (eigenvalues of H times scale), H [k, k], info).
+        max_cols defaults to k (every Ritz value could have |Im| > eps)."""
+        n = self.mesh.n
+        mc = int(k if max_cols is None else max_cols)
+        hess = np.zeros(k * k)
+        ritz = np.zeros(2 * k)
+        vh = np.zeros(2 * n * max(mc, 1))
+        info = capi.GpVhInfo()
+        capi.check(self.lib.gp_find_vh(self.h, int(k), float(scale), float(eps), _p(hess, f64p), _p(ritz, f64p), mc,
+                                       _p(vh, f64p), C.byref(info)), "gp_find_vh")
+        cols = info.leng_vh if mc > 0 else 0
+        V = (vh[0::2] + 1j * vh[1::2])[:n * cols].reshape(cols, n).T
+        stats = {f: getattr(info, f) for f, _ in capi.GpVhInfo._fields_}
+        return V, ritz[0::2] + 1j * ritz[1::2], hess.reshape(k, k).T, stats
+
     def solve(self, freqs, ports=None, precision=capi.GP_PREC_MIXED, rtol=1e-8, batch=0, restart=16,
               max_iters=2000, want_x=False, verbose=0, smooth_edge=0, smooth_aux=0, sell_min_rows=0,
-              aux_post=0, basis=0):
+              aux_post=0, basis=0, graphs=1):
         """Returns (Z[nf, np*np] complex, stats, X or None)."""
         ports = list(range(self.mesh.num_ports)) if ports is None else list(ports)
         freqs = np.ascontiguousarray(np.asarray(freqs, np.float64))
@@ -157,7 +177,7 @@ class Layout:
         self.lib.gp_solve_opts_default(C.byref(o))
         o.rtol, o.precision, o.batch, o.restart, o.max_iters, o.verbose = rtol, precision, batch, restart, max_iters, verbose
         o.smooth_edge, o.smooth_aux, o.sell_min_rows = smooth_edge, smooth_aux, sell_min_rows
-        o.aux_post, o.basis = aux_post, basis
+        o.aux_post, o.basis, o.graphs = aux_post, basis, graphs
         Z = np.zeros(2 * nf * npt * npt)
         X = np.zeros(2 * self.mesh.n * nf * npp) if want_x else None
         st = capi.GpSolveStats()

@@ -20,6 +20,9 @@
  *                                                                   src/fdtd.hpp:4178
  *   gp_z_v0_sweep           fdtdMesh::print_z_V0 (its store)        src/fdtd.hpp:4249
  *   gp_extract_z            fdtdMesh::Construct_Z_V0_Vh             src/fdtd.hpp:4133
+ *   gp_find_vh              fdtdMesh::find_Vh(int k)                src/fdtd.hpp:3577
+ *   gp_hessenberg_eigenvalues / _eigenvector
+ *                           LAPACKE_dhseqr / LAPACKE_dhsein in find_Vh  src/fdtd.hpp:3754,3797
  *   gp_convert_param        Parasitics::convertParam                src/solnoutclass.hpp:1456
  *   gp_write_citi           SolverDataBase::dumpCITI                src/solnoutclass.hpp:6040
  *   gp_write_touchstone     SolverDataBase::dumpTouchstone          src/solnoutclass.hpp:6104
@@ -230,6 +233,10 @@ typedef struct gp_solve_opts {
                                      iterate stay FP64, every restart starts
                                      from the FP64 residual), 1 = FP64.  FP64
                                      precision always keeps an FP64 basis */
+  int32_t graphs;                 /* 1 (default): the preconditioner of each
+                                     Krylov step is captured once in a HIP graph
+                                     and replayed; 0: launched directly.  Off
+                                     while gp_kernel_timer records */
 } gp_solve_opts;
 
 typedef struct gp_solve_stats {
@@ -267,6 +274,9 @@ typedef struct {
   int32_t aux_coarse_eigen;       /* 1: the auxiliary coarsest level is solved through
                                      its pencil's eigenvectors (no per-frequency
                                      inverse); 0: per-frequency Gauss-Jordan       */
+  int64_t graph_captures;         /* preconditioner graphs captured so far         */
+  int64_t graph_replays;          /* preconditioner applications replayed from them */
+  int32_t graph_failed;           /* 1: a capture failed and the solves launch directly */
 } gp_hierarchy_info;
 gp_status gp_hierarchy_info_get(gp_layout *lay, gp_hierarchy_info *out);
 
@@ -404,6 +414,45 @@ gp_status gp_solve_v0(gp_layout *lay, double rtol, double *Z0, double *X, gp_v0_
  * at index k np^2 + i + j np (the reference stores it transposed), with f_k
  * = freqEnd * unit for the last slot and the linear / log grid between. */
 gp_status gp_z_v0_sweep(const gp_mesh *mesh, double *X);
+
+/* ---- Vh high-frequency basis (fdtdMesh::find_Vh(k)) --------------------- */
+typedef struct gp_vh_info {
+  int32_t k;                      /* Arnoldi steps run                             */
+  int32_t leng_vh;                /* Ritz values with |Im| > eps (= Vh's columns)  */
+  double h_next;                  /* |w| after the last step (H's missing h_{k+1,k}) */
+  double arnoldi_ms, eig_ms, vh_ms;
+} gp_vh_info;
+
+/* fdtdMesh::find_Vh(int k) (src/fdtd.hpp:3577-3859): k Arnoldi steps on the
+ * 2n x 2n companion operator of S u + lambda D_sig u + lambda^2 D_eps u = 0,
+ * scaled by `scale` (the reference uses 1e14), from the start vector
+ * 1 / sqrt(2n); the eigenvalues of the k x k Hessenberg matrix (dhseqr, :3754);
+ * eigenvectors of those with |Im| > eps (dhsein, :3797; the reference uses
+ * eps = 1e-6); Vh = V_top * eigenvectors (:3836-3850).  Arnoldi runs on the
+ * device (classical Gram-Schmidt, one re-orthogonalization pass); the k x k
+ * eigenproblem on the host (Francis double-shift QR, inverse iteration).
+ * Outputs, each optional (null skips it):
+ *   hess  k*k, column-major (H[j*k + i] = h_ij, zero below the subdiagonal)
+ *   ritz  2k doubles, the eigenvalues times scale (lambda of the pencil) as
+ *         (re, im) pairs, conjugate pairs positive imaginary part first
+ *   vh    n * max_cols complex (interleaved), column c at vh[2 n c ..], rows in
+ *         the renumbered edge space; a conjugate pair's columns are y, conj(y)
+ *         as the reference stores them, each y scaled so its largest entry is 1.
+ * GP_ERR_ARG if Vh has more than max_cols columns (info->leng_vh still set);
+ * max_cols 0 computes no vectors. */
+gp_status gp_find_vh(gp_layout *lay, int32_t k, double scale, double eps, double *hess, double *ritz,
+                     int64_t max_cols, double *vh, gp_vh_info *info);
+
+/* The two host LAPACK calls of find_Vh, host only.  h is an n x n upper
+ * Hessenberg matrix, column-major (entries below the subdiagonal ignored).
+ * gp_hessenberg_eigenvalues: LAPACKE_dhseqr(job 'E', compz 'N')
+ * (src/fdtd.hpp:3754): all eigenvalues as (re, im) pairs in w[2n], conjugate
+ * pairs adjacent, positive imaginary part first.
+ * gp_hessenberg_eigenvector: LAPACKE_dhsein(side 'R') (src/fdtd.hpp:3797) for
+ * one eigenvalue re + i im: the right eigenvector in v[2n] (complex
+ * interleaved), scaled so its largest entry is 1. */
+gp_status gp_hessenberg_eigenvalues(int32_t n, const double *h, double *w);
+gp_status gp_hessenberg_eigenvector(int32_t n, const double *h, double re, double im, double *v);
 
 /* ---- network-parameter output (host only) ------------------------------- */
 /* Matrices: nf blocks of np x np complex (interleaved), row-major,

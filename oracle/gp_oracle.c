@@ -23,6 +23,9 @@
  *   or_diag/or_rhs A(w) diagonal and J as reference() builds them; checked by
  *                  direct solves (SuperLU) against closed-form low-frequency
  *                  capacitance limits and reciprocity.
+ *   or_vh_coeffs   eps / sig per unknown; the Ritz values built on them match
+ *                  the eigenvalues of find_reference_Vh's dense pencil and
+ *                  the Ritz vectors satisfy the quadratic eigen-equation.
  */
 #include <math.h>
 #include <stdint.h>
@@ -632,6 +635,22 @@ int or_port_j(const gp_mesh *m, int32_t port, double *J) {
 int or_edge_eps(const gp_mesh *m, double *eps) {
   or_grid g = or_make_grid(m);
   for (int64_t e = 0; e < g.Ne; ++e) eps[e] = m->stack_epsn[e / g.El] * OR_EPSILON0;
+  return 0;
+}
+
+/* find_Vh's per-unknown coefficients (src/fdtd.hpp:3601-3605): for unknown
+ * i = mapEdge[e], eps[i] = stackEpsn[L] * EPSILON0 and sig[i] = stackSign[L]
+ * if markEdge[e] != 0 else 0, L = e / (N_edge_s + N_edge_v). */
+int or_vh_coeffs(const gp_mesh *m, double *eps, double *sig) {
+  or_grid g = or_make_grid(m);
+  int64_t *map = or_map_edge(m, &g);
+  for (int64_t e = 0; e < g.Ne; ++e) {
+    if (map[e] < 0) continue;
+    const int64_t L = e / (g.Es + g.Ev);
+    eps[map[e]] = m->stack_epsn[L] * OR_EPSILON0;
+    sig[map[e]] = m->mark_edge[e] != 0 ? m->stack_sign[L] : 0.;
+  }
+  free(map);
   return 0;
 }
 

@@ -44,6 +44,7 @@ def _load():
         lib.or_freq.argtypes = [C.c_void_p, C.c_int32]
         lib.or_port_j.argtypes = [C.c_void_p, C.c_int32, f64p]
         lib.or_edge_eps.argtypes = [C.c_void_p, f64p]
+        lib.or_vh_coeffs.argtypes = [C.c_void_p, f64p, f64p]
         lib.or_extract_z_v0.argtypes = [C.c_void_p, f64p, C.c_int32, f64p]
         lib.or_print_z_v0.argtypes = [C.c_void_p, f64p]
         _lib = lib
@@ -393,3 +394,60 @@ def v0_path(mesh, ops=None, ports=None):
         Z0 = extract_z_v0(mesh, yd, src, Z0)
         yds[src] = yd
     return Z0, yds
+
+
+# ---- Vh basis (fdtdMesh::find_Vh(k), src/fdtd.hpp:3577-3859) ----------------
+
+def vh_coeffs(mesh):
+    """(eps, sigma) per unknown as find_Vh reads them (src/fdtd.hpp:3601-3605)."""
+    e, s = np.zeros(mesh.n), np.zeros(mesh.n)
+    _load().or_vh_coeffs(mesh.ptr, _p(e, f64p), _p(s, f64p))
+    return e, s
+
+
+def vh_operator(mesh, scale=1e14, S=None):
+    """The 2n x 2n operator find_Vh runs Arnoldi on (src/fdtd.hpp:3595-3606):
+    [v1; v2] -> [v2; -S v1 / (eps s^2) - sigma v2 / (eps s)]."""
+    S = stiffness_csr(mesh) if S is None else S
+    eps, sig = vh_coeffs(mesh)
+    n = mesh.n
+    top = sp.hstack([sp.csr_matrix((n, n)), sp.identity(n, format="csr")])
+    bot = sp.hstack([-sp.diags(1.0 / (eps * scale * scale)) @ S, sp.diags(-sig / (eps * scale))])
+    return sp.vstack([top, bot]).tocsr()
+
+
+def find_vh(mesh, k, scale=1e14, eps=1e-6, S=None):
+    """find_Vh(k): k Arnoldi steps with modified Gram-Schmidt from the start
+    vector 1 / sqrt(2n) (src/fdtd.hpp:3589-3636), the eigenvalues of the k x k
+    Hessenberg H (dhseqr, :3754; numpy's LAPACK eigensolver here), the
+    eigenvectors of those with |Im| > eps (dhsein, :3797), and
+    Vh = V_top * eigenvectors (:3836-3850).  Conjugate pairs give the columns
+    y, conj(y) (:3811-3822).  Returns (Vh [n, leng], ritz [k] (times scale),
+    H [k, k], V [k + 1, 2n], h_next)."""
+    M = vh_operator(mesh, scale, S)
+    n2 = M.shape[0]
+    V = np.zeros((k + 1, n2))
+    H = np.zeros((k, k))
+    V[0] = 1.0 / np.sqrt(n2)
+    h_next = 0.0
+    for j in range(1, k + 1):
+        w = M @ V[j - 1]
+        for i in range(j):
+            H[i, j - 1] = V[i] @ w
+            w = w - H[i, j - 1] * V[i]
+        nrm = np.sqrt(w @ w)
+        if j < k:
+            H[j, j - 1] = nrm
+        else:
+            h_next = nrm
+        V[j] = w / nrm
+    lam, Y = np.linalg.eig(H)
+    # dhseqr's order is not numpy's: order conjugate pairs positive part first
+    sel = [i for i in range(k) if abs(lam[i].imag) > eps and lam[i].imag > 0]
+    cols = []
+    for i in sel:
+        y = Y[:, i] / Y[np.argmax(np.abs(Y[:, i])), i]
+        cols += [y, y.conj()]
+    Yc = np.array(cols).T if cols else np.zeros((k, 0), complex)
+    Vh = V[:k, :mesh.n].T @ Yc
+    return Vh, lam * scale, H, V, h_next

@@ -1,0 +1,106 @@
+"""Vh basis on the device (gp_find_vh, fdtdMesh::find_Vh(k), src/fdtd.hpp:3577-3859)
+against the oracle's restatement and find_reference_Vh's dense pencil.
+
+The Hessenberg matrix itself is chaotic in the step count (see test_vh.py), so
+the parity bar is on the converged part: converged Ritz values (residual
+estimate <= 1e-10) are Ritz values of the oracle's run to 1e-10 relative
+(1e-8 in the real conductor-damping cluster) and the complex ones
+eigenvalues of the dense pencil to 1e-9, the matching
+Vh columns span the same eigenvectors as the oracle's (|cos| >= 1 - 1e-9) and solve the quadratic
+eigen-equation; the reported Ritz values are the eigenvalues of the reported
+H; each conjugate pair's second column is the conjugate of its first."""
+import numpy as np
+import pytest
+
+import oracle
+from gds2para_amd import _capi as capi
+from gds2para_amd import mesh as M
+from gds2para_amd.layout import Layout
+from test_vh import _pencil_eigs, converged, match_spectrum, quad_residual
+
+pytestmark = pytest.mark.gpu
+
+SCALE = 1e14
+
+
+def _check(m, k, pencil, conv_tol=1e-10, min_conv=6, min_cplx=6, min_checked=3):
+    print(f"[vh] n={m.n} k={k}", flush=True)
+    lay = Layout(m)
+    Vh, lam, H, info = lay.find_vh(k)
+    lay.close()
+    print(f"[vh] {info}", flush=True)
+    assert info["k"] == k and info["leng_vh"] == Vh.shape[1]
+    # reported Ritz values are the eigenvalues of the reported H
+    match_spectrum(lam / SCALE, np.linalg.eigvals(H), np.abs(H).max())
+    assert info["leng_vh"] == int((np.abs(lam.imag / SCALE) > 1e-6).sum())
+    # H is upper Hessenberg with a positive subdiagonal
+    assert np.all(np.tril(H, -2) == 0) and np.all(np.diag(H, -1) > 0)
+    Vo, lo, Ho, _, hn_o = oracle.find_vh(m, k)
+    conv = converged(H, info["h_next"], conv_tol)
+    conv_o = converged(Ho, hn_o, conv_tol)
+    cg = conv[np.abs(conv.imag) > 1e-6]
+    co = conv_o[np.abs(conv_o.imag) > 1e-6]
+    assert len(conv) >= min_conv and len(conv_o) >= min_conv
+    assert len(cg) >= min_cplx and len(co) >= min_cplx
+    # every converged value of either run is one of the other's Ritz values:
+    # resonances to 1e-10, the conductor-damping cluster (real, tightly
+    # clustered, ill-conditioned) to 1e-8
+    for a, b in ((conv, lo), (conv_o, lam)):
+        for t in a:
+            tol = 1e-10 if abs(t.imag) > 1e-6 else 1e-8
+            assert np.abs(b / SCALE - t).min() <= tol * abs(t), t
+    if pencil is not None:
+        for t in cg:
+            assert np.abs(pencil - t).min() <= 1e-9 * abs(t), t
+    # Vh columns: pairs (y, conj y); converged columns match the oracle's span
+    sel = [l for l in lam if abs(l.imag / SCALE) > 1e-6 and l.imag > 0]
+    assert Vh.shape[1] == 2 * len(sel)
+    sel_o = [l for l in lo if abs(l.imag / SCALE) > 1e-6 and l.imag > 0]
+    S = oracle.stiffness_csr(m)
+    checked = 0
+    for c, l in enumerate(sel):
+        assert np.array_equal(Vh[:, 2 * c + 1], Vh[:, 2 * c].conj())
+        if len(cg) == 0 or np.abs(cg - l / SCALE).min() > 1e-12 * abs(l / SCALE):
+            continue
+        u = Vh[:, 2 * c]
+        assert abs(np.abs(u).max()) > 0
+        assert quad_residual(m, u, l, S) < 1e-8, (l, quad_residual(m, u, l, S))
+        d = [abs(x - l) for x in sel_o]
+        j = int(np.argmin(d))
+        if d[j] <= 1e-9 * abs(l):
+            uo = Vo[:, 2 * j]
+            cos = abs(np.vdot(uo, u)) / (np.linalg.norm(uo) * np.linalg.norm(u))
+            assert cos >= 1 - 1e-9, (l, cos)
+            checked += 1
+    assert checked >= min_checked
+    return info
+
+
+@pytest.mark.parametrize("cells,k", [((3, 3, 6), 60), ((4, 4, 6), 80)])
+def test_find_vh_plate_vs_oracle_and_pencil(cells, k):
+    m = M.Mesh(M.plate_spec(cells))
+    _check(m, k, _pencil_eigs(m))
+
+
+def test_find_vh_manhattan_vs_oracle():
+    """On a Manhattan mesh the conductor-damping cluster converges first: at
+    k = 200 about twenty of its values reach a residual of 1e-8 and no
+    resonance does yet, so the match covers the real cluster and the Vh
+    columns' structure."""
+    m = M.Mesh(M.manhattan_spec((16, 16, 10), n_ports=2, nfreq=2, seed=3))
+    _check(m, 200, None, conv_tol=1e-8, min_conv=10, min_cplx=0, min_checked=0)
+
+
+def test_find_vh_arguments():
+    m = M.Mesh(M.plate_spec((3, 3, 6)))
+    lay = Layout(m)
+    print("[vh] argument errors", flush=True)
+    with pytest.raises(capi.GpError) as e:
+        lay.find_vh(1)
+    assert e.value.code == capi.GP_ERR_ARG
+    with pytest.raises(capi.GpError) as e:
+        lay.find_vh(60, max_cols=2)          # more Vh columns than room for them
+    assert e.value.code == capi.GP_ERR_ARG
+    Vh, lam, H, info = lay.find_vh(60, max_cols=0)     # values only
+    assert Vh.shape[1] == 0 and info["leng_vh"] > 0
+    lay.close()
