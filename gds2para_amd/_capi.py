@@ -164,6 +164,8 @@ SIGNATURES = [
     ("gp_find_vh", i32, [C.c_void_p, i32, f64, f64, P(f64), P(f64), i64, P(f64), P(GpVhInfo)]),
     ("gp_hessenberg_eigenvalues", i32, [i32, P(f64), P(f64)]),
     ("gp_hessenberg_eigenvector", i32, [i32, P(f64), f64, f64, P(f64)]),
+    ("gp_v0_vh_inputs", i32, [C.c_void_p, i32, f64, P(f64), P(f64), P(f64)]),
+    ("gp_solve_vh", i32, [C.c_void_p, P(f64), i64, P(f64), P(f64), P(f64), i32, P(f64), i32, P(f64), P(f64)]),
     ("gp_convert_param", i32, [i32, i32, C.c_char, C.c_char, P(f64), P(f64), P(f64)]),
     ("gp_write_citi", i32, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char, i32, i32, P(f64), P(f64)]),
     ("gp_write_touchstone", i32, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char, i32, i32, P(f64), P(f64), P(f64)]),

@@ -19,6 +19,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <functional>
 #include <unordered_map>
 #include <vector>
 
@@ -133,6 +134,86 @@ double edge_len(const gp_mesh &m, const Grid &g, int64_t e) {
   return dir == 2 ? m.zn[iz + 1] - m.zn[iz] : dir == 1 ? m.xn[ix + 1] - m.xn[ix] : m.yn[iy + 1] - m.yn[iy];
 }
 
+
+// Device buffers of one port's V0 solves (N_edge-long unless noted).  The
+// last three are optional: the test-value ("a") products that find_Vh's
+// per-frequency solve needs (src/matrixCon.cpp:412, :529, :553).
+struct PortBufs {
+  double *J = nullptr, *t = nullptr, *yd1 = nullptr, *yc = nullptr, *yd2 = nullptr;
+  double *bd = nullptr, *xd = nullptr, *bc = nullptr, *xc = nullptr;  // nd / nc long
+  double *ydat = nullptr, *yca = nullptr, *yd2a = nullptr;
+};
+
+typedef std::function<gp_status(RealAMG *, const double *, double *)> RealRun;
+
+// paraGenerator's V0 section for source port src (src/matrixCon.cpp:367-587):
+// J, then
+//   y0d  = Ad^-1 (-V0da^T J) / w0,               yd1 = V0d y0d   (ydat = V0da y0d)
+//   y0c  = Ac^-1 (-V0ca^T J + V0ca^T (-w0 D_eps yd1)), yc = V0c y0c (yca = V0ca y0c)
+//   y0d2 = Ad^-1 (V0da^T (-D_eps yc)),           yd2 = V0d y0d2  (yd2a = V0da y0d2)
+gp_status port_solves(gp_layout *lay, V0Path &P, int32_t src, double w0, PortBufs &b, Arena &tmp,
+                      const RealRun &run) {
+  hipStream_t s = lay->stream;
+  const gp_mesh &m = lay->hm.view;
+  const int64_t ne = P.ne, nd = P.nd, nc = P.nc;
+  const unsigned ge = blocks_for(ne);
+  // J: portDirection of each side, assigned side by side (last assignment wins)
+  std::unordered_map<int64_t, size_t> jm;
+  std::vector<int64_t> ji;
+  std::vector<double> jvh;
+  for (int32_t sd = m.port_side_ptr[src]; sd < m.port_side_ptr[src + 1]; ++sd)
+    for (int64_t q = m.side_edge_ptr[sd]; q < m.side_edge_ptr[sd + 1]; ++q) {
+      auto it = jm.find(m.side_edges[q]);
+      if (it == jm.end()) { jm.emplace(m.side_edges[q], ji.size()); ji.push_back(m.side_edges[q]); jvh.push_back(m.side_dir[sd]); }
+      else jvh[it->second] = m.side_dir[sd];
+    }
+  const int64_t nj = (int64_t)ji.size();
+  GP_HIP(hipMemsetAsync(b.J, 0, sizeof(double) * ne, s));
+  if (nj > 0) {
+    int64_t *jidx = nullptr;
+    double *jv = nullptr;
+    GP_TRY(tmp.alloc(&jidx, nj)); GP_TRY(tmp.alloc(&jv, nj));
+    GP_HIP(hipMemcpyAsync(jidx, ji.data(), sizeof(int64_t) * nj, hipMemcpyHostToDevice, s));
+    GP_HIP(hipMemcpyAsync(jv, jvh.data(), sizeof(double) * nj, hipMemcpyHostToDevice, s));
+    k_scatter<<<blocks_for(nj), 256, 0, s>>>(nj, jidx, jv, b.J);
+    GP_HIP(hipStreamSynchronize(s));
+    tmp.release(jidx, nj);
+    tmp.release(jv, nj);
+  }
+  // y0d = Ad^-1 (-V0da^T J) / w0 ; yd1 = V0d y0d
+  k_mv_wave<<<wave_blocks(nd), 256, 0, s>>>(nd, P.VdT.rp, P.VdT.ci, P.VdT.v[1], b.J, b.bd, 1);
+  GP_TRY(run(P.ad, b.bd, b.xd));
+  k_div<<<blocks_for(nd), 256, 0, s>>>(nd, w0, b.xd);
+  k_mv_wave<<<wave_blocks(ne), 256, 0, s>>>(ne, P.Vd.rp, P.Vd.ci, P.Vd.v[0], b.xd, b.yd1, 0);
+  if (b.ydat) k_mv_wave<<<wave_blocks(ne), 256, 0, s>>>(ne, P.Vd.rp, P.Vd.ci, P.Vd.v[1], b.xd, b.ydat, 0);
+  if (nc > 0) {
+    // y0c = Ac^-1 (-V0ca^T J + V0ca^T (yd1 * (-w0 D_eps))) ; yc = V0c y0c
+    k_scale_eps_w<<<ge, 256, 0, s>>>(ne, lay->g.El, lay->epsn, w0, b.yd1, b.t);
+    k_mv_wave<<<wave_blocks(nc), 256, 0, s>>>(nc, P.VcT.rp, P.VcT.ci, P.VcT.v[1], b.J, b.bc, 1);
+    k_mv_wave<<<wave_blocks(nc), 256, 0, s>>>(nc, P.VcT.rp, P.VcT.ci, P.VcT.v[1], b.t, b.bc, 2);
+    GP_TRY(run(P.ac, b.bc, b.xc));
+    k_mv_wave<<<wave_blocks(ne), 256, 0, s>>>(ne, P.Vc.rp, P.Vc.ci, P.Vc.v[0], b.xc, b.yc, 0);
+    if (b.yca) k_mv_wave<<<wave_blocks(ne), 256, 0, s>>>(ne, P.Vc.rp, P.Vc.ci, P.Vc.v[1], b.xc, b.yca, 0);
+  } else {
+    GP_HIP(hipMemsetAsync(b.yc, 0, sizeof(double) * ne, s));
+    if (b.yca) GP_HIP(hipMemsetAsync(b.yca, 0, sizeof(double) * ne, s));
+  }
+  // y0d2 = Ad^-1 (V0da^T (-yc D_eps)) ; yd2 = V0d y0d2
+  k_neg_eps<<<ge, 256, 0, s>>>(ne, lay->g.El, lay->epsn, b.yc, b.t);
+  k_mv_wave<<<wave_blocks(nd), 256, 0, s>>>(nd, P.VdT.rp, P.VdT.ci, P.VdT.v[1], b.t, b.bd, 0);
+  GP_TRY(run(P.ad, b.bd, b.xd));
+  k_mv_wave<<<wave_blocks(ne), 256, 0, s>>>(ne, P.Vd.rp, P.Vd.ci, P.Vd.v[0], b.xd, b.yd2, 0);
+  if (b.yd2a) k_mv_wave<<<wave_blocks(ne), 256, 0, s>>>(ne, P.Vd.rp, P.Vd.ci, P.Vd.v[1], b.xd, b.yd2a, 0);
+  return GP_OK;
+}
+
+gp_status ensure_path(gp_layout *lay) {
+  GP_TRY(ensure_device());
+  if (!lay->v0) GP_TRY(build_v0(lay));
+  if (lay->v0->v0d_cols < 1) return fail(GP_ERR_ARG, "the layout has no V0d columns");
+  if (!lay->v0path) GP_TRY(setup(lay, &lay->v0path));
+  return GP_OK;
+}
 }  // namespace
 
 int64_t v0path_bytes(const V0Path *p) {
@@ -146,10 +227,7 @@ using namespace gp;
 
 extern "C" gp_status gp_solve_v0(gp_layout *lay, double rtol, double *Z0, double *X, gp_v0_stats *stats) {
   if (!lay || !Z0) return fail(GP_ERR_ARG, "gp_solve_v0: null layout or Z0");
-  GP_TRY(ensure_device());
-  if (!lay->v0) GP_TRY(build_v0(lay));
-  if (lay->v0->v0d_cols < 1) return fail(GP_ERR_ARG, "gp_solve_v0: the layout has no V0d columns");
-  if (!lay->v0path) GP_TRY(setup(lay, &lay->v0path));
+  GP_TRY(ensure_path(lay));
   V0Path &P = *lay->v0path;
   hipStream_t s = lay->stream;
   const gp_mesh &m = lay->hm.view;
@@ -170,8 +248,8 @@ extern "C" gp_status gp_solve_v0(gp_layout *lay, double rtol, double *Z0, double
   struct TGuard { Arena *a; hipStream_t s; ~TGuard() { (void)hipStreamSynchronize(s); a->free_all(); } } tg{&tmp, s};
   const int64_t ne = P.ne, nd = P.nd, nc = P.nc;
   const int64_t nr = std::max<int64_t>((int64_t)redge.size(), 1);
-  double *J, *t, *yd1, *yc, *yd2, *bd, *xd, *bc = nullptr, *xc = nullptr, *gat, *jv;
-  int64_t *ridx, *jidx;
+  double *J, *t, *yd1, *yc, *yd2, *bd, *xd, *bc = nullptr, *xc = nullptr, *gat;
+  int64_t *ridx;
   GP_TRY(tmp.alloc(&J, ne)); GP_TRY(tmp.alloc(&t, ne)); GP_TRY(tmp.alloc(&yd1, ne));
   GP_TRY(tmp.alloc(&yc, ne)); GP_TRY(tmp.alloc(&yd2, ne));
   GP_TRY(tmp.alloc(&bd, nd)); GP_TRY(tmp.alloc(&xd, nd));
@@ -179,13 +257,12 @@ extern "C" gp_status gp_solve_v0(gp_layout *lay, double rtol, double *Z0, double
   GP_TRY(tmp.alloc(&ridx, nr)); GP_TRY(tmp.alloc(&gat, 2 * nr));
   if (!redge.empty())
     GP_HIP(hipMemcpyAsync(ridx, redge.data(), sizeof(int64_t) * redge.size(), hipMemcpyHostToDevice, s));
-  const unsigned ge = blocks_for(ne);
 
   std::vector<double> Zh(2 * (size_t)np * np, 0.), yh(2 * nr);
   int32_t solves = 0, maxit = 0;
   double maxrel = 0.;
   gp_status result = GP_OK;
-  auto run = [&](RealAMG *G, const double *b, double *x) -> gp_status {
+  RealRun run = [&](RealAMG *G, const double *b, double *x) -> gp_status {
     int32_t it = 0;
     double rel = 0.;
     gp_status st = real_amg_solve(G, b, x, tol, 1000, &it, &rel, s);
@@ -195,48 +272,10 @@ extern "C" gp_status gp_solve_v0(gp_layout *lay, double rtol, double *Z0, double
     if (st == GP_ERR_NOCONV) { result = GP_ERR_NOCONV; return GP_OK; }  // reported, x kept (as hypreSolve)
     return st;
   };
+  PortBufs pb;
+  pb.J = J; pb.t = t; pb.yd1 = yd1; pb.yc = yc; pb.yd2 = yd2; pb.bd = bd; pb.xd = xd; pb.bc = bc; pb.xc = xc;
   for (int32_t src = 0; src < np; ++src) {
-    // J: portDirection of each side, assigned side by side (last assignment wins)
-    std::unordered_map<int64_t, size_t> jm;
-    std::vector<int64_t> ji;
-    std::vector<double> jvh;
-    for (int32_t sd = m.port_side_ptr[src]; sd < m.port_side_ptr[src + 1]; ++sd)
-      for (int64_t q = m.side_edge_ptr[sd]; q < m.side_edge_ptr[sd + 1]; ++q) {
-        auto it = jm.find(m.side_edges[q]);
-        if (it == jm.end()) { jm.emplace(m.side_edges[q], ji.size()); ji.push_back(m.side_edges[q]); jvh.push_back(m.side_dir[sd]); }
-        else jvh[it->second] = m.side_dir[sd];
-      }
-    const int64_t nj = (int64_t)ji.size();
-    GP_HIP(hipMemsetAsync(J, 0, sizeof(double) * ne, s));
-    if (nj > 0) {
-      GP_TRY(tmp.alloc(&jidx, nj)); GP_TRY(tmp.alloc(&jv, nj));
-      GP_HIP(hipMemcpyAsync(jidx, ji.data(), sizeof(int64_t) * nj, hipMemcpyHostToDevice, s));
-      GP_HIP(hipMemcpyAsync(jv, jvh.data(), sizeof(double) * nj, hipMemcpyHostToDevice, s));
-      k_scatter<<<blocks_for(nj), 256, 0, s>>>(nj, jidx, jv, J);
-      GP_HIP(hipStreamSynchronize(s));
-      tmp.release(jidx, nj);
-      tmp.release(jv, nj);
-    }
-    // y0d = Ad^-1 (-V0da^T J) / w0 ; yd1 = V0d y0d
-    k_mv_wave<<<wave_blocks(nd), 256, 0, s>>>(nd, P.VdT.rp, P.VdT.ci, P.VdT.v[1], J, bd, 1);
-    GP_TRY(run(P.ad, bd, xd));
-    k_div<<<blocks_for(nd), 256, 0, s>>>(nd, w0, xd);
-    k_mv_wave<<<wave_blocks(ne), 256, 0, s>>>(ne, P.Vd.rp, P.Vd.ci, P.Vd.v[0], xd, yd1, 0);
-    if (nc > 0) {
-      // y0c = Ac^-1 (-V0ca^T J + V0ca^T (yd1 * (-w0 D_eps))) ; yc = V0c y0c
-      k_scale_eps_w<<<ge, 256, 0, s>>>(ne, lay->g.El, lay->epsn, w0, yd1, t);
-      k_mv_wave<<<wave_blocks(nc), 256, 0, s>>>(nc, P.VcT.rp, P.VcT.ci, P.VcT.v[1], J, bc, 1);
-      k_mv_wave<<<wave_blocks(nc), 256, 0, s>>>(nc, P.VcT.rp, P.VcT.ci, P.VcT.v[1], t, bc, 2);
-      GP_TRY(run(P.ac, bc, xc));
-      k_mv_wave<<<wave_blocks(ne), 256, 0, s>>>(ne, P.Vc.rp, P.Vc.ci, P.Vc.v[0], xc, yc, 0);
-    } else {
-      GP_HIP(hipMemsetAsync(yc, 0, sizeof(double) * ne, s));
-    }
-    // y0d2 = Ad^-1 (V0da^T (-yc D_eps)) ; yd2 = V0d y0d2
-    k_neg_eps<<<ge, 256, 0, s>>>(ne, lay->g.El, lay->epsn, yc, t);
-    k_mv_wave<<<wave_blocks(nd), 256, 0, s>>>(nd, P.VdT.rp, P.VdT.ci, P.VdT.v[1], t, bd, 0);
-    GP_TRY(run(P.ad, bd, xd));
-    k_mv_wave<<<wave_blocks(ne), 256, 0, s>>>(ne, P.Vd.rp, P.Vd.ci, P.Vd.v[0], xd, yd2, 0);
+    GP_TRY(port_solves(lay, P, src, w0, pb, tmp, run));
     // yd at the response edges, then Construct_Z_V0 on the host
     if (!redge.empty()) {
       k_gather_yd<<<blocks_for((int64_t)redge.size()), 256, 0, s>>>((int64_t)redge.size(), ridx, yd2, yd1, yc, gat);
@@ -278,6 +317,69 @@ extern "C" gp_status gp_solve_v0(gp_layout *lay, double rtol, double *Z0, double
     stats->ac_rows = nc;
   }
   return result == GP_OK ? GP_OK : fail(GP_ERR_NOCONV, "a V0 real solve stopped above the tolerance");
+}
+
+// The V0 vectors of find_Vh's per-frequency solve for one source port
+// (src/matrixCon.cpp:404-427, :547-568, :569-587), from the same solves as
+// gp_solve_v0; the norms and the final scaling run on the host in the
+// reference's order.
+extern "C" gp_status gp_v0_vh_inputs(gp_layout *lay, int32_t src, double rtol, double *u0, double *u0a,
+                                     double *yd) {
+  if (!lay || !u0 || !u0a || !yd) return fail(GP_ERR_ARG, "gp_v0_vh_inputs: null argument");
+  if (src < 0 || src >= lay->hm.view.num_ports) return fail(GP_ERR_ARG, "gp_v0_vh_inputs: source port out of range");
+  GP_TRY(ensure_path(lay));
+  V0Path &P = *lay->v0path;
+  hipStream_t s = lay->stream;
+  const gp_mesh &m = lay->hm.view;
+  const double tol = rtol > 0 ? rtol : 1e-4;
+  const double w0 = 2 * kPi * m.freq_start * m.freq_unit;
+  const int64_t ne = P.ne, nd = P.nd, nc = P.nc, n = lay->n;
+  Arena tmp;
+  struct TGuard { Arena *a; hipStream_t s; ~TGuard() { (void)hipStreamSynchronize(s); a->free_all(); } } tg{&tmp, s};
+  PortBufs b;
+  GP_TRY(tmp.alloc(&b.J, ne)); GP_TRY(tmp.alloc(&b.t, ne)); GP_TRY(tmp.alloc(&b.yd1, ne));
+  GP_TRY(tmp.alloc(&b.yc, ne)); GP_TRY(tmp.alloc(&b.yd2, ne));
+  GP_TRY(tmp.alloc(&b.ydat, ne)); GP_TRY(tmp.alloc(&b.yca, ne)); GP_TRY(tmp.alloc(&b.yd2a, ne));
+  GP_TRY(tmp.alloc(&b.bd, nd)); GP_TRY(tmp.alloc(&b.xd, nd));
+  if (nc > 0) { GP_TRY(tmp.alloc(&b.bc, nc)); GP_TRY(tmp.alloc(&b.xc, nc)); }
+  bool noconv = false;
+  RealRun run = [&](RealAMG *G, const double *rhs, double *x) -> gp_status {
+    int32_t it = 0;
+    double rel = 0.;
+    gp_status st = real_amg_solve(G, rhs, x, tol, 1000, &it, &rel, s);
+    if (st == GP_ERR_NOCONV) { noconv = true; return GP_OK; }
+    return st;
+  };
+  GP_TRY(port_solves(lay, P, src, w0, b, tmp, run));
+  std::vector<double> yd1(ne), ydat(ne), yd2(ne), yd2a(ne), yc(ne), yca(ne);
+  std::vector<int64_t> mr(n);
+  GP_HIP(hipMemcpyAsync(yd1.data(), b.yd1, sizeof(double) * ne, hipMemcpyDeviceToHost, s));
+  GP_HIP(hipMemcpyAsync(ydat.data(), b.ydat, sizeof(double) * ne, hipMemcpyDeviceToHost, s));
+  GP_HIP(hipMemcpyAsync(yd2.data(), b.yd2, sizeof(double) * ne, hipMemcpyDeviceToHost, s));
+  GP_HIP(hipMemcpyAsync(yd2a.data(), b.yd2a, sizeof(double) * ne, hipMemcpyDeviceToHost, s));
+  GP_HIP(hipMemcpyAsync(yc.data(), b.yc, sizeof(double) * ne, hipMemcpyDeviceToHost, s));
+  GP_HIP(hipMemcpyAsync(yca.data(), b.yca, sizeof(double) * ne, hipMemcpyDeviceToHost, s));
+  GP_HIP(hipMemcpyAsync(mr.data(), lay->map_edge_r, sizeof(int64_t) * n, hipMemcpyDeviceToHost, s));
+  GP_HIP(hipStreamSynchronize(s));
+  // u0 = [V0d y0d, V0d y0d2 + V0c y0c], u0a = [V0da y0d, V0da y0d2 + V0ca y0c],
+  // each column normalised over all N_edge entries, rows by mapEdgeR
+  double nn = 0, nna = 0;
+  for (int64_t e = 0; e < ne; ++e) { nn += yd1[e] * yd1[e]; nna += ydat[e] * ydat[e]; }
+  nn = std::sqrt(nn); nna = std::sqrt(nna);
+  for (int64_t i = 0; i < n; ++i) { u0[i] = yd1[mr[i]] / nn; u0a[i] = ydat[mr[i]] / nna; }
+  nn = 0; nna = 0;
+  for (int64_t e = 0; e < ne; ++e) {
+    nn += (yd2[e] + yc[e]) * (yd2[e] + yc[e]);
+    nna += (yd2a[e] + yca[e]) * (yd2a[e] + yca[e]);
+  }
+  nn = std::sqrt(nn); nna = std::sqrt(nna);
+  for (int64_t i = 0; i < n; ++i) {
+    u0[n + i] = (yd2[mr[i]] + yc[mr[i]]) / nn;
+    u0a[n + i] = (yd2a[mr[i]] + yca[mr[i]]) / nna;
+  }
+  // yd = yd2 - i yd1, then += yc
+  for (int64_t e = 0; e < ne; ++e) { yd[2 * e] = yd2[e] + yc[e]; yd[2 * e + 1] = -yd1[e]; }
+  return noconv ? fail(GP_ERR_NOCONV, "a V0 real solve stopped above the tolerance") : GP_OK;
 }
 
 // print_z_V0's store (src/fdtd.hpp:4249-4316)

@@ -165,6 +165,31 @@ class Layout:
         stats = {f: getattr(info, f) for f, _ in capi.GpVhInfo._fields_}
         return V, ritz[0::2] + 1j * ritz[1::2], hess.reshape(k, k).T, stats
 
+    def v0_vh_inputs(self, port, rtol=0.0):
+        """(u0 [n, 2], u0a [n, 2], yd [N_edge] complex) of paraGenerator's Vh
+        section for one source port (gp_v0_vh_inputs)."""
+        n, ne = self.mesh.n, self.mesh.n_edge
+        u0, u0a, yd = np.zeros(2 * n), np.zeros(2 * n), np.zeros(2 * ne)
+        capi.check(self.lib.gp_v0_vh_inputs(self.h, int(port), float(rtol), _p(u0, f64p), _p(u0a, f64p),
+                                            _p(yd, f64p)), "gp_v0_vh_inputs")
+        return u0.reshape(2, n).T, u0a.reshape(2, n).T, yd[0::2] + 1j * yd[1::2]
+
+    def solve_vh(self, Vh, u0, u0a, yd, port, freqs):
+        """paraGenerator's per-frequency Vh solve (gp_solve_vh): returns
+        (X [nf, n] complex, Z [nf, np*np] complex with column `port` filled)."""
+        n, npt = self.mesh.n, self.mesh.num_ports
+        freqs = np.ascontiguousarray(np.atleast_1d(np.asarray(freqs, np.float64)))
+        nf, L = len(freqs), Vh.shape[1]
+        vh = np.ascontiguousarray(np.stack([Vh.T.real, Vh.T.imag], -1).ravel())
+        u = np.ascontiguousarray(np.asarray(u0, np.float64).T.ravel())
+        ua = np.ascontiguousarray(np.asarray(u0a, np.float64).T.ravel())
+        y = np.ascontiguousarray(np.stack([yd.real, yd.imag], -1).ravel())
+        X = np.zeros(2 * n * nf)
+        Z = np.zeros(2 * nf * npt * npt)
+        capi.check(self.lib.gp_solve_vh(self.h, _p(vh, f64p), L, _p(u, f64p), _p(ua, f64p), _p(y, f64p), int(port),
+                                        _p(freqs, f64p), nf, _p(X, f64p), _p(Z, f64p)), "gp_solve_vh")
+        return (X[0::2] + 1j * X[1::2]).reshape(nf, n), (Z[0::2] + 1j * Z[1::2]).reshape(nf, npt * npt)
+
     def solve(self, freqs, ports=None, precision=capi.GP_PREC_MIXED, rtol=1e-8, batch=0, restart=16,
               max_iters=2000, want_x=False, verbose=0, smooth_edge=0, smooth_aux=0, sell_min_rows=0,
               aux_post=0, basis=0, graphs=1):

@@ -23,6 +23,8 @@
  *   gp_find_vh              fdtdMesh::find_Vh(int k)                src/fdtd.hpp:3577
  *   gp_hessenberg_eigenvalues / _eigenvector
  *                           LAPACKE_dhseqr / LAPACKE_dhsein in find_Vh  src/fdtd.hpp:3754,3797
+ *   gp_v0_vh_inputs         paraGenerator's u0 / u0a / yd           src/matrixCon.cpp:404-427,547-587
+ *   gp_solve_vh             paraGenerator's per-frequency Vh solve  src/matrixCon.cpp:608-801
  *   gp_convert_param        Parasitics::convertParam                src/solnoutclass.hpp:1456
  *   gp_write_citi           SolverDataBase::dumpCITI                src/solnoutclass.hpp:6040
  *   gp_write_touchstone     SolverDataBase::dumpTouchstone          src/solnoutclass.hpp:6104
@@ -453,6 +455,31 @@ gp_status gp_find_vh(gp_layout *lay, int32_t k, double scale, double eps, double
  * interleaved), scaled so its largest entry is 1. */
 gp_status gp_hessenberg_eigenvalues(int32_t n, const double *h, double *w);
 gp_status gp_hessenberg_eigenvector(int32_t n, const double *h, double re, double im, double *v);
+
+/* The V0 vectors paraGenerator's Vh section reads for one source port
+ * (src/matrixCon.cpp:404-427, :547-587), from the same three real solves as
+ * gp_solve_v0 (rtol 0 = the reference's 1e-4):
+ *   u0, u0a  n x 2 real, column-major (u0[c*n + i]), rows by mapEdgeR:
+ *            u0 = [V0d y0d, V0d y0d2 + V0c y0c], u0a = [V0da y0d, V0da y0d2 +
+ *            V0ca y0c], each column divided by its norm over all N_edge entries
+ *   yd       N_edge complex (interleaved), original edge order: the V0 solution
+ *            (V0d y0d2 + V0c y0c) - i V0d y0d */
+gp_status gp_v0_vh_inputs(gp_layout *lay, int32_t source_port, double rtol, double *u0, double *u0a, double *yd);
+
+/* paraGenerator's per-frequency Vh solve (src/matrixCon.cpp:608-801) for one
+ * source port: at each frequency, deflate u0 from Vh, solve the reduced
+ * system Vh'^H (S + D) Vh' y = Vh'^H (J - D y_w), x = y_w + Vh' y with
+ * y_w = Re yd + i Im yd f0 / f, then Construct_Z_V0_Vh(x) into Z.  D uses this
+ * loop's own coefficients (eps from layer (e + N_edge_v) / (N_edge_s + N_edge_v),
+ * sigma = 5.8e7 on conductor edges) and J = -i w on the port's edges.
+ *   vh     n x leng_vh complex, column-major (as gp_find_vh returns it)
+ *   u0/u0a/yd as gp_v0_vh_inputs returns them
+ *   X      n x nf complex, or null (the reference's final_x per frequency)
+ *   Z      nf blocks of np*np complex; column source_port of block k receives
+ *          Construct_Z_V0_Vh as gp_extract_z computes it (zero Z first)
+ * GP_ERR_STATE if a 2 x 2 deflation system or the reduced system is singular. */
+gp_status gp_solve_vh(gp_layout *lay, const double *vh, int64_t leng_vh, const double *u0, const double *u0a,
+                      const double *yd, int32_t source_port, const double *freqs, int32_t nf, double *X, double *Z);
 
 /* ---- network-parameter output (host only) ------------------------------- */
 /* Matrices: nf blocks of np x np complex (interleaved), row-major,

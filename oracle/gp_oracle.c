@@ -654,6 +654,34 @@ int or_vh_coeffs(const gp_mesh *m, double *eps, double *sig) {
   return 0;
 }
 
+/* mapEdge / mapEdgeR (setMapEdge, src/fdtd.hpp:1508-1529) as arrays */
+int or_edge_map(const gp_mesh *m, int64_t *map_out, int64_t *map_r) {
+  or_grid g = or_make_grid(m);
+  int64_t *map = or_map_edge(m, &g);
+  for (int64_t e = 0; e < g.Ne; ++e) {
+    map_out[e] = map[e];
+    if (map[e] >= 0) map_r[map[e]] = e;
+  }
+  free(map);
+  return 0;
+}
+
+/* The per-frequency Vh solve's coefficients (src/matrixCon.cpp:614-622):
+ * eps[i] = stackEpsn[(e + N_edge_v) / (N_edge_s + N_edge_v)] * EPSILON0 and
+ * sig[i] = SIGMA (5.8e7, src/fdtd.hpp:50) if markEdge[e] != 0, else 0, for
+ * unknown i = mapEdge[e]. */
+int or_vh_solve_coeffs(const gp_mesh *m, double *eps, double *sig) {
+  or_grid g = or_make_grid(m);
+  int64_t *map = or_map_edge(m, &g);
+  for (int64_t e = 0; e < g.Ne; ++e) {
+    if (map[e] < 0) continue;
+    eps[map[e]] = m->stack_epsn[(e + g.Ev) / (g.Es + g.Ev)] * OR_EPSILON0;
+    sig[map[e]] = m->mark_edge[e] != 0 ? 5.8e+7 : 0.;
+  }
+  free(map);
+  return 0;
+}
+
 /* Construct_Z_V0 (src/fdtd.hpp:4178-4215): x over ORIGINAL edge ids (N_edge
  * complex, interleaved), accumulated into Z0[resp + np * src] */
 int or_extract_z_v0(const gp_mesh *m, const double *x, int32_t src, double *Z0) {

@@ -45,6 +45,8 @@ def _load():
         lib.or_port_j.argtypes = [C.c_void_p, C.c_int32, f64p]
         lib.or_edge_eps.argtypes = [C.c_void_p, f64p]
         lib.or_vh_coeffs.argtypes = [C.c_void_p, f64p, f64p]
+        lib.or_edge_map.argtypes = [C.c_void_p, i64p, i64p]
+        lib.or_vh_solve_coeffs.argtypes = [C.c_void_p, f64p, f64p]
         lib.or_extract_z_v0.argtypes = [C.c_void_p, f64p, C.c_int32, f64p]
         lib.or_print_z_v0.argtypes = [C.c_void_p, f64p]
         _lib = lib
@@ -451,3 +453,93 @@ def find_vh(mesh, k, scale=1e14, eps=1e-6, S=None):
     Yc = np.array(cols).T if cols else np.zeros((k, 0), complex)
     Vh = V[:k, :mesh.n].T @ Yc
     return Vh, lam * scale, H, V, h_next
+
+
+def edge_map(mesh):
+    """(mapEdge [N_edge], mapEdgeR [n]) as setMapEdge builds them."""
+    m, r = np.empty(mesh.n_edge, np.int64), np.empty(mesh.n, np.int64)
+    _load().or_edge_map(mesh.ptr, _p(m, i64p), _p(r, i64p))
+    return m, r
+
+
+def vh_inputs(mesh, src, ops=None):
+    """u0, u0a (n x 2) and yd (N_edge complex) of paraGenerator's Vh section
+    for one source port (src/matrixCon.cpp:377-427, :434-587), with exact
+    (SuperLU) solves in place of hypreSolve."""
+    ops = v0_operators(mesh) if ops is None else ops
+    Vd, Vda, Vc, Vca = ops["Vd"], ops["Vda"], ops["Vc"], ops["Vca"]
+    lud = spla.splu(ops["Ad"].tocsc())
+    eps = edge_eps(mesh)
+    w0 = 2 * np.pi * mesh.c.freq_start * mesh.c.freq_unit
+    _, mr = edge_map(mesh)
+    J = port_j(mesh, src)
+    y0d = lud.solve(-(Vda.T @ J)) / w0                # :392, :401-403
+    ydt, ydat = Vd @ y0d, Vda @ y0d                   # :411-412
+    n = mesh.n
+    u0, u0a = np.zeros((n, 2)), np.zeros((n, 2))
+    u0[:, 0] = ydt[mr] / np.sqrt(np.sum(ydt * ydt))   # :416-427
+    u0a[:, 0] = ydat[mr] / np.sqrt(np.sum(ydat * ydat))
+    if ops["Ac"].shape[0] > 0:
+        luc = spla.splu(ops["Ac"].tocsc())
+        y0c = luc.solve(-(Vca.T @ J) + Vca.T @ (ydt * (-1.0 * w0 * eps)))   # :434-511
+        yc, yca = Vc @ y0c, Vca @ y0c                                      # :528-529
+    else:
+        yc = yca = np.zeros(mesh.n_edge)
+    y0d2 = lud.solve(Vda.T @ (-yc * eps))             # :536-544
+    yd2, yd2a = Vd @ y0d2, Vda @ y0d2                 # :552-553
+    u0[:, 1] = (yd2 + yc)[mr] / np.sqrt(np.sum((yd2 + yc) * (yd2 + yc)))   # :556-568
+    u0a[:, 1] = (yd2a + yca)[mr] / np.sqrt(np.sum((yd2a + yca) * (yd2a + yca)))
+    yd = (yd2 - 1j * ydt) + yc                        # :570-587
+    return u0, u0a, yd
+
+
+def vh_solve_coeffs(mesh):
+    e, s = np.zeros(mesh.n), np.zeros(mesh.n)
+    _load().or_vh_solve_coeffs(mesh.ptr, _p(e, f64p), _p(s, f64p))
+    return e, s
+
+
+def vh_port_rows(mesh, src):
+    """Rows where the per-frequency Vh solve puts J = -i w: the unknowns of
+    the source port's edges with J[e] != 0 (src/matrixCon.cpp:727-731)."""
+    J = port_j(mesh, src)
+    m, _ = edge_map(mesh)
+    rows = m[np.nonzero(J)[0]]
+    return rows[rows >= 0]
+
+
+def vh_deflate(Vh, u0, u0a, d):
+    """Vh - u0 (u0a^H D u0)^-1 (u0a^H D Vh) (src/matrixCon.cpp:611-690; u0 and
+    u0a are real, so ^H is ^T)."""
+    y_re = u0a.T @ (d[:, None] * Vh)                           # :612-641
+    t4 = u0a.T @ (d[:, None] * u0)                             # :642-661
+    return Vh - u0 @ np.linalg.solve(t4, y_re)                 # :671-690
+
+
+def solve_vh(mesh, Vh, u0, u0a, yd, src, freqs, S=None):
+    """paraGenerator's per-frequency Vh solve (src/matrixCon.cpp:608-801);
+    matrix_multi 'T' is the conjugate transpose (src/findVh.cpp:1205-1218).
+    Returns (X [nf, n], Z [nf, np*np] with column src filled)."""
+    S = stiffness_csr(mesh) if S is None else S
+    eps, sig = vh_solve_coeffs(mesh)
+    _, mr = edge_map(mesh)
+    rows = vh_port_rows(mesh, src)
+    f0 = mesh.c.freq_start * mesh.c.freq_unit
+    ydr = yd[mr]
+    npt = mesh.num_ports
+    Xs, Zs = [], []
+    for f in freqs:
+        w = 2 * np.pi * f
+        d = -(w * w) * eps + 1j * w * sig
+        Vp = vh_deflate(Vh, u0, u0a, d)                        # :611-690
+        T = S @ Vp + d[:, None] * Vp                           # :692-722
+        m_h = Vp.conj().T @ T                                  # :723-724
+        Jc = np.zeros(mesh.n, complex)
+        Jc[rows] = -1j * w                                     # :726-731
+        yw = ydr.real + 1j * ydr.imag * f0 / f
+        rhs = Vp.conj().T @ Jc - Vp.conj().T @ (d * yw)         # :732-776
+        y = np.linalg.solve(m_h, rhs)                          # :777-781
+        x = yw + Vp @ y                                        # :782-786
+        Xs.append(x)
+        Zs.append(extract_z(mesh, x, src))
+    return np.array(Xs), np.array(Zs)

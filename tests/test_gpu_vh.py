@@ -104,3 +104,88 @@ def test_find_vh_arguments():
     Vh, lam, H, info = lay.find_vh(60, max_cols=0)     # values only
     assert Vh.shape[1] == 0 and info["leng_vh"] > 0
     lay.close()
+
+
+# ---- per-frequency Vh solve (gp_v0_vh_inputs, gp_solve_vh) ----------------------
+
+from test_vh import VH_SOLVE_MESHES, exact_in_span_case, span_tol  # noqa: E402
+
+
+@pytest.fixture(scope="module", params=sorted(VH_SOLVE_MESHES))
+def vh_setup(request):
+    spec, k = VH_SOLVE_MESHES[request.param]
+    m = M.Mesh(spec())
+    u0, u0a, yd = oracle.vh_inputs(m, 0)
+    return request.param, m, k, u0, u0a, yd
+
+
+def test_v0_vh_inputs_match_oracle(vh_setup):
+    """u0, u0a and yd from the device's V0 solves (real solves at rtol 1e-12)
+    against the oracle's exact (SuperLU) solves."""
+    name, m, k, u0, u0a, yd = vh_setup
+    print(f"[vh-solve] {name} v0 inputs", flush=True)
+    lay = Layout(m)
+    g0, ga, gy = lay.v0_vh_inputs(0, rtol=1e-12)
+    lay.close()
+    for a, b in ((g0, u0), (ga, u0a)):
+        assert np.abs(a - b).max() <= 1e-8 * np.abs(b).max()
+    assert np.abs(gy - yd).max() <= 1e-8 * np.abs(yd).max()
+
+
+def test_solve_vh_matches_oracle(vh_setup):
+    """Identical inputs (the oracle's Vh, u0, u0a, yd): x and Z to 1e-9 at
+    100 MHz, 1 GHz and 10 GHz."""
+    name, m, k, u0, u0a, yd = vh_setup
+    Vh = oracle.find_vh(m, k)[0]
+    fs = [1e8, 1e9, 1e10]
+    Xo, Zo = oracle.solve_vh(m, Vh, u0, u0a, yd, 0, fs)
+    print(f"[vh-solve] {name} L={Vh.shape[1]} vs oracle", flush=True)
+    lay = Layout(m)
+    X, Z = lay.solve_vh(Vh, u0, u0a, yd, 0, fs)
+    lay.close()
+    for i in range(len(fs)):
+        ex = np.linalg.norm(X[i] - Xo[i]) / np.linalg.norm(Xo[i])
+        ez = np.abs(Z[i] - Zo[i]).max() / np.abs(Zo[i]).max()
+        print(f"[vh-solve] f={fs[i]:.0e} x {ex:.2e} Z {ez:.2e}", flush=True)
+        assert ex <= 1e-9 and ez <= 1e-9
+
+
+@pytest.mark.parametrize("f", [1e8, 1e9, 1e10])
+def test_solve_vh_exact_in_span(vh_setup, f):
+    """A basis that holds the exact correction: the device returns the
+    refined direct solution of the loop's own system, to the method's FP64
+    accuracy."""
+    name, m, k, u0, u0a, yd = vh_setup
+    Vh, ua, xs = exact_in_span_case(m, f, u0, yd)
+    print(f"[vh-solve] {name} exact-in-span f={f:.0e}", flush=True)
+    lay = Layout(m)
+    X, Z = lay.solve_vh(Vh, u0, ua, yd, 0, [f])
+    lay.close()
+    assert np.linalg.norm(X[0] - xs) <= span_tol(f) * np.linalg.norm(xs)
+
+
+def test_vh_pipeline_against_fullwave(vh_setup):
+    """find_Vh, the V0 vectors and the per-frequency solve, all on the device:
+    Z within 1e-3 of the refined full-wave direct solve (the loop's own eps
+    layer and SIGMA make it a slightly different system) and within 1e-4 of
+    the oracle's pipeline (the two Arnoldi bases agree on their converged
+    part only)."""
+    name, m, k, u0, u0a, yd = vh_setup
+    fs = [1e8, 1e9, 1e10]
+    print(f"[vh-solve] {name} device pipeline", flush=True)
+    lay = Layout(m)
+    Vh, lam, H, info = lay.find_vh(k)
+    g0, ga, gy = lay.v0_vh_inputs(0, rtol=1e-12)
+    X, Z = lay.solve_vh(Vh, g0, ga, gy, 0, fs)
+    lay.close()
+    Vo = oracle.find_vh(m, k)[0]
+    _, Zo = oracle.solve_vh(m, Vo, u0, u0a, yd, 0, fs)
+    _, Zd = oracle.direct_solve(m, fs, ports=[0], refine=True)
+    npt = m.num_ports
+    for i in range(len(fs)):
+        col = slice(0, npt)                          # column of source port 0
+        zd = Zd[i][col]
+        e_d = np.abs(Z[i][col] - zd).max() / np.abs(zd).max()
+        e_o = np.abs(Z[i][col] - Zo[i][col]).max() / np.abs(Zo[i][col]).max()
+        print(f"[vh-solve] f={fs[i]:.0e} vs full-wave {e_d:.2e} vs oracle pipeline {e_o:.2e}", flush=True)
+        assert e_d <= 1e-3 and e_o <= 1e-4

@@ -247,3 +247,129 @@ def test_oracle_vh_planted_sigma_mistake_is_caught():
     conv = converged(H, np.linalg.norm(w), 1e-10)
     assert len(conv) > 0
     assert max(np.abs(mu - t).min() / abs(t) for t in conv) > 1e-6
+
+
+# ---- per-frequency Vh solve (src/matrixCon.cpp:608-801) --------------------------
+
+VH_SOLVE_MESHES = {
+    "plate": (lambda: M.plate_spec((4, 4, 6)), 80),
+    "manhattan": (lambda: M.manhattan_spec((16, 16, 10), n_ports=2, nfreq=2, seed=3), 120),
+}
+
+
+def exact_in_span_case(m, f, u0, yd, seed=1):
+    """Inputs for which the Vh solve's answer is known: x* solves
+    (S + D(w)) x = J with the loop's own coefficients (SuperLU with
+    double-double refinement), Vh holds
+    v = x* - y_w and two random columns, and u0a is random but orthogonal to
+    Re(D v) and Im(D v), so the deflation leaves v in the span.  The Galerkin
+    solution on y_w + span(Vh') is then x* (S y_w = 0: yd is a gradient field)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    S = oracle.stiffness_csr(m)
+    eps, sig = oracle.vh_solve_coeffs(m)
+    w = 2 * np.pi * f
+    d = -(w * w) * eps + 1j * w * sig
+    Jc = np.zeros(m.n, complex)
+    Jc[oracle.vh_port_rows(m, 0)] = -1j * w
+    A = (S + sp.diags(d)).tocsc()
+    xr, xi = oracle.refined_solve_dd(spla.splu(A), A, Jc)   # low-frequency breakdown: refine
+    xs = (xr[0] + xr[1]) + 1j * (xi[0] + xi[1])
+    _, mr = oracle.edge_map(m)
+    f0 = m.c.freq_start * m.c.freq_unit
+    yw = yd[mr].real + 1j * yd[mr].imag * f0 / f
+    v = xs - yw
+    rng = np.random.default_rng(seed)
+    Q = np.linalg.qr(np.stack([(d * v).real, (d * v).imag], 1))[0]
+    ua = rng.standard_normal((m.n, 2))
+    ua -= Q @ (Q.T @ ua)
+    Vh = np.stack([v, rng.standard_normal(m.n) + 1j * rng.standard_normal(m.n), rng.standard_normal(m.n) + 0j], 1)
+    return Vh, ua, xs
+
+
+@pytest.fixture(scope="module", params=sorted(VH_SOLVE_MESHES))
+def vh_case(request):
+    spec, k = VH_SOLVE_MESHES[request.param]
+    m = M.Mesh(spec())
+    u0, u0a, yd = oracle.vh_inputs(m, 0)
+    return request.param, m, k, u0, u0a, yd
+
+
+def test_oracle_vh_inputs_are_v0_fields(vh_case):
+    """u0's columns and yd lie in the null space of S (gradient fields, S V0 = 0)
+    to rounding, each u0 column has norm <= 1 over the kept rows (normalised
+    over all N_edge entries), and yd's Construct_Z_V0 is the V0 path's Z0."""
+    name, m, k, u0, u0a, yd = vh_case
+    S = oracle.stiffness_csr(m)
+    smax = abs(S).max()
+    _, mr = oracle.edge_map(m)
+    for c in range(2):
+        assert np.linalg.norm(S @ u0[:, c]) <= 1e-14 * smax * np.linalg.norm(u0[:, c])
+        assert np.linalg.norm(u0[:, c]) <= 1 + 1e-12 and np.linalg.norm(u0a[:, c]) <= 1 + 1e-12
+    assert np.linalg.norm(S @ yd[mr]) <= 1e-14 * smax * np.linalg.norm(yd[mr])
+    Z0, _ = oracle.v0_path(m, ports=[0])
+    Zy = oracle.extract_z_v0(m, yd, 0, np.zeros(m.num_ports ** 2, complex))
+    np.testing.assert_allclose(Zy, Z0, rtol=0, atol=1e-12 * np.abs(Z0).max())
+
+
+def test_oracle_vh_deflation_properties(vh_case):
+    """Vh' = Vh - u0 Y is D-orthogonal to u0a (u0a^T D Vh' = 0) and differs
+    from Vh only inside span(u0)."""
+    name, m, k, u0, u0a, yd = vh_case
+    Vh = oracle.find_vh(m, k)[0]
+    eps, sig = oracle.vh_solve_coeffs(m)
+    for f in (1e8, 1e10):
+        w = 2 * np.pi * f
+        d = -(w * w) * eps + 1j * w * sig
+        Vp = oracle.vh_deflate(Vh, u0, u0a, d)
+        lhs = u0a.T @ (d[:, None] * Vp)
+        assert np.abs(lhs).max() <= 1e-6 * np.abs(u0a.T @ (d[:, None] * Vh)).max()
+        Y = np.linalg.lstsq(u0, Vh - Vp, rcond=None)[0]
+        assert np.linalg.norm(Vh - Vp - u0 @ Y) <= 1e-12 * np.linalg.norm(Vh - Vp)
+
+
+def span_tol(f):
+    """Accuracy of the Vh solve in FP64 when the answer is in the span: forming
+    (S + D) Vh' cancels |S v| down to |D v|, so rounding grows like
+    |S| / (w^2 eps), about 1e-5 relative at 100 MHz on these meshes."""
+    return 5e-5 * (1e8 / f) ** 2 + 1e-8
+
+
+@pytest.mark.parametrize("f", [1e8, 1e9, 1e10])
+def test_oracle_solve_vh_exact_in_span(vh_case, f):
+    name, m, k, u0, u0a, yd = vh_case
+    Vh, ua, xs = exact_in_span_case(m, f, u0, yd)
+    X, Z = oracle.solve_vh(m, Vh, u0, ua, yd, 0, [f])
+    assert np.linalg.norm(X[0] - xs) <= span_tol(f) * np.linalg.norm(xs)
+    Zs = oracle.extract_z(m, xs, 0)
+    assert np.abs(Z[0] - Zs).max() <= span_tol(f) * np.abs(Zs).max()
+
+
+@pytest.mark.parametrize("basis", ["find_vh", "random"])
+def test_oracle_solve_vh_galerkin_residual(vh_case, basis):
+    """The residual J - (S + D) x of the Vh solution is conjugate-orthogonal
+    to the deflated basis, Vh'^H r = -Vh'^H S y_w ~ 0, within the rounding of
+    the products (|S| |x| bounds that of S x).  find_Vh's basis is closed
+    under conjugation (pairs y, conj y), where ^T and ^H give the same
+    solution; the random complex basis is not, so it tells them apart."""
+    name, m, k, u0, u0a, yd = vh_case
+    if basis == "find_vh":
+        Vh = oracle.find_vh(m, k)[0]
+    else:
+        rng = np.random.default_rng(4)
+        Vh = rng.standard_normal((m.n, 6)) + 1j * rng.standard_normal((m.n, 6))
+    S = oracle.stiffness_csr(m)
+    eps, sig = oracle.vh_solve_coeffs(m)
+    rows = oracle.vh_port_rows(m, 0)
+    for f in (1e8, 1e10):
+        X, _ = oracle.solve_vh(m, Vh, u0, u0a, yd, 0, [f], S)
+        w = 2 * np.pi * f
+        d = -(w * w) * eps + 1j * w * sig
+        Jc = np.zeros(m.n, complex)
+        Jc[rows] = -1j * w
+        r = Jc - (S @ X[0] + d * X[0])
+        Vp = oracle.vh_deflate(Vh, u0, u0a, d)
+        g = Vp.conj().T @ r
+        rnd = 64 * 2.2e-16 * np.linalg.norm(abs(S) @ np.abs(X[0]))
+        scale = np.linalg.norm(Vp, axis=0) * (1e-8 * (np.linalg.norm(Jc) + np.linalg.norm(d * X[0])) + rnd)
+        assert np.all(np.abs(g) <= scale), np.max(np.abs(g) / scale)
