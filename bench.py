@@ -229,7 +229,7 @@ def measure(workload, steps, warmup, a, per_frequency):
     ports = list(range(npt))
     kw = dict(ports=ports, precision=capi.GP_PREC_MIXED, rtol=a.rtol, batch=a.batch, restart=a.restart,
               max_iters=a.max_iters, sell_min_rows=a.sell_min_rows, smooth_edge=a.smooth_edge,
-              smooth_aux=a.smooth_aux, basis=1 if a.basis64 else 0)
+              smooth_aux=a.smooth_aux, basis=1 if a.basis64 else 0, graphs=1 if a.graphs else 0)
 
     def step(t):
         fs = freqs[[t % nf]] if per_frequency else freqs
@@ -361,6 +361,8 @@ def main():
     ap.add_argument("--sell-min-rows", type=int, default=0,
                     help="operators with at least this many rows use the sliced-ELL kernels (0: library default)")
     ap.add_argument("--basis64", action="store_true", help="FP64 Krylov basis on the mixed path (default FP32)")
+    ap.add_argument("--no-graphs", dest="graphs", action="store_false",
+                    help="launch the preconditioner directly instead of replaying captured HIP graphs")
     ap.add_argument("--serving-only", action="store_true", help=argparse.SUPPRESS)
     a = ap.parse_args()
     if a.serving_only:                   # the child process of serving_leg_guarded

@@ -1271,6 +1271,9 @@ gp_status autotune(gp_layout *lay, const gp_solve_opts &o) {
   to.max_iters = m;
   to.verbose = 0;
   to.basis = 0;
+  // trial solves launch directly: each trial sizes a new workspace, so a
+  // captured graph would be timed once and never replayed
+  to.graphs = 0;
   std::vector<int32_t> ports(1, 0);
   auto run = [&](int k, int lanes, double *us) -> gp_status {
     QueueIO io;
