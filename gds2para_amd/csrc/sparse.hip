@@ -486,14 +486,27 @@ __global__ void k_sell_fill(int64_t nr, const int64_t *__restrict__ rp, const in
 }
 
 // bad[0] += 1 for every stored column index outside [0, nc); bad[1], bad[2]:
-// smallest and largest index seen
-__global__ void k_sell_check(int64_t n, int64_t nc, const int32_t *__restrict__ ci, int *__restrict__ bad) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  const int c = ci[e];
-  if (c < 0 || (int64_t)c >= nc) atomicAdd(bad, 1);
-  atomicMin(bad + 1, c);
-  atomicMax(bad + 2, c);
+// smallest and largest index seen.  Each wavefront reduces its 64 lanes
+// (a grid-stride loop of them) first and issues one atomic of each kind.
+__global__ void __launch_bounds__(256) k_sell_check(int64_t n, int64_t nc, const int32_t *__restrict__ ci,
+                                                    int *__restrict__ bad) {
+  int cnt = 0, lo = 2147483647, hi = -2147483647 - 1;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = ci[e];
+    cnt += (c < 0 || (int64_t)c >= nc) ? 1 : 0;
+    lo = min(lo, c);
+    hi = max(hi, c);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    cnt += __shfl_xor(cnt, off, 64);
+    lo = min(lo, __shfl_xor(lo, off, 64));
+    hi = max(hi, __shfl_xor(hi, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (cnt) atomicAdd(bad, cnt);
+    atomicMin(bad + 1, lo);
+    atomicMax(bad + 2, hi);
+  }
 }
 
 __global__ void k_d2f_sell(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
@@ -536,7 +549,8 @@ gp_status sell_from_csr(Arena &A, const Csr &a, Sell *out, hipStream_t s, const 
     int *bad = nullptr, hb[3] = {0, 2147483647, -1};
     if ((st = tmp.alloc(&bad, 3)) != GP_OK) break;
     if (hipMemcpyAsync(bad, hb, sizeof(hb), hipMemcpyHostToDevice, s) != hipSuccess) { st = fail(GP_ERR_HIP, "copy"); break; }
-    if (out->nent) k_sell_check<<<blocks_for(out->nent), 256, 0, s>>>(out->nent, a.nc, out->ci, bad);
+    if (out->nent)
+      k_sell_check<<<(unsigned)std::min<int64_t>(blocks_for(out->nent), 4096), 256, 0, s>>>(out->nent, a.nc, out->ci, bad);
     if (hipMemcpyAsync(hb, bad, sizeof(hb), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) { st = fail(GP_ERR_HIP, "sell fill"); break; }
     if (hb[0]) {
