@@ -107,6 +107,28 @@ WAITING_FN = C.CFUNCTYPE(i64, C.c_void_p)
 PUSH_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(GpStreamColumn), C.POINTER(C.c_double), C.POINTER(GpColumnReport))
 
 
+class GpRequestReport(C.Structure):
+    _fields_ = [("npt", i32), ("columns", i32), ("converged", i32), ("rejected", i32), ("max_iters", i32),
+                ("max_rel_residual", f64), ("arrival_s", f64), ("done_s", f64)]
+
+
+class GpServerInfo(C.Structure):
+    _fields_ = [("resident_bytes", i64), ("resident_layouts", i32), ("evictions", i64), ("backlog", i64)]
+
+
+SB_ESTIMATE = C.CFUNCTYPE(i64, C.c_void_p, i32)
+SB_OPEN = C.CFUNCTYPE(i32, C.c_void_p, i32, C.POINTER(C.c_void_p))
+SB_BYTES = C.CFUNCTYPE(i64, C.c_void_p, C.c_void_p)
+SB_NUM_PORTS = C.CFUNCTYPE(i32, C.c_void_p, i32, C.c_void_p)
+SB_CLOSE = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p)
+SB_STREAM = C.CFUNCTYPE(i32, C.c_void_p, C.c_void_p, PULL_FN, WAITING_FN, PUSH_FN, C.c_void_p)
+
+
+class GpServerBackend(C.Structure):
+    _fields_ = [("user", C.c_void_p), ("estimate", SB_ESTIMATE), ("open", SB_OPEN), ("bytes", SB_BYTES),
+                ("num_ports", SB_NUM_PORTS), ("close", SB_CLOSE), ("stream", SB_STREAM)]
+
+
 class GpSolveStats(C.Structure):
     _fields_ = [
         ("columns", i32), ("converged", i32), ("max_iters_used", i32), ("batches", i32),
@@ -157,6 +179,12 @@ SIGNATURES = [
     ("gp_solve_workspace_bytes", i32, [C.c_void_p, i32, i32, i32, P(i64)]),
     ("gp_solve_stream", i32, [C.c_void_p, P(GpSolveOpts), i32, PULL_FN, WAITING_FN, PUSH_FN, C.c_void_p,
                               P(GpSolveStats)]),
+    ("gp_server_create", i32, [P(GpServerBackend), P(GpSolveOpts), i32, i64, P(C.c_void_p)]),
+    ("gp_server_add_mesh", i32, [C.c_void_p, P(GpMesh), P(i32)]),
+    ("gp_server_submit", i32, [C.c_void_p, i32, P(i32), i32, P(f64), i32, P(i64)]),
+    ("gp_server_wait", i32, [C.c_void_p, i64, f64, P(f64), P(GpRequestReport)]),
+    ("gp_server_info_get", i32, [C.c_void_p, P(GpServerInfo)]),
+    ("gp_server_destroy", None, [C.c_void_p]),
     ("gp_extract_z", i32, [P(GpMesh), P(f64), i32, P(f64)]),
     ("gp_solve_real", i32, [P(i64), P(i64), P(f64), i64, i64, P(f64), P(f64), f64, P(i32), P(f64)]),
     ("gp_solve_v0", i32, [C.c_void_p, f64, P(f64), P(f64), P(GpV0Stats)]),

@@ -362,6 +362,184 @@ class GpuBackend:
                    "gp_solve_stream")
 
 
+class CServer:
+    """The library's server (gp_server_*: admission, eviction and the layouts'
+    stream threads in C++), with Server's interface: submit(Request),
+    wait(timeout), close(), evictions.
+
+    backend=None serves the library's own layouts: `layouts` maps each key
+    to a GenSpec, generated here and handed to gp_server_add_mesh.
+    Otherwise `backend` follows Server's protocol (estimate, open, bytes,
+    num_ports, close, stream) and is called from the server's threads.  Each
+    submitted request is collected by a thread of its own blocked in
+    gp_server_wait, which fills Z, the reports and `remaining` when it ends."""
+
+    def __init__(self, backend, budget_bytes, slots=16, precision=capi.GP_PREC_MIXED, rtol=1e-8, restart=16,
+                 max_iters=2000, layouts=None, clock=time.monotonic):
+        self.lib = capi.load()
+        self.clock = clock
+        self.backend = backend
+        self.keys, self.names = {}, []      # request key <-> server key
+        self.handles = {}                   # handle id -> backend handle
+        self._next_handle = 1
+        self.meshes = []
+        self._collectors = []
+        self.error = None
+        opts = capi.GpSolveOpts()
+        self.lib.gp_solve_opts_default(C.byref(opts))
+        opts.rtol, opts.precision, opts.restart, opts.max_iters = rtol, precision, restart, max_iters
+        h = C.c_void_p()
+        if backend is None:
+            capi.check(self.lib.gp_server_create(None, C.byref(opts), slots, int(budget_bytes), C.byref(h)),
+                       "gp_server_create")
+            self.h = h
+            from . import mesh as M
+            for name, spec in (layouts or {}).items():
+                m = M.Mesh(spec)
+                k = C.c_int32()
+                capi.check(self.lib.gp_server_add_mesh(self.h, m.ptr, C.byref(k)), "gp_server_add_mesh")
+                self.meshes.append(m)
+                self.keys[name] = k.value
+                self.names.append(name)
+        else:
+            self._cb = self._callbacks()
+            be = capi.GpServerBackend(None, *self._cb)
+            self._be = be
+            capi.check(self.lib.gp_server_create(C.byref(be), C.byref(opts), slots, int(budget_bytes), C.byref(h)),
+                       "gp_server_create")
+            self.h = h
+
+    # ---- a Python backend behind the C callbacks ----------------------------
+    def _key(self, name):
+        if name not in self.keys:
+            if self.backend is None:
+                raise KeyError(f"unknown layout {name}")
+            self.keys[name] = len(self.names)
+            self.names.append(name)
+        return self.keys[name]
+
+    def _callbacks(self):
+        be = self.backend
+
+        def estimate(user, key):
+            return int(be.estimate(self.names[key]))
+
+        def open_(user, key, hp):
+            try:
+                obj = be.open(self.names[key])
+            except Exception as e:           # reported by wait() as the open's failure
+                self.error = f"opening {self.names[key]}: {e}"
+                return capi.GP_ERR_STATE
+            hid = self._next_handle
+            self._next_handle += 1
+            self.handles[hid] = obj
+            hp[0] = hid
+            return capi.GP_OK
+
+        def nbytes(user, hid):
+            return int(be.bytes(self.handles[hid]))
+
+        def num_ports(user, key, hid):
+            return int(be.num_ports(self.handles[hid]))
+
+        def close(user, hid):
+            be.close(self.handles.pop(hid))
+
+        def stream(user, hid, pull_fp, waiting_fp, push_fp, ctx):
+            col = capi.GpStreamColumn()
+
+            def pull(block):
+                r = pull_fp(ctx, C.byref(col), 1 if block else 0)
+                return ((col.port, col.freq, col.tag) if r == 1 else None), r
+
+            def waiting():
+                return waiting_fp(ctx)
+
+            def push(tag, z, rep):
+                c = capi.GpStreamColumn(0, 0.0, tag)
+                r = capi.GpColumnReport(int(rep.get("iters", 0)), int(rep.get("converged", 0)),
+                                        float(rep.get("rel_residual", -1.0)), 0.0, 0.0)
+                if z is None:
+                    push_fp(ctx, C.byref(c), None, C.byref(r))
+                else:
+                    a = np.ascontiguousarray(np.stack([np.real(z), np.imag(z)], -1).ravel())
+                    push_fp(ctx, C.byref(c), a.ctypes.data_as(C.POINTER(C.c_double)), C.byref(r))
+
+            try:
+                be.stream(self.handles[hid], pull, waiting, push)
+            except Exception as e:
+                self.error = f"stream: {e}"
+                return capi.GP_ERR_STATE
+            return capi.GP_OK
+
+        return (capi.SB_ESTIMATE(estimate), capi.SB_OPEN(open_), capi.SB_BYTES(nbytes),
+                capi.SB_NUM_PORTS(num_ports), capi.SB_CLOSE(close), capi.SB_STREAM(stream))
+
+    # ---- public ------------------------------------------------------------
+    def submit(self, req: Request) -> int:
+        key = self._key(req.layout)
+        ports = np.ascontiguousarray(np.asarray(req.ports, np.int32))
+        freqs = np.ascontiguousarray(np.asarray(req.freqs, np.float64))
+        rid = C.c_int64()
+        req.arrival = self.clock()
+        req.remaining = len(ports) * len(freqs)
+        capi.check(self.lib.gp_server_submit(self.h, key, ports.ctypes.data_as(C.POINTER(C.c_int32)), len(ports),
+                                             freqs.ctypes.data_as(C.POINTER(C.c_double)), len(freqs),
+                                             C.byref(rid)), "gp_server_submit")
+        req.rid = rid.value
+        t = threading.Thread(target=self._collect, args=(req,), daemon=True)
+        self._collectors.append(t)
+        t.start()
+        return req.rid
+
+    def _collect(self, req):
+        rep = capi.GpRequestReport()
+        st = self.lib.gp_server_wait(self.h, req.rid, -1.0, None, C.byref(rep))     # done; npt sizes Z
+        if st == capi.GP_OK:
+            npt = max(rep.npt, 1)
+            buf = np.zeros(2 * len(req.freqs) * len(req.ports) * npt)
+            st = self.lib.gp_server_wait(self.h, req.rid, -1.0, buf.ctypes.data_as(C.POINTER(C.c_double)),
+                                         C.byref(rep))
+        if st != capi.GP_OK:
+            req.error = self.error or self.lib.gp_last_error().decode(errors="replace")
+            self.error = self.error or req.error
+            return
+        npt = rep.npt
+        z = (buf[0::2] + 1j * buf[1::2]).reshape(len(req.freqs), len(req.ports), max(npt, 1))
+        req.Z = np.zeros((len(req.freqs), npt, npt), complex)
+        for pi, p in enumerate(req.ports):
+            if 0 <= p < npt:
+                req.Z[:, :, p] = z[:, pi, :]
+        req.reports = dict(converged=rep.converged, rejected=rep.rejected, max_iters=rep.max_iters,
+                           max_rel_residual=rep.max_rel_residual)
+        if rep.rejected:
+            req.error = "rejected column"
+        req.done = req.arrival + (rep.done_s - rep.arrival_s)
+        req.remaining = 0
+
+    def wait(self, timeout=None):
+        """Blocks until every submitted request has been collected."""
+        end = None if timeout is None else self.clock() + timeout
+        for t in list(self._collectors):
+            left = None if end is None else max(0.0, end - self.clock())
+            t.join(left)
+            if t.is_alive():
+                raise TimeoutError("requests still running")
+        if self.error:
+            raise RuntimeError(self.error)
+
+    @property
+    def evictions(self):
+        info = capi.GpServerInfo()
+        capi.check(self.lib.gp_server_info_get(self.h, C.byref(info)), "gp_server_info_get")
+        return info.evictions
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gp_server_destroy(self.h)
+            self.h = None
+
+
 def poisson_arrivals(rate_hz, n, seed=0):
     """Arrival offsets (s) of a seeded Poisson stream."""
     rng = np.random.default_rng(seed)

@@ -379,6 +379,68 @@ typedef void (*gp_push_fn)(void *user, const gp_stream_column *col, const double
 gp_status gp_solve_stream(gp_layout *lay, const gp_solve_opts *opts, int32_t slots, gp_pull_fn pull,
                           gp_waiting_fn waiting, gp_push_fn push, void *user, gp_solve_stats *stats);
 
+/* ---- request serving over resident layouts (gp_server) -------------------
+ * No counterpart in the reference.  A server keeps layouts resident while
+ * their bytes fit `budget_bytes` and runs one gp_solve_stream per resident
+ * layout on a thread of its own, so columns of new requests join running
+ * solves at the next FGMRES restart and layouts run side by side.  Admission
+ * (opening a layout, evicting idle layouts least recently used first) runs on
+ * a thread of its own; a layout larger than the budget opens only when nothing
+ * else is resident.  Requests of one layout are queued in arrival order.
+ *
+ * The backend says how a layout key becomes resident.  Null: the library's
+ * own -- key k is the k-th mesh given to gp_server_add_mesh, opened with
+ * gp_layout_create, gp_assemble_stiffness and gp_prepare_solver(opts->precision),
+ * sized by gp_layout_device_bytes + gp_solve_workspace_bytes, solved with
+ * gp_solve_stream(opts, slots).  A caller backend supplies all six callbacks
+ * (they run on the server's threads; `stream` must honour gp_solve_stream's
+ * pull / waiting / push contract and pass `ctx` to them). */
+typedef struct gp_server gp_server;
+typedef struct gp_server_backend {
+  void *user;
+  int64_t (*estimate)(void *user, int32_t key);               /* bytes before opening */
+  int32_t (*open)(void *user, int32_t key, void **handle);    /* gp_status */
+  int64_t (*bytes)(void *user, void *handle);                 /* bytes once open, incl. workspace */
+  int32_t (*num_ports)(void *user, int32_t key, void *handle);
+  void (*close)(void *user, void *handle);
+  int32_t (*stream)(void *user, void *handle, gp_pull_fn pull, gp_waiting_fn waiting, gp_push_fn push,
+                    void *ctx);                                /* gp_status */
+} gp_server_backend;
+typedef struct gp_request_report {
+  int32_t npt;                    /* ports of the layout (responses per column)    */
+  int32_t columns;                /* ports x frequencies of the request            */
+  int32_t converged;              /* columns that met rtol                         */
+  int32_t rejected;               /* columns gp_solve_stream rejected (bad port / frequency) */
+  int32_t max_iters;
+  double max_rel_residual;
+  double arrival_s, done_s;       /* seconds on the server's clock (since create)  */
+} gp_request_report;
+typedef struct gp_server_info {
+  int64_t resident_bytes;         /* resident layouts plus the one being opened    */
+  int32_t resident_layouts;
+  int64_t evictions;
+  int64_t backlog;                /* requests waiting for their layout             */
+} gp_server_info;
+/* opts: the library backend's solve options (null: gp_solve_opts_default). */
+gp_status gp_server_create(const gp_server_backend *backend, const gp_solve_opts *opts, int32_t slots,
+                           int64_t budget_bytes, gp_server **out);
+/* Library backend only; the mesh must outlive the server. */
+gp_status gp_server_add_mesh(gp_server *srv, const gp_mesh *mesh, int32_t *key);
+/* Queues a request; returns at once with its id. */
+gp_status gp_server_submit(gp_server *srv, int32_t key, const int32_t *ports, int32_t np, const double *freqs,
+                           int32_t nf, int64_t *request);
+/* Waits for one request (timeout_s < 0: no limit) and collects it: Z receives
+ * nf*np columns of the layout's npt port responses, column (f, p) at
+ * Z[2*((f*np + p)*npt + resp)] (complex interleaved; zeros for a rejected
+ * column).  With Z null the request is waited for and reported (rep->npt
+ * sizes Z) but not collected.  GP_ERR_STATE on a time-out or when a layout
+ * failed to open or solve (the message names it). */
+gp_status gp_server_wait(gp_server *srv, int64_t request, double timeout_s, double *Z, gp_request_report *rep);
+gp_status gp_server_info_get(gp_server *srv, gp_server_info *out);
+/* Stops admission, lets every resident layout finish its queued columns, and
+ * closes them. */
+void gp_server_destroy(gp_server *srv);
+
 /* Construct_Z_V0_Vh for one solution vector x (n complex, renumbered space):
  * accumulates column src of the np*np block Zk (complex interleaved). Host only. */
 gp_status gp_extract_z(const gp_mesh *mesh, const double *x, int32_t source_port, double *Zk);

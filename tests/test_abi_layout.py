@@ -19,6 +19,8 @@ PAIRS = {
     "gp_device_info": capi.GpDeviceInfo, "gp_v0_sizes": capi.GpV0Sizes, "gp_solve_opts": capi.GpSolveOpts,
     "gp_solve_stats": capi.GpSolveStats, "gp_hierarchy_info": capi.GpHierarchyInfo,
     "gp_autotune_info": capi.GpAutotuneInfo, "gp_vh_info": capi.GpVhInfo,
+    "gp_request_report": capi.GpRequestReport, "gp_server_info": capi.GpServerInfo,
+    "gp_server_backend": capi.GpServerBackend,
     "gp_column_report": capi.GpColumnReport, "gp_stream_column": capi.GpStreamColumn, "gp_v0_stats": capi.GpV0Stats,
 }
 

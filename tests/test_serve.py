@@ -14,7 +14,16 @@ import numpy as np
 import pytest
 
 from gds2para_amd import _capi as capi
-from gds2para_amd.serve import Request, Server, poisson_arrivals, run_stream
+from gds2para_amd.serve import CServer, Request, Server, poisson_arrivals, run_stream
+
+
+@pytest.fixture(params=["python", "library"])
+def make_server(request):
+    """The scheduler under test: serve.Server, or the library's gp_server
+    (admission and streams in C++) driving the same Python backend."""
+    if request.param == "python":
+        return lambda be, budget: Server(be, budget)
+    return lambda be, budget: CServer(be, budget)
 
 
 class MockBackend:
@@ -70,9 +79,9 @@ class MockBackend:
             push(tag, self.zcol(key, p, f, self.nports), dict(iters=1, converged=1, rel_residual=0.0))
 
 
-def test_serve_completes_and_orders():
+def test_serve_completes_and_orders(make_server):
     be = MockBackend({"a": 10, "b": 10})
-    srv = Server(be, budget_bytes=100)
+    srv = make_server(be, 100)
     reqs = [Request("a" if i % 2 else "b", [i % 3], [1e9, 2e9]) for i in range(8)]
     run_stream(srv, reqs, poisson_arrivals(200.0, len(reqs), seed=1))
     for r in reqs:
@@ -84,9 +93,9 @@ def test_serve_completes_and_orders():
     assert sorted(be.closed) == ["a", "b"] and not be.live
 
 
-def test_serve_budget_evicts_idle_layouts():
+def test_serve_budget_evicts_idle_layouts(make_server):
     be = MockBackend({"a": 60, "b": 60, "c": 60}, delay=0.002)
-    srv = Server(be, budget_bytes=100)       # one layout at a time
+    srv = make_server(be, 100)               # one layout at a time
     reqs = [Request(k, [0, 1], [1e8]) for k in ("a", "b", "c", "a")]
     run_stream(srv, reqs, [0.0, 0.05, 0.1, 0.15])
     assert all(r.remaining == 0 for r in reqs)
@@ -95,28 +104,29 @@ def test_serve_budget_evicts_idle_layouts():
     srv.close()
 
 
-def test_serve_backlog_admitted_when_last_layout_goes_idle():
+def test_serve_backlog_admitted_when_last_layout_goes_idle(make_server):
     """Budget for one layout: a slow request for a, then at once one for b.
     b can only be admitted once a is idle, and a goes idle inside its own
     stream's last push; admission must still run then (it once ran only on
     the stream thread, which could not evict its own layout, and nothing
     retried the backlog afterwards)."""
     be = MockBackend({"a": 60, "b": 60}, slots=1, delay=0.05)
-    srv = Server(be, budget_bytes=100)
+    srv = make_server(be, 100)
     a = Request("a", [0], [1e9])
     b = Request("b", [1], [1e9])
     srv.submit(a)
     srv.submit(b)
     try:
         srv.wait(timeout=10)
+        ev = srv.evictions
     finally:
         srv.close()
     assert a.remaining == 0 and b.remaining == 0
     assert be.opened == ["a", "b"] and be.closed[0] == "a"
-    assert srv.evictions == 1
+    assert ev == 1
 
 
-def test_serve_open_runs_off_the_lock():
+def test_serve_open_runs_off_the_lock(make_server):
     """Opening a layout (seconds on the GPU) holds neither the server lock nor
     another layout's stream: requests on a resident layout complete while a
     second layout is still being opened."""
@@ -127,7 +137,7 @@ def test_serve_open_runs_off_the_lock():
             return super().open(key)
 
     be = SlowOpen({"fast": 1, "slow": 1}, slots=2, delay=0.005)
-    srv = Server(be, budget_bytes=100)
+    srv = make_server(be, 100)
     warm = Request("fast", [0], [1e9])
     srv.submit(warm)
     srv.wait(timeout=5)
@@ -144,13 +154,13 @@ def test_serve_open_runs_off_the_lock():
     assert dt < 0.5, dt
 
 
-def test_serve_same_alone_and_in_mix_mock():
+def test_serve_same_alone_and_in_mix_mock(make_server):
     be = MockBackend({"a": 1, "b": 1}, slots=3)
-    alone = Server(be, 100)
+    alone = make_server(be, 100)
     r0 = Request("a", [0, 2], [5e8, 1e9])
     run_stream(alone, [r0], [0.0])
     alone.close()
-    mix = Server(be, 100)
+    mix = make_server(be, 100)
     reqs = [Request("b", [1], [1e9]), Request("a", [0, 2], [5e8, 1e9]), Request("a", [1], [2e9])]
     run_stream(mix, reqs, [0.0, 0.001, 0.002])
     mix.close()
