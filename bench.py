@@ -123,21 +123,27 @@ def cpu_baseline():
     return out
 
 
-def serving_leg(n_requests, slots=16):
+def serving_leg(n_requests, slots=16, python_server=False):
     """The first n_requests of the seeded serving stream (gds2para_amd.stream:
     24 Manhattan layouts, 5e4..2e7 edges log-uniform, 64 ports and a 40-point
     10 MHz..10 GHz grid each; 1-64 ports x 1-40 frequencies per request,
     log-uniform), arriving as a Poisson stream at 0.5 requests/s and served with
-    continuous batching under a 260 GB budget.  The layouts the prefix names
-    are opened (assembly, V0 basis, hierarchy) before the first arrival, which
-    is what a warm server holds; latency = completion - arrival per request."""
+    continuous batching under a 260 GB budget by the library's server
+    (gp_server_*: admission, eviction and one stream thread per layout in C++),
+    or by the Python scheduler serve.Server with python_server.  The layouts
+    the prefix names are opened (assembly, V0 basis, hierarchy) before the
+    first arrival, which is what a warm server holds; latency = completion -
+    arrival per request."""
     from gds2para_amd import stream as ST
-    from gds2para_amd.serve import GpuBackend, Request, Server, run_stream
+    from gds2para_amd.serve import CServer, GpuBackend, Request, Server, run_stream
     sp = ST.StreamSpec(n_requests=n_requests, rate_hz=0.5)
     specs = ST.layouts(sp)
     raw, offs = ST.requests(sp)
     keys = sorted({k for k, _, _ in raw})
-    srv = Server(GpuBackend({k: specs[k] for k in keys}, slots=slots), budget_bytes=260 << 30)
+    if python_server:
+        srv = Server(GpuBackend({k: specs[k] for k in keys}, slots=slots), budget_bytes=260 << 30)
+    else:
+        srv = CServer(None, 260 << 30, slots=slots, layouts={k: specs[k] for k in keys})
     t = time.perf_counter()
     for k in keys:                       # warm server: the named layouts resident
         srv.submit(Request(k, [0], [1e9]))
@@ -152,9 +158,13 @@ def serving_leg(n_requests, slots=16):
     srv.close()
     lat = [r.latency for r in reqs]
     cols = sum(len(r.ports) * len(r.freqs) for r in reqs)
-    conv = sum(1 for r in reqs for rep in r.reports.values() if rep["converged"] == 1)
+    if python_server:
+        conv = sum(1 for r in reqs for rep in r.reports.values() if rep["converged"] == 1)
+    else:
+        conv = sum(int(r.reports.get("converged", 0)) for r in reqs)
     sizes = {k: ST.edges_of(specs[k]) for k in keys}
     return {"generator": "gds2para_amd.stream (24 layouts 5e4-2e7 edges, 1-64 ports x 1-40 freqs, Poisson)",
+            "server": "serve.Server (Python)" if python_server else "gp_server (C++)",
             "requests": len(reqs), "of_stream": 1000, "columns": cols, "converged": f"{conv}/{cols}",
             "layouts": sizes, "arrival_rate_hz": sp.rate_hz, "slots": slots, "budget_GB": 260,
             "evictions": evictions, "warm_open_s": warm_s,
@@ -162,11 +172,13 @@ def serving_leg(n_requests, slots=16):
             "columns_per_s": cols / dt, "wall_s": dt}
 
 
-def serving_leg_guarded(n_requests, limit_s=240):
+def serving_leg_guarded(n_requests, limit_s=240, python_server=False):
     """serving_leg in a child process with a time limit: a stream that stalls
     is stopped and reported as such instead of holding up the benchmark line."""
     import subprocess
     cmd = [sys.executable, os.path.abspath(__file__), "--serving-only", "--serve-requests", str(n_requests)]
+    if python_server:
+        cmd.append("--serve-python")
     try:
         p = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=limit_s)
     except subprocess.TimeoutExpired:
@@ -363,10 +375,12 @@ def main():
     ap.add_argument("--basis64", action="store_true", help="FP64 Krylov basis on the mixed path (default FP32)")
     ap.add_argument("--no-graphs", dest="graphs", action="store_false",
                     help="launch the preconditioner directly instead of replaying captured HIP graphs")
+    ap.add_argument("--serve-python", action="store_true",
+                    help="serve the stream with the Python scheduler instead of the library's server")
     ap.add_argument("--serving-only", action="store_true", help=argparse.SUPPRESS)
     a = ap.parse_args()
     if a.serving_only:                   # the child process of serving_leg_guarded
-        print(json.dumps(serving_leg(a.serve_requests)), flush=True)
+        print(json.dumps(serving_leg(a.serve_requests, python_server=a.serve_python)), flush=True)
         return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported "
@@ -428,7 +442,7 @@ def main():
         line["graphs"] = graphs_leg()
     if a.serve_requests > 0:
         print(f"[bench] serving leg: {a.serve_requests} requests", file=sys.stderr, flush=True)
-        line["serving"] = serving_leg_guarded(a.serve_requests)
+        line["serving"] = serving_leg_guarded(a.serve_requests, python_server=a.serve_python)
     if not a.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline()

@@ -430,3 +430,76 @@ def test_serve_gpu_stream_generator():
             for p in r.ports:
                 a, b = r.Z[fi, :, p], Za[fi, p, :]
                 assert np.abs(a - b).max() / np.abs(b).max() < 1e-6, (r.layout, r.freqs[fi], p)
+
+
+def _check_against_alone(specs, reqs, nports=3):
+    """Each request's Z equals the same columns solved alone by Layout.solve."""
+    from gds2para_amd import mesh as M
+    from gds2para_amd.layout import Layout
+    for key, spec in specs.items():
+        lay = Layout(M.Mesh(spec))
+        for r in (q for q in reqs if q.layout == key):
+            Z, st, _ = lay.solve(r.freqs, ports=r.ports, precision=capi.GP_PREC_MIXED, rtol=1e-8, batch=4)
+            Za = Z.reshape(len(r.freqs), nports, nports)     # [f, src, resp]
+            for fi in range(len(r.freqs)):
+                for p in r.ports:
+                    a, b = r.Z[fi, :, p], Za[fi, p, :]
+                    assert np.abs(a - b).max() / np.abs(b).max() < 1e-6, (key, r.freqs[fi], p)
+        lay.close()
+
+
+@pytest.mark.gpu
+def test_cserver_gpu_same_alone_and_in_mix():
+    """The library's server with its own backend (gp_server_add_mesh layouts,
+    gp_solve_stream on the server's stream threads): a Poisson stream of
+    requests over two layouts returns, for every request, the Z of the same
+    columns solved alone."""
+    from gds2para_amd import mesh as M
+    specs = {"m11": M.manhattan_spec((24, 20, 10), n_ports=3, nfreq=3, f_range=(1e8, 1e10), seed=11),
+             "m12": M.manhattan_spec((20, 24, 10), n_ports=3, nfreq=3, f_range=(1e8, 1e10), seed=12)}
+    rng = np.random.default_rng(5)
+    reqs = []
+    for i in range(10):
+        key = "m11" if i % 3 else "m12"
+        ports = sorted(rng.choice(3, size=int(rng.integers(1, 4)), replace=False).tolist())
+        freqs = sorted(rng.choice([1e8, 1e9, 3e9, 1e10], size=int(rng.integers(1, 3)), replace=False).tolist())
+        reqs.append(Request(key, ports, freqs))
+    print("[cserver] streaming 10 requests over 2 layouts, 4 slots", flush=True)
+    srv = CServer(None, 200 << 30, slots=4, rtol=1e-8, layouts=specs)
+    run_stream(srv, reqs, poisson_arrivals(20.0, len(reqs), seed=3))
+    ev = srv.evictions
+    srv.close()
+    lats = [r.latency for r in reqs]
+    print(f"[cserver] latency p50 {np.percentile(lats, 50):.3f} s p99 {np.percentile(lats, 99):.3f} s, "
+          f"evictions {ev}", flush=True)
+    assert ev == 0
+    for r in reqs:
+        assert not r.error and r.remaining == 0, r.error
+        assert r.reports["converged"] == len(r.ports) * len(r.freqs)
+        assert r.reports["max_rel_residual"] <= 1e-8
+    _check_against_alone(specs, reqs)
+
+
+@pytest.mark.gpu
+def test_cserver_gpu_evicts_under_budget():
+    """The library's server under a budget that holds one layout at a time
+    (each estimate ~94 MB, budget 110 MB): requests alternate between three
+    layouts, so idle layouts are evicted and reopened, and every request still
+    returns the Z of its columns solved alone."""
+    from gds2para_amd import mesh as M
+    specs = {"a": M.manhattan_spec((24, 20, 10), n_ports=3, nfreq=3, f_range=(1e8, 1e10), seed=11),
+             "b": M.manhattan_spec((20, 24, 10), n_ports=3, nfreq=3, f_range=(1e8, 1e10), seed=12),
+             "c": M.manhattan_spec((28, 16, 10), n_ports=3, nfreq=3, f_range=(1e8, 1e10), seed=13)}
+    keys = ["a", "b", "c", "a", "c", "b", "a", "b"]
+    reqs = [Request(k, [i % 3], [1e9] if i % 2 else [1e8, 1e10]) for i, k in enumerate(keys)]
+    print(f"[cserver] {len(reqs)} requests over 3 layouts, budget 110 MB", flush=True)
+    srv = CServer(None, 110 << 20, slots=4, rtol=1e-8, layouts=specs)
+    run_stream(srv, reqs, poisson_arrivals(10.0, len(reqs), seed=7))
+    ev = srv.evictions
+    srv.close()
+    print(f"[cserver] evictions {ev}", flush=True)
+    assert ev > 0
+    for r in reqs:
+        assert not r.error and r.remaining == 0, r.error
+        assert r.reports["converged"] == len(r.ports) * len(r.freqs)
+    _check_against_alone(specs, reqs)
