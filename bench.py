@@ -241,7 +241,8 @@ def measure(workload, steps, warmup, a, per_frequency):
     ports = list(range(npt))
     kw = dict(ports=ports, precision=capi.GP_PREC_MIXED, rtol=a.rtol, batch=a.batch, restart=a.restart,
               max_iters=a.max_iters, sell_min_rows=a.sell_min_rows, smooth_edge=a.smooth_edge,
-              smooth_aux=a.smooth_aux, basis=1 if a.basis64 else 0, graphs=1 if a.graphs else 0)
+              smooth_aux=a.smooth_aux, basis=1 if a.basis64 else 0, graphs=1 if a.graphs else 0,
+              edge_lds=1 if a.edge_lds else 0)
 
     def step(t):
         fs = freqs[[t % nf]] if per_frequency else freqs
@@ -346,7 +347,12 @@ def measure(workload, steps, warmup, a, per_frequency):
         "speedup_vs_fp64_single": value / fp64["value"] if fp64 else None,
         "z_rel_diff_mixed_vs_fp64": zdiff,
         "spmv": spmv,
-        "edge0": {"launches": launches, "ms": kms, "bytes": kbytes},
+        "edge0": {"launches": launches, "ms": kms, "bytes": kbytes,
+                  "kernel": "k_edge_march (u staged plane by plane in LDS) for the K u launches"
+                            if a.edge_lds and hinfo["march_block_rows"] else "k_edge_sell",
+                  "march_block_rows": hinfo["march_block_rows"],
+                  "march_outside_share": (hinfo["march_outside"] / hinfo["march_entries"]
+                                          if hinfo["march_entries"] else None)},
     }
     return out
 
@@ -373,6 +379,9 @@ def main():
     ap.add_argument("--sell-min-rows", type=int, default=0,
                     help="operators with at least this many rows use the sliced-ELL kernels (0: library default)")
     ap.add_argument("--basis64", action="store_true", help="FP64 Krylov basis on the mixed path (default FP32)")
+    ap.add_argument("--edge-lds", action="store_true",
+                    help="level-0 edge smoother's K u products from u staged plane by plane in LDS (k_edge_march) "
+                         "instead of gathered through the caches")
     ap.add_argument("--no-graphs", dest="graphs", action="store_false",
                     help="launch the preconditioner directly instead of replaying captured HIP graphs")
     ap.add_argument("--serve-python", action="store_true",

@@ -72,6 +72,7 @@ class GpSolveOpts(C.Structure):
         ("rtol", f64), ("max_iters", i32), ("restart", i32), ("precision", i32),
         ("batch", i32), ("verbose", i32), ("smooth_edge", i32), ("smooth_aux", i32),
         ("sell_min_rows", i64), ("aux_post", i32), ("basis", i32), ("graphs", i32),
+        ("edge_lds", i32),
     ]
 
 
@@ -83,6 +84,7 @@ class GpHierarchyInfo(C.Structure):
         ("coarse_evictions", i64), ("workspace_bytes", i64), ("aux_coarse_blocks", i32),
         ("aux_coarse_max_block", i32), ("coarse_form_ms", f64), ("aux_coarse_eigen", i32),
         ("graph_captures", i64), ("graph_replays", i64), ("graph_failed", i32),
+        ("march_block_rows", i32), ("march_entries", i64), ("march_outside", i64),
     ]
 
 

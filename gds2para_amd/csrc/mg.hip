@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <type_traits>
 
 #include <hipcub/hipcub.hpp>
 
@@ -169,6 +170,123 @@ gp_status level0_order(gp_layout *lay, Arena &A, int32_t **order, hipStream_t s)
   return GP_OK;
 }
 
+// ---- plane march of the level-0 edge smoother: block numbering and codes ----
+constexpr int kMarchHalf = kTile / 2;  // ix columns per block
+
+__global__ void k_march_blk(int64_t n, const int32_t *__restrict__ ord, const int64_t *__restrict__ map_r, Grid g,
+                            int32_t *__restrict__ blk) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int dir; int64_t iz, ix, iy;
+  g.decode(map_r[ord[p]], dir, iz, ix, iy);
+  const int64_t nty = g.Ny / kTile + 1;
+  const int64_t tile = (ix / kTile) * nty + iy / kTile;
+  blk[p] = (int32_t)((tile * (g.Nz + 1) + iz) * 2 + (ix % kTile) / kMarchHalf);
+}
+
+// start[b] = first position of block b (n past the last): position p writes
+// the starts of the blocks (blk[p - 1], blk[p]]; a decreasing block number
+// (positions out of block order) sets *bad
+__global__ void k_march_start(int64_t n, int64_t nb, const int32_t *__restrict__ blk, int32_t *__restrict__ start,
+                              int32_t *__restrict__ bad) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > n) return;
+  const int64_t lo = p == 0 ? -1 : (int64_t)blk[p - 1];
+  const int64_t hi = p == n ? nb : (int64_t)blk[p];
+  if (hi < lo) { *bad = 1; return; }
+  for (int64_t b = lo + 1; b <= hi; ++b) start[b] = (int32_t)p;
+}
+
+__global__ void k_march_inv(int64_t n, const int32_t *__restrict__ ord, int32_t *__restrict__ inv) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) inv[ord[p]] = (int32_t)p;
+}
+
+// Codes of the stored entries of position p's row (the SELL copy's slice
+// p >> 6, lane p & 63); counts, over the rows, the stored entries (high 32
+// bits of *outside) and those read from global memory (low 32 bits)
+__global__ void k_march_code(int64_t n, int nplane, const int64_t *__restrict__ base, const int32_t *__restrict__ w,
+                             const int32_t *__restrict__ ci, const int32_t *__restrict__ inv,
+                             const int32_t *__restrict__ blk, const int32_t *__restrict__ start,
+                             int32_t *__restrict__ code, unsigned long long *__restrict__ outside) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long out = 0;
+  if (p < n) {
+    const int64_t sl = p >> 6;
+    const int lane = (int)(p & 63);
+    const int32_t b = blk[p];
+    const int32_t tb = b >> 1, z = tb % nplane;
+    const int64_t b0 = base[sl];
+    const int wd = w[sl];
+    for (int t = 0; t < wd; ++t) {
+      const int64_t e = b0 + (int64_t)t * 64 + lane;
+      const int32_t c = ci[e], cp = inv[c], bc = blk[cp];
+      const int32_t tc = bc >> 1, zc = tc % nplane;
+      if (tc - zc == tb - z && (bc & 1) == (b & 1) && zc >= z - 1 && zc <= z + 1) {
+        code[e] = ((zc - z + 1) << 16) | (cp - start[bc]);
+      } else {
+        code[e] = (int32_t)(0x80000000u | (uint32_t)c);
+        ++out;
+      }
+    }
+    out += (unsigned long long)wd << 32;
+  }
+  // one atomic per wavefront
+  for (int m = 32; m >= 1; m >>= 1) out += __shfl_xor(out, m);
+  if ((threadIdx.x & 63) == 0 && out) atomicAdd(outside, out);
+}
+
+int march_lds_setup();
+
+gp_status build_edge_march(gp_layout *lay, Arena &A, Hierarchy *H, hipStream_t s) {
+  EdgeMarch &M = H->march;
+  if (M.tried || H->L.size() < 2) return GP_OK;
+  M.tried = true;
+  const Sell &K = H->L[0].Ks;
+  const int64_t n = lay->n;
+  const Grid &g = lay->g;
+  if (!K.nr || K.nr != n || !K.rows || n >= ((int64_t)1 << 31)) return GP_OK;
+  const int64_t nty = g.Ny / kTile + 1, ntx = g.Nx / kTile + 1;
+  M.ntile = ntx * nty;
+  M.nplane = (int32_t)(g.Nz + 1);
+  M.nblk = M.ntile * M.nplane * 2;
+  if (M.nblk >= ((int64_t)1 << 30)) return GP_OK;
+  Arena tmp;
+  int32_t *blk, *inv, *bad;
+  unsigned long long *outside;
+  GP_TRY(tmp.alloc(&blk, n)); GP_TRY(tmp.alloc(&inv, n)); GP_TRY(tmp.alloc(&bad, 1));
+  GP_TRY(tmp.alloc(&outside, 1));
+  GP_TRY(A.alloc(&M.start, M.nblk + 1));
+  GP_TRY(A.alloc(&M.code, std::max<int64_t>(K.nent, 1)));
+  GP_HIP(hipMemsetAsync(bad, 0, sizeof(int32_t), s));
+  GP_HIP(hipMemsetAsync(outside, 0, sizeof(unsigned long long), s));
+  GP_HIP(hipMemsetAsync(M.code, 0, sizeof(int32_t) * std::max<int64_t>(K.nent, 1), s));
+  k_march_blk<<<blocks_for(n), 256, 0, s>>>(n, K.rows, lay->map_edge_r, g, blk);
+  k_march_start<<<blocks_for(n + 1), 256, 0, s>>>(n, M.nblk, blk, M.start, bad);
+  k_march_inv<<<blocks_for(n), 256, 0, s>>>(n, K.rows, inv);
+  k_march_code<<<blocks_for(n), 256, 0, s>>>(n, M.nplane, K.base, K.w, K.ci, inv, blk, M.start, M.code, outside);
+  int32_t bad_h = 0;
+  unsigned long long out_h = 0;
+  std::vector<int32_t> st((size_t)M.nblk + 1);
+  GP_HIP(hipMemcpyAsync(&bad_h, bad, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  GP_HIP(hipMemcpyAsync(&out_h, outside, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  GP_HIP(hipMemcpyAsync(st.data(), M.start, sizeof(int32_t) * st.size(), hipMemcpyDeviceToHost, s));
+  GP_HIP(hipStreamSynchronize(s));
+  tmp.free_all();
+  int32_t maxb = 0;
+  for (int64_t b = 0; b < M.nblk; ++b) maxb = std::max(maxb, st[b + 1] - st[b]);
+  if (bad_h || st[M.nblk] != n || maxb >= (1 << 16)) {  // not in block order: leave the march off
+    M.code = nullptr;
+    M.start = nullptr;
+    return GP_OK;
+  }
+  M.maxb = maxb;
+  M.lds_max = march_lds_setup();
+  M.outside = (int64_t)(out_h & 0xffffffffull);
+  M.entries = (int64_t)(out_h >> 32);
+  return GP_OK;
+}
+
 // Level-0 free nodes in the same tile order (keys from the node's plane and
 // (ix, iy)), for the nodal operators N, G^T and Pn of level 0.
 gp_status level0_node_order(const LevelGrid &L, Arena &A, int32_t **order, hipStream_t s) {
@@ -262,7 +380,7 @@ gp_status aux_coarse_blocks(Arena &A, AuxAMG &X, hipStream_t s) {
 // FP32 copies of every operator the mixed-precision V-cycle reads, added to
 // an existing FP64 hierarchy (so a layout solved in FP64 first and in mixed
 // precision afterwards keeps one hierarchy instead of building a second).
-gp_status hierarchy_add_f32(Arena &A, Hierarchy *H, hipStream_t s) {
+gp_status hierarchy_add_f32(gp_layout *lay, Arena &A, Hierarchy *H, hipStream_t s) {
   if (H->has_f32) return GP_OK;
   const int64_t n = H->L[0].ne;
   GP_TRY(to_f32(A, H->W, n, &H->Wf, s));
@@ -430,7 +548,7 @@ gp_status build_hierarchy(gp_layout *lay, Arena &A, Hierarchy *H, bool want_f32,
   GP_HIP(hipStreamSynchronize(s));
   GP_TRY(build_aux(lay, A, H));
   GP_TRY(aux_coarse_blocks(A, H->aux, s));
-  if (want_f32) GP_TRY(hierarchy_add_f32(A, H, s));
+  if (want_f32) GP_TRY(hierarchy_add_f32(lay, A, H, s));
   {  // dense coarsest data on the device, for the per-frequency inverses
     auto up = [&](const std::vector<double> &h, double **d) -> gp_status {
       GP_TRY(A.alloc(d, std::max<int64_t>((int64_t)h.size(), 1)));
@@ -809,6 +927,29 @@ __device__ __forceinline__ C2<T> l1diag(C2<T> a, T L) {
   return {a.x * f, a.y * f};
 }
 
+// acc -= a x for a real a, and acc -= (ar + i ai) x, with the fused
+// multiply-adds written out: the level-0 edge smoother's two kernels
+// (k_edge_sell, k_edge_march) then round alike whatever the compiler would
+// fuse in each
+template <class T>
+__device__ __forceinline__ void csub_re(C2<T> &acc, T a, const C2<T> &x) {
+  acc.x = fma(-a, x.x, acc.x);
+  acc.y = fma(-a, x.y, acc.y);
+}
+template <class T>
+__device__ __forceinline__ void csub_cx(C2<T> &acc, T ar, T ai, const C2<T> &x) {
+  acc.x -= fma(ar, x.x, -(ai * x.y));
+  acc.y -= fma(ar, x.y, ai * x.x);
+}
+// the l1-Jacobi update u + damp * acc / d_l1 of one slot (same reason)
+template <class T>
+__device__ __forceinline__ C2<T> jacobi_update(const C2<T> &acc, const C2<T> &u0, T c1, T c2, T d0, T d1, T d2, T l0,
+                                               T l1, T l2, T damp) {
+  const C2<T> dq = l1diag<T>({fma(c1, d1, d0), c2 * d2}, fma(c2, l2, fma(fabs(c1), l1, l0)));
+  const C2<T> z = cdiv(acc, dq);
+  return {fma(damp, z.x, u0.x), fma(damp, z.y, u0.y)};
+}
+
 // Products of the long rows (kLongRow), in two passes.  k_long_part: one block
 // per chunk of kLongChunk entries, threads = (column j, entry group g); the
 // groups' sums are combined in LDS in a fixed order into
@@ -1021,7 +1162,7 @@ __global__ void __launch_bounds__(256) k_edge_sell(int64_t nr, int k, SellV<T> K
     if (K.nv == 1) {
       sell_row<T, KS, 1>(K, b0, wd, lane, u, k, j0, [&](T a, T, T, const C2<T>(&x)[KS]) {
 #pragma unroll
-        for (int q = 0; q < KS; ++q) { acc[q].x -= a * x[q].x; acc[q].y -= a * x[q].y; }
+        for (int q = 0; q < KS; ++q) csub_re<T>(acc[q], a, x[q]);
       });
     } else {
       sell_row<T, KS, 3>(K, b0, wd, lane, u, k, j0, [&](T a0, T a1, T a2, const C2<T>(&x)[KS]) {
@@ -1038,22 +1179,14 @@ __global__ void __launch_bounds__(256) k_edge_sell(int64_t nr, int k, SellV<T> K
       C2<T> x[KS];
       load_slots<T, KS>(u + i * k + j0, x);
 #pragma unroll
-      for (int q = 0; q < KS; ++q) {
-        const T ar = c1[q] * de, ai = c2[q] * ds;
-        acc[q].x -= ar * x[q].x - ai * x[q].y;
-        acc[q].y -= ar * x[q].y + ai * x[q].x;
-      }
+      for (int q = 0; q < KS; ++q) csub_cx<T>(acc[q], c1[q] * de, c2[q] * ds, x[q]);
     }
   }
   if (kind != EdgeMode::JACOBI0) {
     if (use_phi) {
       sell_row<T, KS, 2>(B, B.base[sl], B.w[sl], lane, phi, k, j0, [&](T a0, T a1, T, const C2<T>(&x)[KS]) {
 #pragma unroll
-        for (int q = 0; q < KS; ++q) {
-          const T br = c1[q] * a0, bi = c2[q] * a1;
-          acc[q].x -= br * x[q].x - bi * x[q].y;
-          acc[q].y -= br * x[q].y + bi * x[q].x;
-        }
+        for (int q = 0; q < KS; ++q) csub_cx<T>(acc[q], c1[q] * a0, c2[q] * a1, x[q]);
       });
     }
   }
@@ -1061,14 +1194,251 @@ __global__ void __launch_bounds__(256) k_edge_sell(int64_t nr, int k, SellV<T> K
   const T d0 = kd0[i], d1 = kd1[i], d2 = kd2[i], l0 = kl0[i], l1 = kl1[i], l2 = kl2[i];
   C2<T> u0[KS];
   if (kind == EdgeMode::JACOBI) load_slots<T, KS>(u + i * k + j0, u0);
+  else
 #pragma unroll
-  for (int q = 0; q < KS; ++q) {
-    const C2<T> dq = l1diag<T>({d0 + c1[q] * d1, c2[q] * d2}, l0 + fabs(c1[q]) * l1 + c2[q] * l2);
-    const C2<T> z = cdiv(acc[q], dq);
-    if (kind == EdgeMode::JACOBI0) acc[q] = {damp * z.x, damp * z.y};
-    else acc[q] = {u0[q].x + damp * z.x, u0[q].y + damp * z.y};
-  }
+    for (int q = 0; q < KS; ++q) u0[q] = {(T)0, (T)0};
+#pragma unroll
+  for (int q = 0; q < KS; ++q) acc[q] = jacobi_update<T>(acc[q], u0[q], c1[q], c2[q], d0, d1, d2, l0, l1, l2, damp);
   store_slots<T, KS>(out + i * k + j0, acc);
+}
+
+// ---- level-0 edge smoother / residual as a plane march (FP32) --------------
+// k_edge_sell with the K u gathers served from LDS.  A workgroup takes one
+// (tile half, run of planes, group of SW slots) and walks its planes upwards
+// holding u of planes z - 1, z, z + 1 (the rows of blocks (tile, plane,
+// half), SW slots each) in a ring of three LDS plane buffers.  Each row's
+// couplings lie in those three blocks except at the tile half's sides
+// (code[e] < 0: read from global memory as in k_edge_sell).  The next plane's
+// u is loaded into registers while the current plane is computed and written
+// to LDS after it, and each thread computes its rows of a plane together, so
+// the dependent loads of both rows are in flight at once.  Entries are summed
+// in k_edge_sell's order with the same operations: both kernels give the same
+// bits.
+constexpr int kMarchThreads = 768;
+constexpr int kMarchPlanes = 11;                       // planes per run (a run stages one more on each side)
+constexpr int kMarchMaxRows = kMarchHalf * kTile * 3;  // rows of one block: 8 ix x 16 iy x 3 directions
+
+// u of slots [jl, jl + 4) of LDS row off in ring buffer slot; swap reads the
+// two 16-byte halves in the other order (rows of alternate lane quads then
+// touch disjoint LDS banks in each read)
+template <int SW>
+__device__ __forceinline__ void march_lds_row(const float4 *__restrict__ lds, int slot, int maxb, int off, int jl,
+                                              bool swp, C2<float> (&x)[4]) {
+  const float4 *q = lds + ((int64_t)slot * maxb + off) * (SW / 2) + (jl >> 1);
+  float4 v0, v1;
+  if (swp) { v1 = q[1]; v0 = q[0]; } else { v0 = q[0]; v1 = q[1]; }
+  x[0] = {v0.x, v0.y}; x[1] = {v0.z, v0.w}; x[2] = {v1.x, v1.y}; x[3] = {v1.z, v1.w};
+}
+
+template <int LPR>
+__global__ void __launch_bounds__(kMarchThreads)
+    k_edge_march(int k, int nplane, int zchunk, int nzc, int ngroup, int maxb, const int32_t *__restrict__ start,
+                 const int32_t *__restrict__ code, SellV<float> K, SellV<float> B, const float *__restrict__ c1v,
+                 const float *__restrict__ c2v, const float *__restrict__ kd0, const float *__restrict__ kd1,
+                 const float *__restrict__ kd2, const float *__restrict__ kl0, const float *__restrict__ kl1,
+                 const float *__restrict__ kl2, const C2<float> *__restrict__ r, const C2<float> *__restrict__ u,
+                 const C2<float> *__restrict__ phi, C2<float> *__restrict__ out, int mode, int use_phi, float damp) {
+  constexpr int SPL = 4, SW = LPR * SPL, PR = SW / 2, MT = kMarchThreads, RPP = MT / LPR;
+  constexpr int NR = (kMarchMaxRows + RPP - 1) / RPP;       // rows per thread and plane
+  constexpr int NQ = (kMarchMaxRows * PR + MT - 1) / MT;   // 16-byte pieces per thread and plane
+  extern __shared__ float4 lds[];                          // [3][maxb][PR]
+  const int64_t wg = xcd_block_id();
+  const int g = (int)(wg % ngroup);
+  const int64_t rest = wg / ngroup;
+  const int zc = (int)(rest % nzc);
+  const int64_t th = rest / nzc;
+  const int64_t tile = th >> 1;
+  const int half = (int)(th & 1);
+  const int z0 = zc * zchunk, z1 = min(z0 + zchunk, nplane);
+  if (z0 >= nplane) return;  // the whole workgroup leaves together
+  const int tid = (int)threadIdx.x;
+  const int sub = tid % LPR, jl = sub * SPL, j0 = g * SW + jl;
+  const bool swp = LPR == 4 && ((tid >> 2) & 1);
+  float c1[SPL], c2[SPL];
+#pragma unroll
+  for (int q = 0; q < SPL; ++q) { c1[q] = c1v[j0 + q]; c2[q] = c2v[j0 + q]; }
+  const int kind = mode & EdgeMode::KIND;
+  const C2<float> *usrc = u + g * SW;
+
+  // u of block (tile, p, half): fetch into registers, put into ring slot p % 3
+  auto fetch = [&](int p, float4 (&v)[NQ]) {
+    const int64_t b = (tile * nplane + p) * 2 + half;
+    const int st = start[b], tot = (start[b + 1] - st) * PR;
+#pragma unroll
+    for (int a = 0; a < NQ; ++a) {
+      const int q = a * MT + tid;
+      v[a] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (q < tot) {
+        const int row = q / PR, part = q - row * PR;
+        v[a] = reinterpret_cast<const float4 *>(usrc + (int64_t)K.rows[st + row] * k)[part];
+      }
+    }
+  };
+  auto put = [&](int p, const float4 (&v)[NQ]) {
+    const int64_t b = (tile * nplane + p) * 2 + half;
+    const int tot = (start[b + 1] - start[b]) * PR;
+    float4 *dst = lds + (int64_t)(p % 3) * maxb * PR;
+#pragma unroll
+    for (int a = 0; a < NQ; ++a) {
+      const int q = a * MT + tid;
+      if (q < tot) dst[q] = v[a];
+    }
+  };
+
+  float4 pf[NQ];
+  {
+    float4 pb[NQ];
+    if (z0 > 0) fetch(z0 - 1, pb);
+    fetch(z0, pf);
+    if (z0 > 0) put(z0 - 1, pb);
+    put(z0, pf);
+  }
+  if (z0 + 1 < nplane) fetch(z0 + 1, pf);
+  for (int z = z0; z < z1; ++z) {
+    if (z + 1 < nplane) put(z + 1, pf);  // ring slot of plane z - 2, done with since the last barrier
+    __syncthreads();
+    if (z + 2 < nplane && z + 2 <= z1) fetch(z + 2, pf);  // in flight while plane z is computed
+    const int64_t b = (tile * nplane + z) * 2 + half;
+    const int st = start[b], m = start[b + 1] - st;
+    const int zm1 = z - 1;
+
+    int rr[NR], wd[NR], wb[NR], lane[NR];
+    int64_t i[NR], b0[NR], bb[NR];
+    bool ok[NR];
+    C2<float> acc[NR][SPL];
+    int wmax = 0, wbmax = 0;
+#pragma unroll
+    for (int n = 0; n < NR; ++n) {
+      rr[n] = tid / LPR + n * RPP;
+      ok[n] = rr[n] < m;
+      wd[n] = 0; wb[n] = 0; i[n] = 0; b0[n] = 0; bb[n] = 0; lane[n] = 0;
+      if (ok[n]) {
+        const int64_t pos = st + rr[n];
+        const int64_t sl = pos >> 6;
+        lane[n] = (int)(pos & 63);
+        i[n] = K.rows[pos];
+        b0[n] = K.base[sl];
+        wd[n] = K.w[sl];
+        if (use_phi) { bb[n] = B.base[sl]; wb[n] = B.w[sl]; }
+        load_slots<float, SPL>(r + i[n] * k + j0, acc[n]);
+      }
+      wmax = max(wmax, wd[n]);
+      wbmax = max(wbmax, wb[n]);
+    }
+    // K u: two entries of each row in flight per round, each row in entry order
+    for (int t0 = 0; t0 < wmax; t0 += 2) {
+      int32_t c[NR][2];
+      float a[NR][2];
+#pragma unroll
+      for (int n = 0; n < NR; ++n)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          c[n][h] = 0; a[n][h] = 0.f;
+          if (t0 + h < wd[n]) {
+            const int64_t e = b0[n] + (int64_t)(t0 + h) * 64 + lane[n];
+            c[n][h] = __builtin_nontemporal_load(code + e);
+            a[n][h] = __builtin_nontemporal_load(K.v[0] + e);
+          }
+        }
+      C2<float> xv[NR][2][SPL];
+#pragma unroll
+      for (int n = 0; n < NR; ++n)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          if (t0 + h < wd[n]) {
+            if (c[n][h] < 0) load_slots<float, SPL>(u + (int64_t)(c[n][h] & 0x7fffffff) * k + j0, xv[n][h]);
+            else march_lds_row<SW>(lds, (zm1 + (c[n][h] >> 16)) % 3, maxb, c[n][h] & 0xffff, jl, swp, xv[n][h]);
+          }
+        }
+#pragma unroll
+      for (int n = 0; n < NR; ++n)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          if (t0 + h < wd[n])
+#pragma unroll
+            for (int q = 0; q < SPL; ++q) csub_re<float>(acc[n][q], a[n][h], xv[n][h][q]);
+    }
+    // diagonal K_eps, K_sig with the row's own u (plane z's block, offset rr)
+    C2<float> self[NR][SPL];
+#pragma unroll
+    for (int n = 0; n < NR; ++n)
+      if (ok[n]) {
+        march_lds_row<SW>(lds, z % 3, maxb, rr[n], jl, swp, self[n]);
+        if (K.dg[0]) {
+          const float de = K.dg[0][i[n]], ds = K.dg[1][i[n]];
+#pragma unroll
+          for (int q = 0; q < SPL; ++q) csub_cx<float>(acc[n][q], c1[q] * de, c2[q] * ds, self[n][q]);
+        }
+      }
+    // B phi, each row in entry order
+    for (int t = 0; t < wbmax; ++t) {
+      float a0[NR], a1[NR];
+      int32_t cb[NR];
+#pragma unroll
+      for (int n = 0; n < NR; ++n) {
+        a0[n] = 0.f; a1[n] = 0.f; cb[n] = 0;
+        if (t < wb[n]) {
+          const int64_t e = bb[n] + (int64_t)t * 64 + lane[n];
+          cb[n] = __builtin_nontemporal_load(B.ci + e);
+          a0[n] = __builtin_nontemporal_load(B.v[0] + e);
+          a1[n] = __builtin_nontemporal_load(B.v[1] + e);
+        }
+      }
+      C2<float> xb[NR][SPL];
+#pragma unroll
+      for (int n = 0; n < NR; ++n)
+        if (t < wb[n]) load_slots<float, SPL>(phi + (int64_t)cb[n] * k + j0, xb[n]);
+#pragma unroll
+      for (int n = 0; n < NR; ++n)
+        if (t < wb[n])
+#pragma unroll
+          for (int q = 0; q < SPL; ++q) csub_cx<float>(acc[n][q], c1[q] * a0[n], c2[q] * a1[n], xb[n][q]);
+    }
+#pragma unroll
+    for (int n = 0; n < NR; ++n) {
+      if (!ok[n]) continue;
+      if (kind == EdgeMode::JACOBI) {
+        const float d0 = kd0[i[n]], d1 = kd1[i[n]], d2 = kd2[i[n]], l0 = kl0[i[n]], l1 = kl1[i[n]], l2 = kl2[i[n]];
+#pragma unroll
+        for (int q = 0; q < SPL; ++q)
+          acc[n][q] = jacobi_update<float>(acc[n][q], self[n][q], c1[q], c2[q], d0, d1, d2, l0, l1, l2, damp);
+      }
+      store_slots<float, SPL>(out + i[n] * k + j0, acc[n]);
+    }
+    __syncthreads();
+  }
+}
+
+// The march for one level-0 launch that gathers K u (RES or JACOBI without
+// NOK) with FP32 data and k a multiple of 8; false when it does not apply or
+// the device cannot give the workgroup its LDS (the caller then launches
+// k_edge_sell).
+bool edge_march(const Hierarchy &H, int k, const SellV<float> &K, const SellV<float> &B, const float *c1,
+                const float *c2, const float *kd0, const float *kd1, const float *kd2, const float *kl0,
+                const float *kl1, const float *kl2, const C2<float> *r, const C2<float> *u, const C2<float> *phi,
+                C2<float> *out, int mode, int use_phi, float damp, hipStream_t s) {
+  const EdgeMarch &M = H.march;
+  const int kind = mode & EdgeMode::KIND;
+  if (!M.code || !u || kind == EdgeMode::JACOBI0 || (mode & EdgeMode::NOK) || k % 8 || K.nv != 1 || !K.rows) return false;
+  const int lds_max = M.lds_max;
+  const int sw = (k % 16 == 0 && (int64_t)3 * M.maxb * 16 * 8 <= lds_max) ? 16 : 8;
+  const int64_t shm = (int64_t)3 * M.maxb * sw * 8;
+  if (shm > lds_max || M.maxb == 0 || M.maxb > kMarchMaxRows) return false;
+  const int ngroup = k / sw;
+  const int nzc = (M.nplane + kMarchPlanes - 1) / kMarchPlanes;
+  const int zchunk = (M.nplane + nzc - 1) / nzc;
+  const int64_t nwg = M.ntile * 2 * nzc * ngroup;
+  if (nwg >= ((int64_t)1 << 31)) return false;
+  if (sw == 16) {
+    k_edge_march<4><<<(unsigned)nwg, kMarchThreads, (size_t)shm, s>>>(
+        k, M.nplane, zchunk, nzc, ngroup, M.maxb, M.start, M.code, K, B, c1, c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u,
+        phi, out, mode, use_phi, damp);
+  } else {
+    k_edge_march<2><<<(unsigned)nwg, kMarchThreads, (size_t)shm, s>>>(
+        k, M.nplane, zchunk, nzc, ngroup, M.maxb, M.start, M.code, K, B, c1, c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u,
+        phi, out, mode, use_phi, damp);
+  }
+  return true;
 }
 
 // SELL nodal l1-Jacobi (contract of k_node); long rows (lidx >= 0) take their
@@ -1654,6 +2024,22 @@ void long_pass(const MatV<T> &M, int k, const C2<T> *x, C2<T> *part, C2<T> *lp, 
 
 }  // namespace
 
+// LDS a march workgroup may take on the current device (0: none), with the
+// kernels' dynamic-LDS limit raised to it; called while the hierarchy is
+// built, outside any stream capture
+int march_lds_setup() {
+  int d = 0, v = 0;
+  if (hipGetDevice(&d) != hipSuccess || hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, d) != hipSuccess)
+    return 0;
+  if (hipFuncSetAttribute((const void *)k_edge_march<4>, hipFuncAttributeMaxDynamicSharedMemorySize, v) != hipSuccess ||
+      hipFuncSetAttribute((const void *)k_edge_march<2>, hipFuncAttributeMaxDynamicSharedMemorySize, v) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return v;
+}
+
+
 template <class T>
 static gp_status vcycle_level(const Hierarchy &H, VWork<T> &w, hipStream_t s, int l, int nu_pre, int nu_post) {
   const MgLevel &L = H.L[l];
@@ -1698,6 +2084,17 @@ static gp_status vcycle_level(const Hierarchy &H, VWork<T> &w, hipStream_t s, in
   };
   auto tb0 = [&]() { if (kt) kt->begin(s); };
   auto te0 = [&](int mode, int use_phi) { if (kt) kt->end(s, edge_bytes(mode, use_phi)); };
+  // level-0 launches that gather K u go through the plane march when it applies
+  auto edge = [&](const C2<T> *rr, const C2<T> *uu, const C2<T> *pp, C2<T> *oo, int mode, int use_phi, T dmp,
+                  hipStream_t st) {
+    if constexpr (std::is_same<T, float>::value) {
+      if (l == 0 && w.edge_march &&
+          edge_march(H, k, KS, BS, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, rr, uu, pp, oo, mode, use_phi, dmp, st))
+        return;
+    }
+    edge_sell<T>(ne, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, rr, uu, pp, oo, mode,
+                 use_phi, dmp, st);
+  };
   // u and tmp trade places after every Jacobi sweep but the first; start from
   // the buffer that makes the last sweep write the level's own u (no copy)
   if (((nu_pre - 1) + nu_post) & 1) std::swap(u, tmp);
@@ -1708,21 +2105,18 @@ static gp_status vcycle_level(const Hierarchy &H, VWork<T> &w, hipStream_t s, in
     w.u0_ready = false;
   } else {
     tb0();
-    edge_sell<T>(ne, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, nullptr, nullptr, u,
-                 EdgeMode::JACOBI0, 0, damp, s);
+    edge(r, nullptr, nullptr, u, EdgeMode::JACOBI0, 0, damp, s);
     te0(EdgeMode::JACOBI0, 0);
   }
   for (int it = 1; it < nu_pre; ++it) {
     tb0();
-    edge_sell<T>(ne, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, nullptr, tmp,
-                                   EdgeMode::JACOBI, 0, damp, s);
+    edge(r, u, nullptr, tmp, EdgeMode::JACOBI, 0, damp, s);
     te0(EdgeMode::JACOBI, 0);
     std::swap(u, tmp);
   }
   // pre-smoothing on the nodal auxiliary space
   tb0();
-  edge_sell<T>(ne, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, nullptr, res,
-                               EdgeMode::RES, 0, damp, s);
+  edge(r, u, nullptr, res, EdgeMode::RES, 0, damp, s);
   te0(EdgeMode::RES, 0);
   spmv_real_sell<T>(nn, w.sell_min_rows, k, GTS, GT, res, rn, 0, nullptr, w.lp, s);
   node_sell<T>(nn, w.sell_min_rows, k, NS, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, nullptr, phi, damp, nullptr, w.lp, s);
@@ -1734,8 +2128,7 @@ static gp_status vcycle_level(const Hierarchy &H, VWork<T> &w, hipStream_t s, in
   // B phi is subtracted (no second K gather); the result goes to tmp, which
   // is free until the post-smoothing
   tb0();
-  edge_sell<T>(ne, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, res, u, phi, tmp,
-                               EdgeMode::RES | EdgeMode::NOK, 1, damp, s);
+  edge(res, u, phi, tmp, EdgeMode::RES | EdgeMode::NOK, 1, damp, s);
   te0(EdgeMode::RES | EdgeMode::NOK, 1);
   const int64_t nec = H.L[l + 1].ne, nnc = H.L[l + 1].nn;
   spmv_real_sell<T>(nec, w.sell_min_rows, k, PTS, PT, tmp, w.r[l + 1], 0, nullptr, w.lp, s);
@@ -1745,8 +2138,7 @@ static gp_status vcycle_level(const Hierarchy &H, VWork<T> &w, hipStream_t s, in
   (void)nnc;
   // post-smoothing: nodal, then edges
   tb0();
-  edge_sell<T>(ne, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, phi, res,
-                               EdgeMode::RES, 1, damp, s);
+  edge(r, u, phi, res, EdgeMode::RES, 1, damp, s);
   te0(EdgeMode::RES, 1);
   spmv_real_sell<T>(nn, w.sell_min_rows, k, GTS, GT, res, rn, 0, nullptr, w.lp, s);
   node_sell<T>(nn, w.sell_min_rows, k, NS, N, w.c1, w.c2, nd0, nd1, nl0, nl1, rn, nullptr, d1, damp, nullptr, w.lp, s);
@@ -1760,8 +2152,7 @@ static gp_status vcycle_level(const Hierarchy &H, VWork<T> &w, hipStream_t s, in
     // B d1 (the nodal correction just added to phi) is subtracted
     const int md = it == 0 ? (EdgeMode::JACOBI | EdgeMode::NOK) : EdgeMode::JACOBI;
     tb0();
-    edge_sell<T>(ne, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, kd0, kd1, kd2, kl0, kl1, kl2, it == 0 ? res : r, u,
-                 it == 0 ? d1 : phi, tmp, md, 1, damp, s);
+    edge(it == 0 ? res : r, u, it == 0 ? d1 : phi, tmp, md, 1, damp, s);
     te0(md, 1);
     std::swap(u, tmp);
   }
@@ -1868,8 +2259,13 @@ gp_status precond(const Hierarchy &H, VWork<T> &w, hipStream_t s, int nu_pre, in
   const T *kd0 = arr<T>(L0.kd[0], L0.kdf[0]);
   (void)kd0;
   const SellV<T> KS = sview<T>(L0.Ks, L0.Ksf, L0.K, L0.Kf), BS = sview<T>(L0.Bs, L0.Bsf, L0.B, L0.Bf);
-  edge_sell<T>(n, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, w.r0, w.u[0], w.phi[0],
-               w.res[0], EdgeMode::RES, 1, (T)1, s);
+  bool marched = false;
+  if constexpr (std::is_same<T, float>::value)
+    marched = w.edge_march && edge_march(H, k, KS, BS, w.c1, w.c2, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                         w.r0, w.u[0], w.phi[0], w.res[0], EdgeMode::RES, 1, 1.0f, s);
+  if (!marched)
+    edge_sell<T>(n, w.sell_min_rows, k, KS, BS, K, B, w.c1, w.c2, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                 w.r0, w.u[0], w.phi[0], w.res[0], EdgeMode::RES, 1, (T)1, s);
   spmv2_sell<T>(n, w.sell_min_rows, k, BxS, Bx, w.c1, w.c2, nullptr, w.a, w.res[0], 1, nullptr, w.lp, s);
   long_pass<T>(GxT, k, w.res[0], w.lpart, w.lp, s);
   spmv_real_sell<T>(na, w.sell_min_rows, k, GxTS, GxT, w.res[0], w.ar[0], 0, X.GxT.lidx, w.lp, s);

@@ -150,8 +150,29 @@ struct AuxAMG {
   double *V = nullptr, *VT = nullptr, *lam = nullptr;
 };
 
+// Plane march of the level-0 edge smoother (k_edge_march).  The level-0 slot
+// positions fall into blocks (16 x 16-cell tile, plane, half of the tile's ix
+// columns), numbered b = (tile * nplane + plane) * 2 + half in position order,
+// each a contiguous position range [start[b], start[b + 1]).  A workgroup
+// walks one tile half up through the planes holding u of planes z - 1, z,
+// z + 1 in LDS; code[e] tells, for each stored entry e of the level-0 K SELL
+// copy, where its column sits: (plane - z + 1) << 16 | offset in that plane's
+// block, or INT32_MIN | column for a column outside those three blocks (read
+// from global memory).
+struct EdgeMarch {
+  int32_t *code = nullptr;
+  int32_t *start = nullptr;      // [nblk + 1]
+  int64_t ntile = 0, nblk = 0;
+  int32_t nplane = 0, maxb = 0;  // planes; most rows in one block
+  int32_t lds_max = 0;           // LDS one workgroup may take (0: the march is off)
+  int64_t outside = 0;           // entries of real rows whose column is read from global memory
+  int64_t entries = 0;           // entries of real rows
+  bool tried = false;            // built (or found not to apply) once
+};
+
 struct Hierarchy {
   std::vector<MgLevel> L;
+  EdgeMarch march;
   AuxAMG aux;
   // S G and S Gx of the rounded FP64 stiffness, formed exactly (double-double):
   // A (u + G phi + Gx a) = S u + SG phi + SGx a + D (u + G phi + Gx a)
@@ -243,7 +264,11 @@ template <class T>
 gp_status aux_eig_apply(const AuxAMG &X, int k, const double *om, const C2<T> *r, C2<T> *x, C2<double> *yp,
                         hipStream_t s);
 // FP32 copies for the mixed-precision V-cycle, added to an FP64 hierarchy.
-gp_status hierarchy_add_f32(Arena &A, Hierarchy *H, hipStream_t s);
+gp_status hierarchy_add_f32(gp_layout *lay, Arena &A, Hierarchy *H, hipStream_t s);
+// Block starts and entry codes of the level-0 plane march (EdgeMarch), built on
+// the first solve that asks for it; leaves H->march.code null where it does not
+// apply (rows out of block order, no tile order, too many rows).
+gp_status build_edge_march(gp_layout *lay, Arena &A, Hierarchy *H, hipStream_t s);
 gp_status aux_add_f32(Arena &A, Hierarchy *H, hipStream_t s);
 gp_status aux_smoother_data(Arena &A, const Csr &M, double **d, double **l, hipStream_t s);
 gp_status d2f_copy(const double *src, float *dst, int64_t n, hipStream_t s);
@@ -269,6 +294,7 @@ struct VWork {
   bool aux_post = false;  // auxiliary correction after the edge V-cycle too (symmetric)
   bool u0_ready = false;  // level 0's first edge sweep already written (set by precond)
   int64_t sell_min_rows = 1;  // operators with fewer rows use the (row, slot) CSR kernels
+  bool edge_march = false;    // level-0 K u gathers through the LDS plane march (FP32, k % 8 == 0)
   KTimer *kt = nullptr;               // times the level-0 edge smoother launches
   Arena arena;
 };

@@ -693,6 +693,7 @@ uint64_t graph_key(const gp_solve_opts &o, const void *tab, int cap, int lanes) 
   auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
   mix((uint64_t)o.smooth_edge); mix((uint64_t)o.smooth_aux); mix((uint64_t)o.aux_post);
   mix((uint64_t)o.sell_min_rows); mix((uint64_t)(uintptr_t)tab); mix((uint64_t)cap); mix((uint64_t)lanes);
+  mix((uint64_t)o.edge_lds);
   return h;
 }
 
@@ -781,6 +782,8 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
   const OuterOp op = outer_op(lay, H);
   const Csr &G0 = H.L[0].G, &Gx = H.aux.Gx;
 
+  if (std::is_same<T, float>::value && o.edge_lds)
+    GP_TRY(build_edge_march(lay, cx.sv->arena, &cx.sv->H, s));
   SlotWork<T, TV> *W = nullptr;
   GP_TRY((slot_work<T, TV>(cx, k, m, io.want_x(), &W)));
   VWork<T> &vw = W->vw;
@@ -795,6 +798,7 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
   vw.aux_post = o.aux_post > 0;
   vw.kt = &lay->ktimer;
   vw.sell_min_rows = o.sell_min_rows > 0 ? o.sell_min_rows : 1;
+  vw.edge_march = o.edge_lds != 0;
   Z2 *b = W->b, *xu = W->xu, *xphi = W->xphi, *xa = W->xa, *w = W->w, *yd = W->yd;
   C2<TV> *V = W->V, *wa = W->wa;
   C2<T> *Zu = W->Zu, *Zp = W->Zp, *Za = W->Za;
@@ -1360,6 +1364,7 @@ extern "C" void gp_solve_opts_default(gp_solve_opts *o) {
   o->aux_post = 0;
   o->basis = 0;
   o->graphs = 1;
+  o->edge_lds = 0;
 }
 
 extern "C" gp_status gp_prepare_solver(gp_layout *lay, int32_t precision) {
@@ -1369,7 +1374,7 @@ extern "C" gp_status gp_prepare_solver(gp_layout *lay, int32_t precision) {
   if (lay->solver) {
     // an FP64 hierarchy gains the FP32 copies in place
     Solver *sv = lay->solver;
-    GP_TRY(hierarchy_add_f32(sv->arena, &sv->H, lay->stream));
+    GP_TRY(hierarchy_add_f32(lay, sv->arena, &sv->H, lay->stream));
     sv->f32 = true;
     return GP_OK;
   }
@@ -1468,6 +1473,9 @@ extern "C" gp_status gp_hierarchy_info_get(gp_layout *lay, gp_hierarchy_info *ou
   out->graph_captures = lay->solver->graph_captures;
   out->graph_replays = lay->solver->graph_replays;
   out->graph_failed = lay->solver->graph_failed ? 1 : 0;
+  out->march_block_rows = H.march.code ? H.march.maxb : 0;
+  out->march_entries = H.march.entries;
+  out->march_outside = H.march.outside;
   return GP_OK;
 }
 

@@ -111,7 +111,9 @@ class Layout:
                     coarse_evictions=h.coarse_evictions, workspace_bytes=h.workspace_bytes,
                     aux_coarse_blocks=h.aux_coarse_blocks, aux_coarse_max_block=h.aux_coarse_max_block,
                     coarse_form_ms=h.coarse_form_ms, aux_coarse_eigen=h.aux_coarse_eigen,
-                    graph_captures=h.graph_captures, graph_replays=h.graph_replays, graph_failed=h.graph_failed)
+                    graph_captures=h.graph_captures, graph_replays=h.graph_replays, graph_failed=h.graph_failed,
+                    march_block_rows=h.march_block_rows, march_entries=h.march_entries,
+                    march_outside=h.march_outside)
 
     def autotune(self, restart=16):
         """gp_autotune: the mixed path's block width and lanes-per-row cap."""
@@ -192,7 +194,7 @@ class Layout:
 
     def solve(self, freqs, ports=None, precision=capi.GP_PREC_MIXED, rtol=1e-8, batch=0, restart=16,
               max_iters=2000, want_x=False, verbose=0, smooth_edge=0, smooth_aux=0, sell_min_rows=0,
-              aux_post=0, basis=0, graphs=1):
+              aux_post=0, basis=0, graphs=1, edge_lds=0):
         """Returns (Z[nf, np*np] complex, stats, X or None)."""
         ports = list(range(self.mesh.num_ports)) if ports is None else list(ports)
         freqs = np.ascontiguousarray(np.asarray(freqs, np.float64))
@@ -202,7 +204,7 @@ class Layout:
         self.lib.gp_solve_opts_default(C.byref(o))
         o.rtol, o.precision, o.batch, o.restart, o.max_iters, o.verbose = rtol, precision, batch, restart, max_iters, verbose
         o.smooth_edge, o.smooth_aux, o.sell_min_rows = smooth_edge, smooth_aux, sell_min_rows
-        o.aux_post, o.basis, o.graphs = aux_post, basis, graphs
+        o.aux_post, o.basis, o.graphs, o.edge_lds = aux_post, basis, graphs, edge_lds
         Z = np.zeros(2 * nf * npt * npt)
         X = np.zeros(2 * self.mesh.n * nf * npp) if want_x else None
         st = capi.GpSolveStats()

@@ -239,6 +239,14 @@ typedef struct gp_solve_opts {
                                      Krylov step is captured once in a HIP graph
                                      and replayed; 0: launched directly.  Off
                                      while gp_kernel_timer records */
+  int32_t edge_lds;               /* 1: with MIXED and a slot count that is a
+                                     multiple of 8, the level-0 edge smoother's
+                                     K u products read u from planes staged in
+                                     LDS (same bits; its entry codes are built on
+                                     the first such solve, 4 bytes per stored
+                                     entry of level-0 K); 0 (default): u is
+                                     gathered through the caches, faster at 16
+                                     and 32 slots on MI355X */
 } gp_solve_opts;
 
 typedef struct gp_solve_stats {
@@ -279,6 +287,12 @@ typedef struct {
   int64_t graph_captures;         /* preconditioner graphs captured so far         */
   int64_t graph_replays;          /* preconditioner applications replayed from them */
   int32_t graph_failed;           /* 1: a capture failed and the solves launch directly */
+  int32_t march_block_rows;       /* level-0 plane march (edge_lds): most rows in one
+                                     (tile, plane, half) block; 0 = not built (no
+                                     solve with edge_lds yet) or not applicable */
+  int64_t march_entries;          /* stored level-0 K entries of real rows          */
+  int64_t march_outside;          /* of those, read from global memory (outside the
+                                     three staged planes of the row's tile half)   */
 } gp_hierarchy_info;
 gp_status gp_hierarchy_info_get(gp_layout *lay, gp_hierarchy_info *out);
 

@@ -166,3 +166,50 @@ def test_graphs_replay_same_bits(manh):
     assert sa["status"] == sb["status"] == sc["status"] == capi.GP_OK
     assert np.array_equal(Za, Zb) and np.array_equal(Zb, Zc)
     assert sa["iters"] == sb["iters"] == sc["iters"]
+
+
+@pytest.mark.parametrize("batch,graphs,sweeps", [(8, 0, 0), (16, 0, 0), (32, 0, 0), (16, 1, 0), (16, 0, 2)])
+def test_edge_march_same_bits(manh, batch, graphs, sweeps):
+    """The level-0 edge smoother with u staged plane by plane in LDS
+    (edge_lds=1) gives the same bits as the kernel that gathers u through the
+    caches (edge_lds=0): it sums each row's entries in the same order with the
+    same operations.  8 slots take 8-slot workgroups, 16 one 16-slot group, 32
+    two groups; graphs=1 replays the march from captured graphs; two sweeps
+    add the Jacobi sweeps that gather K u."""
+    m, fs, Xr, Zr = manh
+    print(f"[solve] edge march vs gather, batch {batch} graphs {graphs} sweeps {sweeps}", flush=True)
+    lay = Layout(m)
+    kw = dict(precision=capi.GP_PREC_MIXED, batch=batch, graphs=graphs, smooth_edge=sweeps)
+    Za, sa, _ = lay.solve(fs, edge_lds=0, **kw)
+    Zb, sb, _ = lay.solve(fs, edge_lds=1, **kw)
+    info = lay.hierarchy_info()
+    lay.close()
+    print(f"[solve] iterations {sa['iters']} / {sb['iters']}, max |dZ| {np.abs(Za - Zb).max():.3e}, "
+          f"block rows {info['march_block_rows']}, outside {info['march_outside']} of {info['march_entries']}",
+          flush=True)
+    assert info["march_block_rows"] > 0
+    assert 0 < info["march_outside"] < 0.3 * info["march_entries"]
+    assert sa["status"] == sb["status"] == capi.GP_OK
+    assert sa["iters"] == sb["iters"]
+    assert np.array_equal(Za, Zb)
+    for k in range(len(fs)):
+        assert _relerr(Zb[k], Zr[k]) < ZTOL
+
+
+def test_edge_march_same_bits_larger_mesh():
+    """The same on a mesh of 5 x 5 tiles and 13 planes (two runs of planes
+    per tile half), all 8 ports at two frequencies."""
+    m = M.Mesh(M.manhattan_spec((64, 64, 12), 8, 10, (1e7, 1e10), 2001))
+    fs = m.freqs()[[0, 9]]
+    print("[solve] edge march vs gather, 64x64x12", flush=True)
+    lay = Layout(m)
+    Za, sa, _ = lay.solve(fs, precision=capi.GP_PREC_MIXED, batch=16, edge_lds=0)
+    Zb, sb, _ = lay.solve(fs, precision=capi.GP_PREC_MIXED, batch=16, edge_lds=1)
+    info = lay.hierarchy_info()
+    lay.close()
+    print(f"[solve] iterations {sa['iters']} / {sb['iters']}, max |dZ| {np.abs(Za - Zb).max():.3e}, "
+          f"block rows {info['march_block_rows']}, outside {info['march_outside']} of {info['march_entries']}",
+          flush=True)
+    assert info["march_block_rows"] > 0
+    assert sa["iters"] == sb["iters"]
+    assert np.array_equal(Za, Zb)
