@@ -438,7 +438,9 @@ extern "C" gp_status gp_server_wait(gp_server *srv, int64_t request, double time
     if (!srv->error.empty()) return fail(GP_ERR_STATE, "gp_server_wait: admission: " + srv->error);
     for (auto &kv : srv->res)
       if (!kv.second->error.empty()) return fail(GP_ERR_STATE, "gp_server_wait: " + kv.second->error);
-    Request &q = srv->reqs[request];
+    auto cur = srv->reqs.find(request);  // another waiter may have collected it meanwhile
+    if (cur == srv->reqs.end()) return fail(GP_ERR_STATE, "gp_server_wait: request collected by another waiter");
+    Request &q = cur->second;
     if (q.remaining == 0) {
       if (rep) *rep = q.rep;
       if (Z) {

@@ -447,8 +447,9 @@ gp_status gp_server_submit(gp_server *srv, int32_t key, const int32_t *ports, in
  * nf*np columns of the layout's npt port responses, column (f, p) at
  * Z[2*((f*np + p)*npt + resp)] (complex interleaved; zeros for a rejected
  * column).  With Z null the request is waited for and reported (rep->npt
- * sizes Z) but not collected.  GP_ERR_STATE on a time-out or when a layout
- * failed to open or solve (the message names it). */
+ * sizes Z) but not collected.  GP_ERR_STATE on a time-out, when a layout
+ * failed to open or solve (the message names it), or when another thread
+ * collected the request first. */
 gp_status gp_server_wait(gp_server *srv, int64_t request, double timeout_s, double *Z, gp_request_report *rep);
 gp_status gp_server_info_get(gp_server *srv, gp_server_info *out);
 /* Stops admission, lets every resident layout finish its queued columns, and

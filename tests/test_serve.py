@@ -213,6 +213,30 @@ class SlotLoopBackend(MockBackend):
                    False)
 
 
+def test_cserver_request_collected_once():
+    """gp_server_wait hands a request's Z to one collector: a second collect
+    of the same request is an error, not a stale or empty result."""
+    import ctypes as C
+    be = MockBackend({"a": 10})
+    srv = CServer(be, 100)
+    try:
+        ports = np.array([0, 1], np.int32)
+        freqs = np.array([1e9], np.float64)
+        rid = C.c_int64()
+        capi.check(srv.lib.gp_server_submit(srv.h, srv._key("a"), ports.ctypes.data_as(C.POINTER(C.c_int32)), 2,
+                                            freqs.ctypes.data_as(C.POINTER(C.c_double)), 1, C.byref(rid)),
+                   "gp_server_submit")
+        rep = capi.GpRequestReport()
+        z = np.zeros(2 * 2 * 3)
+        assert srv.lib.gp_server_wait(srv.h, rid.value, 10.0, z.ctypes.data_as(C.POINTER(C.c_double)),
+                                      C.byref(rep)) == capi.GP_OK
+        assert rep.converged == 2 and np.abs(z).sum() > 0
+        assert srv.lib.gp_server_wait(srv.h, rid.value, 1.0, z.ctypes.data_as(C.POINTER(C.c_double)),
+                                      C.byref(rep)) != capi.GP_OK
+    finally:
+        srv.close()
+
+
 def _serve_idle_layout(be):
     """A layout goes idle, then gets a request with fewer columns than slots
     and nothing after it: the request must still finish."""
