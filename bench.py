@@ -123,7 +123,7 @@ def cpu_baseline():
     return out
 
 
-def serving_leg(n_requests, slots=16, python_server=False):
+def serving_leg(n_requests, slots=16, python_server=False, progress=False):
     """The first n_requests of the seeded serving stream (gds2para_amd.stream:
     24 Manhattan layouts, 5e4..2e7 edges log-uniform, 64 ports and a 40-point
     10 MHz..10 GHz grid each; 1-64 ports x 1-40 frequencies per request,
@@ -133,17 +133,24 @@ def serving_leg(n_requests, slots=16, python_server=False):
     or by the Python scheduler serve.Server with python_server.  The layouts
     the prefix names are opened (assembly, V0 basis, hierarchy) before the
     first arrival, which is what a warm server holds; latency = completion -
-    arrival per request."""
+    arrival per request.  progress prints each completed request to stderr."""
     from gds2para_amd import stream as ST
     from gds2para_amd.serve import CServer, GpuBackend, Request, Server, run_stream
     sp = ST.StreamSpec(n_requests=n_requests, rate_hz=0.5)
     specs = ST.layouts(sp)
     raw, offs = ST.requests(sp)
     keys = sorted({k for k, _, _ in raw})
+    t_start = time.perf_counter()
+
+    def done(r):
+        if progress:
+            print(f"[serve] {r.layout} {len(r.ports)} ports x {len(r.freqs)} freqs done at "
+                  f"{time.perf_counter() - t_start:.1f} s{', ' + r.error if r.error else ''}", file=sys.stderr, flush=True)
+
     if python_server:
-        srv = Server(GpuBackend({k: specs[k] for k in keys}, slots=slots), budget_bytes=260 << 30)
+        srv = Server(GpuBackend({k: specs[k] for k in keys}, slots=slots), budget_bytes=260 << 30, on_done=done)
     else:
-        srv = CServer(None, 260 << 30, slots=slots, layouts={k: specs[k] for k in keys})
+        srv = CServer(None, 260 << 30, slots=slots, layouts={k: specs[k] for k in keys}, on_done=done)
     t = time.perf_counter()
     for k in keys:                       # warm server: the named layouts resident
         srv.submit(Request(k, [0], [1e9]))
@@ -386,10 +393,12 @@ def main():
                     help="launch the preconditioner directly instead of replaying captured HIP graphs")
     ap.add_argument("--serve-python", action="store_true",
                     help="serve the stream with the Python scheduler instead of the library's server")
+    ap.add_argument("--serve-progress", action="store_true", help="print each completed serving request to stderr")
     ap.add_argument("--serving-only", action="store_true", help=argparse.SUPPRESS)
     a = ap.parse_args()
     if a.serving_only:                   # the child process of serving_leg_guarded
-        print(json.dumps(serving_leg(a.serve_requests, python_server=a.serve_python)), flush=True)
+        print(json.dumps(serving_leg(a.serve_requests, python_server=a.serve_python, progress=a.serve_progress)),
+              flush=True)
         return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported "

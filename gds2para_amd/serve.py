@@ -75,10 +75,11 @@ class Server:
     (assembly and multigrid setup, seconds at millions of edges) therefore
     holds neither the server lock nor another layout's stream thread."""
 
-    def __init__(self, backend, budget_bytes, clock=time.monotonic):
+    def __init__(self, backend, budget_bytes, clock=time.monotonic, on_done=None):
         self.backend = backend
         self.budget = budget_bytes
         self.clock = clock
+        self.on_done = on_done             # called with each request once it completes
         self.res = {}                      # key -> _Resident
         self.backlog = collections.deque() # requests not yet queued on a layout
         self.requests = {}
@@ -266,6 +267,7 @@ class Server:
             return len(r.q)
 
         def push(tag, z, rep):
+            finished = None
             with self.lock:
                 rid, fi, p = self.tags.pop(tag)
                 req = self.requests[rid]
@@ -277,6 +279,7 @@ class Server:
                 req.remaining -= 1
                 if req.remaining == 0:
                     req.done = self.clock()
+                    finished = req
                 with r.cv:
                     r.inflight -= 1
                     r.last_used = self.clock()
@@ -284,6 +287,8 @@ class Server:
                 if idle and self.backlog:
                     self._wake()            # a waiting request may now evict this layout
                 self.all_done.notify_all()
+            if finished is not None and self.on_done:
+                self.on_done(finished)
 
         try:
             self.backend.stream(r.handle, pull, waiting, push)
@@ -375,9 +380,10 @@ class CServer:
     gp_server_wait, which fills Z, the reports and `remaining` when it ends."""
 
     def __init__(self, backend, budget_bytes, slots=16, precision=capi.GP_PREC_MIXED, rtol=1e-8, restart=16,
-                 max_iters=2000, layouts=None, clock=time.monotonic):
+                 max_iters=2000, layouts=None, clock=time.monotonic, on_done=None):
         self.lib = capi.load()
         self.clock = clock
+        self.on_done = on_done              # called with each request once it is collected
         self.backend = backend
         self.keys, self.names = {}, []      # request key <-> server key
         self.handles = {}                   # handle id -> backend handle
@@ -516,6 +522,8 @@ class CServer:
             req.error = "rejected column"
         req.done = req.arrival + (rep.done_s - rep.arrival_s)
         req.remaining = 0
+        if self.on_done:
+            self.on_done(req)
 
     def wait(self, timeout=None):
         """Blocks until every submitted request has been collected."""
