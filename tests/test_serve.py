@@ -527,3 +527,34 @@ def test_cserver_gpu_evicts_under_budget():
         assert not r.error and r.remaining == 0, r.error
         assert r.reports["converged"] == len(r.ports) * len(r.freqs)
     _check_against_alone(specs, reqs)
+
+
+@pytest.mark.gpu
+def test_cserver_gpu_rejects_bad_port_and_keeps_serving():
+    """A column with a port the layout does not have comes back rejected
+    (counted in the request's report, its Z column left zero); the request's
+    valid columns are solved, and the server serves the next request."""
+    from gds2para_amd import mesh as M
+    spec = M.manhattan_spec((24, 20, 10), n_ports=3, nfreq=3, f_range=(1e8, 1e10), seed=11)
+    print("[cserver] request with port 5 on a 3-port layout", flush=True)
+    srv = CServer(None, 200 << 30, slots=4, rtol=1e-8, layouts={"m": spec})
+    try:
+        bad = Request("m", [0, 5], [1e9])
+        srv.submit(bad)
+        srv.wait(120)
+        assert bad.reports["rejected"] == 1 and bad.reports["converged"] == 1
+        assert bad.error == "rejected column"
+        good = Request("m", [1], [1e9])
+        srv.submit(good)
+        srv.wait(120)
+        assert not good.error and good.reports["converged"] == 1
+    finally:
+        srv.close()
+    _check_against_alone({"m": spec}, [good])
+    # the rejected request's valid column equals the same column solved alone
+    from gds2para_amd.layout import Layout
+    lay = Layout(M.Mesh(spec))
+    Z, st, _ = lay.solve([1e9], ports=[0], precision=capi.GP_PREC_MIXED, rtol=1e-8, batch=4)
+    lay.close()
+    Za = Z.reshape(1, 3, 3)
+    assert np.abs(bad.Z[0, :, 0] - Za[0, 0, :]).max() / np.abs(Za[0, 0, :]).max() < 1e-6
