@@ -203,6 +203,17 @@ __global__ void __launch_bounds__(256) k_outer_sell(OuterOp op, SellV<double> SS
   Z2 acc[KS], xi[KS];
 #pragma unroll
   for (int q = 0; q < KS; ++q) acc[q] = {0., 0.};
+  // The G and Gx rows hold two entries or fewer on nearly every edge: their
+  // first two indices and values are loaded before the S gathers, so the phi
+  // and a gathers after them wait on no index load of their own
+  const int64_t g0 = op.grp[i], g1 = op.grp[i + 1], x0 = op.xrp[i], x1 = op.xrp[i + 1];
+  int32_t gc[2] = {0, 0}, xc[2] = {0, 0};
+  double gw[2] = {0., 0.}, xw[2] = {0., 0.};
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (g0 + h < g1) { gc[h] = op.gci[g0 + h]; gw[h] = op.gv[g0 + h]; }
+    if (x0 + h < x1) { xc[h] = op.xci[x0 + h]; xw[h] = op.xv[x0 + h]; }
+  }
   {
     const int64_t b0 = SS.base[sl];
     const int wd = SS.w[sl];
@@ -218,19 +229,38 @@ __global__ void __launch_bounds__(256) k_outer_sell(OuterOp op, SellV<double> SS
 #pragma unroll
     for (int q = 0; q < KS; ++q) xi[q] = {(double)x[q].x, (double)x[q].y};
   }
-  for (int64_t p = op.grp[i]; p < op.grp[i + 1]; ++p) {
-    const double g = op.gv[p];
-    C2<TZ> x[KS];
-    load_slots<TZ, KS>(phi + (int64_t)op.gci[p] * k + j0, x);
+  {
+    C2<TZ> xg[2][KS];
 #pragma unroll
-    for (int q = 0; q < KS; ++q) { xi[q].x += g * (double)x[q].x; xi[q].y += g * (double)x[q].y; }
-  }
-  for (int64_t p = op.xrp[i]; p < op.xrp[i + 1]; ++p) {
-    const double g = op.xv[p];
-    C2<TZ> x[KS];
-    load_slots<TZ, KS>(a + (int64_t)op.xci[p] * k + j0, x);
+    for (int h = 0; h < 2; ++h)
+      if (g0 + h < g1) load_slots<TZ, KS>(phi + (int64_t)gc[h] * k + j0, xg[h]);
 #pragma unroll
-    for (int q = 0; q < KS; ++q) { xi[q].x += g * (double)x[q].x; xi[q].y += g * (double)x[q].y; }
+    for (int h = 0; h < 2; ++h)
+      if (g0 + h < g1)
+#pragma unroll
+        for (int q = 0; q < KS; ++q) { xi[q].x += gw[h] * (double)xg[h][q].x; xi[q].y += gw[h] * (double)xg[h][q].y; }
+    for (int64_t p = g0 + 2; p < g1; ++p) {
+      const double g = op.gv[p];
+      C2<TZ> x[KS];
+      load_slots<TZ, KS>(phi + (int64_t)op.gci[p] * k + j0, x);
+#pragma unroll
+      for (int q = 0; q < KS; ++q) { xi[q].x += g * (double)x[q].x; xi[q].y += g * (double)x[q].y; }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+      if (x0 + h < x1) load_slots<TZ, KS>(a + (int64_t)xc[h] * k + j0, xg[h]);
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+      if (x0 + h < x1)
+#pragma unroll
+        for (int q = 0; q < KS; ++q) { xi[q].x += xw[h] * (double)xg[h][q].x; xi[q].y += xw[h] * (double)xg[h][q].y; }
+    for (int64_t p = x0 + 2; p < x1; ++p) {
+      const double g = op.xv[p];
+      C2<TZ> x[KS];
+      load_slots<TZ, KS>(a + (int64_t)op.xci[p] * k + j0, x);
+#pragma unroll
+      for (int q = 0; q < KS; ++q) { xi[q].x += g * (double)x[q].x; xi[q].y += g * (double)x[q].y; }
+    }
   }
   if (exact) {
     for (int64_t p = op.sgrp[i]; p < op.sgrp[i + 1]; ++p) {
