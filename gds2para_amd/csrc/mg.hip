@@ -1202,6 +1202,104 @@ __global__ void __launch_bounds__(256) k_edge_sell(int64_t nr, int k, SellV<T> K
   store_slots<T, KS>(out + i * k + j0, acc);
 }
 
+// The edge launches that gather no K u (NOK: r - B phi, and the Jacobi
+// sweep from such a residual; JACOBI0: r / d_l1) stream r and out and gather
+// two phi rows per edge.  They were bound by the bytes in flight, not by the
+// texture path's rate: here each lane group takes two rows (positions p and
+// p + half), whose residual, B indices and values, diagonals and u are all
+// loaded before either row's phi gathers.  Per row the same operations in
+// the same order as k_edge_sell.
+constexpr int64_t kNok2MinRows = (int64_t)1 << 22;
+
+template <class T, int SPL, int LPR>
+__global__ void __launch_bounds__(256) k_edge_nok2(int64_t nr, int64_t half, int k, SellV<T> K, SellV<T> B,
+                                                   const T *__restrict__ c1v, const T *__restrict__ c2v,
+                                                   const T *__restrict__ kd0, const T *__restrict__ kd1,
+                                                   const T *__restrict__ kd2, const T *__restrict__ kl0,
+                                                   const T *__restrict__ kl1, const T *__restrict__ kl2,
+                                                   const C2<T> *__restrict__ r, const C2<T> *__restrict__ u,
+                                                   const C2<T> *__restrict__ phi, C2<T> *__restrict__ out, int mode,
+                                                   int use_phi, T damp) {
+  constexpr int KS = SPL, NR = 2, PFB = 2;
+  const int64_t t = xcd_block_id() * blockDim.x + threadIdx.x;
+  const int64_t q0 = t / LPR;
+  if (q0 >= half) return;
+  const int sub = (int)(t - q0 * LPR);
+  const int j0 = ((int)blockIdx.y * LPR + sub) * SPL;
+  const int kind = mode & EdgeMode::KIND;
+  const bool bphi = use_phi && kind != EdgeMode::JACOBI0;
+  T c1[KS], c2[KS];
+#pragma unroll
+  for (int q = 0; q < KS; ++q) { c1[q] = c1v[j0 + q]; c2[q] = c2v[j0 + q]; }
+  bool ok[NR];
+  int64_t i[NR], bb[NR];
+  int wb[NR], lane[NR];
+  C2<T> acc[NR][KS], u0[NR][KS];
+  T d[NR][6];
+  int32_t cb[NR][PFB];
+  T a0[NR][PFB], a1[NR][PFB];
+#pragma unroll
+  for (int n = 0; n < NR; ++n) {
+    const int64_t pos = q0 + n * half;
+    ok[n] = pos < nr;
+    i[n] = 0; bb[n] = 0; wb[n] = 0; lane[n] = 0;
+    if (!ok[n]) continue;
+    i[n] = K.rows ? (int64_t)K.rows[pos] : pos;
+    const int64_t sl = pos >> 6;
+    lane[n] = (int)(pos & 63);
+    if (bphi) { bb[n] = B.base[sl]; wb[n] = B.w[sl]; }
+  }
+#pragma unroll
+  for (int n = 0; n < NR; ++n) {
+    if (!ok[n]) continue;
+    load_slots<T, KS>(r + i[n] * k + j0, acc[n]);
+    if (kind != EdgeMode::RES) {
+      d[n][0] = kd0[i[n]]; d[n][1] = kd1[i[n]]; d[n][2] = kd2[i[n]];
+      d[n][3] = kl0[i[n]]; d[n][4] = kl1[i[n]]; d[n][5] = kl2[i[n]];
+    }
+    if (kind == EdgeMode::JACOBI) load_slots<T, KS>(u + i[n] * k + j0, u0[n]);
+    else
+#pragma unroll
+      for (int q = 0; q < KS; ++q) u0[n][q] = {(T)0, (T)0};
+#pragma unroll
+    for (int h = 0; h < PFB; ++h) {
+      cb[n][h] = 0; a0[n][h] = (T)0; a1[n][h] = (T)0;
+      if (h < wb[n]) {
+        const int64_t e = bb[n] + (int64_t)h * 64 + lane[n];
+        cb[n][h] = __builtin_nontemporal_load(B.ci + e);
+        a0[n][h] = __builtin_nontemporal_load(B.v[0] + e);
+        a1[n][h] = __builtin_nontemporal_load(B.v[1] + e);
+      }
+    }
+  }
+  C2<T> xb[NR][PFB][KS];
+#pragma unroll
+  for (int n = 0; n < NR; ++n)
+#pragma unroll
+    for (int h = 0; h < PFB; ++h)
+      if (h < wb[n]) load_slots<T, KS>(phi + (int64_t)cb[n][h] * k + j0, xb[n][h]);
+#pragma unroll
+  for (int n = 0; n < NR; ++n) {
+    if (!ok[n]) continue;
+#pragma unroll
+    for (int h = 0; h < PFB; ++h)
+      if (h < wb[n])
+#pragma unroll
+        for (int q = 0; q < KS; ++q) csub_cx<T>(acc[n][q], c1[q] * a0[n][h], c2[q] * a1[n][h], xb[n][h][q]);
+    auto fb = [&](T b0v, T b1v, T, const C2<T>(&x)[KS]) {
+#pragma unroll
+      for (int q = 0; q < KS; ++q) csub_cx<T>(acc[n][q], c1[q] * b0v, c2[q] * b1v, x[q]);
+    };
+    for (int tt = PFB; tt < wb[n]; ++tt) sell_batch<T, KS, 2, 1, 1>(B, bb[n], tt, lane[n], phi, k, j0, fb);
+    if (kind != EdgeMode::RES)
+#pragma unroll
+      for (int q = 0; q < KS; ++q)
+        acc[n][q] = jacobi_update<T>(acc[n][q], u0[n][q], c1[q], c2[q], d[n][0], d[n][1], d[n][2], d[n][3], d[n][4],
+                                     d[n][5], damp);
+    store_slots<T, KS>(out + i[n] * k + j0, acc[n]);
+  }
+}
+
 // ---- level-0 edge smoother / residual as a plane march (FP32) --------------
 // k_edge_sell with the K u gathers served from LDS.  A workgroup takes one
 // (tile half, run of planes, group of SW slots) and walks its planes upwards
@@ -1951,6 +2049,26 @@ void edge_sell(int64_t nr, int64_t smin, int k, const SellV<T> &K, const SellV<T
     k_edge<T><<<blocks_for(nr * k), 256, 0, s>>>(nr, k, Kc, Bc, c1, c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, phi, out,
                                                  mode, use_phi, damp);
     return;
+  }
+  const int kind = mode & EdgeMode::KIND;
+  // no K u on an operator whose vectors stream from HBM: two rows per lane
+  // group.  At 400x400x32 the level-0 launches (15.4M rows) took 1252 us
+  // against 2010 us; on level 1 (1.9M rows, its vectors held in the caches)
+  // the single-row kernel is faster (276 us against 374 us)
+  if (((mode & EdgeMode::NOK) || kind == EdgeMode::JACOBI0) && nr >= kNok2MinRows) {
+    const int64_t half = (nr + 1) / 2;
+    const Lanes ln = lanes_shape<T>(half, k);
+    if (ln.sp == 4) {
+#define GP_NOK2(L) k_edge_nok2<T, 4, L><<<ln.grid, 256, 0, s>>>(nr, half, k, K, B, c1, c2, kd0, kd1, kd2, kl0, kl1, kl2, \
+                                                                r, u, phi, out, mode, use_phi, damp)
+      if (ln.lpr == 16) GP_NOK2(16);
+      else if (ln.lpr == 8) GP_NOK2(8);
+      else if (ln.lpr == 4) GP_NOK2(4);
+      else if (ln.lpr == 2) GP_NOK2(2);
+      else GP_NOK2(1);
+#undef GP_NOK2
+      return;
+    }
   }
   GP_LANES_DISPATCH(k_edge_sell, nr, k, K, B, c1, c2, kd0, kd1, kd2, kl0, kl1, kl2, r, u, phi, out, mode, use_phi, damp);
 }
