@@ -410,18 +410,40 @@ __global__ void __launch_bounds__(256) k_mdots(int64_t n, int k, const C2<TV> *_
 #pragma unroll
   for (int q = 0; q <= kDotChunk; ++q) acc[q] = {0., 0.};
   const int64_t step = (int64_t)gridDim.x * rows_per;
-  for (int64_t i = (int64_t)blockIdx.x * rows_per + threadIdx.x / k; i < n; i += step) {
-    const C2<TW> wv = w[i * k + j];
+  // two rows per pass, every load of both issued before the products: one
+  // row per pass left each wavefront waiting a full memory latency per row
+  auto row = [&](const C2<TW> &wv, const C2<TV> (&v)[kDotChunk]) {
     const Z2 b = {(double)wv.x, (double)wv.y};
 #pragma unroll
     for (int q = 0; q < kDotChunk; ++q)
       if (q < nt) {
-        const C2<TV> v = V[(int64_t)(t0 + q) * vstride + i * k + j];
-        const Z2 a = {(double)v.x, (double)v.y};
+        const Z2 a = {(double)v[q].x, (double)v[q].y};
         acc[q].x += a.x * b.x + a.y * b.y;
         acc[q].y += a.x * b.y - a.y * b.x;
       }
     if (do_self) acc[kDotChunk].x += b.x * b.x + b.y * b.y;
+  };
+  int64_t i = (int64_t)blockIdx.x * rows_per + threadIdx.x / k;
+  for (; i + step < n; i += 2 * step) {
+    const int64_t i2 = i + step;
+    const C2<TW> w1 = w[i * k + j], w2 = w[i2 * k + j];
+    C2<TV> v1[kDotChunk], v2[kDotChunk];
+#pragma unroll
+    for (int q = 0; q < kDotChunk; ++q)
+      if (q < nt) {
+        v1[q] = V[(int64_t)(t0 + q) * vstride + i * k + j];
+        v2[q] = V[(int64_t)(t0 + q) * vstride + i2 * k + j];
+      }
+    row(w1, v1);
+    row(w2, v2);
+  }
+  if (i < n) {
+    const C2<TW> w1 = w[i * k + j];
+    C2<TV> v1[kDotChunk];
+#pragma unroll
+    for (int q = 0; q < kDotChunk; ++q)
+      if (q < nt) v1[q] = V[(int64_t)(t0 + q) * vstride + i * k + j];
+    row(w1, v1);
   }
   const int nout = nvec + self;
   for (int q = 0; q <= kDotChunk; ++q) {
@@ -463,6 +485,7 @@ __global__ void k_gs_update(int64_t n, int k, int nvec, const C2<TV> *__restrict
   int j = (int)(q % k);
   const C2<TW> w0 = w[q];
   Z2 acc = {(double)w0.x, (double)w0.y};
+#pragma unroll 4
   for (int t = 0; t < nvec; ++t) {
     const C2<TV> a = V[(int64_t)t * vstride + q];
     const Z2 c = h[t * k + j];
@@ -486,6 +509,7 @@ __global__ void k_xupdate(int64_t n, int k, int nvec, const C2<TZ> *__restrict__
   if (q >= n * k) return;
   int j = (int)(q % k);
   Z2 acc = x[q];
+#pragma unroll 4
   for (int t = 0; t < nvec; ++t) {
     const C2<TZ> a = Zb[(int64_t)t * zstride + q];
     const Z2 c = y[t * k + j];
