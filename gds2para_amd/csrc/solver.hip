@@ -501,6 +501,40 @@ __global__ void k_gs_update(int64_t n, int k, int nvec, const C2<TV> *__restrict
   }
 }
 
+// k_gs_update for an even k, two adjacent slots per thread: one 16-byte load
+// of FP32 basis data per vector instead of two 8-byte ones; per slot the
+// same operations in the same order
+template <class TV, class TW, class TO, class TR>
+__global__ void k_gs_update2(int64_t n, int k, int nvec, const C2<TV> *__restrict__ V, int64_t vstride,
+                             const Z2 *__restrict__ h, const double *__restrict__ scale, const C2<TW> *w, C2<TO> *out,
+                             const double *__restrict__ W, C2<TR> *__restrict__ rout) {
+  const int64_t q0 = 2 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+  if (q0 >= n * k) return;
+  const int j0 = (int)(q0 % k);
+  Z2 acc[2];
+#pragma unroll
+  for (int e = 0; e < 2; ++e) acc[e] = {(double)w[q0 + e].x, (double)w[q0 + e].y};
+#pragma unroll 4
+  for (int t = 0; t < nvec; ++t) {
+    const float4 v4 = *reinterpret_cast<const float4 *>(V + (int64_t)t * vstride + q0);
+    const C2<TV> a[2] = {{(TV)v4.x, (TV)v4.y}, {(TV)v4.z, (TV)v4.w}};
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const Z2 c = h[t * k + j0 + e];
+      acc[e].x -= (double)a[e].x * c.x - (double)a[e].y * c.y;
+      acc[e].y -= (double)a[e].x * c.y + (double)a[e].y * c.x;
+    }
+  }
+  const double wt = rout ? W[q0 / k] : 0.;
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const double sc = scale[j0 + e];
+    const C2<TO> o = {(TO)(acc[e].x * sc), (TO)(acc[e].y * sc)};
+    out[q0 + e] = o;
+    if (rout) rout[q0 + e] = {(TR)(wt * (double)o.x), (TR)(wt * (double)o.y)};
+  }
+}
+
 // x += sum_t Z_t y_t (per column)
 template <class TZ>
 __global__ void k_xupdate(int64_t n, int k, int nvec, const C2<TZ> *__restrict__ Zb, int64_t zstride,
@@ -877,6 +911,9 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
   } pg{cx.sv->cc, pinned};
 
   const unsigned gnk = blocks_for(n * k), gnnk = blocks_for(nn * k), gnak = blocks_for(na * k);
+  // the paired update reads 16-byte pieces of an FP32 basis: k even
+  const bool gs2 = sizeof(TV) == 4 && k % 2 == 0;
+  const unsigned gnk2 = blocks_for((n * k + 1) / 2);
   // slot state
   std::vector<int32_t> cid(k, -1), its(k, 0), fsl(k, 0);
   std::vector<Column> ccol(k);
@@ -1110,7 +1147,8 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
       if (reorth) {
         std::fill(sch.begin(), sch.end(), 1.0);
         GP_HIP(hipMemcpyAsync(sc_d, sch.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-        k_gs_update<TV, TV, TV, T><<<gnk, 256, 0, s>>>(n, k, i + 1, V, n * k, dots.hd, sc_d, wa, wa, nullptr, nullptr);
+        if (gs2) k_gs_update2<TV, TV, TV, T><<<gnk2, 256, 0, s>>>(n, k, i + 1, V, n * k, dots.hd, sc_d, wa, wa, nullptr, nullptr);
+        else k_gs_update<TV, TV, TV, T><<<gnk, 256, 0, s>>>(n, k, i + 1, V, n * k, dots.hd, sc_d, wa, wa, nullptr, nullptr);
         GP_TRY(dots.run(s, n, k, V, n * k, i + 1, wa, 1, h2));
         for (int j = 0; j < k; ++j) {
           double hh = 0.;
@@ -1153,7 +1191,8 @@ gp_status solve_slots(SolveCtx &cx, ColumnIO &io, int k) {
       // V_{i+1} = (w - V h) / |w - V h|, with the coefficients of the last pass
       // (dots.hd holds them: the second pass overwrote the first when it ran)
       GP_HIP(hipMemcpyAsync(sc_d, sch.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-      k_gs_update<TV, TV, TV, T><<<gnk, 256, 0, s>>>(n, k, i + 1, V, n * k, dots.hd, sc_d, wa, Vn, H.W, vw.r[0]);
+      if (gs2) k_gs_update2<TV, TV, TV, T><<<gnk2, 256, 0, s>>>(n, k, i + 1, V, n * k, dots.hd, sc_d, wa, Vn, H.W, vw.r[0]);
+      else k_gs_update<TV, TV, TV, T><<<gnk, 256, 0, s>>>(n, k, i + 1, V, n * k, dots.hd, sc_d, wa, Vn, H.W, vw.r[0]);
       used = i + 1;
       if (all_in) break;
       // with columns waiting, restart once half of the slots are done
