@@ -443,7 +443,7 @@ def main():
     line.update(h)
     line["copy_probe_GBs"] = copy_gbs
     line["roofline"] = {"bound": "hbm",
-                        "kernel": "level-0 edge smoother (k_edge_sell<float, 4, 4>: 4 lanes per row, 4 slots per lane)",
+                        "kernel": "level-0 edge smoother: k_edge_sell<float, 4, 4> for the K u launches, k_edge_nok2<float, 4, 4> for the others (4 lanes per row, 4 slots per lane)",
                         "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": ach / HBM_PEAK_GBS if ach else None,
                         "traffic": traffic, "traffic_source": traffic_src, "launches": launches,
