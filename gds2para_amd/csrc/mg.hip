@@ -1723,6 +1723,119 @@ __global__ void __launch_bounds__(256) k_spmv2_sell(int64_t nr, int k, SellV<T> 
   }
 }
 
+// k_spmv2_sell (one entry group, no long rows) on an operator of
+// kNok2MinRows rows or more (the level-0 Bx product): two rows per lane
+// group, their indices and values, r or y and the epilogue's diagonals
+// loaded before the row gathers, as in k_edge_nok2.  Per row the entries are
+// summed in the same order; the compiler may fuse the complex products'
+// multiply-adds differently from k_spmv2_sell (Z of a 400x400x32 solve moved
+// by 5e-15 relative, same iterations).
+template <class T, int SPL, int LPR>
+__global__ void __launch_bounds__(256) k_spmv2_sell2(int64_t nr, int64_t half, int k, SellV<T> M,
+                                                     const T *__restrict__ c1v, const T *__restrict__ c2v,
+                                                     const C2<T> *__restrict__ r, const C2<T> *__restrict__ x,
+                                                     C2<T> *__restrict__ y, int mode, Jac0<T> jz) {
+  constexpr int KS = SPL, NR = 2, PF = 4;
+  const int64_t t = xcd_block_id() * blockDim.x + threadIdx.x;
+  const int64_t q0 = t / LPR;
+  if (q0 >= half) return;
+  const int j0 = ((int)blockIdx.y * LPR + (int)(t - q0 * LPR)) * SPL;
+  T c1[KS], c2[KS];
+#pragma unroll
+  for (int q = 0; q < KS; ++q) { c1[q] = c1v[j0 + q]; c2[q] = c2v[j0 + q]; }
+  bool ok[NR];
+  int64_t i[NR], b0[NR];
+  int wd[NR], lane[NR];
+  int32_t c[NR][PF];
+  T a0[NR][PF], a1[NR][PF];
+  C2<T> o[NR][KS], acc[NR][KS];
+  T d[NR][6];
+#pragma unroll
+  for (int n = 0; n < NR; ++n) {
+    const int64_t pos = q0 + n * half;
+    ok[n] = pos < nr;
+    i[n] = 0; b0[n] = 0; wd[n] = 0; lane[n] = 0;
+#pragma unroll
+    for (int q = 0; q < KS; ++q) acc[n][q] = {0, 0};
+    if (!ok[n]) continue;
+    i[n] = M.rows ? (int64_t)M.rows[pos] : pos;
+    const int64_t sl = pos >> 6;
+    lane[n] = (int)(pos & 63);
+    b0[n] = M.base[sl];
+    wd[n] = M.w[sl];
+  }
+#pragma unroll
+  for (int n = 0; n < NR; ++n) {
+#pragma unroll
+    for (int h = 0; h < PF; ++h) {
+      c[n][h] = 0; a0[n][h] = (T)0; a1[n][h] = (T)0;
+      if (h < wd[n]) {
+        const int64_t e = b0[n] + (int64_t)h * 64 + lane[n];
+        c[n][h] = __builtin_nontemporal_load(M.ci + e);
+        a0[n][h] = __builtin_nontemporal_load(M.v[0] + e);
+        a1[n][h] = __builtin_nontemporal_load(M.v[1] + e);
+      }
+    }
+    if (!ok[n]) continue;
+    if (mode == 0) load_slots<T, KS>(r + i[n] * k + j0, o[n]);
+    else if (mode == 1) load_slots<T, KS>(y + i[n] * k + j0, o[n]);
+    if (jz.u) {
+      d[n][0] = jz.kd0[i[n]]; d[n][1] = jz.kd1[i[n]]; d[n][2] = jz.kd2[i[n]];
+      d[n][3] = jz.kl0[i[n]]; d[n][4] = jz.kl1[i[n]]; d[n][5] = jz.kl2[i[n]];
+    }
+  }
+#pragma unroll
+  for (int t0 = 0; t0 < PF; t0 += 2) {  // two gathers per row in flight
+    C2<T> xv[NR][2][KS];
+#pragma unroll
+    for (int n = 0; n < NR; ++n)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        if (t0 + h < wd[n]) load_slots<T, KS>(x + (int64_t)c[n][t0 + h] * k + j0, xv[n][h]);
+#pragma unroll
+    for (int n = 0; n < NR; ++n)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        if (t0 + h < wd[n])
+#pragma unroll
+          for (int q = 0; q < KS; ++q) {
+            const T ar = c1[q] * a0[n][t0 + h], ai = c2[q] * a1[n][t0 + h];
+            acc[n][q].x += ar * xv[n][h][q].x - ai * xv[n][h][q].y;
+            acc[n][q].y += ar * xv[n][h][q].y + ai * xv[n][h][q].x;
+          }
+  }
+#pragma unroll
+  for (int n = 0; n < NR; ++n) {
+    if (!ok[n]) continue;
+    auto f = [&](T b0v, T b1v, T, const C2<T>(&xr)[KS]) {
+#pragma unroll
+      for (int q = 0; q < KS; ++q) {
+        const T ar = c1[q] * b0v, ai = c2[q] * b1v;
+        acc[n][q].x += ar * xr[q].x - ai * xr[q].y;
+        acc[n][q].y += ar * xr[q].y + ai * xr[q].x;
+      }
+    };
+    for (int tt = PF; tt < wd[n]; ++tt) sell_batch<T, KS, 2, 1, 1>(M, b0[n], tt, lane[n], x, k, j0, f);
+#pragma unroll
+    for (int q = 0; q < KS; ++q) {
+      if (mode == 2) o[n][q] = acc[n][q];
+      else o[n][q] = {o[n][q].x - acc[n][q].x, o[n][q].y - acc[n][q].y};
+    }
+    store_slots<T, KS>(y + i[n] * k + j0, o[n]);
+    if (jz.u) {
+      C2<T> uo[KS];
+#pragma unroll
+      for (int q = 0; q < KS; ++q) {
+        const C2<T> dq = l1diag<T>({d[n][0] + c1[q] * d[n][1], c2[q] * d[n][2]},
+                                   d[n][3] + fabs(c1[q]) * d[n][4] + c2[q] * d[n][5]);
+        const C2<T> z = cdiv(o[n][q], dq);
+        uo[q] = {jz.damp * z.x, jz.damp * z.y};
+      }
+      store_slots<T, KS>(jz.u + i[n] * k + j0, uo);
+    }
+  }
+}
+
 // Nodal l1-Jacobi on N(w) = -w^2 N_eps + i w N_sig:
 //   first (d == null): d_out = rn / dn;  else d_out = d + (rn - N d) / dn
 template <class T>
@@ -2122,6 +2235,20 @@ bool spmv2_sell(int64_t nr, int64_t smin, int k, const SellV<T> &M, const MatV<T
   if (nr < smin) {
     k_spmv2<T><<<blocks_for(nr * k), 256, 0, s>>>(nr, k, Mc, c1, c2, r, x, y, mode, lp);
     return false;
+  }
+  if (!lidx && nr >= kNok2MinRows) {  // large operator, no long rows: two rows per lane group
+    const int64_t half = (nr + 1) / 2;
+    const Lanes ln = lanes_shape<T>(half, k);
+    if (ln.sp == 4 && entry_split<T>(M, ln) == 1) {
+#define GP_SPMV2X(L) k_spmv2_sell2<T, 4, L><<<ln.grid, 256, 0, s>>>(nr, half, k, M, c1, c2, r, x, y, mode, jz)
+      if (ln.lpr == 16) GP_SPMV2X(16);
+      else if (ln.lpr == 8) GP_SPMV2X(8);
+      else if (ln.lpr == 4) GP_SPMV2X(4);
+      else if (ln.lpr == 2) GP_SPMV2X(2);
+      else GP_SPMV2X(1);
+#undef GP_SPMV2X
+      return jz.u != nullptr;
+    }
   }
   GP_LANES_ES_DISPATCH(k_spmv2_sell, M, nr, k, M, c1, c2, r, x, y, mode, lidx, lp, jz);
   return jz.u != nullptr;
