@@ -263,20 +263,35 @@ __global__ void __launch_bounds__(256) k_outer_sell(OuterOp op, SellV<double> SS
     }
   }
   if (exact) {
-    for (int64_t p = op.sgrp[i]; p < op.sgrp[i + 1]; ++p) {
-      const double g = op.sgv[p];
-      C2<TZ> x[KS];
-      load_slots<TZ, KS>(phi + (int64_t)op.sgci[p] * k + j0, x);
+    // rows of S G and S Gx, four entries at a time: their indices and values
+    // first, then the four row gathers in flight together; order unchanged
+    auto walk = [&](const int64_t *__restrict__ rp, const int32_t *__restrict__ ci, const double *__restrict__ v,
+                    const C2<TZ> *__restrict__ src) {
+      const int64_t p1 = rp[i + 1];
+      int64_t p = rp[i];
+      for (; p + 4 <= p1; p += 4) {
+        int32_t cc[4];
+        double gg[4];
 #pragma unroll
-      for (int q = 0; q < KS; ++q) { acc[q].x += g * (double)x[q].x; acc[q].y += g * (double)x[q].y; }
-    }
-    for (int64_t p = op.sxrp[i]; p < op.sxrp[i + 1]; ++p) {
-      const double g = op.sxv[p];
-      C2<TZ> x[KS];
-      load_slots<TZ, KS>(a + (int64_t)op.sxci[p] * k + j0, x);
+        for (int h = 0; h < 4; ++h) { cc[h] = ci[p + h]; gg[h] = v[p + h]; }
+        C2<TZ> x[4][KS];
 #pragma unroll
-      for (int q = 0; q < KS; ++q) { acc[q].x += g * (double)x[q].x; acc[q].y += g * (double)x[q].y; }
-    }
+        for (int h = 0; h < 4; ++h) load_slots<TZ, KS>(src + (int64_t)cc[h] * k + j0, x[h]);
+#pragma unroll
+        for (int h = 0; h < 4; ++h)
+#pragma unroll
+          for (int q = 0; q < KS; ++q) { acc[q].x += gg[h] * (double)x[h][q].x; acc[q].y += gg[h] * (double)x[h][q].y; }
+      }
+      for (; p < p1; ++p) {
+        const double g = v[p];
+        C2<TZ> x[KS];
+        load_slots<TZ, KS>(src + (int64_t)ci[p] * k + j0, x);
+#pragma unroll
+        for (int q = 0; q < KS; ++q) { acc[q].x += g * (double)x[q].x; acc[q].y += g * (double)x[q].y; }
+      }
+    };
+    walk(op.sgrp, op.sgci, op.sgv, phi);
+    walk(op.sxrp, op.sxci, op.sxv, a);
   }
   const double sii = op.sdiag[i], er = op.epsr[i], sg = op.sig[i];
   Z2 bb[KS];
