@@ -46,6 +46,26 @@ __global__ void k_diag(int64_t n, const int64_t *__restrict__ rp, const int32_t 
   d[i] = x;
 }
 
+// Off-diagonal copy of a one-valued CSR: cnt[i] = entries of row i off the
+// diagonal; then the entries in their order, and d[i] = M_ii
+__global__ void k_offdiag_count(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                int64_t *__restrict__ cnt) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int64_t c = 0;
+  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) c += ci[p] != i;
+  cnt[i] = c;
+}
+__global__ void k_offdiag_fill(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                               const double *__restrict__ v, const int64_t *__restrict__ rp2, int32_t *__restrict__ ci2,
+                               double *__restrict__ v2) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int64_t q = rp2[i];
+  for (int64_t p = rp[i]; p < rp[i + 1]; ++p)
+    if (ci[p] != i) { ci2[q] = ci[p]; v2[q] = v[p]; ++q; }
+}
+
 // B0 = W * [eps_r eps0, sigma] * G   (row scaling of the gradient)
 __global__ void k_level0_B(int64_t n, const int64_t *__restrict__ rp, const double *__restrict__ g,
                            const double *__restrict__ W, const double *__restrict__ epsr, const double *__restrict__ sig,
@@ -388,6 +408,7 @@ gp_status hierarchy_add_f32(gp_layout *lay, Arena &A, Hierarchy *H, hipStream_t 
     GP_TRY(csr_to_float(A, L.K, &L.Kf, s));
     for (int q = 0; q < 2; ++q)
       if (L.K.dg[q]) GP_TRY(to_f32(A, L.K.dg[q], L.ne, &L.Kf.dg[q], s));
+    if (L.K.sdg) GP_TRY(to_f32(A, L.K.sdg, L.ne, &L.Kf.sdg, s));
     if (L.Ks.nr) {
       GP_TRY(sell_to_float(A, L.Ks, &L.Ksf, s)); GP_TRY(sell_to_float(A, L.Bs, &L.Bsf, s));
       GP_TRY(sell_to_float(A, L.Ns, &L.Nsf, s)); GP_TRY(sell_to_float(A, L.GTs, &L.GTsf, s));
@@ -521,6 +542,10 @@ gp_status build_hierarchy(gp_layout *lay, Arena &A, Hierarchy *H, bool want_f32,
     A.release(L0.K.v[1], L0.K.nnz);
     A.release(L0.K.v[2], L0.K.nnz);
     L0.K.nv = 1;
+    // and the stiffness diagonal too: the smoother loads row i of u for K_eps
+    // and K_sig anyway, so K_S's diagonal entry would gather that row twice
+    GP_TRY(A.alloc(&L0.K.sdg, n));
+    k_diag<<<blocks_for(n), 256, 0, s>>>(n, L0.K.rp, L0.K.ci, L0.K.v[0], L0.K.sdg);
   }
   if (H->L.size() > 1) {
     GP_TRY(level0_order(lay, A, &H->ord_e, s));
@@ -529,7 +554,24 @@ gp_status build_hierarchy(gp_layout *lay, Arena &A, Hierarchy *H, bool want_f32,
   for (size_t l = 0; l + 1 < H->L.size(); ++l) {  // the coarsest level is solved densely
     MgLevel &L = H->L[l];
     const int32_t *oe = l == 0 ? H->ord_e : nullptr, *on = l == 0 ? H->ord_n : nullptr;
-    GP_TRY(sell_from_csr(A, L.K, &L.Ks, s, "L.K", oe));
+    if (l == 0 && L.K.sdg) {  // level 0: the SELL copy of K holds the off-diagonal entries
+      Arena tmp;
+      Csr K2 = L.K;
+      K2.sdg = nullptr; K2.dg[0] = K2.dg[1] = nullptr;
+      int64_t *cnt;
+      GP_TRY(tmp.alloc(&cnt, L.K.nr)); GP_TRY(tmp.alloc(&K2.rp, L.K.nr + 1));
+      k_offdiag_count<<<blocks_for(L.K.nr), 256, 0, s>>>(L.K.nr, L.K.rp, L.K.ci, cnt);
+      GP_TRY(exclusive_scan_i64(cnt, K2.rp, L.K.nr, s, tmp));
+      GP_HIP(hipMemcpyAsync(&K2.nnz, K2.rp + L.K.nr, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+      GP_HIP(hipStreamSynchronize(s));
+      GP_TRY(tmp.alloc(&K2.ci, std::max<int64_t>(K2.nnz, 1))); GP_TRY(tmp.alloc(&K2.v[0], std::max<int64_t>(K2.nnz, 1)));
+      k_offdiag_fill<<<blocks_for(L.K.nr), 256, 0, s>>>(L.K.nr, L.K.rp, L.K.ci, L.K.v[0], K2.rp, K2.ci, K2.v[0]);
+      GP_TRY(sell_from_csr(A, K2, &L.Ks, s, "L.K off-diagonal", oe));
+      GP_HIP(hipStreamSynchronize(s));
+      L.K.nsdg = L.K.nnz - K2.nnz;
+    } else {
+      GP_TRY(sell_from_csr(A, L.K, &L.Ks, s, "L.K", oe));
+    }
     GP_TRY(sell_from_csr(A, L.B, &L.Bs, s, "L.B", oe));
     GP_TRY(sell_from_csr(A, L.N, &L.Ns, s, "L.N", on));
     GP_TRY(sell_from_csr(A, L.GT, &L.GTs, s, "L.GT", on));
@@ -1174,12 +1216,18 @@ __global__ void __launch_bounds__(256) k_edge_sell(int64_t nr, int k, SellV<T> K
         }
       });
     }
-    if (K.dg[0]) {  // diagonal K_eps, K_sig
+    if (K.dg[0]) {  // diagonal K_eps, K_sig (and K_S when kept out of the entries)
       const T de = K.dg[0][i], ds = K.dg[1][i];
       C2<T> x[KS];
       load_slots<T, KS>(u + i * k + j0, x);
+      if (K.dgs) {
+        const T dsg = K.dgs[i];
 #pragma unroll
-      for (int q = 0; q < KS; ++q) csub_cx<T>(acc[q], c1[q] * de, c2[q] * ds, x[q]);
+        for (int q = 0; q < KS; ++q) csub_cx<T>(acc[q], fma(c1[q], de, dsg), c2[q] * ds, x[q]);
+      } else {
+#pragma unroll
+        for (int q = 0; q < KS; ++q) csub_cx<T>(acc[q], c1[q] * de, c2[q] * ds, x[q]);
+      }
     }
   }
   if (kind != EdgeMode::JACOBI0) {
@@ -1464,8 +1512,14 @@ __global__ void __launch_bounds__(kMarchThreads)
         march_lds_row<SW>(lds, z % 3, maxb, rr[n], jl, swp, self[n]);
         if (K.dg[0]) {
           const float de = K.dg[0][i[n]], ds = K.dg[1][i[n]];
+          if (K.dgs) {
+            const float dsg = K.dgs[i[n]];
 #pragma unroll
-          for (int q = 0; q < SPL; ++q) csub_cx<float>(acc[n][q], c1[q] * de, c2[q] * ds, self[n][q]);
+            for (int q = 0; q < SPL; ++q) csub_cx<float>(acc[n][q], fmaf(c1[q], de, dsg), c2[q] * ds, self[n][q]);
+          } else {
+#pragma unroll
+            for (int q = 0; q < SPL; ++q) csub_cx<float>(acc[n][q], c1[q] * de, c2[q] * ds, self[n][q]);
+          }
         }
       }
     // B phi, each row in entry order
@@ -2046,7 +2100,7 @@ template <>
 SellV<double> sview<double>(const Sell &c, const SellF &, const Csr &kc, const CsrF &) {
   SellV<double> v; v.nr = c.nr; v.nslice = c.nslice; v.nent = c.nent; v.base = c.base; v.w = c.w; v.ci = c.ci; v.nv = c.nv;
   for (int q = 0; q < c.nv; ++q) v.v[q] = c.v[q];
-  v.dg[0] = kc.dg[0]; v.dg[1] = kc.dg[1];
+  v.dg[0] = kc.dg[0]; v.dg[1] = kc.dg[1]; v.dgs = kc.sdg;
   v.rows = c.rows;
   return v;
 }
@@ -2054,7 +2108,7 @@ template <>
 SellV<float> sview<float>(const Sell &, const SellF &f, const Csr &, const CsrF &kf) {
   SellV<float> v; v.nr = f.nr; v.nslice = f.nslice; v.nent = f.nent; v.base = f.base; v.w = f.w; v.ci = f.ci; v.nv = f.nv;
   for (int q = 0; q < f.nv; ++q) v.v[q] = f.v[q];
-  v.dg[0] = kf.dg[0]; v.dg[1] = kf.dg[1];
+  v.dg[0] = kf.dg[0]; v.dg[1] = kf.dg[1]; v.dgs = kf.sdg;
   v.rows = f.rows;
   return v;
 }
@@ -2318,8 +2372,9 @@ static gp_status vcycle_level(const Hierarchy &H, VWork<T> &w, hipStream_t s, in
     double b = vb * ne * 2;                                                     // r in, out
     if (kind != EdgeMode::JACOBI0) {
       if (use_k) {
-        b += (double)L.K.nnz * (4 + L.K.nv * tb) + 8.0 * (ne + 1) + vb * ne;     // K, u
+        b += (double)(L.K.nnz - L.K.nsdg) * (4 + L.K.nv * tb) + 8.0 * (ne + 1) + vb * ne;  // K, u
         if (L.K.dg[0]) b += 2 * tb * ne;                                        // diagonal K_eps, K_sig
+        if (L.K.sdg) b += tb * ne;                                              // diagonal K_S
       }
       if (use_phi) b += (double)L.B.nnz * (4 + 2 * tb) + 8.0 * (ne + 1) + vb * nn;
     }

@@ -94,6 +94,7 @@ struct SellV {  // read-only view of a Sell in precision T
   const T *v[3] = {nullptr, nullptr, nullptr};
   int nv = 0;
   const T *dg[2] = {nullptr, nullptr};  // diagonal value blocks 1, 2 (level 0)
+  const T *dgs = nullptr;               // diagonal of block 0, kept out of the entries (level 0)
   const int32_t *rows = nullptr;        // slot position -> row (nullptr: identity)
 };
 

@@ -32,6 +32,10 @@ struct Csr {
   int32_t *lqp = nullptr;
   // when set, value blocks 1 and 2 are diagonal and live here (one per row)
   double *dg[2] = {nullptr, nullptr};
+  // when set (level 0), the diagonal of value block 0; the SELL copy then
+  // holds the off-diagonal entries only (nsdg diagonal entries moved here)
+  double *sdg = nullptr;
+  int64_t nsdg = 0;
 };
 
 // FP32 copy of a Csr's values; row pointers and columns are shared.
@@ -47,6 +51,7 @@ struct CsrF {
   const int64_t *lcb = nullptr, *lce = nullptr;
   const int32_t *lqp = nullptr;
   float *dg[2] = {nullptr, nullptr};
+  float *sdg = nullptr;
 };
 
 // Sliced ELL with 64-row slices (one wavefront): slice s holds w[s] entry
