@@ -18,6 +18,7 @@ P = C.POINTER
 GP_OK, GP_ERR_ARG, GP_ERR_HIP, GP_ERR_NOMEM, GP_ERR_NOCONV, GP_ERR_STATE, GP_ERR_NODEVICE = range(7)
 GP_GEN_MICROSTRIP, GP_GEN_MANHATTAN, GP_GEN_PLATE = 0, 1, 2
 GP_PREC_FP64, GP_PREC_MIXED = 0, 1
+GP_SMOOTH_L1_JACOBI, GP_SMOOTH_CHEBYSHEV = 0, 1
 
 
 class GpMesh(C.Structure):
@@ -189,6 +190,7 @@ SIGNATURES = [
     ("gp_server_destroy", None, [C.c_void_p]),
     ("gp_extract_z", i32, [P(GpMesh), P(f64), i32, P(f64)]),
     ("gp_solve_real", i32, [P(i64), P(i64), P(f64), i64, i64, P(f64), P(f64), f64, P(i32), P(f64)]),
+    ("gp_solve_real_ex", i32, [P(i64), P(i64), P(f64), i64, i64, P(f64), P(f64), f64, i32, P(i32), P(f64)]),
     ("gp_solve_v0", i32, [C.c_void_p, f64, P(f64), P(f64), P(GpV0Stats)]),
     ("gp_z_v0_sweep", i32, [P(GpMesh), P(f64)]),
     ("gp_find_vh", i32, [C.c_void_p, i32, f64, f64, P(f64), P(f64), i64, P(f64), P(GpVhInfo)]),

@@ -15,8 +15,12 @@
 //     4/(3 lambda) D^-1 A (lambda from power iteration), truncated (entries
 //     below kTrunc of the row's largest dropped, the row's own aggregate
 //     kept, the row sum restored), Galerkin R A P;
-//   * l1-Jacobi smoothing, two sweeps each side; dense FP64 inverse on the
-//     coarsest level (<= 512 unknowns).
+//   * smoothing, two matrix products each side: l1-Jacobi sweeps (the
+//     default) or two steps of the Chebyshev recurrence on D^-1 A over
+//     [0.3 b, b], b = 1.1 lambda (gp_solve_real_ex; the reference picks
+//     BoomerAMG's relaxation with HYPRE_BoomerAMGSetRelaxType,
+//     src/hypreSolve.cpp:230); dense FP64 inverse on the coarsest level
+//     (<= 512 unknowns).
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -33,6 +37,13 @@ namespace {
 constexpr double kTheta = 0.25;
 constexpr double kTrunc = 0.2;
 constexpr int64_t kCoarsest = 512;
+// Chebyshev smoothing: steps per side (the matrix products of two Jacobi
+// sweeps) and its interval [kChebLower b, b] on the spectrum of D^-1 A, with
+// b = kChebUpper times the power-iteration estimate of its largest eigenvalue
+// (the estimate approaches it from below)
+constexpr int kChebSteps = 2;
+constexpr double kChebUpper = 1.1;
+constexpr double kChebLower = 0.3;
 
 __global__ void k_r_diag(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
                          const double *__restrict__ v, double *__restrict__ d, double *__restrict__ dl1) {
@@ -221,6 +232,53 @@ __global__ void k_r_jacobi_long(const int32_t *__restrict__ lrows, const int64_t
   if (threadIdx.x == 0) xo[i] = x[i] + (r[i] - s) / dl1[i];
 }
 
+// Chebyshev smoothing (the three-term recurrence written with a direction
+// vector dd, Saad, Iterative Methods for Sparse Linear Systems, 2nd ed.,
+// Alg. 12.1, preconditioned with D^-1), first step from x = 0:
+// dd = c2 D^-1 r, xo = dd
+__global__ void k_r_cheb0(int64_t n, const double *__restrict__ dg, const double *__restrict__ r, double c2,
+                          double *__restrict__ dd, double *__restrict__ xo) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double q = dg[i] != 0. ? c2 * r[i] / dg[i] : 0.;
+  dd[i] = q;
+  xo[i] = q;
+}
+
+// later steps: dd = c1 dd + c2 D^-1 (r - A x), xo = x + dd (c1 = 0 starts the
+// recurrence from a nonzero x; dd is then not read)
+__device__ __forceinline__ void cheb_row(int64_t i, double s, const double *__restrict__ dg,
+                                         const double *__restrict__ r, const double *__restrict__ x, double c1,
+                                         double c2, double *__restrict__ dd, double *__restrict__ xo) {
+  double q = dg[i] != 0. ? c2 * (r[i] - s) / dg[i] : 0.;
+  if (c1 != 0.) q += c1 * dd[i];
+  dd[i] = q;
+  xo[i] = x[i] + q;
+}
+
+__global__ void k_r_cheb(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                         const double *__restrict__ v, const double *__restrict__ dg, const double *__restrict__ r,
+                         const double *__restrict__ x, double c1, double c2, double *__restrict__ dd,
+                         double *__restrict__ xo) {
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kRLanes;
+  const int lane = threadIdx.x & (kRLanes - 1);
+  if (i >= n) return;
+  const int64_t b = rp[i], e = rp[i + 1];
+  if (e - b > kRLong) return;
+  const double s = row_dot8(b, e, lane, ci, v, x);
+  if (lane == 0) cheb_row(i, s, dg, r, x, c1, c2, dd, xo);
+}
+
+__global__ void k_r_cheb_long(const int32_t *__restrict__ lrows, const int64_t *__restrict__ rp,
+                              const int32_t *__restrict__ ci, const double *__restrict__ v,
+                              const double *__restrict__ dg, const double *__restrict__ r,
+                              const double *__restrict__ x, double c1, double c2, double *__restrict__ dd,
+                              double *__restrict__ xo) {
+  const int64_t i = lrows[blockIdx.x];
+  const double s = row_dot_block(rp[i], rp[i + 1], ci, v, x);
+  if (threadIdx.x == 0) cheb_row(i, s, dg, r, x, c1, c2, dd, xo);
+}
+
 // x = inv r on the coarsest level, one wavefront per row (coalesced rows)
 __global__ void k_r_dense(int64_t n, const double *__restrict__ inv, const double *__restrict__ r, double *__restrict__ x) {
   const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
@@ -297,12 +355,16 @@ struct RLevel {
   Csr A, P, R;
   double *d = nullptr, *dl1 = nullptr;
   double *r = nullptr, *x = nullptr, *t = nullptr;
+  // Chebyshev smoothing: direction vector and the coefficients of each step
+  double *dd = nullptr;
+  double c1[kChebSteps] = {}, c2[kChebSteps] = {};
   // rows longer than kRLong of A, P and R (device lists) and their counts
   int32_t *la = nullptr, *lp = nullptr, *lr = nullptr;
   int64_t nla = 0, nlp = 0, nlr = 0;
 };
 
 struct RealAMG {
+  int smoother = GP_SMOOTH_L1_JACOBI;
   std::vector<RLevel> L;
   double *inv = nullptr;  // coarsest, dense n x n
   // FGMRES workspace: basis V (m+1 vectors), preconditioned basis Z (m),
@@ -344,6 +406,34 @@ void jacobi_r(const RLevel &L, const double *x, double *xo, hipStream_t s) {
   if (!x) { k_r_jacobi0<<<blocks_for(n), 256, 0, s>>>(n, L.dl1, L.r, xo); return; }
   k_r_jacobi<<<blocks_for(n * kRLanes), 256, 0, s>>>(n, L.A.rp, L.A.ci, L.A.v[0], L.dl1, L.r, x, xo);
   if (L.nla) k_r_jacobi_long<<<(unsigned)L.nla, 256, 0, s>>>(L.la, L.A.rp, L.A.ci, L.A.v[0], L.dl1, L.r, x, xo);
+}
+
+// the Chebyshev coefficients for lambda, the estimate of D^-1 A's largest
+// eigenvalue: interval [alpha, beta], theta its centre, delta its half width,
+// sigma = theta / delta; step 0: c1 = 0, c2 = 1 / theta; step k: rho_k =
+// 1 / (2 sigma - rho_(k-1)) from rho_0 = 1 / sigma, c1 = rho_k rho_(k-1),
+// c2 = 2 rho_k / delta
+void cheb_coeffs(double lam, double *c1, double *c2) {
+  const double beta = kChebUpper * lam, alpha = kChebLower * beta;
+  const double theta = 0.5 * (beta + alpha), delta = 0.5 * (beta - alpha), sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  c1[0] = 0.;
+  c2[0] = 1.0 / theta;
+  for (int k = 1; k < kChebSteps; ++k) {
+    const double rn = 1.0 / (2.0 * sigma - rho);
+    c1[k] = rn * rho;
+    c2[k] = 2.0 * rn / delta;
+    rho = rn;
+  }
+}
+
+// step k of the Chebyshev smoother from x (null: x = 0, step 0 only) into xo
+void cheb_r(const RLevel &L, int k, const double *x, double *xo, hipStream_t s) {
+  const int64_t n = L.A.nr;
+  if (!x) { k_r_cheb0<<<blocks_for(n), 256, 0, s>>>(n, L.d, L.r, L.c2[0], L.dd, xo); return; }
+  k_r_cheb<<<blocks_for(n * kRLanes), 256, 0, s>>>(n, L.A.rp, L.A.ci, L.A.v[0], L.d, L.r, x, L.c1[k], L.c2[k], L.dd, xo);
+  if (L.nla)
+    k_r_cheb_long<<<(unsigned)L.nla, 256, 0, s>>>(L.la, L.A.rp, L.A.ci, L.A.v[0], L.d, L.r, x, L.c1[k], L.c2[k], L.dd, xo);
 }
 
 // one pairwise matching pass on M: tentative P (n x ncoarse)
@@ -426,6 +516,10 @@ gp_status build_real_amg(RealAMG &G, Csr A0, hipStream_t s) {
       if (Pt.nc > (int64_t)(0.8 * (double)n)) { st = GP_ERR_NOCONV; break; }  // stalled: stop coarsening
       // P = Pt - 4/(3 lambda) D^-1 A Pt
       const double lam = power_lambda(tmp, F.A, F.d, s);
+      if (G.smoother == GP_SMOOTH_CHEBYSHEV) {
+        cheb_coeffs(lam, F.c1, F.c2);
+        if ((st = A.alloc(&F.dd, n)) != GP_OK) break;
+      }
       if ((st = csr_alloc(tmp, n, n, F.A.nnz, 1, &DA)) != GP_OK) break;
       if (hipMemcpyAsync(DA.rp, F.A.rp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToDevice, s) != hipSuccess ||
           hipMemcpyAsync(DA.ci, F.A.ci, sizeof(int32_t) * F.A.nnz, hipMemcpyDeviceToDevice, s) != hipSuccess) { st = fail(GP_ERR_HIP, "copy"); break; }
@@ -478,15 +572,17 @@ void vcycle_real(const RealAMG &G, int l, hipStream_t s) {
     k_r_dense<<<blocks_for(n * 64), 256, 0, s>>>(n, G.inv, L.r, L.x);
     return;
   }
-  jacobi_r(L, nullptr, L.x, s);
-  jacobi_r(L, L.x, L.t, s);
+  static_assert(kChebSteps == 2, "the smoothing steps alternate between x and t twice");
+  const bool cheb = G.smoother == GP_SMOOTH_CHEBYSHEV;
+  if (cheb) { cheb_r(L, 0, nullptr, L.x, s); cheb_r(L, 1, L.x, L.t, s); }
+  else { jacobi_r(L, nullptr, L.x, s); jacobi_r(L, L.x, L.t, s); }
   spmv_r(L.A, L.la, L.nla, L.t, L.r, L.x, 1, s);      // x <- r - A t
   const RLevel &C = G.L[l + 1];
   spmv_r(L.R, L.lr, L.nlr, L.x, nullptr, C.r, 0, s);
   vcycle_real(G, l + 1, s);
   spmv_r(L.P, L.lp, L.nlp, C.x, nullptr, L.t, 2, s);  // t += P xc
-  jacobi_r(L, L.t, L.x, s);
-  jacobi_r(L, L.x, L.t, s);
+  if (cheb) { cheb_r(L, 0, L.t, L.x, s); cheb_r(L, 1, L.x, L.t, s); }
+  else { jacobi_r(L, L.t, L.x, s); jacobi_r(L, L.x, L.t, s); }
   (void)hipMemcpyAsync(L.x, L.t, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
 }
 
@@ -508,11 +604,13 @@ constexpr int kRestart = 30;
 }  // namespace
 
 gp_status real_amg_create(int64_t n, int64_t nnz, const int64_t *rows, const int64_t *cols, const double *vals,
-                          RealAMG **out, hipStream_t s) {
+                          int smoother, RealAMG **out, hipStream_t s) {
   *out = nullptr;
   if (n < 1 || nnz < 1) return fail(GP_ERR_ARG, "empty real matrix");
   if (n > 2147483647LL) return fail(GP_ERR_ARG, "more than 2^31 unknowns");
+  if (smoother != GP_SMOOTH_L1_JACOBI && smoother != GP_SMOOTH_CHEBYSHEV) return fail(GP_ERR_ARG, "unknown real smoother");
   RealAMG *G = new RealAMG();
+  G->smoother = smoother;
   Csr A0;
   const double *vv[1] = {vals};
   gp_status st = csr_from_coo(G->A, n, n, nnz, rows, cols, vv, 1, &A0, s);
@@ -617,7 +715,14 @@ using namespace gp;
 
 extern "C" gp_status gp_solve_real(const int64_t *row, const int64_t *col, const double *val, int64_t nnz, int64_t n,
                                    const double *b, double *x, double rtol, int32_t *iters_out, double *relres_out) {
+  return gp_solve_real_ex(row, col, val, nnz, n, b, x, rtol, GP_SMOOTH_L1_JACOBI, iters_out, relres_out);
+}
+
+extern "C" gp_status gp_solve_real_ex(const int64_t *row, const int64_t *col, const double *val, int64_t nnz,
+                                      int64_t n, const double *b, double *x, double rtol, int32_t smoother,
+                                      int32_t *iters_out, double *relres_out) {
   if (!row || !col || !val || !b || !x || n < 1 || nnz < 1) return fail(GP_ERR_ARG, "bad real solve arguments");
+  if (smoother != GP_SMOOTH_L1_JACOBI && smoother != GP_SMOOTH_CHEBYSHEV) return fail(GP_ERR_ARG, "unknown real smoother");
   if (n > 2147483647LL) return fail(GP_ERR_ARG, "more than 2^31 unknowns");
   for (int64_t q = 0; q < nnz; ++q)
     if (row[q] < 0 || row[q] >= n || col[q] < 0 || col[q] >= n) return fail(GP_ERR_ARG, "COO index out of range");
@@ -637,7 +742,7 @@ extern "C" gp_status gp_solve_real(const int64_t *row, const int64_t *col, const
   GP_HIP(hipMemcpyAsync(dv, val, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
   GP_HIP(hipMemcpyAsync(bd, b, sizeof(double) * n, hipMemcpyHostToDevice, s));
   RealAMG *G = nullptr;
-  GP_TRY(real_amg_create(n, nnz, dr, dc, dv, &G, s));
+  GP_TRY(real_amg_create(n, nnz, dr, dc, dv, smoother, &G, s));
   struct GGuard { RealAMG *g; ~GGuard() { real_amg_destroy(g); } } gg{G};
   gp_status st = real_amg_solve(G, bd, xd, tol, 1000, iters_out, relres_out, s);
   if (st != GP_OK && st != GP_ERR_NOCONV) return st;

@@ -10,9 +10,10 @@ namespace gp {
 struct RealAMG;
 
 // Builds the hierarchy of the n x n matrix given as device COO triplets
-// (duplicates summed).  The solver owns its copy of the matrix.
+// (duplicates summed), smoothing with smoother (gp_real_smoother).  The
+// solver owns its copy of the matrix.
 gp_status real_amg_create(int64_t n, int64_t nnz, const int64_t *rows, const int64_t *cols, const double *vals,
-                          RealAMG **out, hipStream_t s);
+                          int smoother, RealAMG **out, hipStream_t s);
 // x = A^-1 b (device vectors of n) by FGMRES(30) preconditioned with one
 // V-cycle, from x = 0, until ||b - A x|| <= tol ||b|| or maxit iterations.
 // Returns GP_ERR_NOCONV when the tolerance is not met (x holds the last iterate).

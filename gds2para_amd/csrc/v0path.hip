@@ -46,6 +46,9 @@ int64_t v0path_bytes(const V0Path *p);
 
 namespace {
 
+// smoother of the Ad and Ac multigrid (gp_real_smoother)
+constexpr int kV0Smoother = GP_SMOOTH_L1_JACOBI;
+
 // y = A x (mode 0), y = -(A x) (mode 1), y = y + A x (mode 2); one wavefront
 // per row, so the long rows of V0d^T (the conductor aggregate columns of V0d)
 // are split over 64 lanes
@@ -118,8 +121,9 @@ gp_status setup(gp_layout *lay, V0Path **out) {
     }
     // Ad: the kept entries only; the reference's trailing (0, 0, 0) slots are
     // never read by hypreSolve's row-by-row insertion (src/hypreSolve.cpp:47-59)
-    if ((st = real_amg_create(P->nd, V.ad.n, V.ad.row, V.ad.col, V.ad.val, &P->ad, s)) != GP_OK) break;
-    if (P->nc > 0 && (st = real_amg_create(P->nc, V.ac.n, V.ac.row, V.ac.col, V.ac.val, &P->ac, s)) != GP_OK) break;
+    if ((st = real_amg_create(P->nd, V.ad.n, V.ad.row, V.ad.col, V.ad.val, kV0Smoother, &P->ad, s)) != GP_OK) break;
+    if (P->nc > 0 && (st = real_amg_create(P->nc, V.ac.n, V.ac.row, V.ac.col, V.ac.val, kV0Smoother, &P->ac, s)) != GP_OK)
+      break;
     if (hipStreamSynchronize(s) != hipSuccess) { st = fail(GP_ERR_HIP, "V0 path setup"); break; }
   } while (0);
   if (st != GP_OK) { delete P; return st; }

@@ -251,11 +251,14 @@ def z_v0_sweep(mesh, Z0):
     return x[0::2] + 1j * x[1::2]
 
 
-def solve_real(row, col, val, n, b, rtol=0.0):
+def solve_real(row, col, val, n, b, rtol=0.0, smoother=None):
     """hypreSolve on the device (gp_solve_real): real A x = b, A as COO triplets.
 
-    rtol = 0 selects the reference's 1e-4.  Returns (x, iterations, relative
-    residual); raises GpError unless the tolerance is met."""
+    rtol = 0 selects the reference's 1e-4.  smoother (GP_SMOOTH_L1_JACOBI or
+    GP_SMOOTH_CHEBYSHEV) picks the multigrid's relaxation through
+    gp_solve_real_ex; None keeps gp_solve_real's l1-Jacobi.  Returns (x,
+    iterations, relative residual); raises GpError unless the tolerance is
+    met."""
     lib = capi.load()
     row = np.ascontiguousarray(row, np.int64)
     col = np.ascontiguousarray(col, np.int64)
@@ -263,7 +266,12 @@ def solve_real(row, col, val, n, b, rtol=0.0):
     b = np.ascontiguousarray(b, np.float64)
     x = np.zeros(int(n))
     it, rr = C.c_int32(), C.c_double()
-    capi.check(lib.gp_solve_real(_p(row, i64p), _p(col, i64p), _p(val, f64p), len(val), int(n), _p(b, f64p),
-                                 _p(x, f64p), float(rtol), C.byref(it), C.byref(rr)), "gp_solve_real")
+    if smoother is None:
+        capi.check(lib.gp_solve_real(_p(row, i64p), _p(col, i64p), _p(val, f64p), len(val), int(n), _p(b, f64p),
+                                     _p(x, f64p), float(rtol), C.byref(it), C.byref(rr)), "gp_solve_real")
+    else:
+        capi.check(lib.gp_solve_real_ex(_p(row, i64p), _p(col, i64p), _p(val, f64p), len(val), int(n),
+                                        _p(b, f64p), _p(x, f64p), float(rtol), int(smoother), C.byref(it),
+                                        C.byref(rr)), "gp_solve_real_ex")
     return x, it.value, rr.value
 

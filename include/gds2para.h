@@ -16,6 +16,7 @@
  *   gp_apply_fullwave       reference()'s A(w), applied             src/generateStiff.cpp:583-626
  *   gp_solve_real           int hypreSolve(fdtdMesh*, myint*, myint*, double*, myint,
  *                                          double*, myint, double*)  src/hypreSolver.h:14
+ *   gp_solve_real_ex        hypreSolve with HYPRE_BoomerAMGSetRelaxType's choice  src/hypreSolve.cpp:230
  *   gp_solve_v0             paraGenerator's V0 section + Construct_Z_V0   src/matrixCon.cpp:366-560,
  *                                                                   src/fdtd.hpp:4178
  *   gp_z_v0_sweep           fdtdMesh::print_z_V0 (its store)        src/fdtd.hpp:4249
@@ -466,6 +467,23 @@ gp_status gp_extract_z(const gp_mesh *mesh, const double *x, int32_t source_port
 gp_status gp_solve_real(const int64_t *row, const int64_t *col, const double *val, int64_t nnz,
                         int64_t n, const double *b, double *x, double rtol,
                         int32_t *iters_out, double *relres_out);
+
+/* Smoother of the real solve's multigrid, chosen per solve as the reference
+ * chooses BoomerAMG's with HYPRE_BoomerAMGSetRelaxType (src/hypreSolve.cpp:230,
+ * hybrid symmetric Gauss-Seidel there).  Both do two matrix products per side
+ * of each level. */
+typedef enum gp_real_smoother {
+  GP_SMOOTH_L1_JACOBI = 0,        /* l1-Jacobi sweeps (gp_solve_real's)           */
+  GP_SMOOTH_CHEBYSHEV = 1         /* Chebyshev steps on D^-1 A, interval [0.3 b, b],
+                                     b = 1.1 x its power-iteration lambda_max    */
+} gp_real_smoother;
+
+/* gp_solve_real with the smoother given (gp_real_smoother; anything else is
+ * GP_ERR_ARG).  Same interface as hypreSolve (src/hypreSolve.cpp:9) plus the
+ * relaxation choice. */
+gp_status gp_solve_real_ex(const int64_t *row, const int64_t *col, const double *val, int64_t nnz,
+                           int64_t n, const double *b, double *x, double rtol, int32_t smoother,
+                           int32_t *iters_out, double *relres_out);
 
 /* ---- V0 quasi-static path (paraGenerator's V0 section) ------------------ */
 typedef struct gp_v0_stats {
