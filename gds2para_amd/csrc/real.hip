@@ -45,18 +45,47 @@ constexpr int kChebSteps = 2;
 constexpr double kChebUpper = 1.1;
 constexpr double kChebLower = 0.3;
 
+// The setup kernels below give each row one wavefront (kWave lanes striding
+// over its entries): Ad's conductor-aggregate rows couple to every dielectric
+// node around a conductor, and one thread per row left each launch waiting
+// on those few rows.  The wavefronts stride over the rows, so any n fits the
+// grid of wave_blocks(n).
+constexpr int kWave = 64;
+
+unsigned wave_blocks(int64_t n) {
+  return (unsigned)std::min<int64_t>(std::max<int64_t>((n * kWave + 255) / 256, 1), 1 << 20);
+}
+
+__device__ __forceinline__ int64_t wave_first() { return ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; }
+__device__ __forceinline__ int64_t wave_count() { return ((int64_t)gridDim.x * blockDim.x) / kWave; }
+
+__device__ __forceinline__ double wave_sum(double x) {
+  for (int o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
+
+__device__ __forceinline__ double wave_max(double x) {
+  for (int o = kWave / 2; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o));
+  return x;
+}
+
 __global__ void k_r_diag(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
                          const double *__restrict__ v, double *__restrict__ d, double *__restrict__ dl1) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double a = 0., off = 0.;
-  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) {
-    if (ci[p] == (int32_t)i) a += v[p];
-    else off += fabs(v[p]);
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int64_t i = wave_first(); i < n; i += wave_count()) {  // i is the same on every lane
+    double a = 0., off = 0.;
+    for (int64_t p = rp[i] + lane; p < rp[i + 1]; p += kWave) {
+      if (ci[p] == (int32_t)i) a += v[p];
+      else off += fabs(v[p]);
+    }
+    a = wave_sum(a);
+    off = wave_sum(off);
+    if (lane == 0) {
+      d[i] = a;
+      const double m = fabs(a);
+      dl1[i] = m > 0 ? a * (1.0 + off / m) : (off > 0 ? off : 1.0);
+    }
   }
-  d[i] = a;
-  const double m = fabs(a);
-  dl1[i] = m > 0 ? a * (1.0 + off / m) : (off > 0 ? off : 1.0);
 }
 
 // symmetric perturbation in [1, 1 + 2^-10) of the strength of edge {i, j}:
@@ -70,34 +99,44 @@ __device__ __forceinline__ double edge_jitter(int64_t i, int64_t j) {
   return 1.0 + (double)(h >> 11) * (1.0 / 9007199254740992.0) * (1.0 / 1024.0);
 }
 
-// each unmatched node picks its strongest unmatched strong neighbour
+// each unmatched node picks its strongest unmatched strong neighbour (the
+// largest jittered score, the smallest index among equal ones: max and the
+// comparisons are exact, so the lanes' partial picks combine to the pick of
+// one pass over the row)
 __global__ void k_pick(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
                        const double *__restrict__ v, const double *__restrict__ d, const int32_t *__restrict__ matched,
                        int32_t *__restrict__ pick) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int32_t best = -1;
-  double bs = 0.;
-  if (!matched[i]) {
-    double rmax = 0.;
-    for (int64_t p = rp[i]; p < rp[i + 1]; ++p) {
-      const int32_t j = ci[p];
-      if (j == (int32_t)i) continue;
-      const double den = sqrt(fabs(d[i] * d[j]));
-      rmax = fmax(rmax, den > 0 ? fabs(v[p]) / den : 0.);
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int64_t i = wave_first(); i < n; i += wave_count()) {  // i is the same on every lane
+    int32_t best = -1;
+    double bs = 0.;
+    if (!matched[i]) {
+      const int64_t b = rp[i], e = rp[i + 1];
+      double rmax = 0.;
+      for (int64_t p = b + lane; p < e; p += kWave) {
+        const int32_t j = ci[p];
+        if (j == (int32_t)i) continue;
+        const double den = sqrt(fabs(d[i] * d[j]));
+        rmax = fmax(rmax, den > 0 ? fabs(v[p]) / den : 0.);
+      }
+      const double thr = kTheta * wave_max(rmax);
+      for (int64_t p = b + lane; p < e; p += kWave) {
+        const int32_t j = ci[p];
+        if (j == (int32_t)i || matched[j]) continue;
+        const double den = sqrt(fabs(d[i] * d[j]));
+        const double sc0 = den > 0 ? fabs(v[p]) / den : 0.;
+        if (!(sc0 > 0.) || sc0 < thr) continue;
+        const double sc = sc0 * edge_jitter(i, j);
+        if (sc > bs || (sc == bs && j < best)) { bs = sc; best = j; }
+      }
+      for (int o = kWave / 2; o > 0; o >>= 1) {
+        const double s2 = __shfl_xor(bs, o);
+        const int32_t j2 = __shfl_xor(best, o);
+        if (s2 > bs || (s2 == bs && j2 < best)) { bs = s2; best = j2; }
+      }
     }
-    const double thr = kTheta * rmax;
-    for (int64_t p = rp[i]; p < rp[i + 1]; ++p) {
-      const int32_t j = ci[p];
-      if (j == (int32_t)i || matched[j]) continue;
-      const double den = sqrt(fabs(d[i] * d[j]));
-      const double sc0 = den > 0 ? fabs(v[p]) / den : 0.;
-      if (!(sc0 > 0.) || sc0 < thr) continue;
-      const double sc = sc0 * edge_jitter(i, j);
-      if (sc > bs || (sc == bs && j < best)) { bs = sc; best = j; }
-    }
+    if (lane == 0) pick[i] = best;
   }
-  pick[i] = best;
 }
 
 // mutual picks become pairs
@@ -136,11 +175,13 @@ __global__ void k_ptent_pairs(int64_t n, const int32_t *__restrict__ partner, co
 __global__ void k_r_dinv_spmv(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
                               const double *__restrict__ v, const double *__restrict__ d, const double *__restrict__ x,
                               double *__restrict__ y) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double s = 0.;
-  for (int64_t p = rp[i]; p < rp[i + 1]; ++p) s += v[p] * x[ci[p]];
-  y[i] = d[i] != 0. ? s / d[i] : 0.;
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int64_t i = wave_first(); i < n; i += wave_count()) {  // i is the same on every lane
+    double s = 0.;
+    for (int64_t p = rp[i] + lane; p < rp[i + 1]; p += kWave) s += v[p] * x[ci[p]];
+    s = wave_sum(s);
+    if (lane == 0) y[i] = d[i] != 0. ? s / d[i] : 0.;
+  }
 }
 
 // rows of A scaled by f / d_i (the smoothing matrix of the prolongation)
@@ -447,7 +488,7 @@ gp_status match_pass(Arena &A, Arena &tmp, const Csr &M, const double *d, Csr *P
   GP_HIP(hipMemsetAsync(mnext, 0, sizeof(int32_t) * n, s));
   GP_HIP(hipMemsetAsync(partner, 0xff, sizeof(int32_t) * n, s));  // -1
   for (int round = 0; round < 6; ++round) {
-    k_pick<<<blocks_for(n), 256, 0, s>>>(n, M.rp, M.ci, M.v[0], d, matched, pick);
+    k_pick<<<wave_blocks(n), 256, 0, s>>>(n, M.rp, M.ci, M.v[0], d, matched, pick);
     k_pair<<<blocks_for(n), 256, 0, s>>>(n, pick, partner, mnext);
     k_copy_i32<<<blocks_for(n), 256, 0, s>>>(n, mnext, matched);
   }
@@ -471,7 +512,7 @@ double power_lambda(Arena &tmp, const Csr &M, const double *d, hipStream_t s) {
   (void)hipMemcpyAsync(x, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, s);
   double lam = 1.0;
   for (int it = 0; it < 12; ++it) {
-    k_r_dinv_spmv<<<blocks_for(n), 256, 0, s>>>(n, M.rp, M.ci, M.v[0], d, x, y);
+    k_r_dinv_spmv<<<wave_blocks(n), 256, 0, s>>>(n, M.rp, M.ci, M.v[0], d, x, y);
     (void)hipMemcpyAsync(h.data(), y, sizeof(double) * n, hipMemcpyDeviceToHost, s);
     (void)hipStreamSynchronize(s);
     double nr = 0;
@@ -495,7 +536,7 @@ gp_status build_real_amg(RealAMG &G, Csr A0, hipStream_t s) {
     RLevel &F = G.L.back();
     const int64_t n = F.A.nr;
     GP_TRY(A.alloc(&F.d, n)); GP_TRY(A.alloc(&F.dl1, n));
-    k_r_diag<<<blocks_for(n), 256, 0, s>>>(n, F.A.rp, F.A.ci, F.A.v[0], F.d, F.dl1);
+    k_r_diag<<<wave_blocks(n), 256, 0, s>>>(n, F.A.rp, F.A.ci, F.A.v[0], F.d, F.dl1);
     GP_TRY(A.alloc(&F.r, n)); GP_TRY(A.alloc(&F.x, n)); GP_TRY(A.alloc(&F.t, n));
     GP_TRY(long_rows(A, F.A, &F.la, &F.nla, s));
     if (n <= kCoarsest || G.L.size() >= 24) break;
@@ -510,7 +551,7 @@ gp_status build_real_amg(RealAMG &G, Csr A0, hipStream_t s) {
       if ((st = spgemm(tmp, R1, AP1, &M2, s)) != GP_OK) break;
       double *d2 = nullptr, *l2 = nullptr;
       if ((st = tmp.alloc(&d2, M2.nr)) != GP_OK || (st = tmp.alloc(&l2, M2.nr)) != GP_OK) break;
-      k_r_diag<<<blocks_for(M2.nr), 256, 0, s>>>(M2.nr, M2.rp, M2.ci, M2.v[0], d2, l2);
+      k_r_diag<<<wave_blocks(M2.nr), 256, 0, s>>>(M2.nr, M2.rp, M2.ci, M2.v[0], d2, l2);
       if ((st = match_pass(tmp, tmp, M2, d2, &P2, s)) != GP_OK) break;
       if ((st = spgemm(tmp, P1, P2, &Pt, s)) != GP_OK) break;
       if (Pt.nc > (int64_t)(0.8 * (double)n)) { st = GP_ERR_NOCONV; break; }  // stalled: stop coarsening
