@@ -61,3 +61,28 @@ def test_struct_layout_matches_header(c_layout, cname):
     assert C.sizeof(py) == c_layout[(cname, ".")], (cname, C.sizeof(py), c_layout[(cname, ".")])
     for f in py._fields_:
         assert getattr(py, f[0]).offset == c_layout[(cname, f[0])], (cname, f[0])
+
+
+def test_enum_values_match_header(tmp_path):
+    """Every enumerator of the header's enums has a constant of the same name
+    and value in _capi.py (gcc prints the header's values)."""
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    header = open(os.path.join(ROOT, "include", "gds2para.h")).read()
+    names = []
+    for body in re.findall(r"^typedef enum(?: gp_[a-z0-9_]+)? \{(.*?)^\} gp_[a-z0-9_]+;", header, flags=re.M | re.S):
+        body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+        names += re.findall(r"\b(GP_[A-Z0-9_]+)\b", body)
+    assert names
+    lines = ['#include <stdio.h>', '#include "gds2para.h"', "int main(void) {"]
+    lines += [f'  printf("{nm} %d\\n", (int){nm});' for nm in names]
+    lines += ["  return 0;", "}"]
+    src, exe = tmp_path / "enums.c", tmp_path / "enums"
+    src.write_text("\n".join(lines) + "\n")
+    subprocess.run([cc, "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    for line in out.splitlines():
+        nm, value = line.split()
+        assert hasattr(capi, nm), f"{nm} is not mirrored in _capi.py"
+        assert getattr(capi, nm) == int(value), (nm, getattr(capi, nm), value)
