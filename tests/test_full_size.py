@@ -33,6 +33,8 @@ def test_full_size_reciprocity_and_v0_agreement():
         Zs[f] = Z[0]
     print("[full] V0 path", flush=True)
     Z0, X, vst = lay.solve_v0(rtol=1e-10)
+    print(f"[full] V0 path: setup {vst['setup_ms']:.0f} ms, {vst['solves']} solves in {vst['solve_ms']:.0f} ms, "
+          f"max {vst['max_iters']} iterations", flush=True)
     assert vst["max_rel_residual"] <= 1e-10 and vst["solves"] == 3 * npt
     Z0m = Z0.reshape(npt, npt)
     assert np.abs(Z0m - Z0m.T).max() / np.abs(Z0m).max() < 1e-9
