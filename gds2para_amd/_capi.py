@@ -192,6 +192,7 @@ SIGNATURES = [
     ("gp_solve_real", i32, [P(i64), P(i64), P(f64), i64, i64, P(f64), P(f64), f64, P(i32), P(f64)]),
     ("gp_solve_real_ex", i32, [P(i64), P(i64), P(f64), i64, i64, P(f64), P(f64), f64, i32, P(i32), P(f64)]),
     ("gp_solve_v0", i32, [C.c_void_p, f64, P(f64), P(f64), P(GpV0Stats)]),
+    ("gp_v0_set_smoother", i32, [C.c_void_p, i32]),
     ("gp_z_v0_sweep", i32, [P(GpMesh), P(f64)]),
     ("gp_find_vh", i32, [C.c_void_p, i32, f64, f64, P(f64), P(f64), i64, P(f64), P(GpVhInfo)]),
     ("gp_hessenberg_eigenvalues", i32, [i32, P(f64), P(f64)]),

@@ -149,8 +149,10 @@ struct gp_layout {
   gp::CsrD S;
   // V0 basis
   gp::V0Data *v0 = nullptr;
-  // V0 quasi-static path: V0 operators in CSR and the Ad / Ac hierarchies
+  // V0 quasi-static path: V0 operators in CSR and the Ad / Ac hierarchies,
+  // built with the smoother v0_smoother names (gp_real_smoother)
   gp::V0Path *v0path = nullptr;
+  int32_t v0_smoother = GP_SMOOTH_CHEBYSHEV;
   // solver hierarchy
   gp::Solver *solver = nullptr;
 };

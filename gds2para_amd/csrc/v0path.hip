@@ -32,6 +32,7 @@ struct V0Path {
   Arena A;
   Csr Vd, VdT, Vc, VcT;            // v[0] = V0 values, v[1] = the test ("a") values
   RealAMG *ad = nullptr, *ac = nullptr;
+  int smoother = GP_SMOOTH_CHEBYSHEV;  // of ad and ac
   int64_t nd = 0, nc = 0, ne = 0;
   double setup_ms = 0.;
   ~V0Path() {
@@ -45,9 +46,6 @@ void destroy_v0path(V0Path *p) { delete p; }
 int64_t v0path_bytes(const V0Path *p);
 
 namespace {
-
-// smoother of the Ad and Ac multigrid (gp_real_smoother)
-constexpr int kV0Smoother = GP_SMOOTH_L1_JACOBI;
 
 // y = A x (mode 0), y = -(A x) (mode 1), y = y + A x (mode 2); one wavefront
 // per row, so the long rows of V0d^T (the conductor aggregate columns of V0d)
@@ -109,6 +107,7 @@ gp_status setup(gp_layout *lay, V0Path **out) {
   P->nd = V.v0d_cols;
   P->nc = V.v0c_cols;
   P->ne = lay->g.Ne;
+  P->smoother = lay->v0_smoother;
   gp_status st = GP_OK;
   do {
     const double *dv[2] = {V.v0d.val, V.v0d.aval};
@@ -121,8 +120,8 @@ gp_status setup(gp_layout *lay, V0Path **out) {
     }
     // Ad: the kept entries only; the reference's trailing (0, 0, 0) slots are
     // never read by hypreSolve's row-by-row insertion (src/hypreSolve.cpp:47-59)
-    if ((st = real_amg_create(P->nd, V.ad.n, V.ad.row, V.ad.col, V.ad.val, kV0Smoother, &P->ad, s)) != GP_OK) break;
-    if (P->nc > 0 && (st = real_amg_create(P->nc, V.ac.n, V.ac.row, V.ac.col, V.ac.val, kV0Smoother, &P->ac, s)) != GP_OK)
+    if ((st = real_amg_create(P->nd, V.ad.n, V.ad.row, V.ad.col, V.ad.val, P->smoother, &P->ad, s)) != GP_OK) break;
+    if (P->nc > 0 && (st = real_amg_create(P->nc, V.ac.n, V.ac.row, V.ac.col, V.ac.val, P->smoother, &P->ac, s)) != GP_OK)
       break;
     if (hipStreamSynchronize(s) != hipSuccess) { st = fail(GP_ERR_HIP, "V0 path setup"); break; }
   } while (0);
@@ -228,6 +227,19 @@ int64_t v0path_bytes(const V0Path *p) {
 }  // namespace gp
 
 using namespace gp;
+
+extern "C" gp_status gp_v0_set_smoother(gp_layout *lay, int32_t smoother) {
+  if (!lay) return fail(GP_ERR_ARG, "null layout");
+  if (smoother != GP_SMOOTH_L1_JACOBI && smoother != GP_SMOOTH_CHEBYSHEV) return fail(GP_ERR_ARG, "unknown real smoother");
+  lay->v0_smoother = smoother;
+  // a hierarchy built with the other smoother is rebuilt by the next V0 call
+  if (lay->v0path && lay->v0path->smoother != smoother) {
+    GP_HIP(hipStreamSynchronize(lay->stream));
+    destroy_v0path(lay->v0path);
+    lay->v0path = nullptr;
+  }
+  return GP_OK;
+}
 
 extern "C" gp_status gp_solve_v0(gp_layout *lay, double rtol, double *Z0, double *X, gp_v0_stats *stats) {
   if (!lay || !Z0) return fail(GP_ERR_ARG, "gp_solve_v0: null layout or Z0");

@@ -134,6 +134,12 @@ class Layout:
                    "gp_apply_fullwave")
         return (yi[0::2] + 1j * yi[1::2]).reshape(len(freqs), n)
 
+    def set_v0_smoother(self, smoother):
+        """gp_v0_set_smoother: GP_SMOOTH_CHEBYSHEV (the default) or
+        GP_SMOOTH_L1_JACOBI for the V0 path's Ad / Ac multigrid; a hierarchy
+        built with the other one is rebuilt by the next V0 call."""
+        capi.check(self.lib.gp_v0_set_smoother(self.h, int(smoother)), "gp_v0_set_smoother")
+
     def solve_v0(self, rtol=0.0):
         """paraGenerator's V0 quasi-static solve (gp_solve_v0).
 

@@ -19,6 +19,7 @@
  *   gp_solve_real_ex        hypreSolve with HYPRE_BoomerAMGSetRelaxType's choice  src/hypreSolve.cpp:230
  *   gp_solve_v0             paraGenerator's V0 section + Construct_Z_V0   src/matrixCon.cpp:366-560,
  *                                                                   src/fdtd.hpp:4178
+ *   gp_v0_set_smoother      HYPRE_BoomerAMGSetRelaxType for the V0 solves  src/hypreSolve.cpp:230
  *   gp_z_v0_sweep           fdtdMesh::print_z_V0 (its store)        src/fdtd.hpp:4249
  *   gp_extract_z            fdtdMesh::Construct_Z_V0_Vh             src/fdtd.hpp:4133
  *   gp_find_vh              fdtdMesh::find_Vh(int k)                src/fdtd.hpp:3577
@@ -505,6 +506,14 @@ typedef struct gp_v0_stats {
  * reference's 1e-4.  Builds the V0 basis if the layout has none; the V0
  * operators and the Ad / Ac hierarchies stay with the layout for later calls. */
 gp_status gp_solve_v0(gp_layout *lay, double rtol, double *Z0, double *X, gp_v0_stats *stats);
+
+/* The smoother (gp_real_smoother) of the V0 path's Ad and Ac multigrid, the
+ * relaxation the reference sets with HYPRE_BoomerAMGSetRelaxType for those
+ * hypreSolve calls (src/hypreSolve.cpp:230).  GP_SMOOTH_CHEBYSHEV until set
+ * (fewer iterations than l1-Jacobi at the same products per level); a
+ * hierarchy built with the other smoother is dropped and rebuilt by the next
+ * gp_solve_v0 / gp_v0_vh_inputs. */
+gp_status gp_v0_set_smoother(gp_layout *lay, int32_t smoother);
 
 /* print_z_V0's store (src/fdtd.hpp:4249-4316), host only: X[0 .. np*np) holds
  * Z0 on entry; slot k >= 1 receives Re Z0[j + i np] + i Im Z0[j + i np] f0 / f_k

@@ -91,6 +91,11 @@ def test_sweep_wrapper_matches_oracle():
     assert np.array_equal(z_v0_sweep(m, Z0), oracle.print_z_v0(m, Z0))
 
 
+def test_v0_set_smoother_rejects_bad_arguments(lib):
+    # argument checks come before any device call, so this runs without a GPU
+    assert lib.gp_v0_set_smoother(None, capi.GP_SMOOTH_CHEBYSHEV) == capi.GP_ERR_ARG
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", list(SPECS))
 def test_gpu_v0_matches_oracle(name):
@@ -111,6 +116,14 @@ def test_gpu_v0_matches_oracle(name):
     Z1, X1, st1 = lay.solve_v0(rtol=1e-10)
     assert st1["setup_ms"] == st["setup_ms"]
     assert np.array_equal(Z1, Z0) and np.array_equal(X1, X)
+    # l1-Jacobi smoothing (the default is Chebyshev) rebuilds the Ad / Ac
+    # multigrid and meets the same bar
+    lay.set_v0_smoother(capi.GP_SMOOTH_L1_JACOBI)
+    Z2, X2, st2 = lay.solve_v0(rtol=1e-10)
+    print(f"[v0path] {name} l1-Jacobi: max {st2['max_iters']} iterations (Chebyshev {st['max_iters']})", flush=True)
+    assert st2["max_rel_residual"] <= 1e-10
+    assert _rel(Z2, Zo) < 1e-6, _rel(Z2, Zo)
+    assert _rel(X2, Xo) < 1e-6
     lay.close()
 
 
