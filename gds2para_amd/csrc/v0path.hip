@@ -231,13 +231,13 @@ using namespace gp;
 extern "C" gp_status gp_v0_set_smoother(gp_layout *lay, int32_t smoother) {
   if (!lay) return fail(GP_ERR_ARG, "null layout");
   if (smoother != GP_SMOOTH_L1_JACOBI && smoother != GP_SMOOTH_CHEBYSHEV) return fail(GP_ERR_ARG, "unknown real smoother");
-  lay->v0_smoother = smoother;
   // a hierarchy built with the other smoother is rebuilt by the next V0 call
   if (lay->v0path && lay->v0path->smoother != smoother) {
     GP_HIP(hipStreamSynchronize(lay->stream));
     destroy_v0path(lay->v0path);
     lay->v0path = nullptr;
   }
+  lay->v0_smoother = smoother;
   return GP_OK;
 }
 
